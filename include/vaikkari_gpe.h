@@ -1,0 +1,211 @@
+/*
+ * vaikkari_gpe.h -- C ABI of the MI355X (gfx950) Gross-Pitaevskii stepper.
+ *
+ * The reference (markus-kivioja/vaikkari) has no plugin or FFI layer: its
+ * single-device solver is the free function
+ *   uint integrateInTime(const VortexState&, ddouble block_scale,
+ *                        const Vector3& minp, const Vector3& maxp,
+ *                        ddouble iteration_period, uint number_of_iterations)
+ * (koodit/GrossPitaevskii/GPUGrossPitaevskii_BCC.cu:218) around one kernel
+ *   update(PitchedPtr next, PitchedPtr prev, PitchedPtr pot, int2* lapind,
+ *          double2 lapfacs, double g, uint3 dims)
+ * (GPUGrossPitaevskii_BCC.cu:176).  This header replaces both: a solver object
+ * owns the device state (the reference's cudaMalloc3D buffers,
+ * GPUGrossPitaevskii_BCC.cu:269-287) and gpe_step() replaces the host loop of
+ * update<<<>>> launches (GPUGrossPitaevskii_BCC.cu:475-481).
+ * gpe_integrate_in_time() is the drop-in for integrateInTime itself.
+ *
+ * Discretisation (identical to the reference): body-centred-cubic Delaunay
+ * mesh, psi on the 12 dual nodes (tetrahedron circumcentres) of every cubic
+ * cell, 4-neighbour DEC Laplacian with Hodge weight 1.5/b^2, homogeneous
+ * Dirichlet (zero) values outside the nx*ny*nz cell box.
+ *
+ * Host array order everywhere: [k][j][i][l] with the class l (0..11) fastest,
+ * exactly the reference's Psi0/pot order without padding
+ * (GPUGrossPitaevskii_BCC.cu:257).  Complex arrays interleave (re, im).
+ *
+ * Errors are returned as gpe_status; nothing exits the process.  The text of
+ * the last error of the calling thread is gpe_last_error().  One solver must
+ * not be used from two host threads at once.
+ */
+#ifndef VAIKKARI_GPE_H
+#define VAIKKARI_GPE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPE_CLASSES 12
+#define GPE_FACES 4
+
+typedef enum gpe_status {
+    GPE_OK = 0,
+    GPE_EINVAL = 1,    /* bad argument or configuration */
+    GPE_ENOMEM = 2,    /* device or host allocation failed */
+    GPE_EDEVICE = 3,   /* HIP runtime error */
+    GPE_EUNSTABLE = 4, /* non-finite values or a time step beyond the stability bound */
+    GPE_EBUSY = 5,     /* resource budget exhausted (batch / serving admission) */
+    GPE_ENOTSUP = 6,   /* combination not supported */
+    GPE_EIO = 7        /* file could not be read */
+} gpe_status;
+
+typedef enum gpe_precision {
+    GPE_FP64 = 0,     /* complex FP64 storage and arithmetic (parity path) */
+    GPE_FP32_BULK = 1 /* complex FP32 storage and stencil; FP64 reductions */
+} gpe_precision;
+
+typedef enum gpe_mode {
+    GPE_REAL_TIME = 0, /* leapfrog, GPUGrossPitaevskii_BCC.cu:475-481 */
+    GPE_IMAG_TIME = 1  /* Euler step + FP64 renormalisation every step */
+} gpe_mode;
+
+typedef enum gpe_potential_kind {
+    GPE_POT_ARRAY = 0, /* V streamed from an array given by gpe_set_potential */
+    GPE_POT_TRAP = 1   /* V = 0.5*(x^2 + y^2 + ratio^2 z^2) evaluated in-kernel
+                          (potentialV3, GPUGrossPitaevskii_BCC.cu:33-36) */
+} gpe_potential_kind;
+
+/* Kernel family.  GPE_KERNEL_AUTO picks the fastest valid one. */
+typedef enum gpe_kernel {
+    GPE_KERNEL_AUTO = 0,
+    GPE_KERNEL_GATHER = 1, /* one thread per cell, all neighbours via global loads */
+    GPE_KERNEL_ZMARCH = 2, /* one thread per (i,j) column marching in z, register reuse */
+    GPE_KERNEL_FUSED = 3   /* odd and even half-updates fused in one z-march pass */
+} gpe_kernel;
+
+typedef struct gpe_config {
+    uint32_t nx, ny, nz;   /* cubic cells (reference xsize, ysize, zsize) */
+    uint32_t precision;    /* gpe_precision */
+    double block_scale;    /* b; cube edge a = sqrt(8)*b (GPUGrossPitaevskii_BCC.cu:70) */
+    double dt;             /* time step (real time) or dtau (imaginary time), unscaled */
+    double g;              /* interaction strength, unscaled */
+    double origin[3];      /* p0: corner of cell (0,0,0) (GPUGrossPitaevskii_BCC.cu:227) */
+    uint32_t potential;    /* gpe_potential_kind */
+    uint32_t mode;         /* gpe_mode */
+    double trap_ratio;     /* RATIO of the in-kernel trap (GPUGrossPitaevskii_BCC.cu:13) */
+    uint32_t kernel;       /* gpe_kernel */
+    uint32_t reserved;
+} gpe_config;
+
+typedef struct gpe_solver gpe_solver;
+
+typedef struct gpe_solver_info {
+    uint64_t nodes;          /* 12*nx*ny*nz */
+    uint64_t device_bytes;   /* resident device memory of this solver */
+    uint64_t steps_done;     /* steps taken since creation */
+    uint32_t kernel;         /* gpe_kernel in use */
+    uint32_t launches_per_step;
+    double bytes_per_node_step; /* algorithmic HBM bytes per node-step of the hot kernel */
+} gpe_solver_info;
+
+/* ---- solver lifecycle ------------------------------------------------ */
+gpe_status gpe_create(const gpe_config *cfg, gpe_solver **out);
+void gpe_destroy(gpe_solver *s);
+
+/* Unscaled potential V (nodes doubles).  Switches the solver to
+ * GPE_POT_ARRAY; the library multiplies by dt like
+ * GPUGrossPitaevskii_BCC.cu:307. */
+gpe_status gpe_set_potential(gpe_solver *s, const double *V);
+
+/* even, odd: 2*nodes doubles.  odd may be NULL in imaginary-time mode. */
+gpe_status gpe_set_state(gpe_solver *s, const double *even, const double *odd);
+gpe_status gpe_get_state(gpe_solver *s, double *even, double *odd);
+
+/* Switch mode and time step.  Entering real time requires an odd field:
+ * set one with gpe_set_state or gpe_init_odd. */
+gpe_status gpe_set_mode(gpe_solver *s, uint32_t mode, double dt);
+
+/* odd = even * exp(+i*mu*dt/2): the reference's start of the leapfrog
+ * (odd = even * getPhase(-dt/2), GPUGrossPitaevskii_BCC.cu:325,342-343). */
+gpe_status gpe_init_odd(gpe_solver *s, double mu);
+
+/* nsteps time steps.  Real time: nsteps x (odd update, even update).
+ * Imaginary time: nsteps x (Euler step, FP64 renormalisation). */
+gpe_status gpe_step(gpe_solver *s, uint64_t nsteps);
+
+/* As gpe_step, timed with HIP events on the solver's stream; elapsed_ms is
+ * the device time of the whole enqueue, launches the number of hot-kernel
+ * launches it contained. */
+gpe_status gpe_time_steps(gpe_solver *s, uint64_t nsteps, double *elapsed_ms, uint64_t *launches);
+
+/* FP64 observables of field 0 (even) or 1 (odd):
+ * out = {N, K, P, I}: N = w*sum|psi|^2, K = w*sum 1/2 Re(conj(psi)(-L psi)),
+ * P = w*sum V|psi|^2, I = w*sum g|psi|^4, w = sqrt(32/9) b^3.
+ * Energy E = K + P + I/2; chemical potential mu = (K + P + I)/N. */
+gpe_status gpe_observables(gpe_solver *s, int field, double out[4]);
+
+gpe_status gpe_synchronize(gpe_solver *s);
+gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info);
+
+const char *gpe_status_string(gpe_status st);
+const char *gpe_last_error(void);
+
+/* ---- BCC dual-mesh geometry (host) ------------------------------------ */
+/* Class positions q_l (node at b*q_l/sqrt(8) in the cell), 12x3 ints
+ * (GPUGrossPitaevskii_BCC.cu:73-89). */
+void gpe_bcc_positions(int32_t out[36]);
+/* Neighbour table used by the kernels, 48 x (dx, dy, dz, class), in the
+ * face order of GPUGrossPitaevskii_BCC.cu:93-140. */
+void gpe_bcc_stencil(int32_t out[192]);
+/* Neighbour sets derived from the positions alone (all nodes at distance b),
+ * 48 x (dx, dy, dz, class), sorted per class.  Returns the number of
+ * neighbours found (48 for a valid mesh). */
+int gpe_bcc_stencil_derived(int32_t out[192]);
+
+/* ---- reference initial state: VortexState (host) ---------------------- */
+typedef struct gpe_vortex_state gpe_vortex_state;
+/* VortexState::load (koodit/GrossPitaevskii/VortexState.cpp:49-72). */
+gpe_status gpe_vortex_state_load(const char *path, gpe_vortex_state **out);
+void gpe_vortex_state_free(gpe_vortex_state *vs);
+/* {kappa, g, rsize, zsize, rplug, rstep, zstep, mu} */
+void gpe_vortex_state_params(const gpe_vortex_state *vs, double out[8]);
+/* VortexState::getPsi(Vector3) (VortexState.cpp:204-212). */
+void gpe_vortex_state_psi(const gpe_vortex_state *vs, double x, double y, double z, double out[2]);
+/* VortexState::getPhase (VortexState.cpp:198-202). */
+void gpe_vortex_state_phase(const gpe_vortex_state *vs, double time, double out[2]);
+/* searchFunctionMax, searchMaxR, searchMaxZ (VortexState.cpp:240-280). */
+double gpe_vortex_state_search_max(const gpe_vortex_state *vs);
+double gpe_vortex_state_search_max_r(const gpe_vortex_state *vs, double eps);
+double gpe_vortex_state_search_max_z(const gpe_vortex_state *vs, double eps);
+/* integrateCurvature (VortexState.cpp:313-340). */
+double gpe_vortex_state_curvature(const gpe_vortex_state *vs);
+
+/* Grid and origin of integrateInTime (GPUGrossPitaevskii_BCC.cu:223-227). */
+void gpe_reference_dims(double block_scale, const double minp[3], const double maxp[3], uint32_t dims[3],
+                        double origin[3]);
+
+/* Scalars of the reference setup. */
+typedef struct gpe_reference_run {
+    uint32_t dims[3];
+    uint32_t steps_per_iteration; /* GPUGrossPitaevskii_BCC.cu:298 */
+    double origin[3];
+    double dt;                    /* GPUGrossPitaevskii_BCC.cu:299 */
+    double g;                     /* unscaled */
+    double maxpot;                /* GPUGrossPitaevskii_BCC.cu:248-263 */
+    double lapfac_dt, lapfac0_dt; /* GPUGrossPitaevskii_BCC.cu:291-306 */
+} gpe_reference_run;
+
+/* Setup of integrateInTime, GPUGrossPitaevskii_BCC.cu:245-348: psi0 from
+ * the VortexState, V = potentialV3 with the given trap ratio, dt rule, the
+ * seeded disk noise and odd = even*getPhase(-dt/2).  Any array may be NULL
+ * (then only run is filled).  V is unscaled. */
+gpe_status gpe_reference_setup(const gpe_vortex_state *vs, double block_scale, const double minp[3],
+                               const double maxp[3], double iteration_period, double trap_ratio, uint32_t seed,
+                               gpe_reference_run *run, double *even, double *odd, double *V);
+
+/* Drop-in for integrateInTime (GPUGrossPitaevskii_BCC.cu:218-499) without the
+ * picture/volume output: sets up as above, then runs number_of_iterations x
+ * steps_per_iteration leapfrog steps on the GPU.  precision selects the
+ * kernel family.  even_out/odd_out (may be NULL) receive the final fields. */
+gpe_status gpe_integrate_in_time(const gpe_vortex_state *vs, double block_scale, const double minp[3],
+                                 const double maxp[3], double iteration_period, uint32_t number_of_iterations,
+                                 double trap_ratio, uint32_t precision, gpe_reference_run *run, double *even_out,
+                                 double *odd_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VAIKKARI_GPE_H */

@@ -1,0 +1,572 @@
+/*
+ * gpe_oracle.c -- CPU restatement of the reference Gross-Pitaevskii stepper.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * only as the checker (or as the timed CPU baseline).  The product library
+ * (vaikkari_amd/) never links, loads or calls anything in oracle/.
+ *
+ * Build:  gcc -O2 -fopenmp -ffp-contract=off -fPIC -shared  (oracle/Makefile)
+ * -ffp-contract=off keeps every a*b+c as two rounded operations, so the
+ * arithmetic below is exactly the sequence written.
+ *
+ * What it restates (reference = markus-kivioja/vaikkari):
+ *   - class positions            koodit/GrossPitaevskii/GPUGrossPitaevskii_BCC.cu:73-89
+ *   - 48-entry neighbour table   GPUGrossPitaevskii_BCC.cu:90-142 (cell offsets, not bytes)
+ *   - kernel `update`            GPUGrossPitaevskii_BCC.cu:176-216
+ *   - leapfrog loop              GPUGrossPitaevskii_BCC.cu:475-481
+ *   - setup / dt rule / noise    GPUGrossPitaevskii_BCC.cu:218-348
+ *   - VortexState load/getPsi/getFunction/getPhase/searches/curvature
+ *                                koodit/GrossPitaevskii/VortexState.cpp:49-72,198-280,313-340
+ *   - MT19937 + getUniformCircle jukan_koodit/Types/Random.cpp:12-127
+ *   - Complex multiply           jukan_koodit/Types/Complex.hpp:57-63,95-98
+ *
+ * What it defines (absent in the reference; pinned by closed forms and
+ * invariants in tests/):
+ *   - observables N, K, P, I (norm, kinetic, potential, interaction sums)
+ *   - imaginary-time Euler step with per-step FP64 renormalisation
+ *
+ * Storage: the reference's array-of-structures, 12 complex values per cubic
+ * cell, with a one-cell zero halo (GPUGrossPitaevskii_BCC.cu:269-287).  The
+ * exported functions take unpadded host arrays in [k][j][i][l] order (class
+ * fastest, complex interleaved re,im) and pad internally.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+#define OR_CLASSES 12
+#define OR_FACES 4
+
+/* Class positions q_l; the node sits at b*q_l/sqrt(8) inside the cell
+ * (GPUGrossPitaevskii_BCC.cu:76-88). */
+static const int OR_Q[OR_CLASSES][3] = {
+    {1, 3, 7}, {7, 1, 3}, {7, 3, 1}, {7, 3, 5}, {7, 5, 3}, {1, 7, 3},
+    {3, 7, 1}, {3, 7, 5}, {5, 7, 3}, {3, 1, 7}, {3, 5, 7}, {5, 3, 7}};
+
+/* Neighbour table in the reference's face order: for class l, faces
+ * 4l..4l+3 give (cell offset dx,dy,dz ; neighbour class).  The reference
+ * stores byte offsets built from nx,ny,nz (GPUGrossPitaevskii_BCC.cu:93-140);
+ * here the same entries are written as cell offsets. */
+static const int OR_NB[OR_CLASSES * OR_FACES][4] = {
+    /* 0 */ {0, 0, 0, 9}, {0, 0, 0, 10}, {-1, 0, 1, 2}, {-1, 0, 0, 3},
+    /* 1 */ {0, 0, 0, 2}, {0, 0, 0, 3}, {1, -1, 0, 5}, {0, -1, 0, 8},
+    /* 2 */ {0, 0, 0, 1}, {0, 0, 0, 4}, {1, 0, -1, 0}, {0, 0, -1, 11},
+    /* 3 */ {0, 0, 0, 1}, {0, 0, 0, 4}, {1, 0, 0, 0}, {0, 0, 0, 11},
+    /* 4 */ {0, 0, 0, 2}, {0, 0, 0, 3}, {1, 0, 0, 5}, {0, 0, 0, 8},
+    /* 5 */ {0, 0, 0, 6}, {0, 0, 0, 7}, {-1, 1, 0, 1}, {-1, 0, 0, 4},
+    /* 6 */ {0, 0, 0, 5}, {0, 0, 0, 8}, {0, 1, -1, 9}, {0, 0, -1, 10},
+    /* 7 */ {0, 0, 0, 5}, {0, 0, 0, 8}, {0, 1, 0, 9}, {0, 0, 0, 10},
+    /* 8 */ {0, 0, 0, 6}, {0, 0, 0, 7}, {0, 1, 0, 1}, {0, 0, 0, 4},
+    /* 9 */ {0, 0, 0, 0}, {0, 0, 0, 11}, {0, -1, 1, 6}, {0, -1, 0, 7},
+    /* 10 */ {0, 0, 0, 0}, {0, 0, 0, 11}, {0, 0, 1, 6}, {0, 0, 0, 7},
+    /* 11 */ {0, 0, 0, 9}, {0, 0, 0, 10}, {0, 0, 1, 2}, {0, 0, 0, 3}};
+
+/* Hodge weight of every dual edge (GPUGrossPitaevskii_BCC.cu:141). */
+#define OR_HODGE 1.5
+/* Node measure factor: volume of one dual cell = VOLUME * b^3
+ * (GPUGrossPitaevskii_BCC.cu:71). */
+#define OR_VOLUME_SQ (32.0 / 9.0)
+
+void or_positions(int *q) { memcpy(q, OR_Q, sizeof(OR_Q)); }
+void or_neighbours(int *nb) { memcpy(nb, OR_NB, sizeof(OR_NB)); }
+
+/* ------------------------------------------------------------------ */
+/* Padded array-of-structures: cell (i,j,k) of an X*Y*Z grid lives at  */
+/* padded index ((k+1)*(Y+2) + (j+1))*(X+2) + (i+1); 12 complex each. */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int X, Y, Z;
+    size_t sx, sy, sz; /* strides in cells */
+} or_grid;
+
+static or_grid or_mkgrid(int X, int Y, int Z) {
+    or_grid g;
+    g.X = X; g.Y = Y; g.Z = Z;
+    g.sx = 1; g.sy = (size_t)(X + 2); g.sz = (size_t)(X + 2) * (size_t)(Y + 2);
+    return g;
+}
+static size_t or_ncell_padded(const or_grid *g) { return g->sz * (size_t)(g->Z + 2); }
+static size_t or_cell(const or_grid *g, int i, int j, int k) {
+    return (size_t)(k + 1) * g->sz + (size_t)(j + 1) * g->sy + (size_t)(i + 1);
+}
+
+/* host [k][j][i][l] complex -> padded AoS */
+static double *or_pad_complex(const double *h, const or_grid *g) {
+    double *p = (double *)calloc(or_ncell_padded(g) * OR_CLASSES * 2, sizeof(double));
+    if (!p) return NULL;
+    for (int k = 0; k < g->Z; k++)
+        for (int j = 0; j < g->Y; j++)
+            for (int i = 0; i < g->X; i++) {
+                size_t src = (((size_t)k * g->Y + j) * g->X + i) * OR_CLASSES * 2;
+                memcpy(p + or_cell(g, i, j, k) * OR_CLASSES * 2, h + src, OR_CLASSES * 2 * sizeof(double));
+            }
+    return p;
+}
+static void or_unpad_complex(double *h, const double *p, const or_grid *g) {
+    for (int k = 0; k < g->Z; k++)
+        for (int j = 0; j < g->Y; j++)
+            for (int i = 0; i < g->X; i++) {
+                size_t dst = (((size_t)k * g->Y + j) * g->X + i) * OR_CLASSES * 2;
+                memcpy(h + dst, p + or_cell(g, i, j, k) * OR_CLASSES * 2, OR_CLASSES * 2 * sizeof(double));
+            }
+}
+static double *or_pad_real(const double *h, const or_grid *g) {
+    double *p = (double *)calloc(or_ncell_padded(g) * OR_CLASSES, sizeof(double));
+    if (!p) return NULL;
+    for (int k = 0; k < g->Z; k++)
+        for (int j = 0; j < g->Y; j++)
+            for (int i = 0; i < g->X; i++) {
+                size_t src = (((size_t)k * g->Y + j) * g->X + i) * OR_CLASSES;
+                memcpy(p + or_cell(g, i, j, k) * OR_CLASSES, h + src, OR_CLASSES * sizeof(double));
+            }
+    return p;
+}
+
+/* One half-update, GPUGrossPitaevskii_BCC.cu:205-215:
+ *   sum = ((n0 + n1) + n2) + n3
+ *   s   = lapfac*sum + ((lapfac0 + pot) + g*|prev|^2) * prev
+ *   next += (s.y, -s.x)                      i.e. next -= i*s
+ * mode 1 (imaginary time, defined by this build): next = prev - s. */
+static void or_update_padded(double *next, const double *prev, const double *pot, const or_grid *g,
+                             double lapfac, double lapfac0, double gg, int imag) {
+    const long long soff[3] = {(long long)g->sx, (long long)g->sy, (long long)g->sz};
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < g->Z; k++) {
+        for (int j = 0; j < g->Y; j++) {
+            for (int i = 0; i < g->X; i++) {
+                const size_t c = or_cell(g, i, j, k);
+                for (int l = 0; l < OR_CLASSES; l++) {
+                    double sx = 0.0, sy = 0.0;
+                    for (int f = 0; f < OR_FACES; f++) {
+                        const int *e = OR_NB[l * OR_FACES + f];
+                        const size_t nc = (size_t)((long long)c + e[0] * soff[0] + e[1] * soff[1] + e[2] * soff[2]);
+                        const double *v = prev + (nc * OR_CLASSES + (size_t)e[3]) * 2;
+                        if (f == 0) { sx = v[0]; sy = v[1]; }
+                        else { sx = sx + v[0]; sy = sy + v[1]; }
+                    }
+                    const double *pv = prev + (c * OR_CLASSES + (size_t)l) * 2;
+                    const double px = pv[0], py = pv[1];
+                    const double normsq = px * px + py * py;
+                    const double coef = (lapfac0 + pot[c * OR_CLASSES + (size_t)l]) + gg * normsq;
+                    const double s_x = lapfac * sx + coef * px;
+                    const double s_y = lapfac * sy + coef * py;
+                    double *nv = next + (c * OR_CLASSES + (size_t)l) * 2;
+                    if (imag) {
+                        nv[0] = px - s_x;
+                        nv[1] = py - s_y;
+                    } else {
+                        nv[0] = nv[0] + s_y;
+                        nv[1] = nv[1] + (-s_x);
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* Real-time leapfrog, GPUGrossPitaevskii_BCC.cu:475-481: odd <- odd - i*dt*H(even)even,
+ * then even <- even - i*dt*H(odd)odd.  pot, lapfac, lapfac0 and g are
+ * already multiplied by dt (GPUGrossPitaevskii_BCC.cu:304-307). */
+int or_leapfrog(double *even, double *odd, const double *pot_dt, int X, int Y, int Z,
+                double lapfac, double lapfac0, double g_dt, long long nsteps) {
+    or_grid gr = or_mkgrid(X, Y, Z);
+    double *pe = or_pad_complex(even, &gr), *po = or_pad_complex(odd, &gr), *pp = or_pad_real(pot_dt, &gr);
+    if (!pe || !po || !pp) { free(pe); free(po); free(pp); return 1; }
+    for (long long s = 0; s < nsteps; s++) {
+        or_update_padded(po, pe, pp, &gr, lapfac, lapfac0, g_dt, 0);
+        or_update_padded(pe, po, pp, &gr, lapfac, lapfac0, g_dt, 0);
+    }
+    or_unpad_complex(even, pe, &gr);
+    or_unpad_complex(odd, po, &gr);
+    free(pe); free(po); free(pp);
+    return 0;
+}
+
+/* One half-update only (next updated in place), for unit tests. */
+int or_half_update(double *next, const double *prev, const double *pot_dt, int X, int Y, int Z,
+                   double lapfac, double lapfac0, double g_dt, int imag) {
+    or_grid gr = or_mkgrid(X, Y, Z);
+    double *pn = or_pad_complex(next, &gr), *pv = or_pad_complex(prev, &gr), *pp = or_pad_real(pot_dt, &gr);
+    if (!pn || !pv || !pp) { free(pn); free(pv); free(pp); return 1; }
+    or_update_padded(pn, pv, pp, &gr, lapfac, lapfac0, g_dt, imag);
+    or_unpad_complex(next, pn, &gr);
+    free(pn); free(pv); free(pp);
+    return 0;
+}
+
+/* Neumaier-compensated accumulator. */
+typedef struct { double s, c; } or_acc;
+static void or_add(or_acc *a, double x) {
+    double t = a->s + x;
+    if (fabs(a->s) >= fabs(x)) a->c += (a->s - t) + x;
+    else a->c += (x - t) + a->s;
+    a->s = t;
+}
+static double or_val(const or_acc *a) { return a->s + a->c; }
+
+/* Observables of one field (defined by this build; the reference has none):
+ *   N = w * sum |psi|^2
+ *   K = w * sum 1/2 Re(conj(psi) * (-L psi)),  (L psi)_i = (1.5/b^2) sum_f (psi_f - psi_i)
+ *   P = w * sum V |psi|^2
+ *   I = w * sum g |psi|^4
+ * with node measure w = sqrt(32/9) b^3.  Energy E = K + P + I/2, chemical
+ * potential mu = (K + P + I)/N.  V is the unscaled potential, g unscaled.
+ * Sums run sequentially in [k][j][i][l] order with Neumaier compensation. */
+int or_observables(const double *psi, const double *V, int X, int Y, int Z, double block_scale, double g, double *out) {
+    or_grid gr = or_mkgrid(X, Y, Z);
+    double *pp = or_pad_complex(psi, &gr);
+    if (!pp) return 1;
+    const double w = sqrt(OR_VOLUME_SQ) * block_scale * block_scale * block_scale;
+    const double lap = OR_HODGE / (block_scale * block_scale);
+    or_acc aN = {0, 0}, aK = {0, 0}, aP = {0, 0}, aI = {0, 0};
+    for (int k = 0; k < Z; k++)
+        for (int j = 0; j < Y; j++)
+            for (int i = 0; i < X; i++) {
+                const size_t c = or_cell(&gr, i, j, k);
+                for (int l = 0; l < OR_CLASSES; l++) {
+                    double dx = 0.0, dy = 0.0;
+                    const double *pv = pp + (c * OR_CLASSES + (size_t)l) * 2;
+                    for (int f = 0; f < OR_FACES; f++) {
+                        const int *e = OR_NB[l * OR_FACES + f];
+                        const size_t nc = (size_t)((long long)c + e[0] * (long long)gr.sx + e[1] * (long long)gr.sy + e[2] * (long long)gr.sz);
+                        const double *v = pp + (nc * OR_CLASSES + (size_t)e[3]) * 2;
+                        dx += v[0] - pv[0];
+                        dy += v[1] - pv[1];
+                    }
+                    const double n2 = pv[0] * pv[0] + pv[1] * pv[1];
+                    const double vi = V[(((size_t)k * Y + j) * X + i) * OR_CLASSES + (size_t)l];
+                    or_add(&aN, n2);
+                    /* 1/2 Re(conj(psi) * (-lap * d)) */
+                    or_add(&aK, -0.5 * lap * (pv[0] * dx + pv[1] * dy));
+                    or_add(&aP, vi * n2);
+                    or_add(&aI, g * n2 * n2);
+                }
+            }
+    out[0] = w * or_val(&aN);
+    out[1] = w * or_val(&aK);
+    out[2] = w * or_val(&aP);
+    out[3] = w * or_val(&aI);
+    free(pp);
+    return 0;
+}
+
+/* Exact leapfrog invariant Q = Re sum conj(even)*odd (unweighted). */
+double or_overlap(const double *a, const double *b, long long nvals) {
+    or_acc q = {0, 0};
+    for (long long n = 0; n < nvals; n++) or_add(&q, a[2 * n] * b[2 * n] + a[2 * n + 1] * b[2 * n + 1]);
+    return or_val(&q);
+}
+
+/* Imaginary-time Euler steps (defined by this build):
+ *   psi <- psi - dtau*H(psi)psi      (same s as the real-time update)
+ *   N   = w * sum |psi|^2            (Neumaier, sequential)
+ *   psi <- psi * (1/sqrt(N))
+ * pot, lapfac, lapfac0, g are already multiplied by dtau.  out_norms (may
+ * be NULL) receives N before each renormalisation. */
+int or_imag_steps(double *psi, const double *pot_dt, int X, int Y, int Z, double lapfac, double lapfac0,
+                  double g_dt, double block_scale, long long nsteps, double *out_norms) {
+    or_grid gr = or_mkgrid(X, Y, Z);
+    double *pa = or_pad_complex(psi, &gr), *pb = or_pad_complex(psi, &gr), *pp = or_pad_real(pot_dt, &gr);
+    if (!pa || !pb || !pp) { free(pa); free(pb); free(pp); return 1; }
+    const double w = sqrt(OR_VOLUME_SQ) * block_scale * block_scale * block_scale;
+    for (long long s = 0; s < nsteps; s++) {
+        or_update_padded(pb, pa, pp, &gr, lapfac, lapfac0, g_dt, 1);
+        or_acc a = {0, 0};
+        for (int k = 0; k < Z; k++)
+            for (int j = 0; j < Y; j++)
+                for (int i = 0; i < X; i++) {
+                    const double *v = pb + or_cell(&gr, i, j, k) * OR_CLASSES * 2;
+                    for (int l = 0; l < OR_CLASSES; l++) or_add(&a, v[2 * l] * v[2 * l] + v[2 * l + 1] * v[2 * l + 1]);
+                }
+        const double N = w * or_val(&a);
+        if (out_norms) out_norms[s] = N;
+        const double sc = 1.0 / sqrt(N);
+        for (int k = 0; k < Z; k++)
+            for (int j = 0; j < Y; j++)
+                for (int i = 0; i < X; i++) {
+                    double *v = pb + or_cell(&gr, i, j, k) * OR_CLASSES * 2;
+                    for (int l = 0; l < 2 * OR_CLASSES; l++) v[l] = v[l] * sc;
+                }
+        double *t = pa; pa = pb; pb = t;
+    }
+    or_unpad_complex(psi, pa, &gr);
+    free(pa); free(pb); free(pp);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* VortexState (koodit/GrossPitaevskii/VortexState.cpp)                */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    uint32_t kappa;
+    double g;
+    uint32_t rsize, zsize;
+    double rplug, rstep, zstep, mu;
+    double *f; /* f[ir*zsize + iz] */
+} or_vs;
+
+/* VortexState::load, VortexState.cpp:49-72 (packed binary fields). */
+or_vs *or_vs_load(const char *path) {
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return NULL;
+    or_vs *s = (or_vs *)calloc(1, sizeof(or_vs));
+    int ok = 1;
+    ok &= fread(&s->kappa, 4, 1, fp) == 1;
+    ok &= fread(&s->g, 8, 1, fp) == 1;
+    ok &= fread(&s->rsize, 4, 1, fp) == 1;
+    ok &= fread(&s->zsize, 4, 1, fp) == 1;
+    ok &= fread(&s->rplug, 8, 1, fp) == 1;
+    ok &= fread(&s->rstep, 8, 1, fp) == 1;
+    ok &= fread(&s->zstep, 8, 1, fp) == 1;
+    ok &= fread(&s->mu, 8, 1, fp) == 1;
+    if (ok) {
+        size_t n = (size_t)s->rsize * s->zsize;
+        s->f = (double *)malloc(n * sizeof(double));
+        ok &= s->f && fread(s->f, 8, n, fp) == n;
+    }
+    fclose(fp);
+    if (!ok) { free(s->f); free(s); return NULL; }
+    return s;
+}
+void or_vs_free(or_vs *s) { if (s) { free(s->f); free(s); } }
+void or_vs_info(const or_vs *s, double *out) {
+    out[0] = s->kappa; out[1] = s->g; out[2] = s->rsize; out[3] = s->zsize;
+    out[4] = s->rplug; out[5] = s->rstep; out[6] = s->zstep; out[7] = s->mu;
+}
+
+/* VortexState::getFunction, VortexState.cpp:220-238 (bilinear in r,|z|). */
+double or_vs_function(const or_vs *s, double r, double z) {
+    const double rr = (r - s->rplug) / s->rstep;
+    if (rr < 0.0) return 0.0;
+    const uint32_t ir = (uint32_t)rr;
+    if (ir >= s->rsize) return 0.0;
+    const double dr = rr - (double)ir;
+    const double zz = fabs(z) / s->zstep;
+    const uint32_t iz = (uint32_t)zz;
+    if (iz >= s->zsize) return 0.0;
+    const double dz = zz - (double)iz;
+    const uint32_t ii = ir * s->zsize + iz;
+    const double fiir = (ir + 1 < s->rsize ? s->f[ii + s->zsize] : 0.0);
+    const double fiiz = (iz + 1 < s->zsize ? s->f[ii + 1] : 0.0);
+    const double fiirz = (ir + 1 < s->rsize && iz + 1 < s->zsize ? s->f[ii + s->zsize + 1] : 0.0);
+    return (1.0 - dz) * ((1.0 - dr) * s->f[ii] + dr * fiir) + dz * ((1.0 - dr) * fiiz + dr * fiirz);
+}
+
+/* VortexState::getPsi(Vector3), VortexState.cpp:204-212: f(r,z) * ((x+iy)/r)^kappa
+ * by repeated Complex *= (Complex.hpp:57-63). */
+void or_vs_psi(const or_vs *s, double x, double y, double z, double *out) {
+    const double r = sqrt(x * x + y * y);
+    if (r < 1e-30) { out[0] = 0.0; out[1] = 0.0; return; }
+    double cr = or_vs_function(s, r, z), ci = 0.0;
+    const double nr = x / r, ni = y / r;
+    for (uint32_t j = 0; j < s->kappa; j++) {
+        const double rr = cr;
+        cr = rr * nr - ci * ni;
+        ci = ci * nr + rr * ni;
+    }
+    out[0] = cr; out[1] = ci;
+}
+
+/* VortexState::getPhase, VortexState.cpp:198-202. */
+void or_vs_phase(const or_vs *s, double time, double *out) {
+    const double ph = time * s->mu;
+    out[0] = cos(ph);
+    out[1] = -sin(ph);
+}
+
+/* searchFunctionMax / searchMaxR / searchMaxZ, VortexState.cpp:240-280. */
+double or_vs_search_max(const or_vs *s) {
+    double m = 0.0;
+    for (size_t ii = 0; ii < (size_t)s->rsize * s->zsize; ii++) if (s->f[ii] > m) m = s->f[ii];
+    return m;
+}
+double or_vs_search_max_r(const or_vs *s, double eps) {
+    for (size_t ii = (size_t)s->rsize * s->zsize; ii-- > 0;)
+        if (s->f[ii] > eps) return s->rplug + (double)(uint32_t)(ii / s->zsize) * s->rstep;
+    return 0.0;
+}
+double or_vs_search_max_z(const or_vs *s, double eps) {
+    for (uint32_t j = s->zsize; j-- > 0;) {
+        size_t ii = j;
+        for (uint32_t i = 0; i < s->rsize; i++) {
+            if (s->f[ii] > eps) return (double)j * s->zstep;
+            ii += s->zsize;
+        }
+    }
+    return 0.0;
+}
+
+/* VortexState::integrateCurvature, VortexState.cpp:313-340. */
+double or_vs_curvature(const or_vs *s) {
+    const double PIx2 = 6.2831853071795864770;
+    const double PIx2drdz = PIx2 * s->rstep * s->zstep;
+    const double kk = (double)(s->kappa * s->kappa);
+    const double laplr = 1.0 / (s->rstep * s->rstep);
+    const double laplz = (s->zsize > 1 ? 1.0 / (s->zstep * s->zstep) : 0.0);
+    double curv = 0.0;
+    uint32_t i, j, ii;
+    for (i = 1, ii = s->zsize; i < s->rsize; i++) {
+        const double r = s->rplug + i * s->rstep;
+        const double lapl = kk / (r * r) + 2.0 * (laplr + laplz);
+        const double gradr = 0.5 / (r * s->rstep);
+        double dv = PIx2drdz * r;
+        for (j = 0; j < s->zsize; j++, ii++) {
+            const double fiir = (i + 1 < s->rsize ? s->f[ii + s->zsize] : 0.0);
+            const double frii = s->f[ii - s->zsize];
+            const double fiiz = (j + 1 < s->zsize ? s->f[ii + 1] : 0.0);
+            const double fzii = (j > 0 ? s->f[ii - 1] : fiiz);
+            const double iicurv = lapl * s->f[ii] + gradr * (frii - fiir) - laplr * (frii + fiir) - laplz * (fzii + fiiz);
+            curv += iicurv * s->f[ii] * dv;
+            if (j == 0) dv *= 2.0;
+        }
+    }
+    return curv;
+}
+
+/* ------------------------------------------------------------------ */
+/* MT19937, jukan_koodit/Types/Random.cpp:12-55,101-118                */
+/* ------------------------------------------------------------------ */
+typedef struct { uint32_t mt[624]; uint32_t i; } or_mt;
+static void or_mt_init(or_mt *r, uint32_t seed) {
+    r->mt[0] = seed;
+    for (r->i = 1; r->i < 624; r->i++) r->mt[r->i] = 1812433253u * (r->mt[r->i - 1] ^ (r->mt[r->i - 1] >> 30)) + r->i;
+}
+static void or_mt_twist(or_mt *r) {
+    static const uint32_t mag[2] = {0u, 0x9908b0dfu};
+    uint32_t i = 0, y;
+    for (; i < 624 - 397; i++) {
+        y = (r->mt[i] & 0x80000000u) | (r->mt[i + 1] & 0x7fffffffu);
+        r->mt[i] = r->mt[i + 397] ^ (y >> 1) ^ mag[y & 1];
+    }
+    for (; i < 623; i++) {
+        y = (r->mt[i] & 0x80000000u) | (r->mt[i + 1] & 0x7fffffffu);
+        r->mt[i] = r->mt[i + 397 - 624] ^ (y >> 1) ^ mag[y & 1];
+    }
+    y = (r->mt[623] & 0x80000000u) | (r->mt[0] & 0x7fffffffu);
+    r->mt[623] = r->mt[396] ^ (y >> 1) ^ mag[y & 1];
+}
+static uint32_t or_mt_next(or_mt *r) {
+    if (r->i >= 624) { or_mt_twist(r); r->i = 0; }
+    uint32_t y = r->mt[r->i++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+/* Random::getUniformCircle, Random.cpp:77-86: int(u)*2^-31 per axis,
+ * rejecting points outside the unit disk. */
+static void or_mt_circle(or_mt *r, double *x, double *y) {
+    const double INT2DOUBLE = 1.0 / 2147483648.0;
+    do {
+        *x = INT2DOUBLE * (double)(int32_t)or_mt_next(r);
+        *y = INT2DOUBLE * (double)(int32_t)or_mt_next(r);
+    } while (*x * *x + *y * *y > 1.0);
+}
+void or_mt_uints(uint32_t seed, long long n, uint32_t *out) {
+    or_mt r; or_mt_init(&r, seed);
+    for (long long i = 0; i < n; i++) out[i] = or_mt_next(&r);
+}
+void or_mt_circles(uint32_t seed, long long n, double *out) {
+    or_mt r; or_mt_init(&r, seed);
+    for (long long i = 0; i < n; i++) or_mt_circle(&r, out + 2 * i, out + 2 * i + 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* Reference setup, GPUGrossPitaevskii_BCC.cu:218-348                  */
+/* ------------------------------------------------------------------ */
+/* Grid size and origin (GPUGrossPitaevskii_BCC.cu:223-227).
+ * out_dims = {X,Y,Z}; out_p0 = p0. */
+void or_ref_dims(double block_scale, const double *minp, const double *maxp, uint32_t *out_dims, double *out_p0) {
+    const double BW = sqrt(8.0);
+    double dom[3], bw[3];
+    for (int a = 0; a < 3; a++) {
+        dom[a] = maxp[a] - minp[a];
+        out_dims[a] = (uint32_t)(dom[a] / (block_scale * BW)) + 1;
+    }
+    for (int a = 0; a < 3; a++) bw[a] = block_scale * (BW * out_dims[a]);
+    for (int a = 0; a < 3; a++) out_p0[a] = ((minp[a] + maxp[a]) - bw[a]) * 0.5;
+}
+
+/* Node position, GPUGrossPitaevskii_BCC.cu:258. */
+static void or_node_pos(const double *p0, double b, uint32_t i, uint32_t j, uint32_t k, int l, double *p) {
+    const double BW = sqrt(8.0), SQ1_8 = 1.0 / sqrt(8.0);
+    p[0] = p0[0] + b * (i * BW + SQ1_8 * OR_Q[l][0]);
+    p[1] = p0[1] + b * (j * BW + SQ1_8 * OR_Q[l][1]);
+    p[2] = p0[2] + b * (k * BW + SQ1_8 * OR_Q[l][2]);
+}
+void or_node_positions(const double *p0, double b, int X, int Y, int Z, double *out) {
+    size_t n = 0;
+    for (int k = 0; k < Z; k++)
+        for (int j = 0; j < Y; j++)
+            for (int i = 0; i < X; i++)
+                for (int l = 0; l < OR_CLASSES; l++, n++) or_node_pos(p0, b, i, j, k, l, out + 3 * n);
+}
+
+/* potentialV3, GPUGrossPitaevskii_BCC.cu:33-36. */
+static double or_potential(const double *p, double ratio) {
+    return 0.5 * (p[0] * p[0] + p[1] * p[1] + ratio * ratio * p[2] * p[2]);
+}
+
+/* Full reference setup for a loaded VortexState: fills even, odd (complex,
+ * host order) and pot_dt (V*dt) and the scalars
+ *   scal = {steps_per_iteration, dt, lapfac*dt, lapfac0*dt, g*dt, maxpot}.
+ * Arrays must hold X*Y*Z*12 (complex: *2) doubles for the dims from
+ * or_ref_dims.  Follows GPUGrossPitaevskii_BCC.cu:245-266 (psi0, V, maxpot),
+ * :291-307 (dt rule, prescale) and :325-345 (noise, odd phase). */
+int or_ref_setup(const or_vs *s, double block_scale, const double *minp, const double *maxp, double period,
+                 double ratio, uint32_t seed, double *even, double *odd, double *pot_dt, double *scal) {
+    uint32_t d[3];
+    double p0[3];
+    or_ref_dims(block_scale, minp, maxp, d, p0);
+    const size_t n = (size_t)d[0] * d[1] * d[2] * OR_CLASSES;
+    double *psi0 = (double *)malloc(n * 2 * sizeof(double));
+    if (!psi0) return 1;
+    const double g = s->g;
+    double maxpot = 0.0;
+    size_t ii = 0;
+    for (uint32_t k = 0; k < d[2]; k++)
+        for (uint32_t j = 0; j < d[1]; j++)
+            for (uint32_t i = 0; i < d[0]; i++)
+                for (int l = 0; l < OR_CLASSES; l++, ii++) {
+                    double p[3];
+                    or_node_pos(p0, block_scale, i, j, k, l, p);
+                    or_vs_psi(s, p[0], p[1], p[2], psi0 + 2 * ii);
+                    pot_dt[ii] = or_potential(p, ratio);
+                    const double poti = pot_dt[ii] + g * (psi0[2 * ii] * psi0[2 * ii] + psi0[2 * ii + 1] * psi0[2 * ii + 1]);
+                    if (poti > maxpot) maxpot = poti;
+                }
+    double lapfac = -0.5 * OR_HODGE / (block_scale * block_scale);
+    double lapfac0 = 4.0 * (-lapfac);
+    const uint32_t steps = (uint32_t)(period * (maxpot + lapfac0)) + 1;
+    const double dt = period / (double)steps;
+    for (ii = 0; ii < n; ii++) pot_dt[ii] *= dt;
+    double ph[2];
+    or_vs_phase(s, -0.5 * dt, ph);
+    or_mt r;
+    or_mt_init(&r, seed);
+    for (ii = 0; ii < n; ii++) {
+        double cx, cy;
+        or_mt_circle(&r, &cx, &cy);
+        cx = 0.01 * cx;
+        cy = 0.01 * cy;
+        const double nr = cx + 1.0, ni = cy;
+        const double pr = psi0[2 * ii], pi = psi0[2 * ii + 1];
+        const double er = pr * nr - pi * ni, ei = pi * nr + pr * ni;
+        even[2 * ii] = er;
+        even[2 * ii + 1] = ei;
+        odd[2 * ii] = ph[0] * er - ph[1] * ei;
+        odd[2 * ii + 1] = ph[1] * er + ph[0] * ei;
+    }
+    free(psi0);
+    scal[0] = steps;
+    scal[1] = dt;
+    scal[2] = lapfac * dt;
+    scal[3] = lapfac0 * dt;
+    scal[4] = g * dt;
+    scal[5] = maxpot;
+    return 0;
+}

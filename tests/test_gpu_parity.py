@@ -1,0 +1,249 @@
+"""GPU parity: the HIP stepper (through the C ABI) against the pinned CPU
+oracle on the same seeded inputs.
+
+Bars: FP64 real-time updates are bit-identical to the oracle (same
+operation sequence, FMA contraction off); FP64 paths that reduce over the
+grid (imaginary-time renormalisation, observables) match to <= 1e-12
+relative; FP32-bulk matches the FP64 oracle to the tolerance written in
+each test.
+"""
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import vaikkari_amd as va
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = ["gather", "zmarch"]
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def case(dims, h, g, dt, ratio, seed, sigma=1.5, noise=1e-3, mu=10.0):
+    b = h / math.sqrt(8)
+    origin = va.centered_origin(dims, b)
+    even = va.gaussian(dims, origin, b, sigma, noise, seed)
+    odd = va.phase_shift(dims, even, mu, dt)
+    V = va.trap_potential(dims, origin, b, ratio)
+    return b, origin, even, odd, V
+
+
+def announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("potential", ["trap", "array"])
+@pytest.mark.parametrize("dims", [(7, 5, 9), (65, 3, 2), (1, 1, 1), (3, 66, 11)])
+def test_fp64_real_time_bit_exact(orc, kernel, potential, dims):
+    announce(f"fp64 real {kernel} {potential} {dims}")
+    h, g, dt, ratio = 0.5, 300.0, 1e-3, 1.5
+    b, origin, even, odd, V = case(dims, h, g, dt, ratio, seed=11, sigma=1.0)
+    s = va.Solver(dims, b, dt, g, origin, "fp64", "real", potential, ratio, kernel)
+    if potential == "array":
+        s.set_potential(V)
+    s.set_state(even, odd)
+    s.step(25)
+    ge, go = s.get_state()
+    lf, lf0, gdt = va.leapfrog_coefs(b, dt, g)
+    oe, oo = orc.leapfrog(even, odd, V * dt, dims, lf, lf0, gdt, 25)
+    np.testing.assert_array_equal(ge, oe)
+    np.testing.assert_array_equal(go, oo)
+    s.close()
+
+
+def test_fp64_small_config_1000_steps(orc):
+    """32^3 cells (393,216 nodes), h = 0.25, g = 500, dt = 1e-3, 1000 steps, seed 54363:
+    relative L2 <= 1e-12 on both leapfrog levels (observed: 0, bit-identical)."""
+    dims, h, g, dt = (32, 32, 32), 0.25, 500.0, 1e-3
+    announce("fp64 32^3 x 1000")
+    b, origin, even, odd, V = case(dims, h, g, dt, 1.0, seed=54363)
+    s = va.Solver(dims, b, dt, g, origin, "fp64", "real", "trap", 1.0)
+    s.set_state(even, odd)
+    s.step(1000)
+    ge, go = s.get_state()
+    lf, lf0, gdt = va.leapfrog_coefs(b, dt, g)
+    oe, oo = orc.leapfrog(even, odd, V * dt, dims, lf, lf0, gdt, 1000)
+    assert rel_l2(ge, oe) <= 1e-12 and rel_l2(go, oo) <= 1e-12
+    np.testing.assert_array_equal(ge, oe)
+    s.close()
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_fp64_imaginary_time(orc, kernel):
+    dims, h, g = (12, 10, 14), 0.4, 200.0
+    announce(f"fp64 imag {kernel}")
+    b, origin, even, _, V = case(dims, h, g, 1e-3, 2.0, seed=5, sigma=1.2)
+    dtau = 0.5 / (3.0 / b ** 2 + V.max() + g * np.max(even[0::2] ** 2 + even[1::2] ** 2))
+    s = va.Solver(dims, b, dtau, g, origin, "fp64", "imag", "trap", 2.0, kernel)
+    s.set_state(even)
+    s.step(60)
+    ge = s.get_state(odd=False)
+    lf, lf0, gdt = va.leapfrog_coefs(b, dtau, g)
+    oe, _ = orc.imag_steps(even, V * dtau, dims, lf, lf0, gdt, b, 60)
+    assert rel_l2(ge, oe) <= 1e-12
+    assert va.field_norm(dims, b, ge) == pytest.approx(1.0, abs=1e-13)
+    s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("potential", ["trap", "array"])
+def test_observables(orc, precision, potential):
+    dims, h, g = (9, 8, 10), 0.45, 700.0
+    announce(f"observables {precision} {potential}")
+    b, origin, even, odd, V = case(dims, h, g, 1e-3, 1.3, seed=9, sigma=1.1)
+    s = va.Solver(dims, b, 1e-3, g, origin, precision, "real", potential, 1.3)
+    if potential == "array":
+        s.set_potential(V)
+    s.set_state(even, odd)
+    ref = orc.observables(even, V, dims, b, g)
+    got = s.observables(0)
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    for k, r in zip("NKPI", ref):
+        assert got[k] == pytest.approx(r, rel=tol), k
+    s.close()
+
+
+def test_init_odd_matches_host_phase(orc):
+    dims = (5, 6, 7)
+    b, origin, even, _, V = case(dims, 0.5, 100.0, 1e-3, 1.0, seed=2)
+    s = va.Solver(dims, b, 1e-3, 100.0, origin, "fp64", "real", "trap", 1.0)
+    s.set_state(even)
+    s.init_odd(12.5)
+    _, go = s.get_state()
+    np.testing.assert_array_equal(go, va.phase_shift(dims, even, 12.5, 1e-3))
+    s.close()
+
+
+def test_fp32_bulk_error_vs_fp64_oracle(orc):
+    """FP32 storage + FP32 stencil, 32^3 x 1000 steps: psi relative L2 error vs the FP64
+    oracle stated and bounded by 1e-5; the leapfrog invariant drifts by < 1e-5 relative."""
+    dims, h, g, dt = (32, 32, 32), 0.25, 500.0, 1e-3
+    announce("fp32 32^3 x 1000")
+    b, origin, even, odd, V = case(dims, h, g, dt, 1.0, seed=54363)
+    s = va.Solver(dims, b, dt, g, origin, "fp32", "real", "trap", 1.0)
+    s.set_state(even, odd)
+    s.step(1000)
+    ge, go = s.get_state()
+    lf, lf0, gdt = va.leapfrog_coefs(b, dt, g)
+    oe, oo = orc.leapfrog(even, odd, V * dt, dims, lf, lf0, gdt, 1000)
+    err = rel_l2(ge, oe)
+    print(f"[fp32] rel-L2 psi error after 1000 steps: {err:.3e}")
+    assert err <= 1e-5
+    q0, q1 = orc.overlap(even, odd), orc.overlap(ge, go)
+    assert abs(q1 - q0) <= 1e-5 * abs(q0)
+    s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_deterministic(precision):
+    dims = (33, 17, 20)
+    b, origin, even, odd, V = case(dims, 0.4, 900.0, 1e-3, 1.0, seed=4)
+    outs = []
+    for _ in range(2):
+        s = va.Solver(dims, b, 1e-3, 900.0, origin, precision, "real", "trap", 1.0)
+        s.set_state(even, odd)
+        s.step(40)
+        outs.append(s.get_state())
+        s.close()
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+def test_full_size_fp64_invariant():
+    """BCC 128^3 cells (25,165,824 nodes), h = 0.1, g = 2000, trap 0.5(x^2+y^2+4z^2),
+    dt = 2e-4: the exact leapfrog invariant Re<even,odd> holds to 1e-12 relative over 200 steps
+    and the norm stays within the O(dt^2) leapfrog oscillation."""
+    dims, h, g, dt = (128, 128, 128), 0.1, 2000.0, 2e-4
+    announce("fp64 128^3 invariant")
+    b, origin, even, odd, V = case(dims, h, g, dt, 2.0, seed=1, sigma=1.2)
+    s = va.Solver(dims, b, dt, g, origin, "fp64", "real", "trap", 2.0)
+    s.set_state(even, odd)
+    q0 = float(np.dot(even, odd))
+    n0 = s.observables(0)["N"]
+    s.step(200)
+    ge, go = s.get_state()
+    q1 = float(np.dot(ge, go))
+    assert abs(q1 - q0) <= 1e-12 * abs(q0)
+    assert s.observables(0)["N"] == pytest.approx(n0, rel=1e-3)
+    assert np.all(np.isfinite(ge))
+    s.close()
+
+
+def write_vortex_state(path, kappa=1, g=400.0, rsize=60, zsize=80, rstep=0.1, zstep=0.1, mu=7.5):
+    """A VortexState file in the reference's binary format (VortexState.cpp:74-96) holding
+    f(r,z) = r^kappa exp(-(r^2+z^2)/2) (this test's own data)."""
+    r = np.arange(rsize) * rstep
+    z = np.arange(zsize) * zstep
+    f = (r[:, None] ** kappa) * np.exp(-(r[:, None] ** 2 + z[None, :] ** 2) / 2)
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<IdIIdddd", kappa, g, rsize, zsize, 0.0, rstep, zstep, mu))
+        fh.write(f.astype("<f8").tobytes())
+
+
+def test_integrate_in_time_drop_in(orc, tmp_path):
+    """integrateInTime on a VortexState file: setup + 2 iterations of steps_per_iteration
+    leapfrog steps, bit-identical to the oracle's setup and leapfrog."""
+    path = str(tmp_path / "vs.dat")
+    write_vortex_state(path)
+    vs = va.VortexState()
+    assert vs.load(path)
+    eps = 1e-5 * vs.searchFunctionMax()
+    mr, mz = vs.searchMaxR(eps), vs.searchMaxZ(eps)
+    bs = 0.15
+    minp, maxp = (-mr, -mr, -mz), (mr, mr, mz)
+    announce("integrateInTime drop-in")
+    rc, run, ge, go = va.integrateInTime(vs, bs, minp, maxp, 0.05, 2, trap_ratio=0.7, return_fields=True)
+    assert rc == 0
+    info, even, odd, V = va.reference_setup(vs, bs, minp, maxp, 0.05, 0.7)
+    assert run == info
+    oe, oo = orc.leapfrog(even, odd, V * info["dt"], info["dims"], info["lapfac_dt"], info["lapfac0_dt"],
+                          info["g"] * info["dt"], 2 * info["steps_per_iteration"])
+    np.testing.assert_array_equal(ge, oe)
+    np.testing.assert_array_equal(go, oo)
+
+
+def test_imag_then_real_workflow(orc):
+    """Ground state by imaginary time, vortex imprint, then real time: the config-2 workflow
+    at 16^3, every stage against the oracle."""
+    dims, h, g, ratio = (16, 16, 16), 0.3, 800.0, 2.0
+    announce("imag->real workflow")
+    b, origin, even, _, V = case(dims, h, g, 1e-3, ratio, seed=3, sigma=1.0)
+    dtau = 0.5 / (3.0 / b ** 2 + V.max())
+    s = va.Solver(dims, b, dtau, g, origin, "fp64", "imag", "trap", ratio)
+    s.set_state(even)
+    s.step(50)
+    ge = s.get_state(odd=False)
+    lf, lf0, gdt = va.leapfrog_coefs(b, dtau, g)
+    oe, _ = orc.imag_steps(even, V * dtau, dims, lf, lf0, gdt, b, 50)
+    assert rel_l2(ge, oe) <= 1e-12
+    obs = s.observables(0)
+    ref = orc.observables(oe, V, dims, b, g)
+    assert obs["N"] == pytest.approx(ref[0], rel=1e-12)
+    mu = (ref[1] + ref[2] + ref[3]) / ref[0]
+    assert obs["mu"] == pytest.approx(mu, rel=1e-12)
+    # real time from the imaginary-time result with an imprinted vortex line
+    dt = 5e-4
+    psi = va.vortex_line(dims, origin, b, 0.1, -0.2, ge.copy())
+    s.set_mode("real", dt)
+    s.set_state(psi)
+    s.init_odd(mu)
+    s.step(100)
+    re, ro = s.get_state()
+    lf, lf0, gdt = va.leapfrog_coefs(b, dt, g)
+    oe2, oo2 = orc.leapfrog(psi, va.phase_shift(dims, psi, mu, dt), V * dt, dims, lf, lf0, gdt, 100)
+    np.testing.assert_array_equal(re, oe2)
+    np.testing.assert_array_equal(ro, oo2)
+    s.close()

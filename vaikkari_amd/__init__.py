@@ -1,0 +1,30 @@
+"""vaikkari_amd -- MI355X (gfx950) Gross-Pitaevskii stepper on the BCC dual mesh.
+
+The compute lives in libvaikkari_gpe.so (HIP kernels + C ABI,
+include/vaikkari_gpe.h); this package is the host-side mirror of the
+reference's solver interface over that ABI.
+"""
+from .solver import (  # noqa: F401
+    Solver,
+    VortexState,
+    as_complex,
+    bcc_positions,
+    bcc_stencil,
+    bcc_stencil_derived,
+    centered_origin,
+    field_norm,
+    gaussian,
+    integrate_in_time,
+    integrateInTime,
+    leapfrog_coefs,
+    node_measure,
+    node_positions,
+    nodes_of,
+    phase_shift,
+    reference_dims,
+    reference_setup,
+    thomas_fermi,
+    trap_potential,
+    vortex_line,
+    vortex_ring,
+)

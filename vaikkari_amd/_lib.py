@@ -1,0 +1,170 @@
+"""ctypes binding of libvaikkari_gpe.so (declarations in include/vaikkari_gpe.h
+and include/vaikkari_gpe_recipes.h).
+
+The native library is the only implementation: if it is missing, importing
+this module raises.  There is no Python fallback for any computation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libvaikkari_gpe.so")
+
+GPE_CLASSES = 12
+GPE_FACES = 4
+
+# gpe_status
+OK, EINVAL, ENOMEM, EDEVICE, EUNSTABLE, EBUSY, ENOTSUP, EIO = range(8)
+# gpe_precision
+FP64, FP32_BULK = 0, 1
+# gpe_mode
+REAL_TIME, IMAG_TIME = 0, 1
+# gpe_potential_kind
+POT_ARRAY, POT_TRAP = 0, 1
+# gpe_kernel
+KERNEL_AUTO, KERNEL_GATHER, KERNEL_ZMARCH, KERNEL_FUSED = 0, 1, 2, 3
+
+
+class GpeConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("block_scale", C.c_double),
+        ("dt", C.c_double),
+        ("g", C.c_double),
+        ("origin", C.c_double * 3),
+        ("potential", C.c_uint32),
+        ("mode", C.c_uint32),
+        ("trap_ratio", C.c_double),
+        ("kernel", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class GpeSolverInfo(C.Structure):
+    _fields_ = [
+        ("nodes", C.c_uint64),
+        ("device_bytes", C.c_uint64),
+        ("steps_done", C.c_uint64),
+        ("kernel", C.c_uint32),
+        ("launches_per_step", C.c_uint32),
+        ("bytes_per_node_step", C.c_double),
+    ]
+
+
+class GpeReferenceRun(C.Structure):
+    _fields_ = [
+        ("dims", C.c_uint32 * 3),
+        ("steps_per_iteration", C.c_uint32),
+        ("origin", C.c_double * 3),
+        ("dt", C.c_double),
+        ("g", C.c_double),
+        ("maxpot", C.c_double),
+        ("lapfac_dt", C.c_double),
+        ("lapfac0_dt", C.c_double),
+    ]
+
+
+class GpeError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str):
+        super().__init__(f"{where}: status {status} ({detail})")
+        self.status = status
+
+
+_P = C.c_void_p
+_D = C.c_double
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+_I = C.c_int
+_DP = C.POINTER(C.c_double)
+_U32P = C.POINTER(C.c_uint32)
+_I32P = C.POINTER(C.c_int32)
+
+# (name, restype, argtypes) of every exported symbol of the two headers.
+SIGNATURES = [
+    ("gpe_create", _I, [C.POINTER(GpeConfig), C.POINTER(_P)]),
+    ("gpe_destroy", None, [_P]),
+    ("gpe_set_potential", _I, [_P, _DP]),
+    ("gpe_set_state", _I, [_P, _DP, _DP]),
+    ("gpe_get_state", _I, [_P, _DP, _DP]),
+    ("gpe_set_mode", _I, [_P, _U32, _D]),
+    ("gpe_init_odd", _I, [_P, _D]),
+    ("gpe_step", _I, [_P, _U64]),
+    ("gpe_time_steps", _I, [_P, _U64, _DP, C.POINTER(_U64)]),
+    ("gpe_observables", _I, [_P, _I, _DP]),
+    ("gpe_synchronize", _I, [_P]),
+    ("gpe_get_info", _I, [_P, C.POINTER(GpeSolverInfo)]),
+    ("gpe_status_string", C.c_char_p, [_I]),
+    ("gpe_last_error", C.c_char_p, []),
+    ("gpe_bcc_positions", None, [_I32P]),
+    ("gpe_bcc_stencil", None, [_I32P]),
+    ("gpe_bcc_stencil_derived", _I, [_I32P]),
+    ("gpe_vortex_state_load", _I, [C.c_char_p, C.POINTER(_P)]),
+    ("gpe_vortex_state_free", None, [_P]),
+    ("gpe_vortex_state_params", None, [_P, _DP]),
+    ("gpe_vortex_state_psi", None, [_P, _D, _D, _D, _DP]),
+    ("gpe_vortex_state_phase", None, [_P, _D, _DP]),
+    ("gpe_vortex_state_search_max", _D, [_P]),
+    ("gpe_vortex_state_search_max_r", _D, [_P, _D]),
+    ("gpe_vortex_state_search_max_z", _D, [_P, _D]),
+    ("gpe_vortex_state_curvature", _D, [_P]),
+    ("gpe_reference_dims", None, [_D, _DP, _DP, _U32P, _DP]),
+    ("gpe_reference_setup", _I, [_P, _D, _DP, _DP, _D, _D, _U32, C.POINTER(GpeReferenceRun), _DP, _DP, _DP]),
+    ("gpe_integrate_in_time", _I, [_P, _D, _DP, _DP, _D, _U32, _D, _U32, C.POINTER(GpeReferenceRun), _DP, _DP]),
+    ("gpe_centered_origin", None, [_U32P, _D, _DP]),
+    ("gpe_node_positions", None, [_U32P, _DP, _D, _DP]),
+    ("gpe_trap_potential", None, [_U32P, _DP, _D, _D, _DP]),
+    ("gpe_field_norm", _D, [_U32P, _D, _DP]),
+    ("gpe_recipe_gaussian", _I, [_U32P, _DP, _D, _D, _D, _U64, _DP]),
+    ("gpe_recipe_thomas_fermi", _I, [_U32P, _DP, _D, _D, _D, _DP, _DP]),
+    ("gpe_recipe_vortex_line", _I, [_U32P, _DP, _D, _D, _D, _DP]),
+    ("gpe_recipe_vortex_ring", _I, [_U32P, _DP, _D, _DP, _DP, _D, _DP]),
+    ("gpe_recipe_phase_shift", _I, [_U32P, _DP, _D, _D, _DP]),
+]
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m vaikkari_amd.build` "
+            "(hipcc --offload-arch=gfx950); there is no fallback implementation")
+    lib = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def last_error() -> str:
+    msg = lib.gpe_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(status: int, where: str) -> None:
+    if status != OK:
+        raise GpeError(status, where, f"{lib.gpe_status_string(status).decode()}: {last_error()}")
+
+
+def dptr(a: np.ndarray):
+    """Pointer to a C-contiguous float64 array (None passes NULL)."""
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(_DP)
+
+
+def u32(vals):
+    return (C.c_uint32 * len(vals))(*[int(v) for v in vals])
+
+
+def d3(vals):
+    return (C.c_double * 3)(*[float(v) for v in vals])

@@ -1,0 +1,67 @@
+"""Build recipe of the native library (libvaikkari_gpe.so) for gfx950.
+
+Compiles every HIP/C++ source under vaikkari_amd/csrc with hipcc into one
+shared library next to this file.  Usable as ``python -m vaikkari_amd.build``.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import time
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(PKG_DIR, "csrc")
+LIB_NAME = "libvaikkari_gpe.so"
+LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
+ARCH = os.environ.get("GPE_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = ["gpe_solver.hip", "host_setup.cpp", "integrate.cpp"]
+
+FLAGS = [
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    f"--offload-arch={ARCH}",
+    # FP arithmetic is the sequence written (FP64 parity with the oracle).
+    "-ffp-contract=off",
+    # packed FP32 instructions stay off on this target
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-result",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the HIP toolchain is required to build libvaikkari_gpe.so")
+
+
+def build(verbose: bool = True) -> str:
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB_PATH + ".tmp"
+    cmd = [hipcc(), *FLAGS, "-x", "hip", *srcs, "-o", tmp]
+    t0 = time.time()
+    if verbose:
+        print("[build]", " ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+    if verbose and res.stderr.strip():
+        # warnings, e.g. the host pass not knowing the packed-fp32 feature
+        lines = [ln for ln in res.stderr.splitlines() if "packed-fp32-ops" not in ln]
+        if lines:
+            sys.stderr.write("\n".join(lines[:40]) + "\n")
+    os.replace(tmp, LIB_PATH)
+    if verbose:
+        print(f"[build] {LIB_PATH} ({time.time() - t0:.1f}s)", flush=True)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    build()

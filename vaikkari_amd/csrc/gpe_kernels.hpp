@@ -1,0 +1,428 @@
+// gpe_kernels.hpp -- gfx950 device code of the BCC Gross-Pitaevskii stepper.
+//
+// Field layout in HBM (one buffer per time level, "class-row SoA"):
+//   element (i, j, k, l) at  origin + k*sz + j*sy + l*xp + i
+//   with xp = row pitch (elements), sy = 12*xp, sz = (ny+2)*sy.
+// For each cell row (j,k) the 12 class rows are contiguous, each holding the
+// nx cells of that class in x order, so a 64-lane wavefront reading one
+// class of 64 consecutive cells issues one fully coalesced 1 KiB (FP64) or
+// 512 B (FP32) access.  A neighbour in another cell is the same class row
+// shifted by dx + dy*sy + dz*sz: still coalesced, no gathers.  One zero halo
+// cell surrounds the box on every side (homogeneous Dirichlet boundary, as
+// the never-written halo of the reference's pitched buffers,
+// GPUGrossPitaevskii_BCC.cu:269-287); the x halo sits inside a 128-byte left
+// pad so every interior row starts cache-line aligned.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bcc_stencil.hpp"
+
+namespace gpe {
+
+template <typename T> struct cplx;
+template <> struct cplx<double> { using type = double2; };
+template <> struct cplx<float> { using type = float2; };
+template <typename T> using cplx_t = typename cplx<T>::type;
+
+struct Layout {
+    int nx, ny, nz;
+    int padl;            // elements of left pad (x halo at padl-1)
+    long long xp;        // class-row pitch in elements
+    long long sy;        // 12*xp
+    long long sz;        // (ny+2)*sy
+    long long origin;    // element index of (0,0,0,l=0)
+    long long elems;     // elements per field
+};
+
+// In-kernel trap V = 0.5*(x^2 + y^2 + rr*z^2) with node positions written
+// in the operation order of GPUGrossPitaevskii_BCC.cu:258 (p0 + b*(i*sqrt(8)
+// + q_l/sqrt(8))), so V*dt has the same bits as the host's potentialV3*dt.
+struct Trap {
+    double p0[3];
+    double b;
+    double bw;          // sqrt(8)
+    double rr;          // ratio*ratio
+    double bpos[12][3]; // q_l/sqrt(8)
+};
+
+// Scalars of one solve.  *_dt are multiplied by the time step like the
+// reference (GPUGrossPitaevskii_BCC.cu:304-307).
+struct Coefs {
+    double lapfac_dt, lapfac0_dt, g_dt, dt;
+    double w;          // node measure sqrt(32/9) b^3
+    double lap;        // 1.5/b^2 (unscaled Laplacian weight)
+    double g;          // unscaled
+};
+
+enum { FORM_REF = 0, FORM_DIFF = 1 };   // neighbour-sum forms
+enum { POT_ARRAY = 0, POT_TRAP = 1 };
+enum { MODE_REAL = 0, MODE_IMAG = 1 };
+
+__device__ __forceinline__ double trap_v(const Trap &t, int i, int j, int k, int l) {
+#pragma clang fp contract(off)
+    const double x = t.p0[0] + t.b * ((double)(unsigned)i * t.bw + t.bpos[l][0]);
+    const double y = t.p0[1] + t.b * ((double)(unsigned)j * t.bw + t.bpos[l][1]);
+    const double z = t.p0[2] + t.b * ((double)(unsigned)k * t.bw + t.bpos[l][2]);
+    return 0.5 * (x * x + y * y + t.rr * z * z);
+}
+
+// s = dt*H(psi)psi from the four neighbour values, |psi|^2 and V*dt.
+//  FORM_REF  (FP64 parity): lapfac*((n0+n1)+n2)+n3) + ((lapfac0+pot)+g|psi|^2)psi,
+//            the exact sequence of GPUGrossPitaevskii_BCC.cu:207-213.
+//  FORM_DIFF (FP32 bulk):   lapfac*(sum_f (n_f - psi)) + (pot + g|psi|^2)psi,
+//            algebraically equal (lapfac0 = -4 lapfac) and free of the
+//            cancellation between lapfac*sum and lapfac0*psi.
+template <typename T, int FORM>
+struct Hamiltonian {
+    using C = cplx_t<T>;
+    T lapfac, lapfac0, g;
+    __device__ __forceinline__ C apply(const C n[4], C psi, T pot) const {
+#pragma clang fp contract(off)
+        const T nsq = psi.x * psi.x + psi.y * psi.y;
+        C s;
+        if (FORM == FORM_REF) {
+            T sx = n[0].x + n[1].x;
+            T sy = n[0].y + n[1].y;
+            sx = sx + n[2].x; sy = sy + n[2].y;
+            sx = sx + n[3].x; sy = sy + n[3].y;
+            const T c = (lapfac0 + pot) + g * nsq;
+            s.x = lapfac * sx + c * psi.x;
+            s.y = lapfac * sy + c * psi.y;
+        } else {
+            T dx = (n[0].x - psi.x) + (n[1].x - psi.x);
+            T dy = (n[0].y - psi.y) + (n[1].y - psi.y);
+            dx = dx + (n[2].x - psi.x); dy = dy + (n[2].y - psi.y);
+            dx = dx + (n[3].x - psi.x); dy = dy + (n[3].y - psi.y);
+            const T c = pot + g * nsq;
+            s.x = lapfac * dx + c * psi.x;
+            s.y = lapfac * dy + c * psi.y;
+        }
+        return s;
+    }
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Deterministic block sum of NV FP64 values; result valid in thread 0.
+template <int NV, int NWAVES>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV*NWAVES */) {
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) v[q] = wave_sum(v[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) lds[q * NWAVES + wave] = v[q];
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double s = 0.0;
+            for (int w = 0; w < NWAVES; w++) s += lds[q * NWAVES + w];
+            v[q] = s;
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ cplx_t<T> cscale(cplx_t<T> a, T s) {
+    cplx_t<T> r;
+    r.x = s * a.x;
+    r.y = s * a.y;
+    return r;
+}
+
+// ---------------------------------------------------------------------
+// GATHER: one thread per cell, all 12 classes; in-cell neighbours come from
+// the thread's own registers, the 18 off-cell neighbours per cell from
+// global memory (cache hits for all but first touch).  Grid
+// (ceil(nx/64), ceil(ny/TY), nz), block (64, TY).
+// Real time: next[l] += (s.y, -s.x) in place (GPUGrossPitaevskii_BCC.cu:215).
+// Imaginary time: out = psi - s, psi = scale*prev, block partial of
+// sum|out|^2 to partials[block].
+// ---------------------------------------------------------------------
+template <typename T, int FORM, int POT, int MODE, int TY>
+__global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                                    const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
+                                                    Trap trap, double dt, const double *__restrict__ scale,
+                                                    double *__restrict__ partials) {
+#pragma clang fp contract(off)
+    using C = cplx_t<T>;
+    __shared__ double lds[TY];
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    const bool act = i < L.nx && j < L.ny;
+    double part[1] = {0.0};
+    if (act) {
+        const long long base = L.origin + (long long)k * L.sz + (long long)j * L.sy + i;
+        T s = (T)1;
+        if (MODE == MODE_IMAG) s = (T)(*scale);
+        C own[12];
+#pragma unroll
+        for (int l = 0; l < 12; l++) {
+            own[l] = prev[base + l * L.xp];
+            if (MODE == MODE_IMAG) own[l] = cscale<T>(own[l], s);
+        }
+#pragma unroll
+        for (int l = 0; l < 12; l++) {
+            C n[4];
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                const Face e = bcc_face(l, f);
+                if (face_in_cell(e)) {
+                    n[f] = own[e.m];
+                } else {
+                    n[f] = prev[base + e.m * L.xp + e.dx + e.dy * L.sy + e.dz * L.sz];
+                    if (MODE == MODE_IMAG) n[f] = cscale<T>(n[f], s);
+                }
+            }
+            T pv;
+            if (POT == POT_TRAP) pv = (T)(trap_v(trap, i, j, k, l) * dt);
+            else pv = pot[base + l * L.xp];
+            const C sres = H.apply(n, own[l], pv);
+            const long long at = base + l * L.xp;
+            if (MODE == MODE_REAL) {
+                C nx = next[at];
+                nx.x = nx.x + sres.y;
+                nx.y = nx.y + (-sres.x);
+                next[at] = nx;
+            } else {
+                C o;
+                o.x = own[l].x - sres.x;
+                o.y = own[l].y - sres.y;
+                next[at] = o;
+                part[0] += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+            }
+        }
+    }
+    if (MODE == MODE_IMAG) {
+        block_sum<1, TY>(part, lds);
+        if (threadIdx.x == 0 && threadIdx.y == 0)
+            partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = part[0];
+    }
+}
+
+// ---------------------------------------------------------------------
+// ZMARCH: one thread per (i,j) column walking CZ layers in z.  The thread
+// keeps its own cell's 12 values, classes {10,11} of the layer below and
+// classes {2,6} of the layer above in registers -- exactly the values the
+// (0,0,-1) and (0,0,+1) faces need -- so z-neighbours never touch memory
+// twice; x/y/diagonal neighbours are global loads that hit L1/L2.
+// Grid (ceil(nx/64), ceil(ny/TY), ceil(nz/CZ)), block (64, TY).
+// ---------------------------------------------------------------------
+template <typename T, int FORM, int POT, int MODE, int TY, int CZ>
+__global__ void __launch_bounds__(64 * TY) k_zmarch(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                                    const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
+                                                    Trap trap, double dt, const double *__restrict__ scale,
+                                                    double *__restrict__ partials) {
+#pragma clang fp contract(off)
+    using C = cplx_t<T>;
+    __shared__ double lds[TY];
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k0 = blockIdx.z * CZ;
+    const int k1 = min(k0 + CZ, L.nz);
+    const bool act = i < L.nx && j < L.ny;
+    double part[1] = {0.0};
+    if (act) {
+        T s = (T)1;
+        if (MODE == MODE_IMAG) s = (T)(*scale);
+        const long long col = L.origin + (long long)j * L.sy + i;
+        C cur[12], dn10, dn11;
+        {
+            const long long b0 = col + (long long)k0 * L.sz;
+#pragma unroll
+            for (int l = 0; l < 12; l++) cur[l] = prev[b0 + l * L.xp];
+            dn10 = prev[b0 - L.sz + 10 * L.xp];
+            dn11 = prev[b0 - L.sz + 11 * L.xp];
+            if (MODE == MODE_IMAG) {
+#pragma unroll
+                for (int l = 0; l < 12; l++) cur[l] = cscale<T>(cur[l], s);
+                dn10 = cscale<T>(dn10, s);
+                dn11 = cscale<T>(dn11, s);
+            }
+        }
+        for (int k = k0; k < k1; k++) {
+            const long long base = col + (long long)k * L.sz;
+            C up2 = prev[base + L.sz + 2 * L.xp];
+            C up6 = prev[base + L.sz + 6 * L.xp];
+            if (MODE == MODE_IMAG) {
+                up2 = cscale<T>(up2, s);
+                up6 = cscale<T>(up6, s);
+            }
+#pragma unroll
+            for (int l = 0; l < 12; l++) {
+                C n[4];
+#pragma unroll
+                for (int f = 0; f < 4; f++) {
+                    const Face e = bcc_face(l, f);
+                    if (face_in_cell(e)) {
+                        n[f] = cur[e.m];
+                    } else if (e.dx == 0 && e.dy == 0 && e.dz == -1) {
+                        n[f] = (e.m == 10) ? dn10 : dn11;
+                    } else if (e.dx == 0 && e.dy == 0 && e.dz == 1) {
+                        n[f] = (e.m == 2) ? up2 : up6;
+                    } else {
+                        n[f] = prev[base + e.m * L.xp + e.dx + e.dy * L.sy + e.dz * L.sz];
+                        if (MODE == MODE_IMAG) n[f] = cscale<T>(n[f], s);
+                    }
+                }
+                T pv;
+                if (POT == POT_TRAP) pv = (T)(trap_v(trap, i, j, k, l) * dt);
+                else pv = pot[base + l * L.xp];
+                const C sres = H.apply(n, cur[l], pv);
+                const long long at = base + l * L.xp;
+                if (MODE == MODE_REAL) {
+                    C nx = next[at];
+                    nx.x = nx.x + sres.y;
+                    nx.y = nx.y + (-sres.x);
+                    next[at] = nx;
+                } else {
+                    C o;
+                    o.x = cur[l].x - sres.x;
+                    o.y = cur[l].y - sres.y;
+                    next[at] = o;
+                    part[0] += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+                }
+            }
+            dn10 = cur[10];
+            dn11 = cur[11];
+            if (k + 1 < k1) {
+                const long long nb = base + L.sz;
+#pragma unroll
+                for (int l = 0; l < 12; l++) {
+                    if (l == 2) cur[l] = up2;
+                    else if (l == 6) cur[l] = up6;
+                    else {
+                        cur[l] = prev[nb + l * L.xp];
+                        if (MODE == MODE_IMAG) cur[l] = cscale<T>(cur[l], s);
+                    }
+                }
+            }
+        }
+    }
+    if (MODE == MODE_IMAG) {
+        block_sum<1, TY>(part, lds);
+        if (threadIdx.x == 0 && threadIdx.y == 0)
+            partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = part[0];
+    }
+}
+
+// ---------------------------------------------------------------------
+// Observables of one field: per-block FP64 partial sums of N, K, P, I
+// (see vaikkari_gpe.h), written to partials[4*block + q].  FP32 fields are
+// widened to FP64 before any arithmetic.
+// ---------------------------------------------------------------------
+template <typename T, int POT, int TY>
+__global__ void __launch_bounds__(64 * TY) k_observables(const cplx_t<T> *__restrict__ psi, const T *__restrict__ pot,
+                                                         Layout L, Trap trap, double inv_dt, double lap, double g,
+                                                         double *__restrict__ partials) {
+    using C = cplx_t<T>;
+    __shared__ double lds[4 * TY];
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < L.nx && j < L.ny) {
+        const long long base = L.origin + (long long)k * L.sz + (long long)j * L.sy + i;
+#pragma unroll
+        for (int l = 0; l < 12; l++) {
+            const C c = psi[base + l * L.xp];
+            const double pr = (double)c.x, pi = (double)c.y;
+            double dx = 0.0, dy = 0.0;
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                const Face e = bcc_face(l, f);
+                const C nb = psi[base + e.m * L.xp + e.dx + e.dy * L.sy + e.dz * L.sz];
+                dx += (double)nb.x - pr;
+                dy += (double)nb.y - pi;
+            }
+            const double n2 = pr * pr + pi * pi;
+            double v;
+            if (POT == POT_TRAP) v = trap_v(trap, i, j, k, l);
+            else v = (double)pot[base + l * L.xp] * inv_dt;
+            acc[0] += n2;
+            acc[1] += -0.5 * lap * (pr * dx + pi * dy);
+            acc[2] += v * n2;
+            acc[3] += g * n2 * n2;
+        }
+    }
+    block_sum<4, TY>(acc, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+#pragma unroll
+        for (int q = 0; q < 4; q++) partials[4 * b + q] = acc[q];
+    }
+}
+
+// One block of 256 threads reduces NV interleaved partial streams in a
+// fixed order (Neumaier inside each thread, then a fixed tree), multiplies by
+// w and writes out[q].  For the imaginary-time norm (NV == 1, renorm != 0)
+// it also writes out[1] = 1/sqrt(N), the scale the next step applies to
+// every value it reads.
+template <int NV>
+__global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ partials, long long nblocks, double w,
+                                                  int renorm, double *__restrict__ out) {
+    __shared__ double red[NV][256];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        double s = 0.0, c = 0.0;
+        for (long long b = t; b < nblocks; b += 256) {
+            const double x = partials[NV * b + q];
+            const double u = s + x;
+            c += (fabs(s) >= fabs(x)) ? ((s - u) + x) : ((x - u) + s);
+            s = u;
+        }
+        red[q][t] = s + c;
+    }
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+#pragma unroll
+            for (int q = 0; q < NV; q++) red[q][t] += red[q][t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) out[q] = w * red[q][0];
+        if (renorm) out[1] = 1.0 / sqrt(out[0]);
+    }
+}
+
+// a *= s elementwise over a whole buffer (halo zeros stay zero).
+template <typename T>
+__global__ void k_scale_field(cplx_t<T> *__restrict__ a, long long n, const double *__restrict__ scale) {
+    const T s = (T)(*scale);
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        cplx_t<T> v = a[e];
+        v.x = s * v.x;
+        v.y = s * v.y;
+        a[e] = v;
+    }
+}
+
+// odd = even * (cr + i ci) elementwise (GPUGrossPitaevskii_BCC.cu:342-343).
+template <typename T>
+__global__ void k_phase_copy(cplx_t<T> *__restrict__ dst, const cplx_t<T> *__restrict__ src, long long n, double cr,
+                             double ci) {
+#pragma clang fp contract(off)
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const cplx_t<T> v = src[e];
+        const double x = (double)v.x, y = (double)v.y;
+        cplx_t<T> o;
+        o.x = (T)(cr * x - ci * y);
+        o.y = (T)(ci * x + cr * y);
+        dst[e] = o;
+    }
+}
+
+}  // namespace gpe

@@ -1,0 +1,516 @@
+// gpe_solver.hip -- the solver object behind the C ABI (include/vaikkari_gpe.h):
+// device buffers, kernel selection and the time-stepping loop that replaces
+// the reference's host loop of update<<<>>> launches
+// (koodit/GrossPitaevskii/GPUGrossPitaevskii_BCC.cu:406-486).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "gpe_internal.hpp"
+#include "gpe_kernels.hpp"
+#include "gpe_solver.hpp"
+
+using namespace gpe;
+
+namespace gpe {
+
+#define GPE_HIP_TRY(expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return GPE_EDEVICE;                                                             \
+        }                                                                                   \
+    } while (0)
+
+Layout make_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes) {
+    Layout L{};
+    L.nx = (int)nx;
+    L.ny = (int)ny;
+    L.nz = (int)nz;
+    const long long line = (long long)(128 / elem_bytes);  // elements per 128-byte line
+    L.padl = (int)line;
+    L.xp = ((L.padl + (long long)nx + 1 + line - 1) / line) * line;
+    L.sy = 12LL * L.xp;
+    L.sz = (long long)(ny + 2) * L.sy;
+    L.origin = L.sz + L.sy + L.padl;
+    L.elems = (long long)(nz + 2) * L.sz;
+    return L;
+}
+
+Trap make_trap(const double origin[3], double b, double ratio) {
+    Trap t{};
+    for (int a = 0; a < 3; a++) t.p0[a] = origin[a];
+    t.b = b;
+    t.bw = std::sqrt(8.0);
+    t.rr = ratio * ratio;
+    const double sq18 = 1.0 / std::sqrt(8.0);
+    for (int l = 0; l < 12; l++)
+        for (int a = 0; a < 3; a++) t.bpos[l][a] = sq18 * kClassPos[l][a];
+    return t;
+}
+
+}  // namespace gpe
+
+// ---------------------------------------------------------------------
+// kernel dispatch
+// ---------------------------------------------------------------------
+namespace {
+
+constexpr int kTY = 4;    // waves per block (rows of 64 cells)
+constexpr int kCZ = 8;    // layers per z-march block
+
+template <typename T, int FORM, int POT, int MODE>
+hipError_t launch_update(int kernel, cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const Layout &L,
+                         const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, const double *scale,
+                         double *partials, hipStream_t st) {
+    const dim3 block(64, kTY);
+    if (kernel == GPE_KERNEL_ZMARCH) {
+        const dim3 grid((L.nx + 63) / 64, (L.ny + kTY - 1) / kTY, (L.nz + kCZ - 1) / kCZ);
+        hipLaunchKernelGGL((k_zmarch<T, FORM, POT, MODE, kTY, kCZ>), grid, block, 0, st, next, prev, pot, L, H, trap,
+                           dt, scale, partials);
+    } else {
+        const dim3 grid((L.nx + 63) / 64, (L.ny + kTY - 1) / kTY, L.nz);
+        hipLaunchKernelGGL((k_gather<T, FORM, POT, MODE, kTY>), grid, block, 0, st, next, prev, pot, L, H, trap, dt,
+                           scale, partials);
+    }
+    return hipGetLastError();
+}
+
+long long update_blocks(int kernel, const Layout &L) {
+    const long long bx = (L.nx + 63) / 64, by = (L.ny + kTY - 1) / kTY;
+    const long long bz = kernel == GPE_KERNEL_ZMARCH ? (L.nz + kCZ - 1) / kCZ : L.nz;
+    return bx * by * bz;
+}
+
+long long obs_blocks(const Layout &L) { return (long long)((L.nx + 63) / 64) * ((L.ny + kTY - 1) / kTY) * L.nz; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// solver
+// ---------------------------------------------------------------------
+gpe_solver::~gpe_solver() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto *p : field)
+        if (p) (void)hipFree(p);
+    if (pot) (void)hipFree(pot);
+    if (scal) (void)hipFree(scal);
+    if (partials) (void)hipFree(partials);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+size_t gpe_solver::elem_bytes() const { return cfg.precision == GPE_FP64 ? 16 : 8; }
+size_t gpe_solver::nodes() const { return (size_t)cfg.nx * cfg.ny * cfg.nz * GPE_CLASSES; }
+
+static gpe_status validate(const gpe_config *c) {
+    if (!c) {
+        set_error("gpe_create: null config");
+        return GPE_EINVAL;
+    }
+    if (c->nx == 0 || c->ny == 0 || c->nz == 0 || c->nx > 65536 || c->ny > 65536 || c->nz > 65536) {
+        set_error("gpe_create: grid %u x %u x %u out of range", c->nx, c->ny, c->nz);
+        return GPE_EINVAL;
+    }
+    if (c->precision > GPE_FP32_BULK || c->mode > GPE_IMAG_TIME || c->potential > GPE_POT_TRAP ||
+        c->kernel > GPE_KERNEL_FUSED) {
+        set_error("gpe_create: bad enum value");
+        return GPE_EINVAL;
+    }
+    if (!(c->block_scale > 0) || !(c->dt > 0) || !std::isfinite(c->g) || !std::isfinite(c->trap_ratio)) {
+        set_error("gpe_create: block_scale and dt must be positive, g and trap_ratio finite");
+        return GPE_EINVAL;
+    }
+    return GPE_OK;
+}
+
+gpe_status gpe_solver::init(const gpe_config &c) {
+    cfg = c;
+    L = make_layout(c.nx, c.ny, c.nz, elem_bytes());
+    trap = make_trap(c.origin, c.block_scale, c.trap_ratio);
+    kernel = c.kernel == GPE_KERNEL_AUTO ? GPE_KERNEL_ZMARCH : c.kernel;
+    if (kernel == GPE_KERNEL_FUSED) kernel = GPE_KERNEL_ZMARCH;  // fused pass not available: z-march instead
+    GPE_HIP_TRY(hipGetDevice(&device));
+    GPE_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    GPE_HIP_TRY(hipEventCreate(&ev0));
+    GPE_HIP_TRY(hipEventCreate(&ev1));
+    const size_t fbytes = (size_t)L.elems * elem_bytes();
+    for (auto &p : field) {
+        if (hipMalloc(&p, fbytes) != hipSuccess) {
+            p = nullptr;
+            set_error("gpe_create: cannot allocate %zu bytes of device memory", fbytes);
+            return GPE_ENOMEM;
+        }
+        GPE_HIP_TRY(hipMemsetAsync(p, 0, fbytes, stream));
+    }
+    npartials = std::max(update_blocks(GPE_KERNEL_GATHER, L), obs_blocks(L));
+    GPE_HIP_TRY(hipMalloc(&partials, (size_t)npartials * 4 * sizeof(double)));
+    GPE_HIP_TRY(hipMalloc(&scal, 8 * sizeof(double)));
+    const double init_scal[8] = {1.0, 1.0, 0, 0, 0, 0, 0, 0};
+    GPE_HIP_TRY(hipMemcpyAsync(scal, init_scal, sizeof(init_scal), hipMemcpyHostToDevice, stream));
+    GPE_HIP_TRY(hipStreamSynchronize(stream));
+    set_coefs();
+    return GPE_OK;
+}
+
+void gpe_solver::set_coefs() {
+    const double b = cfg.block_scale;
+    const double lapfac = -0.5 * kHodge / (b * b);
+    const double lapfac0 = (double)GPE_FACES * (-lapfac);
+    co.lapfac_dt = lapfac * cfg.dt;
+    co.lapfac0_dt = lapfac0 * cfg.dt;
+    co.g_dt = cfg.g * cfg.dt;
+    co.dt = cfg.dt;
+    co.w = node_measure(b);
+    co.lap = kHodge / (b * b);
+    co.g = cfg.g;
+}
+
+size_t gpe_solver::device_bytes() const {
+    size_t b = 2 * (size_t)L.elems * elem_bytes();
+    if (pot) b += (size_t)L.elems * (elem_bytes() / 2);
+    return b + (size_t)npartials * 4 * sizeof(double) + 8 * sizeof(double);
+}
+
+// Host [k][j][i][l] complex double <-> device class-row layout.
+template <typename T>
+static void pack_field(const double *h, std::vector<T> &d, const Layout &L) {
+    d.assign((size_t)L.elems * 2, T(0));
+    size_t n = 0;
+    for (int k = 0; k < L.nz; k++)
+        for (int j = 0; j < L.ny; j++)
+            for (int i = 0; i < L.nx; i++)
+                for (int l = 0; l < 12; l++, n++) {
+                    const long long e = L.origin + (long long)k * L.sz + (long long)j * L.sy + l * L.xp + i;
+                    d[2 * e] = (T)h[2 * n];
+                    d[2 * e + 1] = (T)h[2 * n + 1];
+                }
+}
+template <typename T>
+static void unpack_field(const std::vector<T> &d, double *h, const Layout &L) {
+    size_t n = 0;
+    for (int k = 0; k < L.nz; k++)
+        for (int j = 0; j < L.ny; j++)
+            for (int i = 0; i < L.nx; i++)
+                for (int l = 0; l < 12; l++, n++) {
+                    const long long e = L.origin + (long long)k * L.sz + (long long)j * L.sy + l * L.xp + i;
+                    h[2 * n] = (double)d[2 * e];
+                    h[2 * n + 1] = (double)d[2 * e + 1];
+                }
+}
+
+template <typename T>
+gpe_status gpe_solver::upload(int which, const double *h) {
+    std::vector<T> d;
+    pack_field<T>(h, d, L);
+    GPE_HIP_TRY(hipMemcpyAsync(field[which], d.data(), d.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    GPE_HIP_TRY(hipStreamSynchronize(stream));
+    return GPE_OK;
+}
+
+template <typename T>
+gpe_status gpe_solver::download(int which, double *h) {
+    std::vector<T> d((size_t)L.elems * 2);
+    GPE_HIP_TRY(hipMemcpyAsync(d.data(), field[which], d.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+    GPE_HIP_TRY(hipStreamSynchronize(stream));
+    unpack_field<T>(d, h, L);
+    return GPE_OK;
+}
+
+gpe_status gpe_solver::upload_potential() {
+    // V*dt, multiplied on the host exactly as GPUGrossPitaevskii_BCC.cu:307.
+    const size_t eb = elem_bytes() / 2;
+    std::vector<double> d64;
+    std::vector<float> d32;
+    if (eb == 8) d64.assign((size_t)L.elems, 0.0);
+    else d32.assign((size_t)L.elems, 0.0f);
+    size_t n = 0;
+    for (int k = 0; k < L.nz; k++)
+        for (int j = 0; j < L.ny; j++)
+            for (int i = 0; i < L.nx; i++)
+                for (int l = 0; l < 12; l++, n++) {
+                    const long long e = L.origin + (long long)k * L.sz + (long long)j * L.sy + l * L.xp + i;
+                    const double v = V_host[n] * cfg.dt;
+                    if (eb == 8) d64[e] = v;
+                    else d32[e] = (float)v;
+                }
+    if (!pot && hipMalloc(&pot, (size_t)L.elems * eb) != hipSuccess) {
+        pot = nullptr;
+        set_error("gpe_set_potential: cannot allocate the potential");
+        return GPE_ENOMEM;
+    }
+    GPE_HIP_TRY(hipMemcpyAsync(pot, eb == 8 ? (void *)d64.data() : (void *)d32.data(), (size_t)L.elems * eb,
+                               hipMemcpyHostToDevice, stream));
+    GPE_HIP_TRY(hipStreamSynchronize(stream));
+    return GPE_OK;
+}
+
+template <typename T, int FORM>
+gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
+    using C = cplx_t<T>;
+    Hamiltonian<T, FORM> H;
+    H.lapfac = (T)co.lapfac_dt;
+    H.lapfac0 = (T)co.lapfac0_dt;
+    H.g = (T)co.g_dt;
+    const T *p = (const T *)pot;
+    const bool arr = cfg.potential == GPE_POT_ARRAY;
+    uint64_t nl = 0;
+    hipError_t e = hipSuccess;
+    if (cfg.mode == GPE_REAL_TIME) {
+        C *ev = (C *)field[cur], *od = (C *)field[cur ^ 1];
+        for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
+            if (arr) {
+                e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(kernel, od, ev, p, L, H, trap, co.dt, nullptr, nullptr, stream);
+                if (e == hipSuccess)
+                    e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(kernel, ev, od, p, L, H, trap, co.dt, nullptr, nullptr, stream);
+            } else {
+                e = launch_update<T, FORM, POT_TRAP, MODE_REAL>(kernel, od, ev, p, L, H, trap, co.dt, nullptr, nullptr, stream);
+                if (e == hipSuccess)
+                    e = launch_update<T, FORM, POT_TRAP, MODE_REAL>(kernel, ev, od, p, L, H, trap, co.dt, nullptr, nullptr, stream);
+            }
+            nl += 2;
+        }
+    } else {
+        // Euler step into the other buffer; renormalisation folded into the
+        // next read through scal[1] = 1/sqrt(N).
+        for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
+            C *in = (C *)field[cur], *out = (C *)field[cur ^ 1];
+            if (arr)
+                e = launch_update<T, FORM, POT_ARRAY, MODE_IMAG>(kernel, out, in, p, L, H, trap, co.dt, scal + 1, partials, stream);
+            else
+                e = launch_update<T, FORM, POT_TRAP, MODE_IMAG>(kernel, out, in, p, L, H, trap, co.dt, scal + 1, partials, stream);
+            if (e != hipSuccess) break;
+            hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(256), 0, stream, partials, update_blocks(kernel, L), co.w, 1, scal);
+            e = hipGetLastError();
+            cur ^= 1;
+            nl += 1;
+        }
+        if (e == hipSuccess && nsteps > 0) {
+            // leave a normalised field behind: field[cur] *= scal[1], scal[1] = 1
+            hipLaunchKernelGGL((k_scale_field<T>), dim3(1024), dim3(256), 0, stream, (C *)field[cur], L.elems, scal + 1);
+            e = hipGetLastError();
+            const double one = 1.0;
+            if (e == hipSuccess) e = hipMemcpyAsync(scal + 1, &one, sizeof(double), hipMemcpyHostToDevice, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);  // `one` lives on this stack frame
+        }
+    }
+    if (e != hipSuccess) {
+        set_error("gpe_step: launch failed: %s", hipGetErrorString(e));
+        return GPE_EDEVICE;
+    }
+    steps_done += nsteps;
+    if (launches) *launches = nl;
+    return GPE_OK;
+}
+
+gpe_status gpe_solver::step(uint64_t nsteps, uint64_t *launches) {
+    if (cfg.mode == GPE_REAL_TIME && !odd_valid) {
+        set_error("gpe_step: real-time stepping needs an odd field (gpe_set_state or gpe_init_odd)");
+        return GPE_EINVAL;
+    }
+    if (cfg.potential == GPE_POT_ARRAY && !pot) {
+        set_error("gpe_step: potential kind is ARRAY but no potential was set");
+        return GPE_EINVAL;
+    }
+    if (cfg.precision == GPE_FP64) return enqueue_steps<double, FORM_REF>(nsteps, launches);
+    return enqueue_steps<float, FORM_DIFF>(nsteps, launches);
+}
+
+template <typename T>
+gpe_status gpe_solver::observables_t(int which, double out[4]) {
+    const long long nb = obs_blocks(L);
+    const dim3 grid((L.nx + 63) / 64, (L.ny + kTY - 1) / kTY, L.nz), block(64, kTY);
+    const cplx_t<T> *f = (const cplx_t<T> *)field[which];
+    if (cfg.potential == GPE_POT_ARRAY)
+        hipLaunchKernelGGL((k_observables<T, POT_ARRAY, kTY>), grid, block, 0, stream, f, (const T *)pot, L, trap,
+                           1.0 / cfg.dt, co.lap, co.g, partials);
+    else
+        hipLaunchKernelGGL((k_observables<T, POT_TRAP, kTY>), grid, block, 0, stream, f, (const T *)pot, L, trap,
+                           1.0 / cfg.dt, co.lap, co.g, partials);
+    GPE_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_finalize<4>), dim3(1), dim3(256), 0, stream, partials, nb, co.w, 0, scal + 4);
+    GPE_HIP_TRY(hipGetLastError());
+    GPE_HIP_TRY(hipMemcpyAsync(out, scal + 4, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    GPE_HIP_TRY(hipStreamSynchronize(stream));
+    return GPE_OK;
+}
+
+// ---------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------
+extern "C" {
+
+gpe_status gpe_create(const gpe_config *cfg, gpe_solver **out) {
+    if (!out) {
+        set_error("gpe_create: null output pointer");
+        return GPE_EINVAL;
+    }
+    *out = nullptr;
+    gpe_status st = validate(cfg);
+    if (st != GPE_OK) return st;
+    auto *s = new (std::nothrow) gpe_solver();
+    if (!s) return GPE_ENOMEM;
+    st = s->init(*cfg);
+    if (st != GPE_OK) {
+        delete s;
+        return st;
+    }
+    *out = s;
+    return GPE_OK;
+}
+
+void gpe_destroy(gpe_solver *s) { delete s; }
+
+gpe_status gpe_set_potential(gpe_solver *s, const double *V) {
+    if (!s || !V) {
+        set_error("gpe_set_potential: null argument");
+        return GPE_EINVAL;
+    }
+    s->V_host.assign(V, V + s->nodes());
+    for (double v : s->V_host)
+        if (!std::isfinite(v)) {
+            set_error("gpe_set_potential: non-finite potential value");
+            return GPE_EINVAL;
+        }
+    s->cfg.potential = GPE_POT_ARRAY;
+    return s->upload_potential();
+}
+
+gpe_status gpe_set_state(gpe_solver *s, const double *even, const double *odd) {
+    if (!s || !even) {
+        set_error("gpe_set_state: null argument");
+        return GPE_EINVAL;
+    }
+    gpe_status st;
+    const bool f64 = s->cfg.precision == GPE_FP64;
+    st = f64 ? s->upload<double>(s->cur, even) : s->upload<float>(s->cur, even);
+    if (st != GPE_OK) return st;
+    if (odd) {
+        st = f64 ? s->upload<double>(s->cur ^ 1, odd) : s->upload<float>(s->cur ^ 1, odd);
+        if (st != GPE_OK) return st;
+        s->odd_valid = true;
+    } else {
+        s->odd_valid = false;
+    }
+    const double one = 1.0;
+    GPE_HIP_TRY(hipMemcpy(s->scal + 1, &one, sizeof(double), hipMemcpyHostToDevice));
+    return GPE_OK;
+}
+
+gpe_status gpe_get_state(gpe_solver *s, double *even, double *odd) {
+    if (!s) {
+        set_error("gpe_get_state: null solver");
+        return GPE_EINVAL;
+    }
+    const bool f64 = s->cfg.precision == GPE_FP64;
+    if (even) {
+        gpe_status st = f64 ? s->download<double>(s->cur, even) : s->download<float>(s->cur, even);
+        if (st != GPE_OK) return st;
+    }
+    if (odd) {
+        gpe_status st = f64 ? s->download<double>(s->cur ^ 1, odd) : s->download<float>(s->cur ^ 1, odd);
+        if (st != GPE_OK) return st;
+    }
+    return GPE_OK;
+}
+
+gpe_status gpe_set_mode(gpe_solver *s, uint32_t mode, double dt) {
+    if (!s || mode > GPE_IMAG_TIME || !(dt > 0)) {
+        set_error("gpe_set_mode: invalid argument");
+        return GPE_EINVAL;
+    }
+    GPE_HIP_TRY(hipStreamSynchronize(s->stream));
+    const bool dt_changed = dt != s->cfg.dt;
+    if (mode != s->cfg.mode) s->odd_valid = false;
+    s->cfg.mode = mode;
+    s->cfg.dt = dt;
+    s->set_coefs();
+    if (dt_changed && s->pot) return s->upload_potential();
+    return GPE_OK;
+}
+
+gpe_status gpe_init_odd(gpe_solver *s, double mu) {
+    if (!s || !std::isfinite(mu)) {
+        set_error("gpe_init_odd: invalid argument");
+        return GPE_EINVAL;
+    }
+    // getPhase(-dt/2) = (cos(-mu dt/2), -sin(-mu dt/2)) (VortexState.cpp:198-202).
+    const double ph = (-0.5 * s->cfg.dt) * mu;
+    const double cr = std::cos(ph), ci = -std::sin(ph);
+    if (s->cfg.precision == GPE_FP64)
+        hipLaunchKernelGGL((k_phase_copy<double>), dim3(1024), dim3(256), 0, s->stream, (double2 *)s->field[s->cur ^ 1],
+                           (const double2 *)s->field[s->cur], s->L.elems, cr, ci);
+    else
+        hipLaunchKernelGGL((k_phase_copy<float>), dim3(1024), dim3(256), 0, s->stream, (float2 *)s->field[s->cur ^ 1],
+                           (const float2 *)s->field[s->cur], s->L.elems, cr, ci);
+    GPE_HIP_TRY(hipGetLastError());
+    GPE_HIP_TRY(hipStreamSynchronize(s->stream));
+    s->odd_valid = true;
+    return GPE_OK;
+}
+
+gpe_status gpe_step(gpe_solver *s, uint64_t nsteps) {
+    if (!s) {
+        set_error("gpe_step: null solver");
+        return GPE_EINVAL;
+    }
+    return s->step(nsteps, nullptr);
+}
+
+gpe_status gpe_time_steps(gpe_solver *s, uint64_t nsteps, double *elapsed_ms, uint64_t *launches) {
+    if (!s || !elapsed_ms) {
+        set_error("gpe_time_steps: null argument");
+        return GPE_EINVAL;
+    }
+    GPE_HIP_TRY(hipEventRecord(s->ev0, s->stream));
+    gpe_status st = s->step(nsteps, launches);
+    if (st != GPE_OK) return st;
+    GPE_HIP_TRY(hipEventRecord(s->ev1, s->stream));
+    GPE_HIP_TRY(hipEventSynchronize(s->ev1));
+    float ms = 0.0f;
+    GPE_HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    *elapsed_ms = ms;
+    return GPE_OK;
+}
+
+gpe_status gpe_observables(gpe_solver *s, int which, double out[4]) {
+    if (!s || !out || (which != 0 && which != 1)) {
+        set_error("gpe_observables: invalid argument");
+        return GPE_EINVAL;
+    }
+    const int f = which == 0 ? s->cur : (s->cur ^ 1);
+    if (s->cfg.precision == GPE_FP64) return s->observables_t<double>(f, out);
+    return s->observables_t<float>(f, out);
+}
+
+gpe_status gpe_synchronize(gpe_solver *s) {
+    if (!s) return GPE_EINVAL;
+    GPE_HIP_TRY(hipStreamSynchronize(s->stream));
+    return GPE_OK;
+}
+
+gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info) {
+    if (!s || !info) {
+        set_error("gpe_get_info: null argument");
+        return GPE_EINVAL;
+    }
+    info->nodes = s->nodes();
+    info->device_bytes = s->device_bytes();
+    info->steps_done = s->steps_done;
+    info->kernel = (uint32_t)s->kernel;
+    info->launches_per_step = s->cfg.mode == GPE_REAL_TIME ? 2 : 1;
+    const double S = (double)s->elem_bytes();
+    const double P = s->cfg.potential == GPE_POT_ARRAY ? S / 2 : 0.0;
+    // read prev, read next, write next (+ streamed V) per node-update, two
+    // node-updates per node-step.
+    info->bytes_per_node_step = 2.0 * (3.0 * S + P);
+    return GPE_OK;
+}
+
+}  // extern "C"

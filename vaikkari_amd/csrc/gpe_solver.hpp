@@ -1,0 +1,48 @@
+// gpe_solver.hpp -- the opaque gpe_solver of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "gpe_internal.hpp"
+#include "gpe_kernels.hpp"
+
+namespace gpe {
+Layout make_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes);
+Trap make_trap(const double origin[3], double b, double ratio);
+}  // namespace gpe
+
+struct gpe_solver {
+    gpe_config cfg{};
+    gpe::Layout L{};
+    gpe::Trap trap{};
+    gpe::Coefs co{};
+    int kernel = GPE_KERNEL_GATHER;
+    int device = 0;
+    // field[cur] holds the even time level, field[cur^1] the odd one (or the
+    // imaginary-time scratch buffer).
+    void *field[2] = {nullptr, nullptr};
+    int cur = 0;
+    void *pot = nullptr;           // V*dt in the field layout (GPE_POT_ARRAY)
+    std::vector<double> V_host;    // unscaled V, host order (re-scaled on dt change)
+    double *scal = nullptr;        // [0] last N, [1] read scale, [4..7] observables
+    double *partials = nullptr;    // per-block partial sums
+    long long npartials = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t steps_done = 0;
+    bool odd_valid = false;
+
+    ~gpe_solver();
+    gpe_status init(const gpe_config &c);
+    void set_coefs();
+    size_t elem_bytes() const;
+    size_t nodes() const;
+    size_t device_bytes() const;
+    template <typename T> gpe_status upload(int which, const double *h);
+    template <typename T> gpe_status download(int which, double *h);
+    gpe_status upload_potential();
+    template <typename T, int FORM> gpe_status enqueue_steps(uint64_t nsteps, uint64_t *launches);
+    gpe_status step(uint64_t nsteps, uint64_t *launches);
+    template <typename T> gpe_status observables_t(int which, double out[4]);
+};

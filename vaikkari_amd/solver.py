@@ -1,0 +1,364 @@
+"""Host-side mirror of the reference's solver interface over the C ABI.
+
+The reference drives its stepper through ``integrateInTime`` and a
+``VortexState`` (koodit/GrossPitaevskii/GPUGrossPitaevskii_BCC.cu:218,
+koodit/GrossPitaevskii/VortexState.hpp:16-76).  This module exposes the same
+names with the same argument meaning (``VortexState.load``, ``getPsi``,
+``getPhase``, ``searchFunctionMax``, ``searchMaxR``, ``searchMaxZ``,
+``integrateCurvature``, ``integrateInTime``) plus ``Solver``, the object form of
+the stepper (gpe_create ... gpe_destroy).  Fields are flat float64 arrays of
+2*nodes values in the host order [k][j][i][l], complex interleaved.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, lib
+
+PRECISIONS = {"fp64": _lib.FP64, "fp32": _lib.FP32_BULK, "fp32_bulk": _lib.FP32_BULK}
+MODES = {"real": _lib.REAL_TIME, "imag": _lib.IMAG_TIME}
+POTENTIALS = {"array": _lib.POT_ARRAY, "trap": _lib.POT_TRAP}
+KERNELS = {"auto": _lib.KERNEL_AUTO, "gather": _lib.KERNEL_GATHER, "zmarch": _lib.KERNEL_ZMARCH,
+           "fused": _lib.KERNEL_FUSED}
+KERNEL_NAMES = {v: k for k, v in KERNELS.items()}
+
+PIx2 = 6.2831853071795864770
+HODGE = 1.5
+
+
+def nodes_of(dims: Sequence[int]) -> int:
+    return int(dims[0]) * int(dims[1]) * int(dims[2]) * _lib.GPE_CLASSES
+
+
+def node_measure(block_scale: float) -> float:
+    """w = sqrt(32/9) b^3 = a^3/12 (GPUGrossPitaevskii_BCC.cu:71)."""
+    return math.sqrt(32.0 / 9.0) * block_scale ** 3
+
+
+def as_complex(a: np.ndarray) -> np.ndarray:
+    return a.view(np.complex128)
+
+
+class Solver:
+    """One condensate resident on the current GPU (gpe_solver)."""
+
+    def __init__(self, dims: Sequence[int], block_scale: float, dt: float, g: float,
+                 origin: Optional[Sequence[float]] = None, precision: str = "fp64", mode: str = "real",
+                 potential: str = "trap", trap_ratio: float = 1.0, kernel: str = "auto"):
+        self.dims = tuple(int(d) for d in dims)
+        self.block_scale = float(block_scale)
+        if origin is None:
+            origin = centered_origin(self.dims, block_scale)
+        cfg = _lib.GpeConfig()
+        cfg.nx, cfg.ny, cfg.nz = self.dims
+        cfg.precision = PRECISIONS[precision]
+        cfg.block_scale = float(block_scale)
+        cfg.dt = float(dt)
+        cfg.g = float(g)
+        cfg.origin = _lib.d3(origin)
+        cfg.potential = POTENTIALS[potential]
+        cfg.mode = MODES[mode]
+        cfg.trap_ratio = float(trap_ratio)
+        cfg.kernel = KERNELS[kernel]
+        self.cfg = cfg
+        self.origin = tuple(float(o) for o in origin)
+        self._h = C.c_void_p()
+        check(lib.gpe_create(C.byref(cfg), C.byref(self._h)), "gpe_create")
+        self.nodes = nodes_of(self.dims)
+
+    # -- lifecycle -------------------------------------------------------
+    def close(self) -> None:
+        if self._h:
+            lib.gpe_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state -----------------------------------------------------------
+    def set_potential(self, V: np.ndarray) -> None:
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        if V.size != self.nodes:
+            raise ValueError(f"potential has {V.size} values, expected {self.nodes}")
+        check(lib.gpe_set_potential(self._h, dptr(V)), "gpe_set_potential")
+
+    def set_state(self, even: np.ndarray, odd: Optional[np.ndarray] = None) -> None:
+        even = self._field(even)
+        odd = None if odd is None else self._field(odd)
+        check(lib.gpe_set_state(self._h, dptr(even), dptr(odd)), "gpe_set_state")
+
+    def get_state(self, odd: bool = True):
+        e = np.empty(2 * self.nodes)
+        o = np.empty(2 * self.nodes) if odd else None
+        check(lib.gpe_get_state(self._h, dptr(e), dptr(o)), "gpe_get_state")
+        return (e, o) if odd else e
+
+    def set_mode(self, mode: str, dt: float) -> None:
+        check(lib.gpe_set_mode(self._h, MODES[mode], float(dt)), "gpe_set_mode")
+        self.cfg.mode = MODES[mode]
+        self.cfg.dt = float(dt)
+
+    def init_odd(self, mu: float) -> None:
+        check(lib.gpe_init_odd(self._h, float(mu)), "gpe_init_odd")
+
+    # -- stepping --------------------------------------------------------
+    def step(self, nsteps: int) -> None:
+        check(lib.gpe_step(self._h, int(nsteps)), "gpe_step")
+
+    def time_steps(self, nsteps: int):
+        """Enqueue nsteps and return (device ms, hot-kernel launches)."""
+        ms = C.c_double()
+        nl = C.c_uint64()
+        check(lib.gpe_time_steps(self._h, int(nsteps), C.byref(ms), C.byref(nl)), "gpe_time_steps")
+        return ms.value, nl.value
+
+    def synchronize(self) -> None:
+        check(lib.gpe_synchronize(self._h), "gpe_synchronize")
+
+    def observables(self, field: int = 0) -> dict:
+        out = np.zeros(4)
+        check(lib.gpe_observables(self._h, int(field), dptr(out)), "gpe_observables")
+        N, K, P, I = (float(x) for x in out)
+        return {"N": N, "K": K, "P": P, "I": I, "E": K + P + 0.5 * I, "mu": (K + P + I) / N if N else float("nan")}
+
+    def info(self) -> dict:
+        inf = _lib.GpeSolverInfo()
+        check(lib.gpe_get_info(self._h, C.byref(inf)), "gpe_get_info")
+        return {"nodes": inf.nodes, "device_bytes": inf.device_bytes, "steps_done": inf.steps_done,
+                "kernel": KERNEL_NAMES.get(inf.kernel, str(inf.kernel)),
+                "launches_per_step": inf.launches_per_step, "bytes_per_node_step": inf.bytes_per_node_step}
+
+    def _field(self, a: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(a)
+        if a.dtype == np.complex128:
+            a = a.view(np.float64)
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        if a.size != 2 * self.nodes:
+            raise ValueError(f"field has {a.size} doubles, expected {2 * self.nodes}")
+        return a
+
+
+# ---------------------------------------------------------------------------
+# geometry
+# ---------------------------------------------------------------------------
+def bcc_positions() -> np.ndarray:
+    out = (C.c_int32 * 36)()
+    lib.gpe_bcc_positions(out)
+    return np.array(out, dtype=np.int32).reshape(12, 3)
+
+
+def bcc_stencil() -> np.ndarray:
+    out = (C.c_int32 * 192)()
+    lib.gpe_bcc_stencil(out)
+    return np.array(out, dtype=np.int32).reshape(12, 4, 4)
+
+
+def bcc_stencil_derived():
+    out = (C.c_int32 * 192)()
+    n = lib.gpe_bcc_stencil_derived(out)
+    return n, np.array(out, dtype=np.int32).reshape(12, 4, 4)
+
+
+# ---------------------------------------------------------------------------
+# reference initial state
+# ---------------------------------------------------------------------------
+class VortexState:
+    """Mirror of the reference VortexState (koodit/GrossPitaevskii/VortexState.hpp:16-76),
+    loaded from its binary file format (VortexState.cpp:49-72)."""
+
+    def __init__(self):
+        self._h = C.c_void_p()
+
+    def load(self, path: str) -> bool:
+        h = C.c_void_p()
+        st = lib.gpe_vortex_state_load(str(path).encode(), C.byref(h))
+        if st != _lib.OK:
+            return False
+        self.close()
+        self._h = h
+        return True
+
+    def close(self):
+        if self._h:
+            lib.gpe_vortex_state_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _params(self):
+        out = np.zeros(8)
+        lib.gpe_vortex_state_params(self._h, dptr(out))
+        return out
+
+    def getKappa(self) -> int:
+        return int(self._params()[0])
+
+    def getG(self) -> float:
+        return float(self._params()[1])
+
+    def getMu(self) -> float:
+        return float(self._params()[7])
+
+    def getPsi(self, p: Sequence[float]) -> complex:
+        out = np.zeros(2)
+        lib.gpe_vortex_state_psi(self._h, float(p[0]), float(p[1]), float(p[2]), dptr(out))
+        return complex(out[0], out[1])
+
+    def getPhase(self, time: float) -> complex:
+        out = np.zeros(2)
+        lib.gpe_vortex_state_phase(self._h, float(time), dptr(out))
+        return complex(out[0], out[1])
+
+    def searchFunctionMax(self) -> float:
+        return lib.gpe_vortex_state_search_max(self._h)
+
+    def searchMaxR(self, eps: float) -> float:
+        return lib.gpe_vortex_state_search_max_r(self._h, float(eps))
+
+    def searchMaxZ(self, eps: float) -> float:
+        return lib.gpe_vortex_state_search_max_z(self._h, float(eps))
+
+    def integrateCurvature(self) -> float:
+        return lib.gpe_vortex_state_curvature(self._h)
+
+
+def reference_dims(block_scale, minp, maxp):
+    dims = (C.c_uint32 * 3)()
+    origin = (C.c_double * 3)()
+    lib.gpe_reference_dims(float(block_scale), _lib.d3(minp), _lib.d3(maxp), dims, origin)
+    return tuple(dims), tuple(origin)
+
+
+def reference_setup(state: VortexState, block_scale, minp, maxp, iteration_period=1.0, trap_ratio=0.1,
+                    seed=54363, fields=True):
+    """Setup of integrateInTime (GPUGrossPitaevskii_BCC.cu:245-348).
+
+    Returns (run dict, even, odd, V); V is unscaled."""
+    run = _lib.GpeReferenceRun()
+    check(lib.gpe_reference_setup(state._h, float(block_scale), _lib.d3(minp), _lib.d3(maxp),
+                                  float(iteration_period), float(trap_ratio), int(seed), C.byref(run),
+                                  None, None, None), "gpe_reference_setup")
+    info = _run_dict(run)
+    if not fields:
+        return info, None, None, None
+    n = nodes_of(info["dims"])
+    even, odd, V = np.empty(2 * n), np.empty(2 * n), np.empty(n)
+    check(lib.gpe_reference_setup(state._h, float(block_scale), _lib.d3(minp), _lib.d3(maxp),
+                                  float(iteration_period), float(trap_ratio), int(seed), C.byref(run),
+                                  dptr(even), dptr(odd), dptr(V)), "gpe_reference_setup")
+    return info, even, odd, V
+
+
+def _run_dict(run) -> dict:
+    return {"dims": tuple(run.dims), "steps_per_iteration": run.steps_per_iteration, "origin": tuple(run.origin),
+            "dt": run.dt, "g": run.g, "maxpot": run.maxpot, "lapfac_dt": run.lapfac_dt,
+            "lapfac0_dt": run.lapfac0_dt}
+
+
+def integrateInTime(state: VortexState, block_scale: float, minp, maxp, iteration_period: float,
+                    number_of_iterations: int, trap_ratio: float = 0.1, precision: str = "fp64",
+                    return_fields: bool = False):
+    """Drop-in for the reference's integrateInTime (GPUGrossPitaevskii_BCC.cu:218-499).
+
+    Returns 0 like the reference; with return_fields=True returns
+    (0, run, even, odd)."""
+    run = _lib.GpeReferenceRun()
+    even = odd = None
+    if return_fields:
+        info, _, _, _ = reference_setup(state, block_scale, minp, maxp, iteration_period, trap_ratio, fields=False)
+        n = nodes_of(info["dims"])
+        even, odd = np.empty(2 * n), np.empty(2 * n)
+    check(lib.gpe_integrate_in_time(state._h, float(block_scale), _lib.d3(minp), _lib.d3(maxp),
+                                    float(iteration_period), int(number_of_iterations), float(trap_ratio),
+                                    PRECISIONS[precision], C.byref(run), dptr(even), dptr(odd)),
+          "gpe_integrate_in_time")
+    if return_fields:
+        return 0, _run_dict(run), even, odd
+    return 0
+
+
+integrate_in_time = integrateInTime
+
+
+# ---------------------------------------------------------------------------
+# recipes
+# ---------------------------------------------------------------------------
+def centered_origin(dims, block_scale):
+    out = (C.c_double * 3)()
+    lib.gpe_centered_origin(_lib.u32(dims), float(block_scale), out)
+    return tuple(out)
+
+
+def node_positions(dims, origin, block_scale) -> np.ndarray:
+    out = np.empty(3 * nodes_of(dims))
+    lib.gpe_node_positions(_lib.u32(dims), _lib.d3(origin), float(block_scale), dptr(out))
+    return out.reshape(-1, 3)
+
+
+def trap_potential(dims, origin, block_scale, ratio) -> np.ndarray:
+    out = np.empty(nodes_of(dims))
+    lib.gpe_trap_potential(_lib.u32(dims), _lib.d3(origin), float(block_scale), float(ratio), dptr(out))
+    return out
+
+
+def field_norm(dims, block_scale, psi) -> float:
+    return lib.gpe_field_norm(_lib.u32(dims), float(block_scale), dptr(np.ascontiguousarray(psi, np.float64)))
+
+
+def gaussian(dims, origin, block_scale, sigma, noise, seed) -> np.ndarray:
+    out = np.empty(2 * nodes_of(dims))
+    check(lib.gpe_recipe_gaussian(_lib.u32(dims), _lib.d3(origin), float(block_scale), float(sigma),
+                                  float(noise), int(seed), dptr(out)), "gpe_recipe_gaussian")
+    return out
+
+
+def thomas_fermi(dims, origin, block_scale, g, ratio):
+    out = np.empty(2 * nodes_of(dims))
+    mu = C.c_double()
+    check(lib.gpe_recipe_thomas_fermi(_lib.u32(dims), _lib.d3(origin), float(block_scale), float(g), float(ratio),
+                                      dptr(out), C.byref(mu)), "gpe_recipe_thomas_fermi")
+    return out, mu.value
+
+
+def vortex_line(dims, origin, block_scale, x0, y0, psi) -> np.ndarray:
+    check(lib.gpe_recipe_vortex_line(_lib.u32(dims), _lib.d3(origin), float(block_scale), float(x0), float(y0),
+                                     dptr(psi)), "gpe_recipe_vortex_line")
+    return psi
+
+
+def vortex_ring(dims, origin, block_scale, center, axis, radius, psi) -> np.ndarray:
+    check(lib.gpe_recipe_vortex_ring(_lib.u32(dims), _lib.d3(origin), float(block_scale), _lib.d3(center),
+                                     _lib.d3(axis), float(radius), dptr(psi)), "gpe_recipe_vortex_ring")
+    return psi
+
+
+def phase_shift(dims, even, mu, dt) -> np.ndarray:
+    odd = np.empty_like(even)
+    check(lib.gpe_recipe_phase_shift(_lib.u32(dims), dptr(even), float(mu), float(dt), dptr(odd)),
+          "gpe_recipe_phase_shift")
+    return odd
+
+
+def leapfrog_coefs(block_scale: float, dt: float, g: float):
+    """(lapfac*dt, lapfac0*dt, g*dt) as GPUGrossPitaevskii_BCC.cu:291-306 form them."""
+    lapfac = -0.5 * HODGE / (block_scale * block_scale)
+    lapfac0 = 4.0 * (-lapfac)
+    return lapfac * dt, lapfac0 * dt, g * dt
