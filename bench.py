@@ -95,6 +95,8 @@ def main():
     origin = va.centered_origin(dims, b)
     nodes = va.nodes_of(dims)
     log(f"grid {dims} cells, {nodes} nodes")
+    copy_gbs = va.copy_bandwidth(1 << 30, 10)
+    log(f"plain streaming copy: {copy_gbs:.0f} GB/s ({copy_gbs / HBM_PEAK_GBS * 100:.1f}% of the 8 TB/s peak)")
 
     # ---- phase 1: imaginary time, FP64 -----------------------------------
     psi0 = va.gaussian(dims, origin, b, 1.5, 1e-3, 1)
@@ -185,6 +187,7 @@ def main():
         "phase1_imag": {"steps": args.imag_steps, "ms": ms_imag, "site_steps_per_s": nodes * args.imag_steps / (ms_imag * 1e-3),
                         "E": obs["E"], "mu": obs["mu"], "N": obs["N"]},
         "bytes_per_node_step": {"fp64": r64["bytes_per_node_step"], "fp32": r32["bytes_per_node_step"]},
+        "hbm_copy_GBs": copy_gbs,
         "avg_launch_us": {"fp64": r64["avg_launch_us"], "fp32": r32["avg_launch_us"]},
     }
     print(json.dumps(line), flush=True)

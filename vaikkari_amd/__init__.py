@@ -12,6 +12,7 @@ from .solver import (  # noqa: F401
     bcc_stencil,
     bcc_stencil_derived,
     centered_origin,
+    copy_bandwidth,
     field_norm,
     gaussian,
     integrate_in_time,
@@ -25,6 +26,7 @@ from .solver import (  # noqa: F401
     reference_setup,
     thomas_fermi,
     trap_potential,
+    variants,
     vortex_line,
     vortex_ring,
 )

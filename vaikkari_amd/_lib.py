@@ -41,7 +41,7 @@ class GpeConfig(C.Structure):
         ("mode", C.c_uint32),
         ("trap_ratio", C.c_double),
         ("kernel", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("tile", C.c_uint32),
     ]
 
 
@@ -51,6 +51,7 @@ class GpeSolverInfo(C.Structure):
         ("device_bytes", C.c_uint64),
         ("steps_done", C.c_uint64),
         ("kernel", C.c_uint32),
+        ("tile", C.c_uint32),
         ("launches_per_step", C.c_uint32),
         ("bytes_per_node_step", C.c_double),
     ]
@@ -98,6 +99,10 @@ SIGNATURES = [
     ("gpe_observables", _I, [_P, _I, _DP]),
     ("gpe_synchronize", _I, [_P]),
     ("gpe_get_info", _I, [_P, C.POINTER(GpeSolverInfo)]),
+    ("gpe_set_variant", _I, [_P, _U32, _U32]),
+    ("gpe_variant_tiles", _I, [_U32]),
+    ("gpe_variant_shape", None, [_U32, _U32, _I32P]),
+    ("gpe_diag_copy_bandwidth", _I, [_U64, _U32, _DP]),
     ("gpe_status_string", C.c_char_p, [_I]),
     ("gpe_last_error", C.c_char_p, []),
     ("gpe_bcc_positions", None, [_I32P]),
@@ -130,7 +135,7 @@ SIGNATURES = [
 def _load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            f"{LIB_PATH} is missing: build it with `python -m vaikkari_amd.build` "
+            f"{LIB_PATH} is missing: build it with `python vaikkari_amd/build.py` "
             "(hipcc --offload-arch=gfx950); there is no fallback implementation")
     lib = C.CDLL(LIB_PATH)
     for name, res, args in SIGNATURES:

@@ -1,7 +1,7 @@
 """Build recipe of the native library (libvaikkari_gpe.so) for gfx950.
 
 Compiles every HIP/C++ source under vaikkari_amd/csrc with hipcc into one
-shared library next to this file.  Usable as ``python -m vaikkari_amd.build``.
+shared library next to this file.  Run as ``python vaikkari_amd/build.py``.
 """
 from __future__ import annotations
 
@@ -17,7 +17,7 @@ LIB_NAME = "libvaikkari_gpe.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 ARCH = os.environ.get("GPE_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["gpe_solver.hip", "host_setup.cpp", "integrate.cpp"]
+SOURCES = ["gpe_solver.hip", "diag.hip", "host_setup.cpp", "integrate.cpp"]
 
 FLAGS = [
     "-O3",

@@ -138,10 +138,33 @@ __device__ __forceinline__ cplx_t<T> cscale(cplx_t<T> a, T s) {
     return r;
 }
 
+// Logical block coordinates from a 1-D launch.  Workgroups are dispatched
+// round-robin over the 8 XCDs (each with its own L2); with remap on, block
+// lin runs on XCD lin%8 and is given the (lin/8)-th block of that XCD's
+// contiguous share of the (bx fastest, then by, then bz) order, so blocks
+// sharing halo rows and layers share one L2.  The map is a bijection for any
+// block count.
+struct Grid3 {
+    int gx, gy, gz, remap;
+};
+__device__ __forceinline__ void block_coords(const Grid3 &G, int &bx, int &by, int &bz) {
+    const int nb = G.gx * G.gy * G.gz;
+    const int lin = blockIdx.x;
+    int w = lin;
+    if (G.remap) {
+        const int q = nb >> 3, r = nb & 7, xcd = lin & 7, slot = lin >> 3;
+        w = (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+    }
+    bx = w % G.gx;
+    const int t = w / G.gx;
+    by = t % G.gy;
+    bz = t / G.gy;
+}
+
 // ---------------------------------------------------------------------
 // GATHER: one thread per cell, all 12 classes; in-cell neighbours come from
 // the thread's own registers, the 18 off-cell neighbours per cell from
-// global memory (cache hits for all but first touch).  Grid
+// global memory (cache hits for all but first touch).  Logical grid
 // (ceil(nx/64), ceil(ny/TY), nz), block (64, TY).
 // Real time: next[l] += (s.y, -s.x) in place (GPUGrossPitaevskii_BCC.cu:215).
 // Imaginary time: out = psi - s, psi = scale*prev, block partial of
@@ -151,13 +174,15 @@ template <typename T, int FORM, int POT, int MODE, int TY>
 __global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                                     const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
                                                     Trap trap, double dt, const double *__restrict__ scale,
-                                                    double *__restrict__ partials) {
+                                                    double *__restrict__ partials, Grid3 G) {
 #pragma clang fp contract(off)
     using C = cplx_t<T>;
     __shared__ double lds[TY];
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
-    const int k = blockIdx.z;
+    int bx, by, bz;
+    block_coords(G, bx, by, bz);
+    const int i = bx * 64 + threadIdx.x;
+    const int j = by * TY + threadIdx.y;
+    const int k = bz;
     const bool act = i < L.nx && j < L.ny;
     double part[1] = {0.0};
     if (act) {
@@ -205,7 +230,7 @@ __global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next
     if (MODE == MODE_IMAG) {
         block_sum<1, TY>(part, lds);
         if (threadIdx.x == 0 && threadIdx.y == 0)
-            partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = part[0];
+            partials[blockIdx.x] = part[0];
     }
 }
 
@@ -215,19 +240,21 @@ __global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next
 // classes {2,6} of the layer above in registers -- exactly the values the
 // (0,0,-1) and (0,0,+1) faces need -- so z-neighbours never touch memory
 // twice; x/y/diagonal neighbours are global loads that hit L1/L2.
-// Grid (ceil(nx/64), ceil(ny/TY), ceil(nz/CZ)), block (64, TY).
+// Logical grid (ceil(nx/64), ceil(ny/TY), ceil(nz/CZ)), block (64, TY).
 // ---------------------------------------------------------------------
 template <typename T, int FORM, int POT, int MODE, int TY, int CZ>
 __global__ void __launch_bounds__(64 * TY) k_zmarch(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                                     const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
                                                     Trap trap, double dt, const double *__restrict__ scale,
-                                                    double *__restrict__ partials) {
+                                                    double *__restrict__ partials, Grid3 G) {
 #pragma clang fp contract(off)
     using C = cplx_t<T>;
     __shared__ double lds[TY];
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
-    const int k0 = blockIdx.z * CZ;
+    int bx, by, bz;
+    block_coords(G, bx, by, bz);
+    const int i = bx * 64 + threadIdx.x;
+    const int j = by * TY + threadIdx.y;
+    const int k0 = bz * CZ;
     const int k1 = min(k0 + CZ, L.nz);
     const bool act = i < L.nx && j < L.ny;
     double part[1] = {0.0};
@@ -311,7 +338,7 @@ __global__ void __launch_bounds__(64 * TY) k_zmarch(cplx_t<T> *__restrict__ next
     if (MODE == MODE_IMAG) {
         block_sum<1, TY>(part, lds);
         if (threadIdx.x == 0 && threadIdx.y == 0)
-            partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = part[0];
+            partials[blockIdx.x] = part[0];
     }
 }
 

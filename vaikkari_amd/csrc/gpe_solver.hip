@@ -61,30 +61,95 @@ Trap make_trap(const double origin[3], double b, double ratio) {
 // ---------------------------------------------------------------------
 namespace {
 
-constexpr int kTY = 4;    // waves per block (rows of 64 cells)
-constexpr int kCZ = 8;    // layers per z-march block
+constexpr int kTY = 4;  // waves per block of the observables kernel
+
+// Tile shapes selectable per solver (gpe_config.tile; the autotuner sweeps
+// them).  ty = waves (rows of 64 cells) per block, cz = layers per z-march
+// block, remap = XCD-contiguous block order.
+struct TileShape {
+    int ty, cz, remap;
+};
+constexpr TileShape kGatherTiles[] = {{4, 1, 1}, {4, 1, 0}, {1, 1, 1}, {2, 1, 1}, {8, 1, 1}};
+constexpr TileShape kZmarchTiles[] = {{4, 8, 1}, {4, 8, 0}, {1, 8, 1}, {2, 8, 1}, {8, 8, 1},
+                                      {4, 4, 1}, {4, 16, 1}, {2, 16, 1}, {1, 16, 1}, {2, 4, 1}};
+constexpr int kNumGatherTiles = sizeof(kGatherTiles) / sizeof(kGatherTiles[0]);
+constexpr int kNumZmarchTiles = sizeof(kZmarchTiles) / sizeof(kZmarchTiles[0]);
+
+int num_tiles(int kernel) { return kernel == GPE_KERNEL_ZMARCH ? kNumZmarchTiles : kNumGatherTiles; }
+
+TileShape tile_shape(int kernel, int tile) {
+    return kernel == GPE_KERNEL_ZMARCH ? kZmarchTiles[tile] : kGatherTiles[tile];
+}
+
+Grid3 grid_of(int kernel, int tile, const Layout &L) {
+    const TileShape t = tile_shape(kernel, tile);
+    Grid3 G;
+    G.gx = (L.nx + 63) / 64;
+    G.gy = (L.ny + t.ty - 1) / t.ty;
+    G.gz = kernel == GPE_KERNEL_ZMARCH ? (L.nz + t.cz - 1) / t.cz : L.nz;
+    G.remap = t.remap;
+    return G;
+}
+
+template <typename T, int FORM, int POT, int MODE, int TY>
+void launch_gather(cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const Layout &L,
+                   const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, const double *scale, double *partials,
+                   const Grid3 &G, hipStream_t st) {
+    hipLaunchKernelGGL((k_gather<T, FORM, POT, MODE, TY>), dim3(G.gx * G.gy * G.gz), dim3(64, TY), 0, st, next, prev,
+                       pot, L, H, trap, dt, scale, partials, G);
+}
+
+template <typename T, int FORM, int POT, int MODE, int TY, int CZ>
+void launch_zmarch(cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const Layout &L,
+                   const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, const double *scale, double *partials,
+                   const Grid3 &G, hipStream_t st) {
+    hipLaunchKernelGGL((k_zmarch<T, FORM, POT, MODE, TY, CZ>), dim3(G.gx * G.gy * G.gz), dim3(64, TY), 0, st, next,
+                       prev, pot, L, H, trap, dt, scale, partials, G);
+}
 
 template <typename T, int FORM, int POT, int MODE>
-hipError_t launch_update(int kernel, cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const Layout &L,
+hipError_t launch_update(int kernel, int tile, cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const Layout &L,
                          const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, const double *scale,
                          double *partials, hipStream_t st) {
-    const dim3 block(64, kTY);
+    const Grid3 G = grid_of(kernel, tile, L);
+    const TileShape t = tile_shape(kernel, tile);
+#define GPE_ARGS next, prev, pot, L, H, trap, dt, scale, partials, G, st
     if (kernel == GPE_KERNEL_ZMARCH) {
-        const dim3 grid((L.nx + 63) / 64, (L.ny + kTY - 1) / kTY, (L.nz + kCZ - 1) / kCZ);
-        hipLaunchKernelGGL((k_zmarch<T, FORM, POT, MODE, kTY, kCZ>), grid, block, 0, st, next, prev, pot, L, H, trap,
-                           dt, scale, partials);
+        switch (t.ty * 100 + t.cz) {
+            case 108: launch_zmarch<T, FORM, POT, MODE, 1, 8>(GPE_ARGS); break;
+            case 116: launch_zmarch<T, FORM, POT, MODE, 1, 16>(GPE_ARGS); break;
+            case 204: launch_zmarch<T, FORM, POT, MODE, 2, 4>(GPE_ARGS); break;
+            case 208: launch_zmarch<T, FORM, POT, MODE, 2, 8>(GPE_ARGS); break;
+            case 216: launch_zmarch<T, FORM, POT, MODE, 2, 16>(GPE_ARGS); break;
+            case 404: launch_zmarch<T, FORM, POT, MODE, 4, 4>(GPE_ARGS); break;
+            case 408: launch_zmarch<T, FORM, POT, MODE, 4, 8>(GPE_ARGS); break;
+            case 416: launch_zmarch<T, FORM, POT, MODE, 4, 16>(GPE_ARGS); break;
+            case 808: launch_zmarch<T, FORM, POT, MODE, 8, 8>(GPE_ARGS); break;
+            default: return hipErrorInvalidValue;
+        }
     } else {
-        const dim3 grid((L.nx + 63) / 64, (L.ny + kTY - 1) / kTY, L.nz);
-        hipLaunchKernelGGL((k_gather<T, FORM, POT, MODE, kTY>), grid, block, 0, st, next, prev, pot, L, H, trap, dt,
-                           scale, partials);
+        switch (t.ty) {
+            case 1: launch_gather<T, FORM, POT, MODE, 1>(GPE_ARGS); break;
+            case 2: launch_gather<T, FORM, POT, MODE, 2>(GPE_ARGS); break;
+            case 4: launch_gather<T, FORM, POT, MODE, 4>(GPE_ARGS); break;
+            case 8: launch_gather<T, FORM, POT, MODE, 8>(GPE_ARGS); break;
+            default: return hipErrorInvalidValue;
+        }
     }
+#undef GPE_ARGS
     return hipGetLastError();
 }
 
-long long update_blocks(int kernel, const Layout &L) {
-    const long long bx = (L.nx + 63) / 64, by = (L.ny + kTY - 1) / kTY;
-    const long long bz = kernel == GPE_KERNEL_ZMARCH ? (L.nz + kCZ - 1) / kCZ : L.nz;
-    return bx * by * bz;
+long long update_blocks(int kernel, int tile, const Layout &L) {
+    const Grid3 G = grid_of(kernel, tile, L);
+    return (long long)G.gx * G.gy * G.gz;
+}
+
+long long max_update_blocks(const Layout &L) {
+    long long m = 0;
+    for (int t = 0; t < kNumGatherTiles; t++) m = std::max(m, update_blocks(GPE_KERNEL_GATHER, t, L));
+    for (int t = 0; t < kNumZmarchTiles; t++) m = std::max(m, update_blocks(GPE_KERNEL_ZMARCH, t, L));
+    return m;
 }
 
 long long obs_blocks(const Layout &L) { return (long long)((L.nx + 63) / 64) * ((L.ny + kTY - 1) / kTY) * L.nz; }
@@ -136,6 +201,11 @@ gpe_status gpe_solver::init(const gpe_config &c) {
     trap = make_trap(c.origin, c.block_scale, c.trap_ratio);
     kernel = c.kernel == GPE_KERNEL_AUTO ? GPE_KERNEL_ZMARCH : c.kernel;
     if (kernel == GPE_KERNEL_FUSED) kernel = GPE_KERNEL_ZMARCH;  // fused pass not available: z-march instead
+    tile = (int)c.tile;
+    if (tile >= num_tiles(kernel)) {
+        set_error("gpe_create: tile %d out of range for this kernel family", tile);
+        return GPE_EINVAL;
+    }
     GPE_HIP_TRY(hipGetDevice(&device));
     GPE_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     GPE_HIP_TRY(hipEventCreate(&ev0));
@@ -149,7 +219,7 @@ gpe_status gpe_solver::init(const gpe_config &c) {
         }
         GPE_HIP_TRY(hipMemsetAsync(p, 0, fbytes, stream));
     }
-    npartials = std::max(update_blocks(GPE_KERNEL_GATHER, L), obs_blocks(L));
+    npartials = std::max(max_update_blocks(L), obs_blocks(L));
     GPE_HIP_TRY(hipMalloc(&partials, (size_t)npartials * 4 * sizeof(double)));
     GPE_HIP_TRY(hipMalloc(&scal, 8 * sizeof(double)));
     const double init_scal[8] = {1.0, 1.0, 0, 0, 0, 0, 0, 0};
@@ -266,13 +336,13 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
         C *ev = (C *)field[cur], *od = (C *)field[cur ^ 1];
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
             if (arr) {
-                e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(kernel, od, ev, p, L, H, trap, co.dt, nullptr, nullptr, stream);
+                e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(kernel, tile, od, ev, p, L, H, trap, co.dt, nullptr, nullptr, stream);
                 if (e == hipSuccess)
-                    e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(kernel, ev, od, p, L, H, trap, co.dt, nullptr, nullptr, stream);
+                    e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(kernel, tile, ev, od, p, L, H, trap, co.dt, nullptr, nullptr, stream);
             } else {
-                e = launch_update<T, FORM, POT_TRAP, MODE_REAL>(kernel, od, ev, p, L, H, trap, co.dt, nullptr, nullptr, stream);
+                e = launch_update<T, FORM, POT_TRAP, MODE_REAL>(kernel, tile, od, ev, p, L, H, trap, co.dt, nullptr, nullptr, stream);
                 if (e == hipSuccess)
-                    e = launch_update<T, FORM, POT_TRAP, MODE_REAL>(kernel, ev, od, p, L, H, trap, co.dt, nullptr, nullptr, stream);
+                    e = launch_update<T, FORM, POT_TRAP, MODE_REAL>(kernel, tile, ev, od, p, L, H, trap, co.dt, nullptr, nullptr, stream);
             }
             nl += 2;
         }
@@ -282,11 +352,11 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
             C *in = (C *)field[cur], *out = (C *)field[cur ^ 1];
             if (arr)
-                e = launch_update<T, FORM, POT_ARRAY, MODE_IMAG>(kernel, out, in, p, L, H, trap, co.dt, scal + 1, partials, stream);
+                e = launch_update<T, FORM, POT_ARRAY, MODE_IMAG>(kernel, tile, out, in, p, L, H, trap, co.dt, scal + 1, partials, stream);
             else
-                e = launch_update<T, FORM, POT_TRAP, MODE_IMAG>(kernel, out, in, p, L, H, trap, co.dt, scal + 1, partials, stream);
+                e = launch_update<T, FORM, POT_TRAP, MODE_IMAG>(kernel, tile, out, in, p, L, H, trap, co.dt, scal + 1, partials, stream);
             if (e != hipSuccess) break;
-            hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(256), 0, stream, partials, update_blocks(kernel, L), co.w, 1, scal);
+            hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(256), 0, stream, partials, update_blocks(kernel, tile, L), co.w, 1, scal);
             e = hipGetLastError();
             cur ^= 1;
             nl += 1;
@@ -495,6 +565,40 @@ gpe_status gpe_synchronize(gpe_solver *s) {
     return GPE_OK;
 }
 
+gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile) {
+    if (!s || kernel == GPE_KERNEL_AUTO || kernel > GPE_KERNEL_FUSED) {
+        set_error("gpe_set_variant: invalid kernel family");
+        return GPE_EINVAL;
+    }
+    if (kernel == GPE_KERNEL_FUSED) {
+        set_error("gpe_set_variant: fused kernel family not available");
+        return GPE_ENOTSUP;
+    }
+    if ((int)tile >= num_tiles((int)kernel)) {
+        set_error("gpe_set_variant: tile %u out of range", tile);
+        return GPE_EINVAL;
+    }
+    GPE_HIP_TRY(hipStreamSynchronize(s->stream));
+    s->kernel = (int)kernel;
+    s->tile = (int)tile;
+    return GPE_OK;
+}
+
+int gpe_variant_tiles(uint32_t kernel) {
+    if (kernel == GPE_KERNEL_GATHER || kernel == GPE_KERNEL_ZMARCH) return num_tiles((int)kernel);
+    return 0;
+}
+
+void gpe_variant_shape(uint32_t kernel, uint32_t tile, int32_t out[3]) {
+    out[0] = out[1] = out[2] = 0;
+    if ((kernel == GPE_KERNEL_GATHER || kernel == GPE_KERNEL_ZMARCH) && (int)tile < num_tiles((int)kernel)) {
+        const TileShape t = tile_shape((int)kernel, (int)tile);
+        out[0] = t.ty;
+        out[1] = t.cz;
+        out[2] = t.remap;
+    }
+}
+
 gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info) {
     if (!s || !info) {
         set_error("gpe_get_info: null argument");
@@ -504,6 +608,7 @@ gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info) {
     info->device_bytes = s->device_bytes();
     info->steps_done = s->steps_done;
     info->kernel = (uint32_t)s->kernel;
+    info->tile = (uint32_t)s->tile;
     info->launches_per_step = s->cfg.mode == GPE_REAL_TIME ? 2 : 1;
     const double S = (double)s->elem_bytes();
     const double P = s->cfg.potential == GPE_POT_ARRAY ? S / 2 : 0.0;

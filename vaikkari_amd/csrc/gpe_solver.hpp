@@ -18,6 +18,7 @@ struct gpe_solver {
     gpe::Trap trap{};
     gpe::Coefs co{};
     int kernel = GPE_KERNEL_GATHER;
+    int tile = 0;
     int device = 0;
     // field[cur] holds the even time level, field[cur^1] the odd one (or the
     // imaginary-time scratch buffer).

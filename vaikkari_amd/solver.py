@@ -49,7 +49,7 @@ class Solver:
 
     def __init__(self, dims: Sequence[int], block_scale: float, dt: float, g: float,
                  origin: Optional[Sequence[float]] = None, precision: str = "fp64", mode: str = "real",
-                 potential: str = "trap", trap_ratio: float = 1.0, kernel: str = "auto"):
+                 potential: str = "trap", trap_ratio: float = 1.0, kernel: str = "auto", tile: int = 0):
         self.dims = tuple(int(d) for d in dims)
         self.block_scale = float(block_scale)
         if origin is None:
@@ -65,6 +65,7 @@ class Solver:
         cfg.mode = MODES[mode]
         cfg.trap_ratio = float(trap_ratio)
         cfg.kernel = KERNELS[kernel]
+        cfg.tile = int(tile)
         self.cfg = cfg
         self.origin = tuple(float(o) for o in origin)
         self._h = C.c_void_p()
@@ -135,11 +136,14 @@ class Solver:
         N, K, P, I = (float(x) for x in out)
         return {"N": N, "K": K, "P": P, "I": I, "E": K + P + 0.5 * I, "mu": (K + P + I) / N if N else float("nan")}
 
+    def set_variant(self, kernel: str, tile: int) -> None:
+        check(lib.gpe_set_variant(self._h, KERNELS[kernel], int(tile)), "gpe_set_variant")
+
     def info(self) -> dict:
         inf = _lib.GpeSolverInfo()
         check(lib.gpe_get_info(self._h, C.byref(inf)), "gpe_get_info")
         return {"nodes": inf.nodes, "device_bytes": inf.device_bytes, "steps_done": inf.steps_done,
-                "kernel": KERNEL_NAMES.get(inf.kernel, str(inf.kernel)),
+                "kernel": KERNEL_NAMES.get(inf.kernel, str(inf.kernel)), "tile": inf.tile,
                 "launches_per_step": inf.launches_per_step, "bytes_per_node_step": inf.bytes_per_node_step}
 
     def _field(self, a: np.ndarray) -> np.ndarray:
@@ -150,6 +154,25 @@ class Solver:
         if a.size != 2 * self.nodes:
             raise ValueError(f"field has {a.size} doubles, expected {2 * self.nodes}")
         return a
+
+
+def variants():
+    """All (kernel family, tile, shape) combinations the library can run."""
+    out = []
+    for name in ("gather", "zmarch"):
+        k = KERNELS[name]
+        for t in range(lib.gpe_variant_tiles(k)):
+            sh = (C.c_int32 * 3)()
+            lib.gpe_variant_shape(k, t, sh)
+            out.append({"kernel": name, "tile": t, "waves": sh[0], "cz": sh[1], "remap": sh[2]})
+    return out
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20) -> float:
+    """GB/s of a plain device-to-device streaming copy (read + write bytes)."""
+    ms = C.c_double()
+    check(lib.gpe_diag_copy_bandwidth(int(nbytes), int(iters), C.byref(ms)), "gpe_diag_copy_bandwidth")
+    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
 
 # ---------------------------------------------------------------------------
