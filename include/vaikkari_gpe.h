@@ -87,7 +87,7 @@ typedef struct gpe_config {
     uint32_t mode;         /* gpe_mode */
     double trap_ratio;     /* RATIO of the in-kernel trap (GPUGrossPitaevskii_BCC.cu:13) */
     uint32_t kernel;       /* gpe_kernel */
-    uint32_t tile;         /* tile shape index within the kernel family (0 = default) */
+    uint32_t tile;         /* tile shape 1..gpe_variant_tiles(kernel), 0 = the family's default */
 } gpe_config;
 
 typedef struct gpe_solver gpe_solver;
@@ -97,7 +97,7 @@ typedef struct gpe_solver_info {
     uint64_t device_bytes;   /* resident device memory of this solver */
     uint64_t steps_done;     /* steps taken since creation */
     uint32_t kernel;         /* gpe_kernel in use */
-    uint32_t tile;           /* its tile shape index */
+    uint32_t tile;           /* its tile shape (1-based) */
     uint32_t launches_per_step;
     double bytes_per_node_step; /* algorithmic HBM bytes per node-step of the hot kernel */
 } gpe_solver_info;
@@ -144,8 +144,11 @@ gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info);
 /* Kernel variants (the autotuner's search space).  gpe_set_variant switches
  * the solver's kernel family and tile shape between steps; results do not
  * depend on the variant (every variant performs the same arithmetic per
- * node).  gpe_variant_tiles gives the number of tile shapes of a family,
- * gpe_variant_shape its {waves per block, z layers per block, XCD remap}. */
+ * node).  gpe_variant_tiles gives the number of tile shapes of a family
+ * (numbered 1..n; 0 selects the family's default), gpe_variant_shape the
+ * shape {waves per block, z layers per block, XCD remap} of tile 1..n.
+ * GPE_KERNEL_FUSED needs two more field buffers (allocated on selection);
+ * imaginary-time steps of a fused solver run the z-march kernel. */
 gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile);
 int gpe_variant_tiles(uint32_t kernel);
 void gpe_variant_shape(uint32_t kernel, uint32_t tile, int32_t out[3]);

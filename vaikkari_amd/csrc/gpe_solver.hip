@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "gpe_internal.hpp"
+#include "gpe_fused.hpp"
 #include "gpe_kernels.hpp"
 #include "gpe_solver.hpp"
 
@@ -72,13 +73,67 @@ struct TileShape {
 constexpr TileShape kGatherTiles[] = {{4, 1, 1}, {4, 1, 0}, {1, 1, 1}, {2, 1, 1}, {8, 1, 1}};
 constexpr TileShape kZmarchTiles[] = {{4, 8, 1}, {4, 8, 0}, {1, 8, 1}, {2, 8, 1}, {8, 8, 1},
                                       {4, 4, 1}, {4, 16, 1}, {2, 16, 1}, {1, 16, 1}, {2, 4, 1}};
+// fused: ty = waves per block (footprint rows, interior ty-2), cz = layers per block
+constexpr TileShape kFusedTiles[] = {{8, 16, 1}, {4, 16, 1}, {4, 32, 1}, {8, 32, 1}, {8, 8, 1},
+                                     {16, 16, 1}, {4, 8, 1}, {8, 16, 0}, {6, 16, 1}, {6, 32, 1}};
 constexpr int kNumGatherTiles = sizeof(kGatherTiles) / sizeof(kGatherTiles[0]);
 constexpr int kNumZmarchTiles = sizeof(kZmarchTiles) / sizeof(kZmarchTiles[0]);
+constexpr int kNumFusedTiles = sizeof(kFusedTiles) / sizeof(kFusedTiles[0]);
 
-int num_tiles(int kernel) { return kernel == GPE_KERNEL_ZMARCH ? kNumZmarchTiles : kNumGatherTiles; }
+int num_tiles(int kernel) {
+    return kernel == GPE_KERNEL_ZMARCH ? kNumZmarchTiles : kernel == GPE_KERNEL_FUSED ? kNumFusedTiles : kNumGatherTiles;
+}
+
+// Tile index used when the caller asks for tile 0.
+int default_tile(int kernel, uint32_t precision) {
+    if (kernel == GPE_KERNEL_FUSED) return precision == GPE_FP64 ? 1 : 0;
+    return 0;
+}
 
 TileShape tile_shape(int kernel, int tile) {
-    return kernel == GPE_KERNEL_ZMARCH ? kZmarchTiles[tile] : kGatherTiles[tile];
+    return kernel == GPE_KERNEL_ZMARCH ? kZmarchTiles[tile]
+           : kernel == GPE_KERNEL_FUSED ? kFusedTiles[tile]
+                                        : kGatherTiles[tile];
+}
+
+FusedGrid make_fused_grid(const Layout &L, const TileShape &t) {
+    FusedGrid G;
+    G.ntx = (L.nx + 61) / 62;
+    G.w = (L.nx + G.ntx - 1) / G.ntx;
+    G.nty = (L.ny + t.ty - 3) / (t.ty - 2);
+    G.h = (L.ny + G.nty - 1) / G.nty;
+    G.ntz = (L.nz + t.cz - 1) / t.cz;
+    G.cz = (L.nz + G.ntz - 1) / G.ntz;
+    G.remap = t.remap;
+    return G;
+}
+
+template <typename T, int FORM, int POT, int TY>
+void launch_fused_t(cplx_t<T> *eo, cplx_t<T> *oo, const cplx_t<T> *ei, const cplx_t<T> *oi, const T *pot,
+                    const Layout &L, const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, const FusedGrid &G,
+                    hipStream_t st) {
+    hipLaunchKernelGGL((k_fused<T, FORM, POT, TY>), dim3(G.ntx * G.nty * G.ntz), dim3(64, TY), 0, st, eo, oo, ei, oi,
+                       pot, L, H, trap, dt, G);
+}
+
+template <typename T, int FORM, int POT>
+hipError_t launch_fused(int tile, cplx_t<T> *eo, cplx_t<T> *oo, const cplx_t<T> *ei, const cplx_t<T> *oi,
+                        const T *pot, const Layout &L, const Hamiltonian<T, FORM> &H, const Trap &trap, double dt,
+                        hipStream_t st) {
+    const TileShape t = tile_shape(GPE_KERNEL_FUSED, tile);
+    const FusedGrid G = make_fused_grid(L, t);
+    switch (t.ty) {
+        case 4: launch_fused_t<T, FORM, POT, 4>(eo, oo, ei, oi, pot, L, H, trap, dt, G, st); break;
+        case 6: launch_fused_t<T, FORM, POT, 6>(eo, oo, ei, oi, pot, L, H, trap, dt, G, st); break;
+        case 8: launch_fused_t<T, FORM, POT, 8>(eo, oo, ei, oi, pot, L, H, trap, dt, G, st); break;
+        case 16:
+            // 16 rows of FP64 exchange slots exceed the 160 KiB of LDS
+            if constexpr (sizeof(T) == 4) launch_fused_t<T, FORM, POT, 16>(eo, oo, ei, oi, pot, L, H, trap, dt, G, st);
+            else return hipErrorInvalidConfiguration;
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 Grid3 grid_of(int kernel, int tile, const Layout &L) {
@@ -200,24 +255,24 @@ gpe_status gpe_solver::init(const gpe_config &c) {
     L = make_layout(c.nx, c.ny, c.nz, elem_bytes());
     trap = make_trap(c.origin, c.block_scale, c.trap_ratio);
     kernel = c.kernel == GPE_KERNEL_AUTO ? GPE_KERNEL_ZMARCH : c.kernel;
-    if (kernel == GPE_KERNEL_FUSED) kernel = GPE_KERNEL_ZMARCH;  // fused pass not available: z-march instead
-    tile = (int)c.tile;
-    if (tile >= num_tiles(kernel)) {
-        set_error("gpe_create: tile %d out of range for this kernel family", tile);
+    if ((int)c.tile > num_tiles(kernel)) {
+        set_error("gpe_create: tile %u out of range for this kernel family", c.tile);
         return GPE_EINVAL;
     }
+    tile = c.tile == 0 ? default_tile(kernel, c.precision) : (int)c.tile - 1;
     GPE_HIP_TRY(hipGetDevice(&device));
     GPE_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     GPE_HIP_TRY(hipEventCreate(&ev0));
     GPE_HIP_TRY(hipEventCreate(&ev1));
-    const size_t fbytes = (size_t)L.elems * elem_bytes();
-    for (auto &p : field) {
-        if (hipMalloc(&p, fbytes) != hipSuccess) {
-            p = nullptr;
-            set_error("gpe_create: cannot allocate %zu bytes of device memory", fbytes);
-            return GPE_ENOMEM;
+    for (int f = 0; f < 2; f++) {
+        gpe_status st = alloc_field(f);
+        if (st != GPE_OK) return st;
+    }
+    if (kernel == GPE_KERNEL_FUSED) {
+        for (int f = 2; f < 4; f++) {
+            gpe_status st = alloc_field(f);
+            if (st != GPE_OK) return st;
         }
-        GPE_HIP_TRY(hipMemsetAsync(p, 0, fbytes, stream));
     }
     npartials = std::max(max_update_blocks(L), obs_blocks(L));
     GPE_HIP_TRY(hipMalloc(&partials, (size_t)npartials * 4 * sizeof(double)));
@@ -226,6 +281,20 @@ gpe_status gpe_solver::init(const gpe_config &c) {
     GPE_HIP_TRY(hipMemcpyAsync(scal, init_scal, sizeof(init_scal), hipMemcpyHostToDevice, stream));
     GPE_HIP_TRY(hipStreamSynchronize(stream));
     set_coefs();
+    return GPE_OK;
+}
+
+gpe_status gpe_solver::alloc_field(int f) {
+    if (field[f]) return GPE_OK;
+    const size_t fbytes = (size_t)L.elems * elem_bytes();
+    if (hipMalloc(&field[f], fbytes) != hipSuccess) {
+        field[f] = nullptr;
+        set_error("gpe_create: cannot allocate %zu bytes of device memory", fbytes);
+        return GPE_ENOMEM;
+    }
+    // the zero halo of every field is never written afterwards
+    GPE_HIP_TRY(hipMemsetAsync(field[f], 0, fbytes, stream));
+    GPE_HIP_TRY(hipStreamSynchronize(stream));
     return GPE_OK;
 }
 
@@ -243,7 +312,9 @@ void gpe_solver::set_coefs() {
 }
 
 size_t gpe_solver::device_bytes() const {
-    size_t b = 2 * (size_t)L.elems * elem_bytes();
+    size_t b = 0;
+    for (auto *p : field)
+        if (p) b += (size_t)L.elems * elem_bytes();
     if (pot) b += (size_t)L.elems * (elem_bytes() / 2);
     return b + (size_t)npartials * 4 * sizeof(double) + 8 * sizeof(double);
 }
@@ -332,8 +403,21 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
     const bool arr = cfg.potential == GPE_POT_ARRAY;
     uint64_t nl = 0;
     hipError_t e = hipSuccess;
-    if (cfg.mode == GPE_REAL_TIME) {
-        C *ev = (C *)field[cur], *od = (C *)field[cur ^ 1];
+    if (cfg.mode == GPE_REAL_TIME && kernel == GPE_KERNEL_FUSED) {
+        int sp[2], n = 0;
+        for (int f = 0; f < 4; f++)
+            if (f != ie && f != io) sp[n++] = f;
+        for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
+            C *eo = (C *)field[sp[0]], *oo = (C *)field[sp[1]];
+            const C *ei = (const C *)field[ie], *oi = (const C *)field[io];
+            if (arr) e = launch_fused<T, FORM, POT_ARRAY>(tile, eo, oo, ei, oi, p, L, H, trap, co.dt, stream);
+            else e = launch_fused<T, FORM, POT_TRAP>(tile, eo, oo, ei, oi, p, L, H, trap, co.dt, stream);
+            std::swap(ie, sp[0]);
+            std::swap(io, sp[1]);
+            nl += 1;
+        }
+    } else if (cfg.mode == GPE_REAL_TIME) {
+        C *ev = (C *)field[ie], *od = (C *)field[io];
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
             if (arr) {
                 e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(kernel, tile, od, ev, p, L, H, trap, co.dt, nullptr, nullptr, stream);
@@ -348,9 +432,12 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
         }
     } else {
         // Euler step into the other buffer; renormalisation folded into the
-        // next read through scal[1] = 1/sqrt(N).
+        // next read through scal[1] = 1/sqrt(N).  (The fused pass is a
+        // leapfrog step; imaginary time runs the single-update z-march.)
+        const int kernel = this->kernel == GPE_KERNEL_FUSED ? GPE_KERNEL_ZMARCH : this->kernel;
+        const int tile = this->kernel == GPE_KERNEL_FUSED ? 0 : this->tile;
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
-            C *in = (C *)field[cur], *out = (C *)field[cur ^ 1];
+            C *in = (C *)field[ie], *out = (C *)field[io];
             if (arr)
                 e = launch_update<T, FORM, POT_ARRAY, MODE_IMAG>(kernel, tile, out, in, p, L, H, trap, co.dt, scal + 1, partials, stream);
             else
@@ -358,12 +445,12 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
             if (e != hipSuccess) break;
             hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(256), 0, stream, partials, update_blocks(kernel, tile, L), co.w, 1, scal);
             e = hipGetLastError();
-            cur ^= 1;
+            std::swap(ie, io);
             nl += 1;
         }
         if (e == hipSuccess && nsteps > 0) {
-            // leave a normalised field behind: field[cur] *= scal[1], scal[1] = 1
-            hipLaunchKernelGGL((k_scale_field<T>), dim3(1024), dim3(256), 0, stream, (C *)field[cur], L.elems, scal + 1);
+            // leave a normalised field behind: field[ie] *= scal[1], scal[1] = 1
+            hipLaunchKernelGGL((k_scale_field<T>), dim3(1024), dim3(256), 0, stream, (C *)field[ie], L.elems, scal + 1);
             e = hipGetLastError();
             const double one = 1.0;
             if (e == hipSuccess) e = hipMemcpyAsync(scal + 1, &one, sizeof(double), hipMemcpyHostToDevice, stream);
@@ -459,10 +546,10 @@ gpe_status gpe_set_state(gpe_solver *s, const double *even, const double *odd) {
     }
     gpe_status st;
     const bool f64 = s->cfg.precision == GPE_FP64;
-    st = f64 ? s->upload<double>(s->cur, even) : s->upload<float>(s->cur, even);
+    st = f64 ? s->upload<double>(s->ie, even) : s->upload<float>(s->ie, even);
     if (st != GPE_OK) return st;
     if (odd) {
-        st = f64 ? s->upload<double>(s->cur ^ 1, odd) : s->upload<float>(s->cur ^ 1, odd);
+        st = f64 ? s->upload<double>(s->io, odd) : s->upload<float>(s->io, odd);
         if (st != GPE_OK) return st;
         s->odd_valid = true;
     } else {
@@ -480,11 +567,11 @@ gpe_status gpe_get_state(gpe_solver *s, double *even, double *odd) {
     }
     const bool f64 = s->cfg.precision == GPE_FP64;
     if (even) {
-        gpe_status st = f64 ? s->download<double>(s->cur, even) : s->download<float>(s->cur, even);
+        gpe_status st = f64 ? s->download<double>(s->ie, even) : s->download<float>(s->ie, even);
         if (st != GPE_OK) return st;
     }
     if (odd) {
-        gpe_status st = f64 ? s->download<double>(s->cur ^ 1, odd) : s->download<float>(s->cur ^ 1, odd);
+        gpe_status st = f64 ? s->download<double>(s->io, odd) : s->download<float>(s->io, odd);
         if (st != GPE_OK) return st;
     }
     return GPE_OK;
@@ -514,11 +601,11 @@ gpe_status gpe_init_odd(gpe_solver *s, double mu) {
     const double ph = (-0.5 * s->cfg.dt) * mu;
     const double cr = std::cos(ph), ci = -std::sin(ph);
     if (s->cfg.precision == GPE_FP64)
-        hipLaunchKernelGGL((k_phase_copy<double>), dim3(1024), dim3(256), 0, s->stream, (double2 *)s->field[s->cur ^ 1],
-                           (const double2 *)s->field[s->cur], s->L.elems, cr, ci);
+        hipLaunchKernelGGL((k_phase_copy<double>), dim3(1024), dim3(256), 0, s->stream, (double2 *)s->field[s->io],
+                           (const double2 *)s->field[s->ie], s->L.elems, cr, ci);
     else
-        hipLaunchKernelGGL((k_phase_copy<float>), dim3(1024), dim3(256), 0, s->stream, (float2 *)s->field[s->cur ^ 1],
-                           (const float2 *)s->field[s->cur], s->L.elems, cr, ci);
+        hipLaunchKernelGGL((k_phase_copy<float>), dim3(1024), dim3(256), 0, s->stream, (float2 *)s->field[s->io],
+                           (const float2 *)s->field[s->ie], s->L.elems, cr, ci);
     GPE_HIP_TRY(hipGetLastError());
     GPE_HIP_TRY(hipStreamSynchronize(s->stream));
     s->odd_valid = true;
@@ -554,7 +641,7 @@ gpe_status gpe_observables(gpe_solver *s, int which, double out[4]) {
         set_error("gpe_observables: invalid argument");
         return GPE_EINVAL;
     }
-    const int f = which == 0 ? s->cur : (s->cur ^ 1);
+    const int f = which == 0 ? s->ie : s->io;
     if (s->cfg.precision == GPE_FP64) return s->observables_t<double>(f, out);
     return s->observables_t<float>(f, out);
 }
@@ -570,29 +657,31 @@ gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile) {
         set_error("gpe_set_variant: invalid kernel family");
         return GPE_EINVAL;
     }
-    if (kernel == GPE_KERNEL_FUSED) {
-        set_error("gpe_set_variant: fused kernel family not available");
-        return GPE_ENOTSUP;
-    }
-    if ((int)tile >= num_tiles((int)kernel)) {
+    if ((int)tile > num_tiles((int)kernel)) {
         set_error("gpe_set_variant: tile %u out of range", tile);
         return GPE_EINVAL;
     }
     GPE_HIP_TRY(hipStreamSynchronize(s->stream));
+    if (kernel == GPE_KERNEL_FUSED) {
+        for (int f = 0; f < 4; f++) {
+            gpe_status st = s->alloc_field(f);
+            if (st != GPE_OK) return st;
+        }
+    }
     s->kernel = (int)kernel;
-    s->tile = (int)tile;
+    s->tile = tile == 0 ? default_tile((int)kernel, s->cfg.precision) : (int)tile - 1;
     return GPE_OK;
 }
 
 int gpe_variant_tiles(uint32_t kernel) {
-    if (kernel == GPE_KERNEL_GATHER || kernel == GPE_KERNEL_ZMARCH) return num_tiles((int)kernel);
+    if (kernel >= GPE_KERNEL_GATHER && kernel <= GPE_KERNEL_FUSED) return num_tiles((int)kernel);
     return 0;
 }
 
 void gpe_variant_shape(uint32_t kernel, uint32_t tile, int32_t out[3]) {
     out[0] = out[1] = out[2] = 0;
-    if ((kernel == GPE_KERNEL_GATHER || kernel == GPE_KERNEL_ZMARCH) && (int)tile < num_tiles((int)kernel)) {
-        const TileShape t = tile_shape((int)kernel, (int)tile);
+    if (kernel >= GPE_KERNEL_GATHER && kernel <= GPE_KERNEL_FUSED && tile >= 1 && (int)tile <= num_tiles((int)kernel)) {
+        const TileShape t = tile_shape((int)kernel, (int)tile - 1);
         out[0] = t.ty;
         out[1] = t.cz;
         out[2] = t.remap;
@@ -608,8 +697,8 @@ gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info) {
     info->device_bytes = s->device_bytes();
     info->steps_done = s->steps_done;
     info->kernel = (uint32_t)s->kernel;
-    info->tile = (uint32_t)s->tile;
-    info->launches_per_step = s->cfg.mode == GPE_REAL_TIME ? 2 : 1;
+    info->tile = (uint32_t)s->tile + 1;
+    info->launches_per_step = (s->cfg.mode == GPE_REAL_TIME && s->kernel != GPE_KERNEL_FUSED) ? 2 : 1;
     const double S = (double)s->elem_bytes();
     const double P = s->cfg.potential == GPE_POT_ARRAY ? S / 2 : 0.0;
     // read prev, read next, write next (+ streamed V) per node-update, two

@@ -20,10 +20,11 @@ struct gpe_solver {
     int kernel = GPE_KERNEL_GATHER;
     int tile = 0;
     int device = 0;
-    // field[cur] holds the even time level, field[cur^1] the odd one (or the
-    // imaginary-time scratch buffer).
-    void *field[2] = {nullptr, nullptr};
-    int cur = 0;
+    // field[ie] holds the even time level, field[io] the odd one (or the
+    // imaginary-time scratch buffer).  The fused kernel writes the step's
+    // result into the other two buffers and swaps.
+    void *field[4] = {nullptr, nullptr, nullptr, nullptr};
+    int ie = 0, io = 1;
     void *pot = nullptr;           // V*dt in the field layout (GPE_POT_ARRAY)
     std::vector<double> V_host;    // unscaled V, host order (re-scaled on dt change)
     double *scal = nullptr;        // [0] last N, [1] read scale, [4..7] observables
@@ -36,6 +37,7 @@ struct gpe_solver {
 
     ~gpe_solver();
     gpe_status init(const gpe_config &c);
+    gpe_status alloc_field(int f);
     void set_coefs();
     size_t elem_bytes() const;
     size_t nodes() const;

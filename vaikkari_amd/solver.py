@@ -159,9 +159,9 @@ class Solver:
 def variants():
     """All (kernel family, tile, shape) combinations the library can run."""
     out = []
-    for name in ("gather", "zmarch"):
+    for name in ("gather", "zmarch", "fused"):
         k = KERNELS[name]
-        for t in range(lib.gpe_variant_tiles(k)):
+        for t in range(1, lib.gpe_variant_tiles(k) + 1):
             sh = (C.c_int32 * 3)()
             lib.gpe_variant_shape(k, t, sh)
             out.append({"kernel": name, "tile": t, "waves": sh[0], "cz": sh[1], "remap": sh[2]})
