@@ -1,0 +1,90 @@
+/*
+ * vaikkari_gpe_batch.h -- many independent condensates stepped together.
+ *
+ * No counterpart in the reference, which steps exactly one field per
+ * process (GPUGrossPitaevskii_BCC.cu:475-481).  A batch keeps every
+ * instance's state resident in HBM and advances all of them with one set of
+ * persistent kernel launches per step: the fused leapfrog pass for
+ * real-time instances, the z-march Euler update plus a per-instance FP64
+ * renormalisation for imaginary-time instances.  A chunk of steps is
+ * captured once into a hipGraph and replayed.  Each instance's arithmetic
+ * (tile decomposition, reduction order) depends only on that instance, so
+ * its result is bit-identical whatever else shares the batch.
+ *
+ * Instances use the in-kernel harmonic trap V = 0.5(x^2 + y^2 + ratio^2 z^2).
+ * Host array order as in vaikkari_gpe.h.
+ */
+#ifndef VAIKKARI_GPE_BATCH_H
+#define VAIKKARI_GPE_BATCH_H
+
+#include "vaikkari_gpe.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gpe_batch gpe_batch;
+
+typedef enum gpe_init_kind {
+    GPE_INIT_HOST = 0,    /* state supplied with gpe_batch_set_state */
+    GPE_INIT_GAUSSIAN = 1 /* device recipe: (pi sigma^2)^(-3/4) exp(-r^2/(2 sigma^2))
+                             * (1 + noise (xr + i xi)), xr, xi ~ N(0,1) from a
+                             * counter-based generator keyed by (seed, node);
+                             * odd = even exp(i mu0 dt/2) */
+} gpe_init_kind;
+
+typedef struct gpe_instance_config {
+    uint32_t nx, ny, nz;   /* cubic cells */
+    uint32_t mode;         /* gpe_mode */
+    double block_scale;    /* b = h/sqrt(8) */
+    double dt;             /* time step (real) or dtau (imaginary) */
+    double g;
+    double trap_ratio;
+    double origin[3];      /* corner of cell (0,0,0); non-finite -> box centred on 0 */
+    uint64_t steps;        /* steps to run */
+    uint32_t init;         /* gpe_init_kind */
+    uint32_t zlayers;      /* z layers per tile, 0 = default */
+    double sigma, noise, mu0; /* GPE_INIT_GAUSSIAN parameters */
+    uint64_t seed;
+} gpe_instance_config;
+
+typedef struct gpe_instance_info {
+    uint64_t steps_done, steps_left;
+    uint64_t nodes;
+    uint64_t device_bytes;
+    uint32_t mode;
+    uint32_t active;       /* 1 while resident */
+} gpe_instance_info;
+
+/* precision: gpe_precision of every instance; hbm_budget_bytes: instances
+ * are admitted while their resident bytes fit (0 = no limit). */
+gpe_status gpe_batch_create(uint32_t precision, uint64_t hbm_budget_bytes, gpe_batch **out);
+void gpe_batch_destroy(gpe_batch *b);
+
+/* Adds an instance (allocating its fields); GPE_EBUSY when it would exceed
+ * the budget.  *id >= 0 identifies it until gpe_batch_remove. */
+gpe_status gpe_batch_add(gpe_batch *b, const gpe_instance_config *cfg, int32_t *id);
+/* Bytes an instance of this configuration would hold resident. */
+uint64_t gpe_batch_instance_bytes(const gpe_batch *b, const gpe_instance_config *cfg);
+gpe_status gpe_batch_set_state(gpe_batch *b, int32_t id, const double *even, const double *odd);
+gpe_status gpe_batch_get_state(gpe_batch *b, int32_t id, double *even, double *odd);
+gpe_status gpe_batch_observables(gpe_batch *b, int32_t id, double out[4]);
+gpe_status gpe_batch_instance(const gpe_batch *b, int32_t id, gpe_instance_info *info);
+gpe_status gpe_batch_remove(gpe_batch *b, int32_t id);
+
+/* Advances every active instance by min(chunk_steps, steps_left) steps,
+ * `chunks` times in a row; the chunk is one captured hipGraph (re-captured
+ * only when chunk_steps or the instance set changes).  With elapsed_ms
+ * non-NULL the device time of the whole call is returned (HIP events). */
+gpe_status gpe_batch_run(gpe_batch *b, uint32_t chunk_steps, uint32_t chunks, double *elapsed_ms);
+/* Sum over active instances of nodes * min(chunk_steps, steps_left), i.e.
+ * the node-steps the next chunk will perform. */
+uint64_t gpe_batch_chunk_node_steps(const gpe_batch *b, uint32_t chunk_steps);
+/* Number of active instances; bytes resident. */
+int32_t gpe_batch_active(const gpe_batch *b);
+uint64_t gpe_batch_resident_bytes(const gpe_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VAIKKARI_GPE_BATCH_H */

@@ -1,0 +1,73 @@
+/*
+ * vaikkari_gpe_serve.h -- continuous-batching solve server.
+ *
+ * No counterpart in the reference (one solve per process, its state never
+ * leaves the GPU until the run ends).  Requests enter a queue; every tick the
+ * server admits queued requests while their resident bytes fit the HBM
+ * budget (first fit: a request that does not fit does not block smaller
+ * ones behind it), advances every resident solve by one chunk of steps in
+ * shared persistent launches (vaikkari_gpe_batch.h), and retires finished
+ * solves, recording time-to-result.  A solve's result does not depend on
+ * what else is resident.
+ *
+ * Times are seconds on the server's clock (gpe_server_now).
+ */
+#ifndef VAIKKARI_GPE_SERVE_H
+#define VAIKKARI_GPE_SERVE_H
+
+#include "vaikkari_gpe.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gpe_server gpe_server;
+
+typedef struct gpe_request {
+    uint32_t n;          /* n^3 cubic cells */
+    uint32_t mode;       /* gpe_mode: ground state (imaginary time) or real-time evolution */
+    uint64_t steps;
+    double h;            /* cube edge; block_scale = h/sqrt(8) */
+    double dt;           /* time step (dtau for imaginary time) */
+    double g;
+    double trap_ratio;   /* V = 0.5(x^2 + y^2 + ratio^2 z^2) */
+    double sigma, noise, mu0; /* Gaussian initial state, see gpe_init_kind */
+    uint64_t seed;
+    uint32_t keep_result;     /* 1: keep the final state resident until gpe_server_fetch */
+    uint32_t reserved;
+} gpe_request;
+
+typedef struct gpe_completion {
+    uint64_t ticket;
+    double arrival, admitted, finished; /* server clock, seconds */
+    double node_steps;                  /* 12 n^3 * steps */
+    double norm, energy;                /* FP64 observables of the final state */
+} gpe_completion;
+
+typedef struct gpe_server_stats {
+    uint64_t submitted, admitted, completed, queued, active;
+    uint64_t resident_bytes;
+    double node_steps_done;  /* over all ticks */
+    double device_ms;        /* device time of all chunks */
+    double ticks;
+} gpe_server_stats;
+
+gpe_status gpe_server_create(uint32_t precision, uint64_t hbm_budget_bytes, uint32_t chunk_steps,
+                             gpe_server **out);
+void gpe_server_destroy(gpe_server *s);
+double gpe_server_now(const gpe_server *s);
+
+/* Queues a request that arrived at `arrival` (server clock). */
+gpe_status gpe_server_submit(gpe_server *s, const gpe_request *req, double arrival, uint64_t *ticket);
+/* Admit, run one chunk, retire.  *completed = solves finished in this tick. */
+gpe_status gpe_server_tick(gpe_server *s, uint32_t *completed);
+/* Completions since the last call (at most max). */
+gpe_status gpe_server_poll(gpe_server *s, gpe_completion *out, uint32_t max, uint32_t *n);
+/* Final state of a finished keep_result solve (odd may be NULL); frees it. */
+gpe_status gpe_server_fetch(gpe_server *s, uint64_t ticket, double *even, double *odd);
+gpe_status gpe_server_get_stats(const gpe_server *s, gpe_server_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VAIKKARI_GPE_SERVE_H */

@@ -1,0 +1,172 @@
+"""GPU tests of the batched stepper: each instance matches the oracle (or the
+single-instance solver) and is bit-identical alone and in a mixed batch."""
+import math
+
+import numpy as np
+import pytest
+
+import vaikkari_amd as va
+from vaikkari_amd.batch import Batch, instance_config
+from vaikkari_amd.serve import stable_dt
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def host_case(dims, h, g, dt, ratio, seed, sigma=1.2, mu=8.0):
+    b = h / math.sqrt(8)
+    origin = va.centered_origin(dims, b)
+    even = va.gaussian(dims, origin, b, sigma, 1e-3, seed)
+    odd = va.phase_shift(dims, even, mu, dt)
+    V = va.trap_potential(dims, origin, b, ratio)
+    return b, origin, even, odd, V
+
+
+def random_case(dims, h, ratio, seed, amp=0.05):
+    rng = np.random.default_rng(seed)
+    b = h / math.sqrt(8)
+    origin = va.centered_origin(dims, b)
+    n = va.nodes_of(dims)
+    even = amp * rng.uniform(-1, 1, 2 * n)
+    odd = even + 0.01 * amp * rng.uniform(-1, 1, 2 * n)
+    return b, origin, even, odd, va.trap_potential(dims, origin, b, ratio)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_real_time_matches_oracle(orc, precision):
+    announce(f"batch real {precision}")
+    B = Batch(precision)
+    cases = [((13, 9, 21), 0.45, 400.0, 1e-3, 1.5, 1), ((70, 5, 8), 0.4, 900.0, 8e-4, 0.7, 2),
+             ((24, 24, 24), 0.25, 2000.0, 1e-3, 2.0, 3)]
+    ids, refs = [], []
+    lf = {}
+    for dims, h, g, dt, ratio, seed in cases:
+        b, origin, even, odd, V = random_case(dims, h, ratio, seed)
+        cfg = instance_config(dims, h, g, dt, 37, "real", ratio, init="host", origin=origin)
+        i = B.add(cfg)
+        B.set_state(i, even, odd)
+        ids.append(i)
+        c = va.leapfrog_coefs(b, dt, g)
+        refs.append(orc.leapfrog(even, odd, V * dt, dims, *c, 37))
+    B.run(10, 4)  # 40 >= 37: every instance stops at its own step count
+    for i, (oe, oo) in zip(ids, refs):
+        assert B.info(i)["steps_done"] == 37
+        ge, go = B.get_state(i)
+        if precision == "fp64":
+            np.testing.assert_array_equal(ge, oe)
+            np.testing.assert_array_equal(go, oo)
+        else:
+            assert np.linalg.norm(ge - oe) / np.linalg.norm(oe) < 2e-6
+    B.close()
+
+
+def test_batch_imag_time_matches_solver_and_oracle(orc):
+    dims, h, g, ratio = (20, 14, 17), 0.4, 600.0, 1.7
+    announce("batch imag fp64")
+    b, origin, even, _, V = host_case(dims, h, g, 1e-3, ratio, 5)
+    dtau = 0.5 / (3.0 / b ** 2 + V.max())
+    B = Batch("fp64")
+    i = B.add(instance_config(dims, h, g, dtau, 45, "imag", ratio, init="host", origin=origin))
+    B.set_state(i, even)
+    B.run(16, 3)
+    ge, _ = B.get_state(i)
+    s = va.Solver(dims, b, dtau, g, origin, "fp64", "imag", "trap", ratio, "zmarch")
+    s.set_state(even)
+    s.step(45)
+    se = s.get_state(odd=False)
+    np.testing.assert_array_equal(ge, se)
+    lf, lf0, gdt = va.leapfrog_coefs(b, dtau, g)
+    oe, _ = orc.imag_steps(even, V * dtau, dims, lf, lf0, gdt, b, 45)
+    assert np.linalg.norm(ge - oe) / np.linalg.norm(oe) < 1e-12
+    obs = B.observables(i)
+    ref = orc.observables(oe, V, dims, b, g)
+    assert obs["N"] == pytest.approx(ref[0], rel=1e-12)
+    assert obs["E"] == pytest.approx(ref[1] + ref[2] + 0.5 * ref[3], rel=1e-12)
+    s.close()
+    B.close()
+
+
+def _mixed_configs():
+    rng = np.random.default_rng(12)
+    out = []
+    for q in range(7):
+        n = int(rng.choice([8, 12, 16, 24, 33]))
+        mode = "imag" if q % 3 == 0 else "real"
+        g = float(np.exp(rng.uniform(np.log(100), np.log(5000))))
+        ratio = float(rng.uniform(0.5, 4.0))
+        dt = stable_dt(n, 0.25, g, ratio, mode=mode)
+        out.append(dict(dims=(n, n, n), h=0.25, g=g, dt=dt, steps=int(rng.integers(20, 60)), mode=mode,
+                        trap_ratio=ratio, seed=q + 100))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_alone_equals_in_mix(precision):
+    """Each instance's result is bit-identical when stepped alone and inside a mixed batch of
+    real- and imaginary-time instances of other sizes (chunks of uneven length)."""
+    announce(f"alone vs mix {precision}")
+    cfgs = _mixed_configs()
+    mix = Batch(precision)
+    ids = [mix.add(instance_config(c["dims"], c["h"], c["g"], c["dt"], c["steps"], c["mode"], c["trap_ratio"],
+                                   seed=c["seed"], mu0=5.0)) for c in cfgs]
+    mix.run(7, 10)
+    for c, i in zip(cfgs, ids):
+        alone = Batch(precision)
+        j = alone.add(instance_config(c["dims"], c["h"], c["g"], c["dt"], c["steps"], c["mode"], c["trap_ratio"],
+                                      seed=c["seed"], mu0=5.0))
+        alone.run(13, 6)
+        a_e, a_o = alone.get_state(j)
+        m_e, m_o = mix.get_state(i)
+        assert alone.info(j)["steps_done"] == mix.info(i)["steps_done"] == c["steps"]
+        np.testing.assert_array_equal(a_e, m_e)
+        if a_o is not None:
+            np.testing.assert_array_equal(a_o, m_o)
+        alone.close()
+    mix.close()
+
+
+def test_device_gaussian_init_and_budget():
+    announce("batch init + budget")
+    cfg = instance_config((32, 32, 32), 0.25, 500.0, 1e-3, 10, "real", 1.0, sigma=1.5, noise=1e-3, seed=7)
+    B = Batch("fp32", hbm_budget_bytes=0)
+    per = B.instance_bytes(cfg)
+    B.close()
+    B = Batch("fp32", hbm_budget_bytes=2 * per + per // 2)
+    i0, i1 = B.add(cfg), B.add(cfg)
+    assert B.try_add(cfg) is None  # over budget
+    assert B.resident_bytes == 2 * per
+    e, o = B.get_state(i0)
+    dims, b = (32, 32, 32), 0.25 / math.sqrt(8)
+    assert va.field_norm(dims, b, e) == pytest.approx(1.0, abs=2e-3)
+    e1, _ = B.get_state(i1)
+    np.testing.assert_array_equal(e, e1)  # same seed, same state
+    B.remove(i1)
+    assert B.try_add(cfg) is not None
+    B.close()
+
+
+def test_graph_replays_match_eager_chunks():
+    """One captured chunk replayed several times == the same steps in other chunkings."""
+    announce("graph replay consistency")
+    cfg = instance_config((30, 20, 25), 0.3, 800.0, 6e-4, 60, "real", 1.3, seed=3, mu0=4.0)
+    outs = []
+    for chunk, chunks in ((10, 6), (60, 1), (7, 9), (1, 60)):
+        B = Batch("fp64")
+        i = B.add(cfg)
+        for _ in range(chunks):
+            B.run(chunk, 1)
+            B.get_state(i)  # host work between replays
+        outs.append(B.get_state(i))
+        B.close()
+    for e, o in outs[1:]:
+        np.testing.assert_array_equal(e, outs[0][0])
+        np.testing.assert_array_equal(o, outs[0][1])

@@ -1,0 +1,60 @@
+"""CPU tests of the batch/serving host logic: workload generator, time-step
+rule, stability guard and admission arithmetic (no device compute)."""
+import math
+
+import numpy as np
+import pytest
+
+import vaikkari_amd as va
+from vaikkari_amd import _lib
+from vaikkari_amd.batch import instance_config
+from vaikkari_amd.serve import poisson_workload, stable_dt
+
+
+def test_poisson_workload_statistics():
+    w = poisson_workload(seed=5, rate=20.0, duration=600.0)
+    assert abs(len(w) / 600.0 - 20.0) < 1.0
+    assert all(a.arrival < b.arrival for a, b in zip(w, w[1:]))
+    ns = np.array([r.n for r in w])
+    assert ns.min() >= 16 and ns.max() <= 160
+    # log-uniform in n: median near sqrt(16*160)
+    assert abs(np.median(ns) - math.sqrt(16 * 160)) < 6
+    frac_imag = np.mean([r.mode == "imag" for r in w])
+    assert abs(frac_imag - 0.3) < 0.03
+    steps = np.array([r.steps for r in w])
+    assert steps.min() >= 500 and steps.max() <= 20000
+    # seeded: identical replay
+    w2 = poisson_workload(seed=5, rate=20.0, duration=600.0)
+    assert [(r.arrival, r.n, r.seed) for r in w] == [(r.arrival, r.n, r.seed) for r in w2]
+
+
+@pytest.mark.parametrize("n", [16, 24, 32, 64, 96, 160])
+@pytest.mark.parametrize("g", [100.0, 5000.0])
+@pytest.mark.parametrize("ratio", [0.5, 4.0])
+def test_stable_dt_respects_linear_bound(n, g, ratio):
+    h = 0.25
+    b = h / math.sqrt(8)
+    dt = stable_dt(n, h, g, ratio)
+    assert dt <= 1e-3 * 32 / n
+    dims = (n, n, n)
+    V = va.trap_potential((2, 2, 2), va.centered_origin((2, 2, 2), b), b, ratio)  # placeholder shape
+    half = 0.5 * n * h
+    vmax = 0.5 * (2 * half ** 2 + ratio ** 2 * half ** 2)
+    assert dt * (6.0 / b ** 2 + vmax) < 2.0
+
+
+def test_solver_rejects_unstable_step_before_device_work():
+    b = 0.1 / math.sqrt(8)
+    with pytest.raises(_lib.GpeError) as ei:
+        va.Solver((16, 16, 16), b, 1e-3, 100.0)  # dt * 6/b^2 = 4.8 > 2
+    assert ei.value.status == _lib.EUNSTABLE
+
+
+def test_instance_bytes_accounting():
+    from vaikkari_amd.batch import Batch  # noqa: F401  (import only: creating a Batch needs a GPU)
+    c = instance_config((32, 32, 32), 0.25, 500.0, 1e-3, 10, "real")
+    assert c.block_scale == pytest.approx(0.25 / math.sqrt(8))
+    # host-side layout: 4 FP32 fields of (nz+2)*(ny+2)*12*xp complex values
+    xp = ((16 + 32 + 1 + 15) // 16) * 16
+    per_field = 34 * 34 * 12 * xp * 8
+    assert per_field > 12 * 32 ** 3 * 8

@@ -1,5 +1,4 @@
-"""ctypes binding of libvaikkari_gpe.so (declarations in include/vaikkari_gpe.h
-and include/vaikkari_gpe_recipes.h).
+"""ctypes binding of libvaikkari_gpe.so (declarations in include/*.h).
 
 The native library is the only implementation: if it is missing, importing
 this module raises.  There is no Python fallback for any computation.
@@ -27,6 +26,8 @@ REAL_TIME, IMAG_TIME = 0, 1
 POT_ARRAY, POT_TRAP = 0, 1
 # gpe_kernel
 KERNEL_AUTO, KERNEL_GATHER, KERNEL_ZMARCH, KERNEL_FUSED = 0, 1, 2, 3
+# gpe_init_kind
+INIT_HOST, INIT_GAUSSIAN = 0, 1
 
 
 class GpeConfig(C.Structure):
@@ -70,6 +71,62 @@ class GpeReferenceRun(C.Structure):
     ]
 
 
+class GpeInstanceConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
+        ("mode", C.c_uint32),
+        ("block_scale", C.c_double),
+        ("dt", C.c_double),
+        ("g", C.c_double),
+        ("trap_ratio", C.c_double),
+        ("origin", C.c_double * 3),
+        ("steps", C.c_uint64),
+        ("init", C.c_uint32),
+        ("zlayers", C.c_uint32),
+        ("sigma", C.c_double),
+        ("noise", C.c_double),
+        ("mu0", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class GpeInstanceInfo(C.Structure):
+    _fields_ = [
+        ("steps_done", C.c_uint64),
+        ("steps_left", C.c_uint64),
+        ("nodes", C.c_uint64),
+        ("device_bytes", C.c_uint64),
+        ("mode", C.c_uint32),
+        ("active", C.c_uint32),
+    ]
+
+
+class GpeRequest(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32), ("mode", C.c_uint32), ("steps", C.c_uint64),
+        ("h", C.c_double), ("dt", C.c_double), ("g", C.c_double), ("trap_ratio", C.c_double),
+        ("sigma", C.c_double), ("noise", C.c_double), ("mu0", C.c_double),
+        ("seed", C.c_uint64),
+        ("keep_result", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
+class GpeCompletion(C.Structure):
+    _fields_ = [
+        ("ticket", C.c_uint64),
+        ("arrival", C.c_double), ("admitted", C.c_double), ("finished", C.c_double),
+        ("node_steps", C.c_double), ("norm", C.c_double), ("energy", C.c_double),
+    ]
+
+
+class GpeServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
+        ("queued", C.c_uint64), ("active", C.c_uint64), ("resident_bytes", C.c_uint64),
+        ("node_steps_done", C.c_double), ("device_ms", C.c_double), ("ticks", C.c_double),
+    ]
+
+
 class GpeError(RuntimeError):
     def __init__(self, status: int, where: str, detail: str):
         super().__init__(f"{where}: status {status} ({detail})")
@@ -103,6 +160,27 @@ SIGNATURES = [
     ("gpe_variant_tiles", _I, [_U32]),
     ("gpe_variant_shape", None, [_U32, _U32, _I32P]),
     ("gpe_diag_copy_bandwidth", _I, [_U64, _U32, _DP]),
+    ("gpe_batch_create", _I, [_U32, _U64, C.POINTER(_P)]),
+    ("gpe_batch_destroy", None, [_P]),
+    ("gpe_batch_add", _I, [_P, C.POINTER(GpeInstanceConfig), C.POINTER(C.c_int32)]),
+    ("gpe_batch_instance_bytes", _U64, [_P, C.POINTER(GpeInstanceConfig)]),
+    ("gpe_batch_set_state", _I, [_P, C.c_int32, _DP, _DP]),
+    ("gpe_batch_get_state", _I, [_P, C.c_int32, _DP, _DP]),
+    ("gpe_batch_observables", _I, [_P, C.c_int32, _DP]),
+    ("gpe_batch_instance", _I, [_P, C.c_int32, C.POINTER(GpeInstanceInfo)]),
+    ("gpe_batch_remove", _I, [_P, C.c_int32]),
+    ("gpe_batch_run", _I, [_P, _U32, _U32, _DP]),
+    ("gpe_batch_chunk_node_steps", _U64, [_P, _U32]),
+    ("gpe_batch_active", C.c_int32, [_P]),
+    ("gpe_batch_resident_bytes", _U64, [_P]),
+    ("gpe_server_create", _I, [_U32, _U64, _U32, C.POINTER(_P)]),
+    ("gpe_server_destroy", None, [_P]),
+    ("gpe_server_now", _D, [_P]),
+    ("gpe_server_submit", _I, [_P, C.POINTER(GpeRequest), _D, C.POINTER(_U64)]),
+    ("gpe_server_tick", _I, [_P, C.POINTER(_U32)]),
+    ("gpe_server_poll", _I, [_P, C.POINTER(GpeCompletion), _U32, C.POINTER(_U32)]),
+    ("gpe_server_fetch", _I, [_P, _U64, _DP, _DP]),
+    ("gpe_server_get_stats", _I, [_P, C.POINTER(GpeServerStats)]),
     ("gpe_status_string", C.c_char_p, [_I]),
     ("gpe_last_error", C.c_char_p, []),
     ("gpe_bcc_positions", None, [_I32P]),

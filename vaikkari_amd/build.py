@@ -17,7 +17,7 @@ LIB_NAME = "libvaikkari_gpe.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 ARCH = os.environ.get("GPE_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["gpe_solver.hip", "diag.hip", "host_setup.cpp", "integrate.cpp"]
+SOURCES = ["gpe_solver.hip", "gpe_batch.hip", "diag.hip", "host_setup.cpp", "integrate.cpp", "gpe_serve.cpp"]
 
 FLAGS = [
     "-O3",

@@ -1,0 +1,785 @@
+// gpe_batch.hip -- many independent condensates stepped by shared persistent
+// launches (include/vaikkari_gpe_batch.h).
+//
+// Device side: every active instance has a descriptor (layout, trap,
+// coefficients, tile grid, field pointers, steps left, ping-pong parity).
+// Per step three persistent kernels run over fixed grids: the fused
+// leapfrog pass over all tiles of all real-time instances, the z-march
+// Euler update over all tiles of all imaginary-time instances, and the
+// per-instance FP64 norm reduction of the latter.  Work items (instance,
+// tile) are laid out instance-major; each round of the persistent loop hands
+// each XCD a contiguous range of them.  An "advance" kernel at the end of a
+// chunk updates steps-left and parity on the device, so a captured chunk
+// graph can be replayed without host traffic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "gpe_fused.hpp"
+#include "gpe_internal.hpp"
+#include "gpe_kernels.hpp"
+#include "gpe_solver.hpp"
+#include "../../include/vaikkari_gpe_batch.h"
+
+using namespace gpe;
+
+namespace gpe {
+
+template <typename T>
+struct InstDesc {
+    Layout L;
+    Trap trap;
+    Hamiltonian<T, sizeof(T) == 8 ? FORM_REF : FORM_DIFF> H;
+    double dt;
+    FusedGrid fg;  // real time
+    Grid3 zg;      // imaginary time (z-march tiles)
+    int zcz;
+    cplx_t<T> *buf[4];
+    double *partials;
+    double *scal;  // [0] N, [1] read scale
+    double w;
+    int tile0;     // first work item in its kernel's list
+    int ntiles;
+    long long steps_left;
+    int parity;
+};
+
+struct BatchHeader {
+    int n_real, total_real, n_imag, total_imag;
+};
+
+constexpr int kBatchGrid = 2048;  // persistent blocks per kernel
+constexpr int kImagTY = 4;
+constexpr int kFinalizeThreads = 256;
+
+template <typename T>
+__device__ __forceinline__ int find_instance(const InstDesc<T> *__restrict__ D, int n, int w) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (D[mid].tile0 <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <typename T, int TY>
+__global__ void __launch_bounds__(64 * TY)
+    k_batch_fused(const InstDesc<T> *__restrict__ D, const BatchHeader *__restrict__ hdr, int s, int chunk) {
+    constexpr int FORM = sizeof(T) == 8 ? FORM_REF : FORM_DIFF;
+    __shared__ cplx_t<T> ylds[2][6][TY][64];
+    const int n = hdr->n_real, total = hdr->total_real;
+    for (int base = 0; base < total; base += gridDim.x) {
+        const int w = base + remap_block(blockIdx.x, gridDim.x, 1);
+        if (w >= total) continue;
+        const int q = find_instance(D, n, w);
+        const InstDesc<T> &d = D[q];
+        const long long steps = d.steps_left < chunk ? d.steps_left : chunk;
+        if (s >= steps) continue;
+        const int p = (d.parity + s) & 1;
+        fused_tile<T, FORM, POT_TRAP, TY>(d.buf[2 * (1 - p)], d.buf[2 * (1 - p) + 1], d.buf[2 * p], d.buf[2 * p + 1],
+                                          nullptr, d.L, d.H, d.trap, d.dt, d.fg, w - d.tile0, ylds);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64 * kImagTY)
+    k_batch_imag(const InstDesc<T> *__restrict__ D, const BatchHeader *__restrict__ hdr, int s, int chunk) {
+    constexpr int FORM = sizeof(T) == 8 ? FORM_REF : FORM_DIFF;
+    __shared__ double lds[kImagTY];
+    const int n = hdr->n_imag, total = hdr->total_imag;
+    for (int base = 0; base < total; base += gridDim.x) {
+        const int w = base + remap_block(blockIdx.x, gridDim.x, 1);
+        if (w >= total) continue;
+        const int q = find_instance(D, n, w);
+        const InstDesc<T> &d = D[q];
+        const long long steps = d.steps_left < chunk ? d.steps_left : chunk;
+        if (s >= steps) continue;
+        const int p = (d.parity + s) & 1;
+        const int lt = w - d.tile0;
+        const int bx = lt % d.zg.gx, by = (lt / d.zg.gx) % d.zg.gy, bz = lt / (d.zg.gx * d.zg.gy);
+        double part[1];
+        part[0] = zmarch_tile<T, FORM, POT_TRAP, MODE_IMAG, kImagTY>(d.buf[1 - p], d.buf[p], nullptr, d.L, d.H,
+                                                                     d.trap, d.dt, d.scal + 1, bx, by, bz, d.zcz);
+        block_sum<1, kImagTY>(part, lds);
+        if (threadIdx.x == 0 && threadIdx.y == 0) d.partials[lt] = part[0];
+        __syncthreads();  // lds is reused by the next work item
+    }
+}
+
+// Per imaginary-time instance: N = w * sum(partials), scale = 1/sqrt(N), in a
+// fixed order (the same as k_finalize<1>).
+template <typename T>
+__global__ void __launch_bounds__(kFinalizeThreads)
+    k_batch_finalize(const InstDesc<T> *__restrict__ D, const BatchHeader *__restrict__ hdr, int s, int chunk) {
+    __shared__ double red[kFinalizeThreads];
+    const int n = hdr->n_imag;
+    const int t = threadIdx.x;
+    for (int q = blockIdx.x; q < n; q += gridDim.x) {
+        const InstDesc<T> &d = D[q];
+        const long long steps = d.steps_left < chunk ? d.steps_left : chunk;
+        if (s >= steps) continue;
+        double sum = 0.0, c = 0.0;
+        for (int b = t; b < d.ntiles; b += kFinalizeThreads) {
+            const double x = d.partials[b];
+            const double u = sum + x;
+            c += (fabs(sum) >= fabs(x)) ? ((sum - u) + x) : ((x - u) + sum);
+            sum = u;
+        }
+        red[t] = sum + c;
+        __syncthreads();
+        for (int h = kFinalizeThreads / 2; h > 0; h >>= 1) {
+            if (t < h) red[t] += red[t + h];
+            __syncthreads();
+        }
+        if (t == 0) {
+            const double N = d.w * red[0];
+            d.scal[0] = N;
+            d.scal[1] = 1.0 / sqrt(N);
+        }
+        __syncthreads();
+    }
+}
+
+// End of a chunk: steps_left -= steps taken, parity ^= steps & 1.
+template <typename T>
+__global__ void k_batch_advance(InstDesc<T> *__restrict__ R, InstDesc<T> *__restrict__ I,
+                                const BatchHeader *__restrict__ hdr, int chunk) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    InstDesc<T> *d = nullptr;
+    if (q < hdr->n_real) d = &R[q];
+    else if (q - hdr->n_real < hdr->n_imag) d = &I[q - hdr->n_real];
+    if (!d) return;
+    const long long steps = d->steps_left < chunk ? d->steps_left : chunk;
+    d->steps_left -= steps;
+    d->parity = (int)((d->parity + steps) & 1);
+}
+
+// Counter-based N(0,1) pair for node n: splitmix64 of (seed, n), Box-Muller.
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+template <typename T>
+__global__ void k_init_gaussian(cplx_t<T> *__restrict__ even, cplx_t<T> *__restrict__ odd, Layout L, Trap trap,
+                                double inv2s2, double amp, double noise, unsigned long long seed, double cph,
+                                double sph) {
+    const long long cells = (long long)L.nx * L.ny * L.nz;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < cells;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % L.nx), j = (int)((c / L.nx) % L.ny), k = (int)(c / ((long long)L.nx * L.ny));
+        const long long base = L.origin + (long long)k * L.sz + (long long)j * L.sy + i;
+        for (int l = 0; l < 12; l++) {
+            const double x = trap.p0[0] + trap.b * ((double)(unsigned)i * trap.bw + trap.bpos[l][0]);
+            const double y = trap.p0[1] + trap.b * ((double)(unsigned)j * trap.bw + trap.bpos[l][1]);
+            const double z = trap.p0[2] + trap.b * ((double)(unsigned)k * trap.bw + trap.bpos[l][2]);
+            const double a = amp * exp(-(x * x + y * y + z * z) * inv2s2);
+            const unsigned long long h1 = splitmix64(seed ^ (unsigned long long)(c * 12 + l) * 0xD1B54A32D192ED03ULL);
+            const unsigned long long h2 = splitmix64(h1);
+            const double u1 = ((double)(h1 >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+            const double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);
+            const double r = sqrt(-2.0 * log(u1));
+            const double xr = r * cos(6.2831853071795864770 * u2), xi = r * sin(6.2831853071795864770 * u2);
+            const double er = a * (1.0 + noise * xr), ei = a * (noise * xi);
+            cplx_t<T> e, o;
+            e.x = (T)er;
+            e.y = (T)ei;
+            even[base + l * L.xp] = e;
+            if (odd) {
+                o.x = (T)(cph * er - sph * ei);
+                o.y = (T)(sph * er + cph * ei);
+                odd[base + l * L.xp] = o;
+            }
+        }
+    }
+}
+
+}  // namespace gpe
+
+namespace {
+
+#define GPE_HIP_TRY(expr)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return GPE_EDEVICE;                                                                   \
+        }                                                                                         \
+    } while (0)
+
+constexpr int kFusedTY64 = 4;   // fused tile rows, FP64 (LDS and VGPR budget)
+constexpr int kFusedTY32 = 8;   // fused tile rows, FP32
+constexpr int kDefaultRealCZ = 16;
+constexpr int kDefaultImagCZ = 8;
+
+struct BInst {
+    gpe_instance_config cfg{};
+    Layout L{};
+    Trap trap{};
+    Coefs co{};
+    FusedGrid fg{};
+    Grid3 zg{};
+    int zcz = 8;
+    void *buf[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *partials = nullptr;
+    double *scal = nullptr;
+    int ntiles = 0;
+    uint64_t steps_done = 0, steps_left = 0;
+    int parity = 0;
+    bool active = false;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct gpe_batch {
+    uint32_t precision = GPE_FP32_BULK;
+    uint64_t budget = 0;
+    std::vector<BInst> inst;
+    uint64_t resident = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // device descriptor arrays (capacity grows by doubling)
+    void *d_real = nullptr, *d_imag = nullptr;
+    BatchHeader *d_hdr = nullptr;
+    int capacity = 0;
+    void *h_stage = nullptr;  // pinned staging for descriptor uploads
+    size_t stage_bytes = 0;
+    int n_real = 0, n_imag = 0;
+    bool desc_dirty = true;
+    // captured chunk graph
+    hipGraphExec_t gexec = nullptr;
+    uint32_t graph_chunk = 0;
+    bool graph_dirty = true;
+    double *obs_partials = nullptr;
+    long long obs_cap = 0;
+
+    size_t elem() const { return precision == GPE_FP64 ? 16 : 8; }
+    ~gpe_batch();
+};
+
+gpe_batch::~gpe_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto &b : inst) {
+        for (auto *p : b.buf)
+            if (p) (void)hipFree(p);
+        if (b.partials) (void)hipFree(b.partials);
+        if (b.scal) (void)hipFree(b.scal);
+    }
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (d_real) (void)hipFree(d_real);
+    if (d_imag) (void)hipFree(d_imag);
+    if (d_hdr) (void)hipFree(d_hdr);
+    if (h_stage) (void)hipHostFree(h_stage);
+    if (obs_partials) (void)hipFree(obs_partials);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+bool valid_instance(const gpe_instance_config *c) {
+    return c && c->nx > 0 && c->ny > 0 && c->nz > 0 && c->nx <= 4096 && c->ny <= 4096 && c->nz <= 4096 &&
+           c->mode <= GPE_IMAG_TIME && c->block_scale > 0 && c->dt > 0 && std::isfinite(c->g) &&
+           std::isfinite(c->trap_ratio) && c->init <= GPE_INIT_GAUSSIAN;
+}
+
+size_t instance_bytes(const gpe_instance_config *c, size_t elem, Layout *Lout, FusedGrid *fg, Grid3 *zg, int *zcz,
+                      int *ntiles, uint32_t precision) {
+    const Layout L = make_layout(c->nx, c->ny, c->nz, elem);
+    const int nfields = c->mode == GPE_REAL_TIME ? 4 : 2;
+    size_t bytes = (size_t)nfields * (size_t)L.elems * elem;
+    int tiles = 0;
+    if (c->mode == GPE_REAL_TIME) {
+        const int ty = precision == GPE_FP64 ? kFusedTY64 : kFusedTY32;
+        const int cz = c->zlayers ? (int)c->zlayers : kDefaultRealCZ;
+        FusedGrid G;
+        G.ntx = (L.nx + 61) / 62;
+        G.w = (L.nx + G.ntx - 1) / G.ntx;
+        G.nty = (L.ny + ty - 3) / (ty - 2);
+        G.h = (L.ny + G.nty - 1) / G.nty;
+        G.ntz = (L.nz + cz - 1) / cz;
+        G.cz = (L.nz + G.ntz - 1) / G.ntz;
+        G.remap = 0;
+        tiles = G.ntx * G.nty * G.ntz;
+        if (fg) *fg = G;
+    } else {
+        const int cz = c->zlayers ? (int)c->zlayers : kDefaultImagCZ;
+        Grid3 G;
+        G.gx = (L.nx + 63) / 64;
+        G.gy = (L.ny + kImagTY - 1) / kImagTY;
+        G.gz = (L.nz + cz - 1) / cz;
+        G.remap = 0;
+        tiles = G.gx * G.gy * G.gz;
+        bytes += (size_t)tiles * sizeof(double) + 8 * sizeof(double);
+        if (zg) *zg = G;
+        if (zcz) *zcz = cz;
+    }
+    if (Lout) *Lout = L;
+    if (ntiles) *ntiles = tiles;
+    return bytes;
+}
+
+template <typename T>
+void fill_desc(InstDesc<T> &d, const BInst &b) {
+    std::memset(&d, 0, sizeof(d));
+    d.L = b.L;
+    d.trap = b.trap;
+    d.H.lapfac = (T)b.co.lapfac_dt;
+    d.H.lapfac0 = (T)b.co.lapfac0_dt;
+    d.H.g = (T)b.co.g_dt;
+    d.dt = b.co.dt;
+    d.fg = b.fg;
+    d.zg = b.zg;
+    d.zcz = b.zcz;
+    for (int f = 0; f < 4; f++) d.buf[f] = (cplx_t<T> *)b.buf[f];
+    d.partials = b.partials;
+    d.scal = b.scal;
+    d.w = b.co.w;
+    d.ntiles = b.ntiles;
+    d.steps_left = (long long)b.steps_left;
+    d.parity = b.parity;
+}
+
+template <typename T>
+gpe_status upload_desc(gpe_batch *B) {
+    const size_t dsz = sizeof(InstDesc<T>);
+    int nr = 0, ni = 0;
+    for (auto &b : B->inst)
+        if (b.active && b.steps_left > 0) (b.cfg.mode == GPE_REAL_TIME ? nr : ni)++;
+    const int need = std::max(std::max(nr, ni), 1);
+    if (need > B->capacity) {
+        int cap = std::max(64, B->capacity);
+        while (cap < need) cap *= 2;
+        if (B->d_real) (void)hipFree(B->d_real);
+        if (B->d_imag) (void)hipFree(B->d_imag);
+        if (B->h_stage) (void)hipHostFree(B->h_stage);
+        B->d_real = B->d_imag = B->h_stage = nullptr;
+        B->capacity = 0;
+        GPE_HIP_TRY(hipMalloc(&B->d_real, cap * dsz));
+        GPE_HIP_TRY(hipMalloc(&B->d_imag, cap * dsz));
+        B->stage_bytes = 2 * cap * dsz + sizeof(BatchHeader);
+        GPE_HIP_TRY(hipHostMalloc(&B->h_stage, B->stage_bytes));
+        B->capacity = cap;
+        B->graph_dirty = true;  // descriptor addresses changed
+    }
+    GPE_HIP_TRY(hipStreamSynchronize(B->stream));  // the staging buffer is reused
+    InstDesc<T> *R = (InstDesc<T> *)B->h_stage;
+    InstDesc<T> *I = R + B->capacity;
+    BatchHeader *H = (BatchHeader *)(I + B->capacity);
+    int tr = 0, ti = 0;
+    nr = ni = 0;
+    for (auto &b : B->inst) {
+        if (!b.active || b.steps_left == 0) continue;
+        if (b.cfg.mode == GPE_REAL_TIME) {
+            fill_desc(R[nr], b);
+            R[nr].tile0 = tr;
+            tr += b.ntiles;
+            nr++;
+        } else {
+            fill_desc(I[ni], b);
+            I[ni].tile0 = ti;
+            ti += b.ntiles;
+            ni++;
+        }
+    }
+    H->n_real = nr;
+    H->total_real = tr;
+    H->n_imag = ni;
+    H->total_imag = ti;
+    GPE_HIP_TRY(hipMemcpyAsync(B->d_real, R, (size_t)std::max(nr, 1) * dsz, hipMemcpyHostToDevice, B->stream));
+    GPE_HIP_TRY(hipMemcpyAsync(B->d_imag, I, (size_t)std::max(ni, 1) * dsz, hipMemcpyHostToDevice, B->stream));
+    GPE_HIP_TRY(hipMemcpyAsync(B->d_hdr, H, sizeof(BatchHeader), hipMemcpyHostToDevice, B->stream));
+    GPE_HIP_TRY(hipStreamSynchronize(B->stream));
+    B->n_real = nr;
+    B->n_imag = ni;
+    B->desc_dirty = false;
+    return GPE_OK;
+}
+
+template <typename T>
+void enqueue_chunk(gpe_batch *B, uint32_t chunk, hipStream_t st) {
+    const InstDesc<T> *R = (const InstDesc<T> *)B->d_real;
+    const InstDesc<T> *I = (const InstDesc<T> *)B->d_imag;
+    constexpr int TY = sizeof(T) == 8 ? kFusedTY64 : kFusedTY32;
+    for (uint32_t s = 0; s < chunk; s++) {
+        if (B->n_real > 0)
+            hipLaunchKernelGGL((k_batch_fused<T, TY>), dim3(kBatchGrid), dim3(64, TY), 0, st, R, B->d_hdr, (int)s,
+                               (int)chunk);
+        if (B->n_imag > 0) {
+            hipLaunchKernelGGL((k_batch_imag<T>), dim3(kBatchGrid), dim3(64, kImagTY), 0, st, I, B->d_hdr, (int)s,
+                               (int)chunk);
+            hipLaunchKernelGGL((k_batch_finalize<T>), dim3(std::min(B->n_imag, 1024)), dim3(kFinalizeThreads), 0, st,
+                               I, B->d_hdr, (int)s, (int)chunk);
+        }
+    }
+    const int n = B->n_real + B->n_imag;
+    hipLaunchKernelGGL((k_batch_advance<T>), dim3((n + 255) / 256), dim3(256), 0, st, (InstDesc<T> *)B->d_real,
+                       (InstDesc<T> *)B->d_imag, B->d_hdr, (int)chunk);
+}
+
+template <typename T>
+gpe_status run_t(gpe_batch *B, uint32_t chunk, uint32_t chunks, double *elapsed_ms) {
+    if (B->desc_dirty) {
+        gpe_status st = upload_desc<T>(B);
+        if (st != GPE_OK) return st;
+        B->graph_dirty = true;  // instance counts are baked into the launch sequence
+    }
+    if (B->n_real + B->n_imag == 0) {
+        if (elapsed_ms) *elapsed_ms = 0.0;
+        return GPE_OK;
+    }
+    if (B->graph_dirty || B->graph_chunk != chunk || !B->gexec) {
+        if (B->gexec) {
+            (void)hipGraphExecDestroy(B->gexec);
+            B->gexec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        GPE_HIP_TRY(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_chunk<T>(B, chunk, B->stream);
+        GPE_HIP_TRY(hipStreamEndCapture(B->stream, &g));
+        const hipError_t ie = hipGraphInstantiate(&B->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            B->gexec = nullptr;
+            set_error("gpe_batch_run: graph instantiation failed: %s", hipGetErrorString(ie));
+            return GPE_EDEVICE;
+        }
+        B->graph_chunk = chunk;
+        B->graph_dirty = false;
+    }
+    if (elapsed_ms) GPE_HIP_TRY(hipEventRecord(B->ev0, B->stream));
+    for (uint32_t c = 0; c < chunks; c++) GPE_HIP_TRY(hipGraphLaunch(B->gexec, B->stream));
+    if (elapsed_ms) GPE_HIP_TRY(hipEventRecord(B->ev1, B->stream));
+    GPE_HIP_TRY(hipStreamSynchronize(B->stream));
+    if (elapsed_ms) {
+        float ms = 0.0f;
+        GPE_HIP_TRY(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+        *elapsed_ms = ms;
+    }
+    // host mirror of k_batch_advance
+    bool finished = false;
+    for (auto &b : B->inst) {
+        if (!b.active || b.steps_left == 0) continue;
+        const uint64_t take = std::min<uint64_t>(b.steps_left, (uint64_t)chunk * chunks);
+        b.steps_left -= take;
+        b.steps_done += take;
+        b.parity = (int)((b.parity + take) & 1);
+        if (b.steps_left == 0) finished = true;
+    }
+    if (finished) B->desc_dirty = true;  // drop finished instances from the work lists
+    return GPE_OK;
+}
+
+BInst *get_inst(gpe_batch *B, int32_t id) {
+    if (!B || id < 0 || id >= (int32_t)B->inst.size() || !B->inst[id].active) {
+        set_error("batch: no active instance %d", id);
+        return nullptr;
+    }
+    return &B->inst[id];
+}
+
+template <typename T>
+gpe_status init_gaussian(gpe_batch *B, BInst &b) {
+    const double s = b.cfg.sigma > 0 ? b.cfg.sigma : 1.0;
+    const double amp = std::pow(3.14159265358979323846 * s * s, -0.75);
+    const double ph = 0.5 * b.cfg.mu0 * b.cfg.dt;
+    cplx_t<T> *odd = b.cfg.mode == GPE_REAL_TIME ? (cplx_t<T> *)b.buf[1] : nullptr;
+    hipLaunchKernelGGL((k_init_gaussian<T>), dim3(1024), dim3(256), 0, B->stream, (cplx_t<T> *)b.buf[0], odd, b.L,
+                       b.trap, 1.0 / (2.0 * s * s), amp, b.cfg.noise, (unsigned long long)b.cfg.seed, std::cos(ph),
+                       std::sin(ph));
+    GPE_HIP_TRY(hipGetLastError());
+    return GPE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gpe_status gpe_batch_create(uint32_t precision, uint64_t budget, gpe_batch **out) {
+    if (!out || precision > GPE_FP32_BULK) {
+        set_error("gpe_batch_create: invalid argument");
+        return GPE_EINVAL;
+    }
+    *out = nullptr;
+    auto *B = new (std::nothrow) gpe_batch();
+    if (!B) return GPE_ENOMEM;
+    B->precision = precision;
+    B->budget = budget;
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&B->ev0) != hipSuccess || hipEventCreate(&B->ev1) != hipSuccess ||
+        hipMalloc(&B->d_hdr, sizeof(BatchHeader)) != hipSuccess) {
+        set_error("gpe_batch_create: HIP setup failed");
+        delete B;
+        return GPE_EDEVICE;
+    }
+    *out = B;
+    return GPE_OK;
+}
+
+void gpe_batch_destroy(gpe_batch *b) { delete b; }
+
+uint64_t gpe_batch_instance_bytes(const gpe_batch *B, const gpe_instance_config *c) {
+    if (!B || !valid_instance(c)) return 0;
+    return instance_bytes(c, B->elem(), nullptr, nullptr, nullptr, nullptr, nullptr, B->precision);
+}
+
+gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id) {
+    if (!B || !id || !valid_instance(c)) {
+        set_error("gpe_batch_add: invalid instance configuration");
+        return GPE_EINVAL;
+    }
+    BInst b;
+    b.cfg = *c;
+    b.bytes = instance_bytes(c, B->elem(), &b.L, &b.fg, &b.zg, &b.zcz, &b.ntiles, B->precision);
+    if (B->budget && B->resident + b.bytes > B->budget) {
+        set_error("gpe_batch_add: %zu bytes would exceed the budget (%llu of %llu resident)", b.bytes,
+                  (unsigned long long)B->resident, (unsigned long long)B->budget);
+        return GPE_EBUSY;
+    }
+    double origin[3];
+    for (int a = 0; a < 3; a++) origin[a] = c->origin[a];
+    if (!std::isfinite(origin[0]) || !std::isfinite(origin[1]) || !std::isfinite(origin[2])) {
+        const uint32_t dims[3] = {c->nx, c->ny, c->nz};
+        const double a = std::sqrt(8.0) * c->block_scale;
+        for (int d = 0; d < 3; d++) origin[d] = -0.5 * a * dims[d];
+    }
+    {
+        const uint32_t dims[3] = {c->nx, c->ny, c->nz};
+        const double vmax = trap_vmax(dims, origin, c->block_scale, c->trap_ratio);
+        if (!linear_step_stable(c->dt, c->block_scale, vmax)) {
+            set_error("gpe_batch_add: dt = %g violates dt*(6/b^2 + max V) < 2 (max V = %g)", c->dt, vmax);
+            return GPE_EUNSTABLE;
+        }
+    }
+    b.trap = make_trap(origin, c->block_scale, c->trap_ratio);
+    const double bs = c->block_scale;
+    const double lapfac = -0.5 * kHodge / (bs * bs);
+    b.co.lapfac_dt = lapfac * c->dt;
+    b.co.lapfac0_dt = (double)GPE_FACES * (-lapfac) * c->dt;
+    b.co.g_dt = c->g * c->dt;
+    b.co.dt = c->dt;
+    b.co.w = node_measure(bs);
+    b.co.lap = kHodge / (bs * bs);
+    b.co.g = c->g;
+    const int nfields = c->mode == GPE_REAL_TIME ? 4 : 2;
+    const size_t fbytes = (size_t)b.L.elems * B->elem();
+    auto fail = [&](gpe_status st) {
+        for (auto *&p : b.buf)
+            if (p) (void)hipFree(p);
+        if (b.partials) (void)hipFree(b.partials);
+        if (b.scal) (void)hipFree(b.scal);
+        return st;
+    };
+    for (int f = 0; f < nfields; f++) {
+        if (hipMalloc(&b.buf[f], fbytes) != hipSuccess) {
+            b.buf[f] = nullptr;
+            set_error("gpe_batch_add: out of device memory");
+            return fail(GPE_ENOMEM);
+        }
+        if (hipMemsetAsync(b.buf[f], 0, fbytes, B->stream) != hipSuccess) return fail(GPE_EDEVICE);
+    }
+    if (c->mode == GPE_IMAG_TIME) {
+        if (hipMalloc(&b.partials, (size_t)b.ntiles * sizeof(double)) != hipSuccess ||
+            hipMalloc(&b.scal, 8 * sizeof(double)) != hipSuccess) {
+            set_error("gpe_batch_add: out of device memory");
+            return fail(GPE_ENOMEM);
+        }
+        const double init[8] = {1.0, 1.0, 0, 0, 0, 0, 0, 0};
+        if (hipMemcpyAsync(b.scal, init, sizeof(init), hipMemcpyHostToDevice, B->stream) != hipSuccess)
+            return fail(GPE_EDEVICE);
+    }
+    b.steps_left = c->steps;
+    b.active = true;
+    if (c->init == GPE_INIT_GAUSSIAN) {
+        const gpe_status st = B->precision == GPE_FP64 ? init_gaussian<double>(B, b) : init_gaussian<float>(B, b);
+        if (st != GPE_OK) return fail(st);
+    }
+    if (hipStreamSynchronize(B->stream) != hipSuccess) {
+        set_error("gpe_batch_add: initialisation failed");
+        return fail(GPE_EDEVICE);
+    }
+    // reuse a free slot so ids stay small
+    int32_t slot = -1;
+    for (size_t q = 0; q < B->inst.size(); q++)
+        if (!B->inst[q].active && !B->inst[q].buf[0]) {
+            slot = (int32_t)q;
+            break;
+        }
+    if (slot < 0) {
+        B->inst.push_back(b);
+        slot = (int32_t)B->inst.size() - 1;
+    } else {
+        B->inst[slot] = b;
+    }
+    B->resident += b.bytes;
+    B->desc_dirty = true;
+    *id = slot;
+    return GPE_OK;
+}
+
+gpe_status gpe_batch_remove(gpe_batch *B, int32_t id) {
+    BInst *b = get_inst(B, id);
+    if (!b) return GPE_EINVAL;
+    GPE_HIP_TRY(hipStreamSynchronize(B->stream));
+    for (auto *&p : b->buf)
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+    if (b->partials) (void)hipFree(b->partials);
+    if (b->scal) (void)hipFree(b->scal);
+    b->partials = nullptr;
+    b->scal = nullptr;
+    b->active = false;
+    B->resident -= b->bytes;
+    B->desc_dirty = true;
+    return GPE_OK;
+}
+
+gpe_status gpe_batch_set_state(gpe_batch *B, int32_t id, const double *even, const double *odd) {
+    BInst *b = get_inst(B, id);
+    if (!b || !even || (b->cfg.mode == GPE_REAL_TIME && !odd)) {
+        set_error("gpe_batch_set_state: invalid argument");
+        return GPE_EINVAL;
+    }
+    GPE_HIP_TRY(hipStreamSynchronize(B->stream));
+    const int pe = b->cfg.mode == GPE_REAL_TIME ? 2 * b->parity : b->parity;
+    const bool f64 = B->precision == GPE_FP64;
+    gpe_status st = f64 ? upload_field<double>(b->buf[pe], even, b->L, B->stream)
+                        : upload_field<float>(b->buf[pe], even, b->L, B->stream);
+    if (st == GPE_OK && b->cfg.mode == GPE_REAL_TIME)
+        st = f64 ? upload_field<double>(b->buf[pe + 1], odd, b->L, B->stream)
+                 : upload_field<float>(b->buf[pe + 1], odd, b->L, B->stream);
+    if (b->scal) {
+        const double init[2] = {1.0, 1.0};
+        GPE_HIP_TRY(hipMemcpy(b->scal, init, sizeof(init), hipMemcpyHostToDevice));
+    }
+    return st;
+}
+
+gpe_status gpe_batch_get_state(gpe_batch *B, int32_t id, double *even, double *odd) {
+    BInst *b = get_inst(B, id);
+    if (!b) return GPE_EINVAL;
+    GPE_HIP_TRY(hipStreamSynchronize(B->stream));
+    const int pe = b->cfg.mode == GPE_REAL_TIME ? 2 * b->parity : b->parity;
+    if (b->cfg.mode == GPE_IMAG_TIME && b->steps_done > 0) {
+        // apply the pending renormalisation, as gpe_step does at its end
+        const long long n = b->L.elems;
+        if (B->precision == GPE_FP64)
+            hipLaunchKernelGGL((k_scale_field<double>), dim3(1024), dim3(256), 0, B->stream, (double2 *)b->buf[pe], n,
+                               b->scal + 1);
+        else
+            hipLaunchKernelGGL((k_scale_field<float>), dim3(1024), dim3(256), 0, B->stream, (float2 *)b->buf[pe], n,
+                               b->scal + 1);
+        GPE_HIP_TRY(hipGetLastError());
+        const double one = 1.0;
+        GPE_HIP_TRY(hipMemcpyAsync(b->scal + 1, &one, sizeof(double), hipMemcpyHostToDevice, B->stream));
+        GPE_HIP_TRY(hipStreamSynchronize(B->stream));
+        B->desc_dirty = true;
+    }
+    const bool f64 = B->precision == GPE_FP64;
+    gpe_status st = GPE_OK;
+    if (even)
+        st = f64 ? download_field<double>(b->buf[pe], even, b->L, B->stream)
+                 : download_field<float>(b->buf[pe], even, b->L, B->stream);
+    if (st == GPE_OK && odd && b->cfg.mode == GPE_REAL_TIME)
+        st = f64 ? download_field<double>(b->buf[pe + 1], odd, b->L, B->stream)
+                 : download_field<float>(b->buf[pe + 1], odd, b->L, B->stream);
+    return st;
+}
+
+gpe_status gpe_batch_observables(gpe_batch *B, int32_t id, double out[4]) {
+    BInst *b = get_inst(B, id);
+    if (!b || !out) return GPE_EINVAL;
+    GPE_HIP_TRY(hipStreamSynchronize(B->stream));
+    const long long nb = (long long)((b->L.nx + 63) / 64) * ((b->L.ny + 3) / 4) * b->L.nz;
+    if (nb > B->obs_cap) {
+        if (B->obs_partials) (void)hipFree(B->obs_partials);
+        B->obs_partials = nullptr;
+        GPE_HIP_TRY(hipMalloc(&B->obs_partials, (size_t)nb * 4 * sizeof(double) + 8 * sizeof(double)));
+        B->obs_cap = nb;
+    }
+    const int pe = b->cfg.mode == GPE_REAL_TIME ? 2 * b->parity : b->parity;
+    const dim3 grid((b->L.nx + 63) / 64, (b->L.ny + 3) / 4, b->L.nz), block(64, 4);
+    double *res = B->obs_partials + nb * 4;
+    if (B->precision == GPE_FP64)
+        hipLaunchKernelGGL((k_observables<double, POT_TRAP, 4>), grid, block, 0, B->stream,
+                           (const double2 *)b->buf[pe], nullptr, b->L, b->trap, 1.0 / b->co.dt, b->co.lap, b->co.g,
+                           B->obs_partials);
+    else
+        hipLaunchKernelGGL((k_observables<float, POT_TRAP, 4>), grid, block, 0, B->stream,
+                           (const float2 *)b->buf[pe], nullptr, b->L, b->trap, 1.0 / b->co.dt, b->co.lap, b->co.g,
+                           B->obs_partials);
+    GPE_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_finalize<4>), dim3(1), dim3(256), 0, B->stream, B->obs_partials, nb, b->co.w, 0, res);
+    GPE_HIP_TRY(hipGetLastError());
+    double tmp[4];
+    GPE_HIP_TRY(hipMemcpyAsync(tmp, res, sizeof(tmp), hipMemcpyDeviceToHost, B->stream));
+    GPE_HIP_TRY(hipStreamSynchronize(B->stream));
+    double sc = 1.0;
+    if (b->scal && b->steps_done > 0) {
+        // pending renormalisation of an imaginary-time field: N, K, P scale
+        // with s^2, I with s^4
+        GPE_HIP_TRY(hipMemcpy(&sc, b->scal + 1, sizeof(double), hipMemcpyDeviceToHost));
+    }
+    const double s2 = sc * sc;
+    out[0] = tmp[0] * s2;
+    out[1] = tmp[1] * s2;
+    out[2] = tmp[2] * s2;
+    out[3] = tmp[3] * s2 * s2;
+    return GPE_OK;
+}
+
+gpe_status gpe_batch_instance(const gpe_batch *B, int32_t id, gpe_instance_info *info) {
+    if (!B || !info || id < 0 || id >= (int32_t)B->inst.size()) {
+        set_error("gpe_batch_instance: invalid argument");
+        return GPE_EINVAL;
+    }
+    const BInst &b = B->inst[id];
+    info->steps_done = b.steps_done;
+    info->steps_left = b.steps_left;
+    info->nodes = (uint64_t)b.cfg.nx * b.cfg.ny * b.cfg.nz * GPE_CLASSES;
+    info->device_bytes = b.bytes;
+    info->mode = b.cfg.mode;
+    info->active = b.active ? 1u : 0u;
+    return GPE_OK;
+}
+
+gpe_status gpe_batch_run(gpe_batch *B, uint32_t chunk_steps, uint32_t chunks, double *elapsed_ms) {
+    if (!B || chunk_steps == 0) {
+        set_error("gpe_batch_run: invalid argument");
+        return GPE_EINVAL;
+    }
+    if (B->precision == GPE_FP64) return run_t<double>(B, chunk_steps, chunks, elapsed_ms);
+    return run_t<float>(B, chunk_steps, chunks, elapsed_ms);
+}
+
+uint64_t gpe_batch_chunk_node_steps(const gpe_batch *B, uint32_t chunk_steps) {
+    if (!B) return 0;
+    uint64_t t = 0;
+    for (auto &b : B->inst)
+        if (b.active)
+            t += (uint64_t)b.cfg.nx * b.cfg.ny * b.cfg.nz * GPE_CLASSES *
+                 std::min<uint64_t>(b.steps_left, chunk_steps);
+    return t;
+}
+
+int32_t gpe_batch_active(const gpe_batch *B) {
+    if (!B) return 0;
+    int32_t n = 0;
+    for (auto &b : B->inst) n += b.active ? 1 : 0;
+    return n;
+}
+
+uint64_t gpe_batch_resident_bytes(const gpe_batch *B) { return B ? B->resident : 0; }
+
+}  // extern "C"

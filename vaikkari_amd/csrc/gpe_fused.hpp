@@ -152,22 +152,23 @@ __host__ __device__ constexpr int yslot(int m) {
     return m == 1 ? 0 : m == 5 ? 1 : m == 6 ? 2 : m == 7 ? 3 : m == 8 ? 4 : 5;
 }
 
+// Logical tile index -> (bx, by, bz), with the XCD-contiguous order of
+// block_coords when remap is set.
+__device__ __forceinline__ int remap_block(int lin, int nb, int remap) {
+    if (!remap) return lin;
+    const int q = nb >> 3, r = nb & 7, xcd = lin & 7, slot = lin >> 3;
+    return (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+// One tile of the fused step (the whole block works on tile `w`).
 template <typename T, int FORM, int POT, int TY>
-__global__ void __launch_bounds__(64 * TY)
-    k_fused(cplx_t<T> *__restrict__ eout, cplx_t<T> *__restrict__ oout, const cplx_t<T> *__restrict__ ein,
-            const cplx_t<T> *__restrict__ oin, const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H, Trap trap,
-            double dt, FusedGrid G) {
+__device__ __forceinline__ void fused_tile(cplx_t<T> *__restrict__ eout, cplx_t<T> *__restrict__ oout,
+                                           const cplx_t<T> *__restrict__ ein, const cplx_t<T> *__restrict__ oin,
+                                           const T *__restrict__ pot, const Layout &L, const Hamiltonian<T, FORM> &H,
+                                           const Trap &trap, double dt, const FusedGrid &G, int w,
+                                           cplx_t<T> (&ylds)[2][6][TY][64]) {
 #pragma clang fp contract(off)
     using C = cplx_t<T>;
-    __shared__ C ylds[2][6][TY][64];
-
-    // ---- tile coordinates -------------------------------------------------
-    const int nb = G.ntx * G.nty * G.ntz;
-    int w = blockIdx.x;
-    if (G.remap) {
-        const int q = nb >> 3, r = nb & 7, xcd = w & 7, slot = w >> 3;
-        w = (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
-    }
     const int bx = w % G.ntx, by = (w / G.ntx) % G.nty, bz = w / (G.ntx * G.nty);
     const int xs = bx * G.w, ys = by * G.h, k0 = bz * G.cz;
     const int wx = min(G.w, L.nx - xs), hy = min(G.h, L.ny - ys);
@@ -221,17 +222,19 @@ __global__ void __launch_bounds__(64 * TY)
         C E_yp9_now = zero;
         oe.ym7 = oe.ym8 = oe.yp1 = oe.yp9 = oe.ym_up6 = oe.xm_yp1 = oe.xp_ym5 = zero;
         C nEup2 = zero, nEup6 = zero;
+        // lookahead E(t+1) classes 2 and 6: needed also when t itself is the
+        // zero halo layer below the box (t = -1), for the next step's E(t)
+        if (dom && t + 1 >= 0 && t + 1 < L.nz) {
+            nEup2 = ld(ein, t + 1, 2, 0);
+            nEup6 = ld(ein, t + 1, 6, 0);
+            if (tin) oe.ym_up6 = ld(ein, t + 1, 6, -L.sy);
+        }
         if (dom && tin) {
 #pragma unroll
             for (int l = 0; l < 12; l++) {
                 if (l == 2) Ec[l] = Eup2;
                 else if (l == 6) Ec[l] = Eup6;
                 else Ec[l] = ld(ein, t, l, 0);
-            }
-            if (t + 1 < L.nz) {
-                nEup2 = ld(ein, t + 1, 2, 0);
-                nEup6 = ld(ein, t + 1, 6, 0);
-                oe.ym_up6 = ld(ein, t + 1, 6, -L.sy);
             }
             oe.ym7 = ld(ein, t, 7, -L.sy);
             oe.ym8 = ld(ein, t, 8, -L.sy);
@@ -327,6 +330,16 @@ __global__ void __launch_bounds__(64 * TY)
         E_yp9_prev = E_yp9_now;
         O_yp9_prev = O_yp9_now;
     }
+}
+
+template <typename T, int FORM, int POT, int TY>
+__global__ void __launch_bounds__(64 * TY)
+    k_fused(cplx_t<T> *__restrict__ eout, cplx_t<T> *__restrict__ oout, const cplx_t<T> *__restrict__ ein,
+            const cplx_t<T> *__restrict__ oin, const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H, Trap trap,
+            double dt, FusedGrid G) {
+    __shared__ cplx_t<T> ylds[2][6][TY][64];
+    const int w = remap_block(blockIdx.x, G.ntx * G.nty * G.ntz, G.remap);
+    fused_tile<T, FORM, POT, TY>(eout, oout, ein, oin, pot, L, H, trap, dt, G, w, ylds);
 }
 
 }  // namespace gpe

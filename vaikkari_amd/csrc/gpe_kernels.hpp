@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next
     if (MODE == MODE_IMAG) {
         block_sum<1, TY>(part, lds);
         if (threadIdx.x == 0 && threadIdx.y == 0)
-            partials[blockIdx.x] = part[0];
+            partials[bx + G.gx * (by + G.gy * bz)] = part[0];
     }
 }
 
@@ -242,20 +242,19 @@ __global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next
 // twice; x/y/diagonal neighbours are global loads that hit L1/L2.
 // Logical grid (ceil(nx/64), ceil(ny/TY), ceil(nz/CZ)), block (64, TY).
 // ---------------------------------------------------------------------
-template <typename T, int FORM, int POT, int MODE, int TY, int CZ>
-__global__ void __launch_bounds__(64 * TY) k_zmarch(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
-                                                    const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
-                                                    Trap trap, double dt, const double *__restrict__ scale,
-                                                    double *__restrict__ partials, Grid3 G) {
+// One (bx, by, bz) tile of the z-march for the calling thread; returns the
+// thread's sum of |out|^2 (imaginary time) for the block reduction.
+template <typename T, int FORM, int POT, int MODE, int TY>
+__device__ __forceinline__ double zmarch_tile(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                              const T *__restrict__ pot, const Layout &L,
+                                              const Hamiltonian<T, FORM> &H, const Trap &trap, double dt,
+                                              const double *__restrict__ scale, int bx, int by, int bz, int cz) {
 #pragma clang fp contract(off)
     using C = cplx_t<T>;
-    __shared__ double lds[TY];
-    int bx, by, bz;
-    block_coords(G, bx, by, bz);
     const int i = bx * 64 + threadIdx.x;
     const int j = by * TY + threadIdx.y;
-    const int k0 = bz * CZ;
-    const int k1 = min(k0 + CZ, L.nz);
+    const int k0 = bz * cz;
+    const int k1 = min(k0 + cz, L.nz);
     const bool act = i < L.nx && j < L.ny;
     double part[1] = {0.0};
     if (act) {
@@ -335,10 +334,24 @@ __global__ void __launch_bounds__(64 * TY) k_zmarch(cplx_t<T> *__restrict__ next
             }
         }
     }
+    return part[0];
+}
+
+template <typename T, int FORM, int POT, int MODE, int TY, int CZ>
+__global__ void __launch_bounds__(64 * TY) k_zmarch(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                                    const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
+                                                    Trap trap, double dt, const double *__restrict__ scale,
+                                                    double *__restrict__ partials, Grid3 G) {
+    __shared__ double lds[TY];
+    int bx, by, bz;
+    block_coords(G, bx, by, bz);
+    double part[1];
+    part[0] = zmarch_tile<T, FORM, POT, MODE, TY>(next, prev, pot, L, H, trap, dt, scale, bx, by, bz, CZ);
     if (MODE == MODE_IMAG) {
+        // partials in logical tile order, so the norm's summation order does
+        // not depend on the block-to-XCD order
         block_sum<1, TY>(part, lds);
-        if (threadIdx.x == 0 && threadIdx.y == 0)
-            partials[blockIdx.x] = part[0];
+        if (threadIdx.x == 0 && threadIdx.y == 0) partials[bx + G.gx * (by + G.gy * bz)] = part[0];
     }
 }
 

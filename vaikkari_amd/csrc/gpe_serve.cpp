@@ -1,0 +1,209 @@
+// gpe_serve.cpp -- continuous-batching solve server over the batch engine
+// (include/vaikkari_gpe_serve.h).  Host-side scheduling only; all device
+// work goes through gpe_batch_*.
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <new>
+#include <unordered_map>
+#include <vector>
+
+#include "gpe_internal.hpp"
+#include "../../include/vaikkari_gpe_batch.h"
+#include "../../include/vaikkari_gpe_serve.h"
+
+namespace {
+
+struct Pending {
+    uint64_t ticket;
+    gpe_request req;
+    double arrival;
+};
+
+struct Running {
+    uint64_t ticket;
+    gpe_request req;
+    double arrival, admitted;
+    int32_t id;
+};
+
+struct Finished {
+    int32_t id;  // batch instance kept resident for gpe_server_fetch
+    gpe_request req;
+};
+
+}  // namespace
+
+struct gpe_server {
+    gpe_batch *batch = nullptr;
+    uint32_t chunk = 50;
+    std::chrono::steady_clock::time_point t0;
+    uint64_t next_ticket = 1;
+    std::deque<Pending> queue;
+    std::vector<Running> running;
+    std::vector<gpe_completion> done;  // not yet polled
+    std::unordered_map<uint64_t, Finished> kept;
+    gpe_server_stats st{};
+    ~gpe_server() { gpe_batch_destroy(batch); }
+};
+
+static gpe_instance_config to_instance(const gpe_request &r) {
+    gpe_instance_config c{};
+    c.nx = c.ny = c.nz = r.n;
+    c.mode = r.mode;
+    c.block_scale = r.h / std::sqrt(8.0);
+    c.dt = r.dt;
+    c.g = r.g;
+    c.trap_ratio = r.trap_ratio;
+    c.origin[0] = c.origin[1] = c.origin[2] = NAN;  // centred box
+    c.steps = r.steps;
+    c.init = GPE_INIT_GAUSSIAN;
+    c.zlayers = 0;
+    c.sigma = r.sigma;
+    c.noise = r.noise;
+    c.mu0 = r.mu0;
+    c.seed = r.seed;
+    return c;
+}
+
+extern "C" {
+
+gpe_status gpe_server_create(uint32_t precision, uint64_t budget, uint32_t chunk_steps, gpe_server **out) {
+    if (!out || chunk_steps == 0) {
+        gpe::set_error("gpe_server_create: invalid argument");
+        return GPE_EINVAL;
+    }
+    *out = nullptr;
+    auto *s = new (std::nothrow) gpe_server();
+    if (!s) return GPE_ENOMEM;
+    const gpe_status st = gpe_batch_create(precision, budget, &s->batch);
+    if (st != GPE_OK) {
+        delete s;
+        return st;
+    }
+    s->chunk = chunk_steps;
+    s->t0 = std::chrono::steady_clock::now();
+    *out = s;
+    return GPE_OK;
+}
+
+void gpe_server_destroy(gpe_server *s) { delete s; }
+
+double gpe_server_now(const gpe_server *s) {
+    if (!s) return 0.0;
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
+}
+
+gpe_status gpe_server_submit(gpe_server *s, const gpe_request *r, double arrival, uint64_t *ticket) {
+    if (!s || !r || !ticket || r->n == 0 || r->n > 4096 || r->steps == 0 || !(r->h > 0) || !(r->dt > 0) ||
+        r->mode > GPE_IMAG_TIME) {
+        gpe::set_error("gpe_server_submit: invalid request");
+        return GPE_EINVAL;
+    }
+    const gpe_instance_config c = to_instance(*r);
+    if (gpe_batch_instance_bytes(s->batch, &c) == 0) {
+        gpe::set_error("gpe_server_submit: invalid request");
+        return GPE_EINVAL;
+    }
+    *ticket = s->next_ticket++;
+    s->queue.push_back({*ticket, *r, arrival});
+    s->st.submitted++;
+    return GPE_OK;
+}
+
+gpe_status gpe_server_tick(gpe_server *s, uint32_t *completed) {
+    if (!s) return GPE_EINVAL;
+    // admission, first fit in arrival order
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
+        const gpe_instance_config c = to_instance(it->req);
+        int32_t id = -1;
+        const gpe_status st = gpe_batch_add(s->batch, &c, &id);
+        if (st == GPE_EBUSY) {
+            ++it;
+            continue;
+        }
+        if (st != GPE_OK) return st;
+        s->running.push_back({it->ticket, it->req, it->arrival, gpe_server_now(s), id});
+        s->st.admitted++;
+        it = s->queue.erase(it);
+    }
+    uint32_t ndone = 0;
+    if (!s->running.empty()) {
+        const uint64_t ns = gpe_batch_chunk_node_steps(s->batch, s->chunk);
+        double ms = 0.0;
+        const gpe_status st = gpe_batch_run(s->batch, s->chunk, 1, &ms);
+        if (st != GPE_OK) return st;
+        s->st.node_steps_done += (double)ns;
+        s->st.device_ms += ms;
+        s->st.ticks += 1;
+        const double now = gpe_server_now(s);
+        for (size_t q = 0; q < s->running.size();) {
+            Running &r = s->running[q];
+            gpe_instance_info inf;
+            gpe_status e = gpe_batch_instance(s->batch, r.id, &inf);
+            if (e != GPE_OK) return e;
+            if (inf.steps_left > 0) {
+                q++;
+                continue;
+            }
+            double obs[4] = {0, 0, 0, 0};
+            e = gpe_batch_observables(s->batch, r.id, obs);
+            if (e != GPE_OK) return e;
+            gpe_completion cpl;
+            cpl.ticket = r.ticket;
+            cpl.arrival = r.arrival;
+            cpl.admitted = r.admitted;
+            cpl.finished = now;
+            cpl.node_steps = 12.0 * r.req.n * r.req.n * (double)r.req.n * (double)r.req.steps;
+            cpl.norm = obs[0];
+            cpl.energy = obs[1] + obs[2] + 0.5 * obs[3];
+            s->done.push_back(cpl);
+            if (r.req.keep_result) {
+                s->kept[r.ticket] = {r.id, r.req};
+            } else {
+                e = gpe_batch_remove(s->batch, r.id);
+                if (e != GPE_OK) return e;
+            }
+            s->st.completed++;
+            ndone++;
+            r = s->running.back();
+            s->running.pop_back();
+        }
+    }
+    if (completed) *completed = ndone;
+    return GPE_OK;
+}
+
+gpe_status gpe_server_poll(gpe_server *s, gpe_completion *out, uint32_t max, uint32_t *n) {
+    if (!s || !n || (max && !out)) return GPE_EINVAL;
+    const uint32_t k = (uint32_t)std::min<size_t>(max, s->done.size());
+    for (uint32_t i = 0; i < k; i++) out[i] = s->done[i];
+    s->done.erase(s->done.begin(), s->done.begin() + k);
+    *n = k;
+    return GPE_OK;
+}
+
+gpe_status gpe_server_fetch(gpe_server *s, uint64_t ticket, double *even, double *odd) {
+    if (!s) return GPE_EINVAL;
+    auto it = s->kept.find(ticket);
+    if (it == s->kept.end()) {
+        gpe::set_error("gpe_server_fetch: ticket %llu has no kept result", (unsigned long long)ticket);
+        return GPE_EINVAL;
+    }
+    gpe_status st = gpe_batch_get_state(s->batch, it->second.id, even, odd);
+    if (st != GPE_OK) return st;
+    st = gpe_batch_remove(s->batch, it->second.id);
+    s->kept.erase(it);
+    return st;
+}
+
+gpe_status gpe_server_get_stats(const gpe_server *s, gpe_server_stats *out) {
+    if (!s || !out) return GPE_EINVAL;
+    *out = s->st;
+    out->queued = s->queue.size();
+    out->active = s->running.size();
+    out->resident_bytes = gpe_batch_resident_bytes(s->batch);
+    return GPE_OK;
+}
+
+}  // extern "C"

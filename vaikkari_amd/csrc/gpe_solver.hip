@@ -247,6 +247,14 @@ static gpe_status validate(const gpe_config *c) {
         set_error("gpe_create: block_scale and dt must be positive, g and trap_ratio finite");
         return GPE_EINVAL;
     }
+    if (c->potential == GPE_POT_TRAP) {
+        const uint32_t dims[3] = {c->nx, c->ny, c->nz};
+        const double vmax = trap_vmax(dims, c->origin, c->block_scale, c->trap_ratio);
+        if (!linear_step_stable(c->dt, c->block_scale, vmax)) {
+            set_error("gpe_create: dt = %g violates dt*(6/b^2 + max V) < 2 (max V = %g)", c->dt, vmax);
+            return GPE_EUNSTABLE;
+        }
+    }
     return GPE_OK;
 }
 
@@ -346,22 +354,41 @@ static void unpack_field(const std::vector<T> &d, double *h, const Layout &L) {
                 }
 }
 
+namespace gpe {
+
 template <typename T>
-gpe_status gpe_solver::upload(int which, const double *h) {
+gpe_status upload_field(void *dev, const double *h, const Layout &L, hipStream_t st) {
     std::vector<T> d;
     pack_field<T>(h, d, L);
-    GPE_HIP_TRY(hipMemcpyAsync(field[which], d.data(), d.size() * sizeof(T), hipMemcpyHostToDevice, stream));
-    GPE_HIP_TRY(hipStreamSynchronize(stream));
+    GPE_HIP_TRY(hipMemcpyAsync(dev, d.data(), d.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    GPE_HIP_TRY(hipStreamSynchronize(st));
     return GPE_OK;
 }
 
 template <typename T>
-gpe_status gpe_solver::download(int which, double *h) {
+gpe_status download_field(const void *dev, double *h, const Layout &L, hipStream_t st) {
     std::vector<T> d((size_t)L.elems * 2);
-    GPE_HIP_TRY(hipMemcpyAsync(d.data(), field[which], d.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
-    GPE_HIP_TRY(hipStreamSynchronize(stream));
+    GPE_HIP_TRY(hipMemcpyAsync(d.data(), dev, d.size() * sizeof(T), hipMemcpyDeviceToHost, st));
+    GPE_HIP_TRY(hipStreamSynchronize(st));
     unpack_field<T>(d, h, L);
     return GPE_OK;
+}
+
+template gpe_status upload_field<double>(void *, const double *, const Layout &, hipStream_t);
+template gpe_status upload_field<float>(void *, const double *, const Layout &, hipStream_t);
+template gpe_status download_field<double>(const void *, double *, const Layout &, hipStream_t);
+template gpe_status download_field<float>(const void *, double *, const Layout &, hipStream_t);
+
+}  // namespace gpe
+
+template <typename T>
+gpe_status gpe_solver::upload(int which, const double *h) {
+    return upload_field<T>(field[which], h, L, stream);
+}
+
+template <typename T>
+gpe_status gpe_solver::download(int which, double *h) {
+    return download_field<T>(field[which], h, L, stream);
 }
 
 gpe_status gpe_solver::upload_potential() {
@@ -535,6 +562,11 @@ gpe_status gpe_set_potential(gpe_solver *s, const double *V) {
             set_error("gpe_set_potential: non-finite potential value");
             return GPE_EINVAL;
         }
+    const double vmax = *std::max_element(s->V_host.begin(), s->V_host.end());
+    if (!linear_step_stable(s->cfg.dt, s->cfg.block_scale, std::max(vmax, 0.0))) {
+        set_error("gpe_set_potential: dt*(6/b^2 + max V) >= 2 for this potential");
+        return GPE_EUNSTABLE;
+    }
     s->cfg.potential = GPE_POT_ARRAY;
     return s->upload_potential();
 }
@@ -581,6 +613,14 @@ gpe_status gpe_set_mode(gpe_solver *s, uint32_t mode, double dt) {
     if (!s || mode > GPE_IMAG_TIME || !(dt > 0)) {
         set_error("gpe_set_mode: invalid argument");
         return GPE_EINVAL;
+    }
+    if (s->cfg.potential == GPE_POT_TRAP) {
+        const uint32_t dims[3] = {s->cfg.nx, s->cfg.ny, s->cfg.nz};
+        const double vmax = trap_vmax(dims, s->cfg.origin, s->cfg.block_scale, s->cfg.trap_ratio);
+        if (!linear_step_stable(dt, s->cfg.block_scale, vmax)) {
+            set_error("gpe_set_mode: dt = %g violates dt*(6/b^2 + max V) < 2", dt);
+            return GPE_EUNSTABLE;
+        }
     }
     GPE_HIP_TRY(hipStreamSynchronize(s->stream));
     const bool dt_changed = dt != s->cfg.dt;

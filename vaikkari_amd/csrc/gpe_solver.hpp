@@ -10,6 +10,10 @@
 namespace gpe {
 Layout make_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes);
 Trap make_trap(const double origin[3], double b, double ratio);
+// Host [k][j][i][l] complex double <-> device class-row layout of one field
+// (T = double or float), synchronous on stream st.
+template <typename T> gpe_status upload_field(void *dev, const double *h, const Layout &L, hipStream_t st);
+template <typename T> gpe_status download_field(const void *dev, double *h, const Layout &L, hipStream_t st);
 }  // namespace gpe
 
 struct gpe_solver {

@@ -31,6 +31,21 @@ const int kClassPos[GPE_CLASSES][3] = {{1, 3, 7}, {7, 1, 3}, {7, 3, 1}, {7, 3, 5
 
 double node_measure(double b) { return std::sqrt(32.0 / 9.0) * b * b * b; }
 
+double trap_vmax(const uint32_t dims[3], const double origin[3], double b, double ratio) {
+    double vmax = 0.0;
+    for (int ci = 0; ci < 2; ci++)
+        for (int cj = 0; cj < 2; cj++)
+            for (int ck = 0; ck < 2; ck++)
+                for (int l = 0; l < GPE_CLASSES; l++) {
+                    double p[3];
+                    node_position(origin, b, ci ? dims[0] - 1 : 0, cj ? dims[1] - 1 : 0, ck ? dims[2] - 1 : 0, l, p);
+                    vmax = std::max(vmax, trap_potential(p, ratio));
+                }
+    return vmax;
+}
+
+bool linear_step_stable(double dt, double b, double vmax) { return dt * (6.0 / (b * b) + vmax) < 2.0; }
+
 void node_position(const double origin[3], double b, uint32_t i, uint32_t j, uint32_t k, int l, double p[3]) {
     const double bw = std::sqrt(8.0), sq18 = 1.0 / std::sqrt(8.0);
     const uint32_t ijk[3] = {i, j, k};

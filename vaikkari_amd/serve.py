@@ -1,0 +1,177 @@
+"""Continuous-batching solve server (include/vaikkari_gpe_serve.h) and the
+Poisson request workload it is benchmarked on.
+
+Workload (serving form): requests arrive as a seeded Poisson stream; each
+asks for an n^3-cell solve, n log-uniform in [16, 160], steps uniform in
+[500, 20000], g log-uniform in [100, 5000], trap anisotropy uniform in
+[0.5, 4], 30 % imaginary-time ground states and 70 % real-time evolutions,
+cube edge h = 0.25, Gaussian initial state (sigma 1.5, 1e-3 noise).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, lib
+from .solver import MODES, PRECISIONS, nodes_of
+
+
+def stable_dt(n: int, h: float, g: float, ratio: float, sigma: float = 1.5, dt_nominal: Optional[float] = None,
+              mode: str = "real") -> float:
+    """Time step for an n^3-cell request: the nominal 1e-3*(32/n) (real time), capped by the
+    reference's own rule dt <= 1/(max(V + g|psi0|^2) + 3/b^2) (GPUGrossPitaevskii_BCC.cu:298).
+    Imaginary time uses half the cap."""
+    b = h / math.sqrt(8.0)
+    half = 0.5 * n * h
+    vmax = 0.5 * (2 * half * half + ratio * ratio * half * half)
+    rho = (math.pi * sigma * sigma) ** -1.5
+    cap = 1.0 / (vmax + g * rho + 3.0 / (b * b))
+    if mode == "imag":
+        return 0.5 * cap
+    nominal = 1e-3 * 32.0 / n if dt_nominal is None else dt_nominal
+    return min(nominal, cap)
+
+
+@dataclass
+class Request:
+    arrival: float
+    n: int
+    steps: int
+    g: float
+    ratio: float
+    mode: str
+    seed: int
+    h: float = 0.25
+
+    def node_steps(self) -> float:
+        return 12.0 * self.n ** 3 * self.steps
+
+
+def poisson_workload(seed: int, rate: float, duration: float, n_min: int = 16, n_max: int = 160,
+                     steps_min: int = 500, steps_max: int = 20000, imag_frac: float = 0.3) -> List[Request]:
+    rng = np.random.default_rng(seed)
+    t, out, q = 0.0, [], 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        steps = int(rng.integers(steps_min, steps_max + 1))
+        g = float(math.exp(rng.uniform(math.log(100.0), math.log(5000.0))))
+        ratio = float(rng.uniform(0.5, 4.0))
+        mode = "imag" if rng.uniform() < imag_frac else "real"
+        out.append(Request(t, n, steps, g, ratio, mode, seed * 1000003 + q))
+        q += 1
+    return out
+
+
+class Server:
+    def __init__(self, precision: str = "fp32", hbm_budget_bytes: int = 200 * 10 ** 9, chunk_steps: int = 50):
+        self._h = C.c_void_p()
+        check(lib.gpe_server_create(PRECISIONS[precision], int(hbm_budget_bytes), int(chunk_steps),
+                                    C.byref(self._h)), "gpe_server_create")
+
+    def close(self):
+        if self._h:
+            lib.gpe_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def now(self) -> float:
+        return lib.gpe_server_now(self._h)
+
+    def submit(self, r: Request, arrival: Optional[float] = None, keep_result: bool = False) -> int:
+        q = _lib.GpeRequest()
+        q.n = r.n
+        q.mode = MODES[r.mode]
+        q.steps = r.steps
+        q.h = r.h
+        q.dt = stable_dt(r.n, r.h, r.g, r.ratio, mode=r.mode)
+        q.g = r.g
+        q.trap_ratio = r.ratio
+        q.sigma, q.noise, q.mu0 = 1.5, 1e-3, 0.0
+        q.seed = r.seed
+        q.keep_result = 1 if keep_result else 0
+        t = C.c_uint64()
+        check(lib.gpe_server_submit(self._h, C.byref(q), float(r.arrival if arrival is None else arrival),
+                                    C.byref(t)), "gpe_server_submit")
+        return t.value
+
+    def tick(self) -> int:
+        n = C.c_uint32()
+        check(lib.gpe_server_tick(self._h, C.byref(n)), "gpe_server_tick")
+        return n.value
+
+    def poll(self, max_items: int = 4096):
+        buf = (_lib.GpeCompletion * max_items)()
+        n = C.c_uint32()
+        check(lib.gpe_server_poll(self._h, buf, max_items, C.byref(n)), "gpe_server_poll")
+        return [{"ticket": c.ticket, "arrival": c.arrival, "admitted": c.admitted, "finished": c.finished,
+                 "node_steps": c.node_steps, "norm": c.norm, "energy": c.energy} for c in buf[:n.value]]
+
+    def fetch(self, ticket: int, n: int, mode: str):
+        nodes = nodes_of((n, n, n))
+        e = np.empty(2 * nodes)
+        o = np.empty(2 * nodes) if mode == "real" else None
+        check(lib.gpe_server_fetch(self._h, int(ticket), dptr(e), dptr(o)), "gpe_server_fetch")
+        return e, o
+
+    def stats(self) -> dict:
+        s = _lib.GpeServerStats()
+        check(lib.gpe_server_get_stats(self._h, C.byref(s)), "gpe_server_get_stats")
+        return {k: getattr(s, k) for k, _ in _lib.GpeServerStats._fields_}
+
+
+def run_serving(workload: List[Request], duration: float, precision: str = "fp32",
+                hbm_budget_bytes: int = 200 * 10 ** 9, chunk_steps: int = 50, log=None) -> dict:
+    """Replays the workload against the wall clock for `duration` seconds: requests are
+    submitted when their arrival time has passed, the server ticks continuously."""
+    srv = Server(precision, hbm_budget_bytes, chunk_steps)
+    t_start = srv.now()
+    pending = sorted(workload, key=lambda r: r.arrival)
+    k = 0
+    done = []
+    last_log = t_start
+    while True:
+        now = srv.now() - t_start
+        if now >= duration:
+            break
+        while k < len(pending) and pending[k].arrival <= now:
+            srv.submit(pending[k], arrival=t_start + pending[k].arrival)
+            k += 1
+        srv.tick()
+        done.extend(srv.poll())
+        if log and srv.now() - last_log > 10.0:
+            st = srv.stats()
+            log(f"serve t={now:.0f}s completed={st['completed']} active={st['active']} queued={st['queued']} "
+                f"resident={st['resident_bytes'] / 1e9:.1f} GB")
+            last_log = srv.now()
+    wall = srv.now() - t_start
+    st = srv.stats()
+    lat = np.array([d["finished"] - d["arrival"] for d in done]) if done else np.array([])
+    offered = sum(r.node_steps() for r in workload if r.arrival < duration) / duration
+    res = {
+        "duration_s": wall,
+        "submitted": int(st["submitted"]), "completed": int(st["completed"]), "queued_at_end": int(st["queued"]),
+        "active_at_end": int(st["active"]),
+        "served_site_steps_per_s": st["node_steps_done"] / wall if wall > 0 else 0.0,
+        "device_site_steps_per_s": st["node_steps_done"] / (st["device_ms"] * 1e-3) if st["device_ms"] else 0.0,
+        "offered_site_steps_per_s": offered,
+        "p50_s": float(np.percentile(lat, 50)) if lat.size else None,
+        "p99_s": float(np.percentile(lat, 99)) if lat.size else None,
+        "completed_norm_range": [float(min(d["norm"] for d in done)), float(max(d["norm"] for d in done))]
+        if done else None,
+        "ticks": int(st["ticks"]),
+    }
+    srv.close()
+    return res
