@@ -146,7 +146,9 @@ gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info);
  * depend on the variant (every variant performs the same arithmetic per
  * node).  gpe_variant_tiles gives the number of tile shapes of a family
  * (numbered 1..n; 0 selects the family's default), gpe_variant_shape the
- * shape {waves per block, z layers per block, XCD remap} of tile 1..n.
+ * shape {waves per block, z layers per block, flags} of tile 1..n with
+ * flags bit 0 = XCD-contiguous block order, bit 1 = streaming
+ * (non-temporal) access to the updated field.
  * GPE_KERNEL_FUSED needs two more field buffers (allocated on selection);
  * imaginary-time steps of a fused solver run the z-march kernel. */
 gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile);

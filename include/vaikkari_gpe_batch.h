@@ -4,9 +4,9 @@
  * No counterpart in the reference, which steps exactly one field per
  * process (GPUGrossPitaevskii_BCC.cu:475-481).  A batch keeps every
  * instance's state resident in HBM and advances all of them with one set of
- * persistent kernel launches per step: the fused leapfrog pass for
- * real-time instances, the z-march Euler update plus a per-instance FP64
- * renormalisation for imaginary-time instances.  A chunk of steps is
+ * persistent kernel launches per step: the two in-place leapfrog
+ * half-updates for real-time instances, the z-march Euler update plus a
+ * per-instance FP64 renormalisation for imaginary-time instances.  A chunk of steps is
  * captured once into a hipGraph and replayed.  Each instance's arithmetic
  * (tile decomposition, reduction order) depends only on that instance, so
  * its result is bit-identical whatever else shares the batch.

@@ -108,6 +108,26 @@ def test_fused_every_tile(orc, precision, tile):
     s.close()
 
 
+@pytest.mark.parametrize("variant", [(v["kernel"], v["tile"]) for v in va.variants()
+                                     if not (v["kernel"] == "fused" and v["waves"] == 16)])
+def test_every_variant_fp64_bit_exact(orc, variant):
+    """Every (kernel family, tile shape) the autotuner may pick: FP64 bit-identical to the
+    oracle on a ragged grid with full-amplitude boundaries, real and imaginary time."""
+    kernel, tile = variant
+    dims = (66, 7, 11)
+    announce(f"variant {kernel} {tile}")
+    b, origin, even, odd, V = random_case(dims, 0.45, 1.3, seed=31)
+    s = va.Solver(dims, b, 1e-3, 250.0, origin, "fp64", "real", "trap", 1.3, kernel, tile)
+    s.set_state(even, odd)
+    s.step(6)
+    ge, go = s.get_state()
+    lf, lf0, gdt = va.leapfrog_coefs(b, 1e-3, 250.0)
+    oe, oo = orc.leapfrog(even, odd, V * 1e-3, dims, lf, lf0, gdt, 6)
+    np.testing.assert_array_equal(ge, oe)
+    np.testing.assert_array_equal(go, oo)
+    s.close()
+
+
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_fp32_random_field_all_kernels(orc, kernel):
     """FP32 on a field of full amplitude everywhere (boundary layers, rows and columns of

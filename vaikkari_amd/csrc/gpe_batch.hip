@@ -3,10 +3,10 @@
 //
 // Device side: every active instance has a descriptor (layout, trap,
 // coefficients, tile grid, field pointers, steps left, ping-pong parity).
-// Per step three persistent kernels run over fixed grids: the fused
-// leapfrog pass over all tiles of all real-time instances, the z-march
-// Euler update over all tiles of all imaginary-time instances, and the
-// per-instance FP64 norm reduction of the latter.  Work items (instance,
+// Per step persistent kernels run over fixed grids: the two in-place
+// leapfrog half-updates (gather pass) over all tiles of all real-time
+// instances, the z-march Euler update over all tiles of all imaginary-time
+// instances, and the per-instance FP64 norm reduction of the latter.  Work items (instance,
 // tile) are laid out instance-major; each round of the persistent loop hands
 // each XCD a contiguous range of them.  An "advance" kernel at the end of a
 // chunk updates steps-left and parity on the device, so a captured chunk
@@ -19,7 +19,7 @@
 #include <new>
 #include <vector>
 
-#include "gpe_fused.hpp"
+#include "gpe_fused.hpp"  // remap_block
 #include "gpe_internal.hpp"
 #include "gpe_kernels.hpp"
 #include "gpe_solver.hpp"
@@ -35,7 +35,7 @@ struct InstDesc {
     Trap trap;
     Hamiltonian<T, sizeof(T) == 8 ? FORM_REF : FORM_DIFF> H;
     double dt;
-    FusedGrid fg;  // real time
+    Grid3 gg;      // real time (gather tiles, in-place leapfrog)
     Grid3 zg;      // imaginary time (z-march tiles)
     int zcz;
     cplx_t<T> *buf[4];
@@ -54,6 +54,7 @@ struct BatchHeader {
 
 constexpr int kBatchGrid = 2048;  // persistent blocks per kernel
 constexpr int kImagTY = 4;
+constexpr int kRealTY = 4;
 constexpr int kFinalizeThreads = 256;
 
 template <typename T>
@@ -67,11 +68,13 @@ __device__ __forceinline__ int find_instance(const InstDesc<T> *__restrict__ D, 
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
-template <typename T, int TY>
-__global__ void __launch_bounds__(64 * TY)
-    k_batch_fused(const InstDesc<T> *__restrict__ D, const BatchHeader *__restrict__ hdr, int s, int chunk) {
+// One half of a leapfrog step over all tiles of all real-time instances:
+// half 0 updates odd from even, half 1 even from odd (in place, as
+// GPUGrossPitaevskii_BCC.cu:478,480).
+template <typename T>
+__global__ void __launch_bounds__(64 * kRealTY)
+    k_batch_real(const InstDesc<T> *__restrict__ D, const BatchHeader *__restrict__ hdr, int s, int chunk, int half) {
     constexpr int FORM = sizeof(T) == 8 ? FORM_REF : FORM_DIFF;
-    __shared__ cplx_t<T> ylds[2][6][TY][64];
     const int n = hdr->n_real, total = hdr->total_real;
     for (int base = 0; base < total; base += gridDim.x) {
         const int w = base + remap_block(blockIdx.x, gridDim.x, 1);
@@ -80,9 +83,12 @@ __global__ void __launch_bounds__(64 * TY)
         const InstDesc<T> &d = D[q];
         const long long steps = d.steps_left < chunk ? d.steps_left : chunk;
         if (s >= steps) continue;
-        const int p = (d.parity + s) & 1;
-        fused_tile<T, FORM, POT_TRAP, TY>(d.buf[2 * (1 - p)], d.buf[2 * (1 - p) + 1], d.buf[2 * p], d.buf[2 * p + 1],
-                                          nullptr, d.L, d.H, d.trap, d.dt, d.fg, w - d.tile0, ylds);
+        const int lt = w - d.tile0;
+        const int bx = lt % d.gg.gx, by = (lt / d.gg.gx) % d.gg.gy, bz = lt / (d.gg.gx * d.gg.gy);
+        cplx_t<T> *next = d.buf[half == 0 ? 1 : 0];
+        const cplx_t<T> *prev = d.buf[half == 0 ? 0 : 1];
+        gather_tile<T, FORM, POT_TRAP, MODE_REAL, kRealTY>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, nullptr, bx,
+                                                          by, bz);
     }
 }
 
@@ -214,9 +220,6 @@ namespace {
         }                                                                                         \
     } while (0)
 
-constexpr int kFusedTY64 = 4;   // fused tile rows, FP64 (LDS and VGPR budget)
-constexpr int kFusedTY32 = 8;   // fused tile rows, FP32
-constexpr int kDefaultRealCZ = 16;
 constexpr int kDefaultImagCZ = 8;
 
 struct BInst {
@@ -224,7 +227,7 @@ struct BInst {
     Layout L{};
     Trap trap{};
     Coefs co{};
-    FusedGrid fg{};
+    Grid3 gg{};
     Grid3 zg{};
     int zcz = 8;
     void *buf[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -292,25 +295,19 @@ bool valid_instance(const gpe_instance_config *c) {
            std::isfinite(c->trap_ratio) && c->init <= GPE_INIT_GAUSSIAN;
 }
 
-size_t instance_bytes(const gpe_instance_config *c, size_t elem, Layout *Lout, FusedGrid *fg, Grid3 *zg, int *zcz,
-                      int *ntiles, uint32_t precision) {
+size_t instance_bytes(const gpe_instance_config *c, size_t elem, Layout *Lout, Grid3 *gg, Grid3 *zg, int *zcz,
+                      int *ntiles) {
     const Layout L = make_layout(c->nx, c->ny, c->nz, elem);
-    const int nfields = c->mode == GPE_REAL_TIME ? 4 : 2;
-    size_t bytes = (size_t)nfields * (size_t)L.elems * elem;
+    size_t bytes = 2 * (size_t)L.elems * elem;  // even and odd (real time) or ping-pong pair (imaginary time)
     int tiles = 0;
     if (c->mode == GPE_REAL_TIME) {
-        const int ty = precision == GPE_FP64 ? kFusedTY64 : kFusedTY32;
-        const int cz = c->zlayers ? (int)c->zlayers : kDefaultRealCZ;
-        FusedGrid G;
-        G.ntx = (L.nx + 61) / 62;
-        G.w = (L.nx + G.ntx - 1) / G.ntx;
-        G.nty = (L.ny + ty - 3) / (ty - 2);
-        G.h = (L.ny + G.nty - 1) / G.nty;
-        G.ntz = (L.nz + cz - 1) / cz;
-        G.cz = (L.nz + G.ntz - 1) / G.ntz;
+        Grid3 G;
+        G.gx = (L.nx + 63) / 64;
+        G.gy = (L.ny + kRealTY - 1) / kRealTY;
+        G.gz = L.nz;
         G.remap = 0;
-        tiles = G.ntx * G.nty * G.ntz;
-        if (fg) *fg = G;
+        tiles = G.gx * G.gy * G.gz;
+        if (gg) *gg = G;
     } else {
         const int cz = c->zlayers ? (int)c->zlayers : kDefaultImagCZ;
         Grid3 G;
@@ -337,7 +334,7 @@ void fill_desc(InstDesc<T> &d, const BInst &b) {
     d.H.lapfac0 = (T)b.co.lapfac0_dt;
     d.H.g = (T)b.co.g_dt;
     d.dt = b.co.dt;
-    d.fg = b.fg;
+    d.gg = b.gg;
     d.zg = b.zg;
     d.zcz = b.zcz;
     for (int f = 0; f < 4; f++) d.buf[f] = (cplx_t<T> *)b.buf[f];
@@ -409,11 +406,12 @@ template <typename T>
 void enqueue_chunk(gpe_batch *B, uint32_t chunk, hipStream_t st) {
     const InstDesc<T> *R = (const InstDesc<T> *)B->d_real;
     const InstDesc<T> *I = (const InstDesc<T> *)B->d_imag;
-    constexpr int TY = sizeof(T) == 8 ? kFusedTY64 : kFusedTY32;
     for (uint32_t s = 0; s < chunk; s++) {
-        if (B->n_real > 0)
-            hipLaunchKernelGGL((k_batch_fused<T, TY>), dim3(kBatchGrid), dim3(64, TY), 0, st, R, B->d_hdr, (int)s,
-                               (int)chunk);
+        if (B->n_real > 0) {
+            for (int half = 0; half < 2; half++)
+                hipLaunchKernelGGL((k_batch_real<T>), dim3(kBatchGrid), dim3(64, kRealTY), 0, st, R, B->d_hdr, (int)s,
+                                   (int)chunk, half);
+        }
         if (B->n_imag > 0) {
             hipLaunchKernelGGL((k_batch_imag<T>), dim3(kBatchGrid), dim3(64, kImagTY), 0, st, I, B->d_hdr, (int)s,
                                (int)chunk);
@@ -529,7 +527,7 @@ void gpe_batch_destroy(gpe_batch *b) { delete b; }
 
 uint64_t gpe_batch_instance_bytes(const gpe_batch *B, const gpe_instance_config *c) {
     if (!B || !valid_instance(c)) return 0;
-    return instance_bytes(c, B->elem(), nullptr, nullptr, nullptr, nullptr, nullptr, B->precision);
+    return instance_bytes(c, B->elem(), nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id) {
@@ -539,7 +537,7 @@ gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id
     }
     BInst b;
     b.cfg = *c;
-    b.bytes = instance_bytes(c, B->elem(), &b.L, &b.fg, &b.zg, &b.zcz, &b.ntiles, B->precision);
+    b.bytes = instance_bytes(c, B->elem(), &b.L, &b.gg, &b.zg, &b.zcz, &b.ntiles);
     if (B->budget && B->resident + b.bytes > B->budget) {
         set_error("gpe_batch_add: %zu bytes would exceed the budget (%llu of %llu resident)", b.bytes,
                   (unsigned long long)B->resident, (unsigned long long)B->budget);
@@ -570,7 +568,7 @@ gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id
     b.co.w = node_measure(bs);
     b.co.lap = kHodge / (bs * bs);
     b.co.g = c->g;
-    const int nfields = c->mode == GPE_REAL_TIME ? 4 : 2;
+    const int nfields = 2;
     const size_t fbytes = (size_t)b.L.elems * B->elem();
     auto fail = [&](gpe_status st) {
         for (auto *&p : b.buf)
@@ -652,7 +650,7 @@ gpe_status gpe_batch_set_state(gpe_batch *B, int32_t id, const double *even, con
         return GPE_EINVAL;
     }
     GPE_HIP_TRY(hipStreamSynchronize(B->stream));
-    const int pe = b->cfg.mode == GPE_REAL_TIME ? 2 * b->parity : b->parity;
+    const int pe = b->cfg.mode == GPE_REAL_TIME ? 0 : b->parity;
     const bool f64 = B->precision == GPE_FP64;
     gpe_status st = f64 ? upload_field<double>(b->buf[pe], even, b->L, B->stream)
                         : upload_field<float>(b->buf[pe], even, b->L, B->stream);
@@ -670,7 +668,7 @@ gpe_status gpe_batch_get_state(gpe_batch *B, int32_t id, double *even, double *o
     BInst *b = get_inst(B, id);
     if (!b) return GPE_EINVAL;
     GPE_HIP_TRY(hipStreamSynchronize(B->stream));
-    const int pe = b->cfg.mode == GPE_REAL_TIME ? 2 * b->parity : b->parity;
+    const int pe = b->cfg.mode == GPE_REAL_TIME ? 0 : b->parity;
     if (b->cfg.mode == GPE_IMAG_TIME && b->steps_done > 0) {
         // apply the pending renormalisation, as gpe_step does at its end
         const long long n = b->L.elems;
@@ -708,7 +706,7 @@ gpe_status gpe_batch_observables(gpe_batch *B, int32_t id, double out[4]) {
         GPE_HIP_TRY(hipMalloc(&B->obs_partials, (size_t)nb * 4 * sizeof(double) + 8 * sizeof(double)));
         B->obs_cap = nb;
     }
-    const int pe = b->cfg.mode == GPE_REAL_TIME ? 2 * b->parity : b->parity;
+    const int pe = b->cfg.mode == GPE_REAL_TIME ? 0 : b->parity;
     const dim3 grid((b->L.nx + 63) / 64, (b->L.ny + 3) / 4, b->L.nz), block(64, 4);
     double *res = B->obs_partials + nb * 4;
     if (B->precision == GPE_FP64)

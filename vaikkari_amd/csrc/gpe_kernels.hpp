@@ -170,16 +170,17 @@ __device__ __forceinline__ void block_coords(const Grid3 &G, int &bx, int &by, i
 // Imaginary time: out = psi - s, psi = scale*prev, block partial of
 // sum|out|^2 to partials[block].
 // ---------------------------------------------------------------------
-template <typename T, int FORM, int POT, int MODE, int TY>
-__global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
-                                                    const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
-                                                    Trap trap, double dt, const double *__restrict__ scale,
-                                                    double *__restrict__ partials, Grid3 G) {
+// Cell (bx*64 + threadIdx.x, by*TY + threadIdx.y, bz) of the gather pass for
+// the calling thread; returns its sum of |out|^2 (imaginary time).  NT
+// marks the read-modify-write operand as streaming (read once, written
+// once) so it does not displace neighbour rows from the caches.
+template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
+__device__ __forceinline__ double gather_tile(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                              const T *__restrict__ pot, const Layout &L,
+                                              const Hamiltonian<T, FORM> &H, const Trap &trap, double dt,
+                                              const double *__restrict__ scale, int bx, int by, int bz) {
 #pragma clang fp contract(off)
     using C = cplx_t<T>;
-    __shared__ double lds[TY];
-    int bx, by, bz;
-    block_coords(G, bx, by, bz);
     const int i = bx * 64 + threadIdx.x;
     const int j = by * TY + threadIdx.y;
     const int k = bz;
@@ -214,10 +215,21 @@ __global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next
             const C sres = H.apply(n, own[l], pv);
             const long long at = base + l * L.xp;
             if (MODE == MODE_REAL) {
-                C nx = next[at];
+                C nx;
+                if (NT) {
+                    nx.x = __builtin_nontemporal_load(&next[at].x);
+                    nx.y = __builtin_nontemporal_load(&next[at].y);
+                } else {
+                    nx = next[at];
+                }
                 nx.x = nx.x + sres.y;
                 nx.y = nx.y + (-sres.x);
-                next[at] = nx;
+                if (NT) {
+                    __builtin_nontemporal_store(nx.x, &next[at].x);
+                    __builtin_nontemporal_store(nx.y, &next[at].y);
+                } else {
+                    next[at] = nx;
+                }
             } else {
                 C o;
                 o.x = own[l].x - sres.x;
@@ -227,10 +239,23 @@ __global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next
             }
         }
     }
+    return part[0];
+}
+
+template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
+__global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                                    const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
+                                                    Trap trap, double dt, const double *__restrict__ scale,
+                                                    double *__restrict__ partials, Grid3 G) {
+    __shared__ double lds[TY];
+    int bx, by, bz;
+    block_coords(G, bx, by, bz);
+    double part[1];
+    part[0] = gather_tile<T, FORM, POT, MODE, TY, NT>(next, prev, pot, L, H, trap, dt, scale, bx, by, bz);
     if (MODE == MODE_IMAG) {
+        // partials in logical tile order (independent of the block-to-XCD order)
         block_sum<1, TY>(part, lds);
-        if (threadIdx.x == 0 && threadIdx.y == 0)
-            partials[bx + G.gx * (by + G.gy * bz)] = part[0];
+        if (threadIdx.x == 0 && threadIdx.y == 0) partials[bx + G.gx * (by + G.gy * bz)] = part[0];
     }
 }
 

@@ -68,14 +68,15 @@ constexpr int kTY = 4;  // waves per block of the observables kernel
 // them).  ty = waves (rows of 64 cells) per block, cz = layers per z-march
 // block, remap = XCD-contiguous block order.
 struct TileShape {
-    int ty, cz, remap;
+    int ty, cz, remap, nt;  // nt: streaming (non-temporal) read-modify-write of the updated field
 };
-constexpr TileShape kGatherTiles[] = {{4, 1, 1}, {4, 1, 0}, {1, 1, 1}, {2, 1, 1}, {8, 1, 1}};
-constexpr TileShape kZmarchTiles[] = {{4, 8, 1}, {4, 8, 0}, {1, 8, 1}, {2, 8, 1}, {8, 8, 1},
-                                      {4, 4, 1}, {4, 16, 1}, {2, 16, 1}, {1, 16, 1}, {2, 4, 1}};
+constexpr TileShape kGatherTiles[] = {{4, 1, 0, 0}, {4, 1, 1, 0}, {1, 1, 1, 0}, {2, 1, 1, 0}, {8, 1, 1, 0},
+                                      {4, 1, 0, 1}, {4, 1, 1, 1}, {2, 1, 0, 1}, {8, 1, 0, 0}, {2, 1, 0, 0}};
+constexpr TileShape kZmarchTiles[] = {{4, 8, 1, 0}, {4, 8, 0, 0}, {1, 8, 1, 0}, {2, 8, 1, 0}, {8, 8, 1, 0},
+                                      {4, 4, 1, 0}, {4, 16, 1, 0}, {2, 16, 1, 0}, {1, 16, 1, 0}, {2, 4, 1, 0}};
 // fused: ty = waves per block (footprint rows, interior ty-2), cz = layers per block
-constexpr TileShape kFusedTiles[] = {{8, 16, 1}, {4, 16, 1}, {4, 32, 1}, {8, 32, 1}, {8, 8, 1},
-                                     {16, 16, 1}, {4, 8, 1}, {8, 16, 0}, {6, 16, 1}, {6, 32, 1}};
+constexpr TileShape kFusedTiles[] = {{8, 16, 1, 0}, {4, 16, 1, 0}, {4, 32, 1, 0}, {8, 32, 1, 0}, {8, 8, 1, 0},
+                                     {16, 16, 1, 0}, {4, 8, 1, 0}, {8, 16, 0, 0}, {6, 16, 1, 0}, {6, 32, 1, 0}};
 constexpr int kNumGatherTiles = sizeof(kGatherTiles) / sizeof(kGatherTiles[0]);
 constexpr int kNumZmarchTiles = sizeof(kZmarchTiles) / sizeof(kZmarchTiles[0]);
 constexpr int kNumFusedTiles = sizeof(kFusedTiles) / sizeof(kFusedTiles[0]);
@@ -146,12 +147,12 @@ Grid3 grid_of(int kernel, int tile, const Layout &L) {
     return G;
 }
 
-template <typename T, int FORM, int POT, int MODE, int TY>
+template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
 void launch_gather(cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const Layout &L,
                    const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, const double *scale, double *partials,
                    const Grid3 &G, hipStream_t st) {
-    hipLaunchKernelGGL((k_gather<T, FORM, POT, MODE, TY>), dim3(G.gx * G.gy * G.gz), dim3(64, TY), 0, st, next, prev,
-                       pot, L, H, trap, dt, scale, partials, G);
+    hipLaunchKernelGGL((k_gather<T, FORM, POT, MODE, TY, NT>), dim3(G.gx * G.gy * G.gz), dim3(64, TY), 0, st, next,
+                       prev, pot, L, H, trap, dt, scale, partials, G);
 }
 
 template <typename T, int FORM, int POT, int MODE, int TY, int CZ>
@@ -183,11 +184,13 @@ hipError_t launch_update(int kernel, int tile, cplx_t<T> *next, const cplx_t<T> 
             default: return hipErrorInvalidValue;
         }
     } else {
-        switch (t.ty) {
-            case 1: launch_gather<T, FORM, POT, MODE, 1>(GPE_ARGS); break;
-            case 2: launch_gather<T, FORM, POT, MODE, 2>(GPE_ARGS); break;
-            case 4: launch_gather<T, FORM, POT, MODE, 4>(GPE_ARGS); break;
-            case 8: launch_gather<T, FORM, POT, MODE, 8>(GPE_ARGS); break;
+        switch (t.ty * 10 + (MODE == MODE_REAL ? t.nt : 0)) {
+            case 10: launch_gather<T, FORM, POT, MODE, 1>(GPE_ARGS); break;
+            case 20: launch_gather<T, FORM, POT, MODE, 2>(GPE_ARGS); break;
+            case 40: launch_gather<T, FORM, POT, MODE, 4>(GPE_ARGS); break;
+            case 80: launch_gather<T, FORM, POT, MODE, 8>(GPE_ARGS); break;
+            case 21: launch_gather<T, FORM, POT, MODE, 2, true>(GPE_ARGS); break;
+            case 41: launch_gather<T, FORM, POT, MODE, 4, true>(GPE_ARGS); break;
             default: return hipErrorInvalidValue;
         }
     }
@@ -262,7 +265,10 @@ gpe_status gpe_solver::init(const gpe_config &c) {
     cfg = c;
     L = make_layout(c.nx, c.ny, c.nz, elem_bytes());
     trap = make_trap(c.origin, c.block_scale, c.trap_ratio);
-    kernel = c.kernel == GPE_KERNEL_AUTO ? GPE_KERNEL_ZMARCH : c.kernel;
+    // AUTO: the one-thread-per-cell gather pass, the fastest family measured on
+    // MI355X for both precisions (the z-march and fused families stay
+    // selectable).
+    kernel = c.kernel == GPE_KERNEL_AUTO ? GPE_KERNEL_GATHER : c.kernel;
     if ((int)c.tile > num_tiles(kernel)) {
         set_error("gpe_create: tile %u out of range for this kernel family", c.tile);
         return GPE_EINVAL;
@@ -724,7 +730,7 @@ void gpe_variant_shape(uint32_t kernel, uint32_t tile, int32_t out[3]) {
         const TileShape t = tile_shape((int)kernel, (int)tile - 1);
         out[0] = t.ty;
         out[1] = t.cz;
-        out[2] = t.remap;
+        out[2] = t.remap | (t.nt << 1);
     }
 }
 
