@@ -87,8 +87,8 @@ __global__ void __launch_bounds__(64 * kRealTY)
         const int bx = lt % d.gg.gx, by = (lt / d.gg.gx) % d.gg.gy, bz = lt / (d.gg.gx * d.gg.gy);
         cplx_t<T> *next = d.buf[half == 0 ? 1 : 0];
         const cplx_t<T> *prev = d.buf[half == 0 ? 0 : 1];
-        gather_tile<T, FORM, POT_TRAP, MODE_REAL, kRealTY>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, nullptr, bx,
-                                                          by, bz);
+        gather_tile<T, FORM, POT_TRAP, MODE_REAL, kRealTY, true>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, nullptr,
+                                                                bx, by, bz);
     }
 }
 

@@ -70,8 +70,8 @@ constexpr int kTY = 4;  // waves per block of the observables kernel
 struct TileShape {
     int ty, cz, remap, nt;  // nt: streaming (non-temporal) read-modify-write of the updated field
 };
-constexpr TileShape kGatherTiles[] = {{4, 1, 0, 0}, {4, 1, 1, 0}, {1, 1, 1, 0}, {2, 1, 1, 0}, {8, 1, 1, 0},
-                                      {4, 1, 0, 1}, {4, 1, 1, 1}, {2, 1, 0, 1}, {8, 1, 0, 0}, {2, 1, 0, 0}};
+constexpr TileShape kGatherTiles[] = {{4, 1, 0, 1}, {4, 1, 0, 0}, {4, 1, 1, 0}, {1, 1, 1, 0}, {2, 1, 1, 0},
+                                      {8, 1, 1, 0}, {4, 1, 1, 1}, {2, 1, 0, 1}, {8, 1, 0, 1}, {1, 1, 0, 1}};
 constexpr TileShape kZmarchTiles[] = {{4, 8, 1, 0}, {4, 8, 0, 0}, {1, 8, 1, 0}, {2, 8, 1, 0}, {8, 8, 1, 0},
                                       {4, 4, 1, 0}, {4, 16, 1, 0}, {2, 16, 1, 0}, {1, 16, 1, 0}, {2, 4, 1, 0}};
 // fused: ty = waves per block (footprint rows, interior ty-2), cz = layers per block
@@ -189,8 +189,10 @@ hipError_t launch_update(int kernel, int tile, cplx_t<T> *next, const cplx_t<T> 
             case 20: launch_gather<T, FORM, POT, MODE, 2>(GPE_ARGS); break;
             case 40: launch_gather<T, FORM, POT, MODE, 4>(GPE_ARGS); break;
             case 80: launch_gather<T, FORM, POT, MODE, 8>(GPE_ARGS); break;
+            case 11: launch_gather<T, FORM, POT, MODE, 1, true>(GPE_ARGS); break;
             case 21: launch_gather<T, FORM, POT, MODE, 2, true>(GPE_ARGS); break;
             case 41: launch_gather<T, FORM, POT, MODE, 4, true>(GPE_ARGS); break;
+            case 81: launch_gather<T, FORM, POT, MODE, 8, true>(GPE_ARGS); break;
             default: return hipErrorInvalidValue;
         }
     }
