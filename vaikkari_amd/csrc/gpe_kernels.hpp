@@ -234,7 +234,9 @@ __device__ __forceinline__ double gather_tile(cplx_t<T> *__restrict__ next, cons
                 C o;
                 o.x = own[l].x - sres.x;
                 o.y = own[l].y - sres.y;
-                next[at] = o;
+                // written once, read only by the next step: streaming store
+                __builtin_nontemporal_store(o.x, &next[at].x);
+                __builtin_nontemporal_store(o.y, &next[at].y);
                 part[0] += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
             }
         }
@@ -339,7 +341,9 @@ __device__ __forceinline__ double zmarch_tile(cplx_t<T> *__restrict__ next, cons
                     C o;
                     o.x = cur[l].x - sres.x;
                     o.y = cur[l].y - sres.y;
-                    next[at] = o;
+                    // written once, read only by the next step: streaming store
+                    __builtin_nontemporal_store(o.x, &next[at].x);
+                    __builtin_nontemporal_store(o.y, &next[at].y);
                     part[0] += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
                 }
             }
