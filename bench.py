@@ -44,6 +44,15 @@ def parse():
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--batch-instances", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="length of the Poisson serving sample (0 = skip)")
+    ap.add_argument("--suite", default="main", choices=["main", "precision"],
+                    help="main: the JSON line of record; precision: 256^3 FP32-vs-FP64 error study")
+    ap.add_argument("--precision-cells", type=int, default=256)
+    ap.add_argument("--precision-steps", type=int, default=10000)
+    ap.add_argument("--precision-every", type=int, default=1000)
     return ap.parse_args()
 
 
@@ -80,8 +89,129 @@ def cpu_baseline(dims, b, g, dt, ratio, seconds):
                       f"{n2} steps timed ({t3 - t2:.1f} s) minus a 1-step run"}
 
 
+def batched_workload(n_inst, steps, seed=4, log=log):
+    """64 independent condensates: n^3 cells with n uniform in {24,32,48,64,96}, cube edge 0.25,
+    g log-uniform in [100, 5000], trap anisotropy uniform in [0.5, 4], dt = 1e-3*32/n capped by
+    the reference's stability rule, FP32 bulk; Gaussian + noise initial states (device recipe)."""
+    import vaikkari_amd as va
+    from vaikkari_amd.autotune import tune_batch_tile
+    from vaikkari_amd.batch import Batch, instance_config
+    from vaikkari_amd.serve import stable_dt
+
+    rng = np.random.default_rng(seed)
+    cfgs = []
+    for q in range(n_inst):
+        n = int(rng.choice([24, 32, 48, 64, 96]))
+        g = float(np.exp(rng.uniform(np.log(100.0), np.log(5000.0))))
+        ratio = float(rng.uniform(0.5, 4.0))
+        dt = stable_dt(n, 0.25, g, ratio)
+        cfgs.append(instance_config((n, n, n), 0.25, g, dt, steps, "real", ratio, sigma=1.5, noise=1e-3, mu0=0.0,
+                                    seed=seed * 100003 + q))
+
+    def make(waves, cfg_list=cfgs):
+        b = Batch("fp32")
+        b.set_tile(waves)
+        for c in cfg_list:
+            b.add(c)
+        return b
+
+    tuned = tune_batch_tile(lambda w: make(w, cfgs[: max(8, n_inst // 4)]), chunk_steps=10, chunks=2)
+    log(f"batched: tile autotuned to {tuned['waves']} waves ({tuned['ms']})")
+    B = make(tuned["waves"])
+    ids = list(B.dims.keys())
+    n0 = {i: B.observables(i)["N"] for i in ids}
+    chunk = 100
+    B.run(chunk, 1)  # graph capture + first chunk, untimed
+    rest = steps - chunk
+    nchunks, tail = divmod(rest, chunk)
+    node_steps = sum(B.info(i)["nodes"] for i in ids) * rest
+    ms_total = B.run(chunk, nchunks) if nchunks else 0.0
+    if tail:
+        ms_total += B.run(tail, 1)
+    assert all(B.info(i)["steps_left"] == 0 for i in ids)
+    drift = max(abs(B.observables(i)["N"] / n0[i] - 1.0) for i in ids)
+    nodes = sum(B.info(i)["nodes"] for i in ids)
+    B.close()
+    rate = node_steps / (ms_total * 1e-3)
+    gbs = rate * 48.0 / 1e9
+    return {"instances": n_inst, "steps": steps, "nodes_total": nodes, "site_steps_per_s": rate,
+            "timed_ms": ms_total, "timed_steps": rest, "algorithmic_GBs": gbs,
+            "frac_of_peak": gbs / HBM_PEAK_GBS, "tile_waves": tuned["waves"], "max_rel_norm_drift": drift,
+            "dtype": "fp32 bulk", "sizes": sorted(set(int(c.nx) for c in cfgs)), "h": 0.25}
+
+
+def serving_workload(seconds, log=log):
+    from vaikkari_amd.serve import poisson_workload, run_serving
+
+    w = poisson_workload(seed=5, rate=20.0, duration=seconds)
+    res = run_serving(w, duration=seconds, precision="fp32", hbm_budget_bytes=200 * 10 ** 9, chunk_steps=50,
+                      log=log)
+    res["rate_per_s"] = 20.0
+    res["hbm_budget_GB"] = 200
+    res["policy"] = "first-fit admission by resident bytes, FIFO order, 50-step chunks"
+    return res
+
+
+def precision_study(args):
+    """256^3 cells (201,326,592 nodes), h = 0.06, g = 5000, isotropic trap: Thomas-Fermi profile
+    with a seeded vortex ring, dt = 5e-5; FP32 bulk vs the FP64 path (bit-identical to the CPU
+    oracle) every `every` steps: psi relative L2, norm drift and energy drift."""
+    import vaikkari_amd as va
+
+    N = args.precision_cells
+    dims = (N, N, N)
+    h, g, dt = 0.06, 5000.0, 5e-5
+    b = h / math.sqrt(8)
+    origin = va.centered_origin(dims, b)
+    rng = np.random.default_rng(3)
+    psi, mu_tf = va.thomas_fermi(dims, origin, b, g, 1.0)
+    R = float(rng.uniform(2.0, 4.0))
+    ax = rng.normal(size=3)
+    ax /= np.linalg.norm(ax)
+    psi = va.vortex_ring(dims, origin, b, (0.0, 0.0, 0.0), ax, R, psi)
+    log(f"precision: {dims} cells, ring R={R:.3f} axis={ax.round(3).tolist()}, mu_TF={mu_tf:.3f}")
+    s64 = va.Solver(dims, b, dt, g, origin, "fp64", "real", "trap", 1.0)
+    s32 = va.Solver(dims, b, dt, g, origin, "fp32", "real", "trap", 1.0)
+    for s in (s64, s32):
+        s.set_state(psi)
+        s.init_odd(mu_tf)
+    del psi
+    o64, o32 = s64.observables(0), s32.observables(0)
+    rows = []
+    done = 0
+    ms64 = ms32 = 0.0
+    while done < args.precision_steps:
+        k = min(args.precision_every, args.precision_steps - done)
+        ms64 += s64.time_steps(k)[0]
+        ms32 += s32.time_steps(k)[0]
+        done += k
+        e64 = s64.get_state(odd=False)
+        e32 = s32.get_state(odd=False)
+        err = float(np.linalg.norm(e32 - e64) / np.linalg.norm(e64))
+        del e32
+        a64, a32 = s64.observables(0), s32.observables(0)
+        rows.append({"step": done, "psi_rel_l2": err,
+                     "norm_drift_fp64": a64["N"] / o64["N"] - 1, "norm_drift_fp32": a32["N"] / o32["N"] - 1,
+                     "energy_drift_fp64": a64["E"] / o64["E"] - 1, "energy_drift_fp32": a32["E"] / o32["E"] - 1,
+                     "norm_fp32_minus_fp64": a32["N"] - a64["N"], "energy_fp32_minus_fp64": a32["E"] - a64["E"]})
+        log(f"precision step {done}: rel-L2 {err:.3e}, N drift fp64 {rows[-1]['norm_drift_fp64']:.2e} "
+            f"fp32 {rows[-1]['norm_drift_fp32']:.2e}")
+    nodes = va.nodes_of(dims)
+    out = {"suite": "precision", "cells": list(dims), "nodes": nodes, "h": h, "g": g, "dt": dt,
+           "steps": args.precision_steps, "ring_radius": R, "ring_axis": ax.tolist(),
+           "fp64_site_steps_per_s": nodes * done / (ms64 * 1e-3),
+           "fp32_site_steps_per_s": nodes * done / (ms32 * 1e-3),
+           "checkpoints": rows, "max_psi_rel_l2": max(r["psi_rel_l2"] for r in rows)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
+    if args.suite == "precision":
+        if args.gpus != 1:
+            sys.exit(2)
+        precision_study(args)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the stepper shards by z-slabs across GPUs by design, not built here)", file=sys.stderr)
@@ -153,6 +283,17 @@ def main():
                 / np.linalg.norm(results["fp64"]["state"]))
     s64.close()
 
+    batched = None
+    if args.batch_instances > 0:
+        batched = batched_workload(args.batch_instances, args.batch_steps)
+        log(f"batched: {batched['site_steps_per_s'] / 1e9:.1f} G site-steps/s over {batched['instances']} instances "
+            f"({batched['frac_of_peak'] * 100:.1f}% of peak), norm drift {batched['max_rel_norm_drift']:.1e}")
+    serving = None
+    if args.serve_seconds > 0:
+        serving = serving_workload(args.serve_seconds)
+        log(f"serving: {serving['completed']} done, p50 {serving['p50_s']} s, p99 {serving['p99_s']} s, "
+            f"served {serving['served_site_steps_per_s'] / 1e9:.1f} G site-steps/s")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(dims, b, g, dt, ratio, args.cpu_seconds)
@@ -188,6 +329,8 @@ def main():
                         "E": obs["E"], "mu": obs["mu"], "N": obs["N"]},
         "bytes_per_node_step": {"fp64": r64["bytes_per_node_step"], "fp32": r32["bytes_per_node_step"]},
         "hbm_copy_GBs": copy_gbs,
+        "batched": batched,
+        "serving": serving,
         "avg_launch_us": {"fp64": r64["avg_launch_us"], "fp32": r32["avg_launch_us"]},
     }
     print(json.dumps(line), flush=True)

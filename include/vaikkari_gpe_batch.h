@@ -80,6 +80,9 @@ gpe_status gpe_batch_run(gpe_batch *b, uint32_t chunk_steps, uint32_t chunks, do
 /* Sum over active instances of nodes * min(chunk_steps, steps_left), i.e.
  * the node-steps the next chunk will perform. */
 uint64_t gpe_batch_chunk_node_steps(const gpe_batch *b, uint32_t chunk_steps);
+/* Waves per block (2, 4 or 8) of the real-time pass; only while the batch
+ * is empty.  The autotuner picks it. */
+gpe_status gpe_batch_set_tile(gpe_batch *b, uint32_t waves);
 /* Number of active instances; bytes resident. */
 int32_t gpe_batch_active(const gpe_batch *b);
 uint64_t gpe_batch_resident_bytes(const gpe_batch *b);

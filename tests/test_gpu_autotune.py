@@ -1,0 +1,62 @@
+"""GPU tests of the autotuner: it picks a variant that runs, persists it, reuses it, and the
+tuned solver still matches the oracle bit for bit (FP64)."""
+import math
+
+import numpy as np
+import pytest
+
+import vaikkari_amd as va
+from vaikkari_amd import autotune
+from vaikkari_amd.batch import Batch, instance_config
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tune_persist_and_parity(tmp_path):
+    path = str(tmp_path / "tune.json")
+    dims, h, g, dt = (40, 24, 30), 0.3, 400.0, 8e-4
+    b = h / math.sqrt(8)
+    origin = va.centered_origin(dims, b)
+    even = va.gaussian(dims, origin, b, 1.2, 1e-3, 9)
+    odd = va.phase_shift(dims, even, 6.0, dt)
+    s = va.Solver(dims, b, dt, g, origin, "fp64", "real", "trap", 1.0)
+    s.set_state(even, odd)
+    r1 = autotune.tune(s, steps=4, path=path)
+    assert not r1["cached"] and r1["candidates"] >= 10
+    r2 = autotune.tune(s, steps=4, path=path)
+    assert r2["cached"] and (r2["kernel"], r2["tile"]) == (r1["kernel"], r1["tile"])
+    assert s.info()["kernel"] == r1["kernel"]
+    s.set_state(even, odd)
+    s.step(10)
+    ge, go = s.get_state()
+    V = va.trap_potential(dims, origin, b, 1.0)
+    lf, lf0, gdt = va.leapfrog_coefs(b, dt, g)
+    oe, oo = Oracle().leapfrog(even, odd, V * dt, dims, lf, lf0, gdt, 10)
+    np.testing.assert_array_equal(ge, oe)
+    np.testing.assert_array_equal(go, oo)
+    s.close()
+
+
+def test_batch_tile_choice_does_not_change_results():
+    cfg = instance_config((33, 21, 27), 0.3, 700.0, 6e-4, 25, "real", 1.6, seed=4, mu0=3.0)
+    outs = []
+    for w in (2, 4, 8):
+        B = Batch("fp32")
+        B.set_tile(w)
+        i = B.add(cfg)
+        B.run(25, 1)
+        outs.append(B.get_state(i))
+        B.close()
+    for e, o in outs[1:]:
+        np.testing.assert_array_equal(e, outs[0][0])
+        np.testing.assert_array_equal(o, outs[0][1])
+    res = autotune.tune_batch_tile(lambda w: _mk(w, cfg), chunk_steps=5, chunks=1)
+    assert res["waves"] in (2, 4, 8)
+
+
+def _mk(w, cfg):
+    B = Batch("fp32")
+    B.set_tile(w)
+    B.add(cfg)
+    return B

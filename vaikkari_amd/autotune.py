@@ -1,0 +1,105 @@
+"""Tile-shape autotuner.
+
+Times every kernel variant the library offers (gpe_variant_tiles /
+gpe_set_variant) on the caller's own problem -- same grid, precision, mode
+and state -- and keeps the fastest.  Results persist in a JSON file keyed by
+(cells, precision, mode, potential), so a later solve of the same shape
+skips the search.  Every variant performs identical arithmetic per node, so
+the choice changes speed only; the FP64 parity tests check each variant.
+
+The cache file is $VAIKKARI_AUTOTUNE_CACHE, default
+~/.cache/vaikkari_amd/autotune.json.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from typing import Optional
+
+from .solver import Solver, variants
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "vaikkari_amd", "autotune.json")
+
+
+def cache_path() -> str:
+    return os.environ.get("VAIKKARI_AUTOTUNE_CACHE", DEFAULT_CACHE)
+
+
+def _key(s: Solver) -> str:
+    prec = "fp64" if s.cfg.precision == 0 else "fp32"
+    mode = "real" if s.cfg.mode == 0 else "imag"
+    pot = "array" if s.cfg.potential == 0 else "trap"
+    return f"{s.dims[0]}x{s.dims[1]}x{s.dims[2]}/{prec}/{mode}/{pot}"
+
+
+def load_cache(path: Optional[str] = None) -> dict:
+    path = path or cache_path()
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def save_cache(data: dict, path: Optional[str] = None) -> None:
+    path = path or cache_path()
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def tune(s: Solver, steps: int = 20, families=("gather", "zmarch", "fused"), use_cache: bool = True,
+         path: Optional[str] = None) -> dict:
+    """Selects the fastest variant for solver s and leaves it set.
+
+    The solver's state is advanced by the trial steps; callers tune on a
+    scratch solver or reset the state afterwards (set_state)."""
+    key = _key(s)
+    cache = load_cache(path) if use_cache else {}
+    if key in cache:
+        best = cache[key]
+        s.set_variant(best["kernel"], best["tile"])
+        return dict(best, cached=True)
+    results = []
+    real = s.cfg.mode == 0
+    for v in variants():
+        if v["kernel"] not in families:
+            continue
+        if v["kernel"] == "fused" and not real:
+            continue  # imaginary time runs the z-march whatever the family
+        try:
+            s.set_variant(v["kernel"], v["tile"])
+            s.step(2)
+            ms, _ = s.time_steps(steps)
+        except Exception:
+            continue  # e.g. a shape the precision cannot hold in LDS
+        results.append({"kernel": v["kernel"], "tile": v["tile"], "ms_per_step": ms / steps})
+    if not results:
+        raise RuntimeError("autotune: no variant ran")
+    best = min(results, key=lambda r: r["ms_per_step"])
+    s.set_variant(best["kernel"], best["tile"])
+    out = {"kernel": best["kernel"], "tile": best["tile"], "ms_per_step": best["ms_per_step"],
+           "site_steps_per_s": s.nodes / (best["ms_per_step"] * 1e-3), "tuned_at": time.time(),
+           "candidates": len(results)}
+    if use_cache:
+        cache = load_cache(path)
+        cache[key] = out
+        save_cache(cache, path)
+    return dict(out, cached=False)
+
+
+def tune_batch_tile(make_batch, chunk_steps: int = 10, chunks: int = 2, candidates=(2, 4, 8)) -> dict:
+    """Picks the real-time tile (waves per block) of a batch: make_batch(waves) must return a
+    populated Batch; each candidate is timed over chunks x chunk_steps steps."""
+    timings = {}
+    for w in candidates:
+        b = make_batch(w)
+        b.run(chunk_steps, 1)  # capture + warm-up
+        ms = b.run(chunk_steps, chunks)
+        timings[w] = ms
+        b.close()
+    best = min(timings, key=timings.get)
+    return {"waves": best, "ms": timings}

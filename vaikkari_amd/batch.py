@@ -56,6 +56,9 @@ class Batch:
         except Exception:
             pass
 
+    def set_tile(self, waves: int) -> None:
+        check(lib.gpe_batch_set_tile(self._h, int(waves)), "gpe_batch_set_tile")
+
     def instance_bytes(self, cfg: _lib.GpeInstanceConfig) -> int:
         return int(lib.gpe_batch_instance_bytes(self._h, C.byref(cfg)))
 

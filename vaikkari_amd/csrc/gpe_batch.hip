@@ -54,7 +54,7 @@ struct BatchHeader {
 
 constexpr int kBatchGrid = 2048;  // persistent blocks per kernel
 constexpr int kImagTY = 4;
-constexpr int kRealTY = 4;
+constexpr int kRealTY = 4;  // default waves per block of the real-time pass
 constexpr int kFinalizeThreads = 256;
 
 template <typename T>
@@ -71,8 +71,8 @@ __device__ __forceinline__ int find_instance(const InstDesc<T> *__restrict__ D, 
 // One half of a leapfrog step over all tiles of all real-time instances:
 // half 0 updates odd from even, half 1 even from odd (in place, as
 // GPUGrossPitaevskii_BCC.cu:478,480).
-template <typename T>
-__global__ void __launch_bounds__(64 * kRealTY)
+template <typename T, int TY>
+__global__ void __launch_bounds__(64 * TY)
     k_batch_real(const InstDesc<T> *__restrict__ D, const BatchHeader *__restrict__ hdr, int s, int chunk, int half) {
     constexpr int FORM = sizeof(T) == 8 ? FORM_REF : FORM_DIFF;
     const int n = hdr->n_real, total = hdr->total_real;
@@ -87,8 +87,8 @@ __global__ void __launch_bounds__(64 * kRealTY)
         const int bx = lt % d.gg.gx, by = (lt / d.gg.gx) % d.gg.gy, bz = lt / (d.gg.gx * d.gg.gy);
         cplx_t<T> *next = d.buf[half == 0 ? 1 : 0];
         const cplx_t<T> *prev = d.buf[half == 0 ? 0 : 1];
-        gather_tile<T, FORM, POT_TRAP, MODE_REAL, kRealTY, true>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, nullptr,
-                                                                bx, by, bz);
+        gather_tile<T, FORM, POT_TRAP, MODE_REAL, TY, true>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, nullptr, bx,
+                                                           by, bz);
     }
 }
 
@@ -245,6 +245,7 @@ struct BInst {
 struct gpe_batch {
     uint32_t precision = GPE_FP32_BULK;
     uint64_t budget = 0;
+    int real_ty = kRealTY;  // waves per block of the real-time pass (fixed while instances exist)
     std::vector<BInst> inst;
     uint64_t resident = 0;
     hipStream_t stream = nullptr;
@@ -295,15 +296,15 @@ bool valid_instance(const gpe_instance_config *c) {
            std::isfinite(c->trap_ratio) && c->init <= GPE_INIT_GAUSSIAN;
 }
 
-size_t instance_bytes(const gpe_instance_config *c, size_t elem, Layout *Lout, Grid3 *gg, Grid3 *zg, int *zcz,
-                      int *ntiles) {
+size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, Layout *Lout, Grid3 *gg, Grid3 *zg,
+                      int *zcz, int *ntiles) {
     const Layout L = make_layout(c->nx, c->ny, c->nz, elem);
     size_t bytes = 2 * (size_t)L.elems * elem;  // even and odd (real time) or ping-pong pair (imaginary time)
     int tiles = 0;
     if (c->mode == GPE_REAL_TIME) {
         Grid3 G;
         G.gx = (L.nx + 63) / 64;
-        G.gy = (L.ny + kRealTY - 1) / kRealTY;
+        G.gy = (L.ny + real_ty - 1) / real_ty;
         G.gz = L.nz;
         G.remap = 0;
         tiles = G.gx * G.gy * G.gz;
@@ -408,9 +409,21 @@ void enqueue_chunk(gpe_batch *B, uint32_t chunk, hipStream_t st) {
     const InstDesc<T> *I = (const InstDesc<T> *)B->d_imag;
     for (uint32_t s = 0; s < chunk; s++) {
         if (B->n_real > 0) {
-            for (int half = 0; half < 2; half++)
-                hipLaunchKernelGGL((k_batch_real<T>), dim3(kBatchGrid), dim3(64, kRealTY), 0, st, R, B->d_hdr, (int)s,
-                                   (int)chunk, half);
+            for (int half = 0; half < 2; half++) {
+                switch (B->real_ty) {
+                    case 2:
+                        hipLaunchKernelGGL((k_batch_real<T, 2>), dim3(kBatchGrid), dim3(64, 2), 0, st, R, B->d_hdr,
+                                           (int)s, (int)chunk, half);
+                        break;
+                    case 8:
+                        hipLaunchKernelGGL((k_batch_real<T, 8>), dim3(kBatchGrid), dim3(64, 8), 0, st, R, B->d_hdr,
+                                           (int)s, (int)chunk, half);
+                        break;
+                    default:
+                        hipLaunchKernelGGL((k_batch_real<T, 4>), dim3(kBatchGrid), dim3(64, 4), 0, st, R, B->d_hdr,
+                                           (int)s, (int)chunk, half);
+                }
+            }
         }
         if (B->n_imag > 0) {
             hipLaunchKernelGGL((k_batch_imag<T>), dim3(kBatchGrid), dim3(64, kImagTY), 0, st, I, B->d_hdr, (int)s,
@@ -527,7 +540,7 @@ void gpe_batch_destroy(gpe_batch *b) { delete b; }
 
 uint64_t gpe_batch_instance_bytes(const gpe_batch *B, const gpe_instance_config *c) {
     if (!B || !valid_instance(c)) return 0;
-    return instance_bytes(c, B->elem(), nullptr, nullptr, nullptr, nullptr, nullptr);
+    return instance_bytes(c, B->elem(), B->real_ty, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id) {
@@ -537,7 +550,7 @@ gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id
     }
     BInst b;
     b.cfg = *c;
-    b.bytes = instance_bytes(c, B->elem(), &b.L, &b.gg, &b.zg, &b.zcz, &b.ntiles);
+    b.bytes = instance_bytes(c, B->elem(), B->real_ty, &b.L, &b.gg, &b.zg, &b.zcz, &b.ntiles);
     if (B->budget && B->resident + b.bytes > B->budget) {
         set_error("gpe_batch_add: %zu bytes would exceed the budget (%llu of %llu resident)", b.bytes,
                   (unsigned long long)B->resident, (unsigned long long)B->budget);
@@ -779,5 +792,20 @@ int32_t gpe_batch_active(const gpe_batch *B) {
 }
 
 uint64_t gpe_batch_resident_bytes(const gpe_batch *B) { return B ? B->resident : 0; }
+
+gpe_status gpe_batch_set_tile(gpe_batch *B, uint32_t waves) {
+    if (!B || (waves != 2 && waves != 4 && waves != 8)) {
+        set_error("gpe_batch_set_tile: waves must be 2, 4 or 8");
+        return GPE_EINVAL;
+    }
+    for (auto &b : B->inst)
+        if (b.active) {
+            set_error("gpe_batch_set_tile: the batch holds instances");
+            return GPE_EBUSY;
+        }
+    B->real_ty = (int)waves;
+    B->graph_dirty = true;
+    return GPE_OK;
+}
 
 }  // extern "C"
