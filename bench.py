@@ -56,6 +56,21 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(nodes, precision, info):
+    """HBM bytes per launch of the timed kernel on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic_128cubed.md), when the
+    workload and kernel variant are the ones profiled; else None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_128cubed.json")
+    if nodes != 25165824 or info["kernel"] != "gather" or info["tile"] != 1 or not os.path.exists(path):
+        return None
+    key = "k_gather<double,0,1,0,4,true>" if precision == "fp64" else "k_gather<float,1,1,0,4,true>"
+    try:
+        k = json.load(open(path))["kernels"][key]
+    except (OSError, ValueError, KeyError):
+        return None
+    return k["read_bytes"] + k["write_bytes"]
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -272,7 +287,8 @@ def main():
             "ms_per_step": ms / args.steps, "avg_launch_us": avg_launch_s * 1e6,
             "bytes_per_node_step": info["bytes_per_node_step"],
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(nodes, prec, info),
+                         "algorithmic_bytes_per_launch": upd_bytes},
             "state": s.get_state(odd=False),
         }
         log(f"{prec}: {rate / 1e9:.2f} G site-steps/s, {ms / args.steps * 1e3:.1f} us/step, "

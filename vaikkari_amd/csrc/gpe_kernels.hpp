@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next
 // ---------------------------------------------------------------------
 // One (bx, by, bz) tile of the z-march for the calling thread; returns the
 // thread's sum of |out|^2 (imaginary time) for the block reduction.
-template <typename T, int FORM, int POT, int MODE, int TY>
+template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
 __device__ __forceinline__ double zmarch_tile(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                               const T *__restrict__ pot, const Layout &L,
                                               const Hamiltonian<T, FORM> &H, const Trap &trap, double dt,
@@ -333,10 +333,21 @@ __device__ __forceinline__ double zmarch_tile(cplx_t<T> *__restrict__ next, cons
                 const C sres = H.apply(n, cur[l], pv);
                 const long long at = base + l * L.xp;
                 if (MODE == MODE_REAL) {
-                    C nx = next[at];
+                    C nx;
+                    if (NT) {
+                        nx.x = __builtin_nontemporal_load(&next[at].x);
+                        nx.y = __builtin_nontemporal_load(&next[at].y);
+                    } else {
+                        nx = next[at];
+                    }
                     nx.x = nx.x + sres.y;
                     nx.y = nx.y + (-sres.x);
-                    next[at] = nx;
+                    if (NT) {
+                        __builtin_nontemporal_store(nx.x, &next[at].x);
+                        __builtin_nontemporal_store(nx.y, &next[at].y);
+                    } else {
+                        next[at] = nx;
+                    }
                 } else {
                     C o;
                     o.x = cur[l].x - sres.x;
@@ -366,7 +377,7 @@ __device__ __forceinline__ double zmarch_tile(cplx_t<T> *__restrict__ next, cons
     return part[0];
 }
 
-template <typename T, int FORM, int POT, int MODE, int TY, int CZ>
+template <typename T, int FORM, int POT, int MODE, int TY, int CZ, bool NT = false>
 __global__ void __launch_bounds__(64 * TY) k_zmarch(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                                     const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
                                                     Trap trap, double dt, const double *__restrict__ scale,
@@ -375,7 +386,7 @@ __global__ void __launch_bounds__(64 * TY) k_zmarch(cplx_t<T> *__restrict__ next
     int bx, by, bz;
     block_coords(G, bx, by, bz);
     double part[1];
-    part[0] = zmarch_tile<T, FORM, POT, MODE, TY>(next, prev, pot, L, H, trap, dt, scale, bx, by, bz, CZ);
+    part[0] = zmarch_tile<T, FORM, POT, MODE, TY, NT>(next, prev, pot, L, H, trap, dt, scale, bx, by, bz, CZ);
     if (MODE == MODE_IMAG) {
         // partials in logical tile order, so the norm's summation order does
         // not depend on the block-to-XCD order

@@ -72,8 +72,8 @@ struct TileShape {
 };
 constexpr TileShape kGatherTiles[] = {{4, 1, 0, 1}, {4, 1, 0, 0}, {4, 1, 1, 0}, {1, 1, 1, 0}, {2, 1, 1, 0},
                                       {8, 1, 1, 0}, {4, 1, 1, 1}, {2, 1, 0, 1}, {8, 1, 0, 1}, {1, 1, 0, 1}};
-constexpr TileShape kZmarchTiles[] = {{4, 8, 1, 0}, {4, 8, 0, 0}, {1, 8, 1, 0}, {2, 8, 1, 0}, {8, 8, 1, 0},
-                                      {4, 4, 1, 0}, {4, 16, 1, 0}, {2, 16, 1, 0}, {1, 16, 1, 0}, {2, 4, 1, 0}};
+constexpr TileShape kZmarchTiles[] = {{4, 8, 1, 0}, {4, 8, 0, 1}, {1, 8, 1, 0}, {2, 8, 1, 1}, {8, 8, 1, 1},
+                                      {4, 4, 1, 1}, {4, 16, 1, 1}, {2, 16, 1, 0}, {1, 16, 1, 1}, {2, 4, 1, 1}};
 // fused: ty = waves per block (footprint rows, interior ty-2), cz = layers per block
 constexpr TileShape kFusedTiles[] = {{8, 16, 1, 0}, {4, 16, 1, 0}, {4, 32, 1, 0}, {8, 32, 1, 0}, {8, 8, 1, 0},
                                      {16, 16, 1, 0}, {4, 8, 1, 0}, {8, 16, 0, 0}, {6, 16, 1, 0}, {6, 32, 1, 0}};
@@ -155,12 +155,12 @@ void launch_gather(cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const L
                        prev, pot, L, H, trap, dt, scale, partials, G);
 }
 
-template <typename T, int FORM, int POT, int MODE, int TY, int CZ>
+template <typename T, int FORM, int POT, int MODE, int TY, int CZ, bool NT = false>
 void launch_zmarch(cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const Layout &L,
                    const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, const double *scale, double *partials,
                    const Grid3 &G, hipStream_t st) {
-    hipLaunchKernelGGL((k_zmarch<T, FORM, POT, MODE, TY, CZ>), dim3(G.gx * G.gy * G.gz), dim3(64, TY), 0, st, next,
-                       prev, pot, L, H, trap, dt, scale, partials, G);
+    hipLaunchKernelGGL((k_zmarch<T, FORM, POT, MODE, TY, CZ, NT>), dim3(G.gx * G.gy * G.gz), dim3(64, TY), 0, st,
+                       next, prev, pot, L, H, trap, dt, scale, partials, G);
 }
 
 template <typename T, int FORM, int POT, int MODE>
@@ -171,16 +171,24 @@ hipError_t launch_update(int kernel, int tile, cplx_t<T> *next, const cplx_t<T> 
     const TileShape t = tile_shape(kernel, tile);
 #define GPE_ARGS next, prev, pot, L, H, trap, dt, scale, partials, G, st
     if (kernel == GPE_KERNEL_ZMARCH) {
-        switch (t.ty * 100 + t.cz) {
-            case 108: launch_zmarch<T, FORM, POT, MODE, 1, 8>(GPE_ARGS); break;
-            case 116: launch_zmarch<T, FORM, POT, MODE, 1, 16>(GPE_ARGS); break;
-            case 204: launch_zmarch<T, FORM, POT, MODE, 2, 4>(GPE_ARGS); break;
-            case 208: launch_zmarch<T, FORM, POT, MODE, 2, 8>(GPE_ARGS); break;
-            case 216: launch_zmarch<T, FORM, POT, MODE, 2, 16>(GPE_ARGS); break;
-            case 404: launch_zmarch<T, FORM, POT, MODE, 4, 4>(GPE_ARGS); break;
-            case 408: launch_zmarch<T, FORM, POT, MODE, 4, 8>(GPE_ARGS); break;
-            case 416: launch_zmarch<T, FORM, POT, MODE, 4, 16>(GPE_ARGS); break;
-            case 808: launch_zmarch<T, FORM, POT, MODE, 8, 8>(GPE_ARGS); break;
+        const bool nt = MODE == MODE_REAL && t.nt;
+        switch (t.ty * 1000 + t.cz * 10 + (nt ? 1 : 0)) {
+            case 1080: launch_zmarch<T, FORM, POT, MODE, 1, 8>(GPE_ARGS); break;
+            case 1161: launch_zmarch<T, FORM, POT, MODE, 1, 16, true>(GPE_ARGS); break;
+            case 1160: launch_zmarch<T, FORM, POT, MODE, 1, 16>(GPE_ARGS); break;
+            case 2041: launch_zmarch<T, FORM, POT, MODE, 2, 4, true>(GPE_ARGS); break;
+            case 2040: launch_zmarch<T, FORM, POT, MODE, 2, 4>(GPE_ARGS); break;
+            case 2081: launch_zmarch<T, FORM, POT, MODE, 2, 8, true>(GPE_ARGS); break;
+            case 2080: launch_zmarch<T, FORM, POT, MODE, 2, 8>(GPE_ARGS); break;
+            case 2160: launch_zmarch<T, FORM, POT, MODE, 2, 16>(GPE_ARGS); break;
+            case 4041: launch_zmarch<T, FORM, POT, MODE, 4, 4, true>(GPE_ARGS); break;
+            case 4040: launch_zmarch<T, FORM, POT, MODE, 4, 4>(GPE_ARGS); break;
+            case 4080: launch_zmarch<T, FORM, POT, MODE, 4, 8>(GPE_ARGS); break;
+            case 4081: launch_zmarch<T, FORM, POT, MODE, 4, 8, true>(GPE_ARGS); break;
+            case 4161: launch_zmarch<T, FORM, POT, MODE, 4, 16, true>(GPE_ARGS); break;
+            case 4160: launch_zmarch<T, FORM, POT, MODE, 4, 16>(GPE_ARGS); break;
+            case 8081: launch_zmarch<T, FORM, POT, MODE, 8, 8, true>(GPE_ARGS); break;
+            case 8080: launch_zmarch<T, FORM, POT, MODE, 8, 8>(GPE_ARGS); break;
             default: return hipErrorInvalidValue;
         }
     } else {

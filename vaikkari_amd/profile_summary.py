@@ -18,16 +18,16 @@ from collections import defaultdict
 
 
 def short(name: str) -> str:
-    n = name.split("(")[0]
-    return n.replace("void ", "").replace("gpe::", "").replace("(anonymous namespace)::", "")
+    n = name.replace("(anonymous namespace)::", "").replace("void ", "").replace("gpe::", "")
+    return n.split("(")[0]
 
 
 def stats(db: str):
     con = sqlite3.connect(db)
     rows = con.execute("select name, total_calls, total_duration, average, percentage from top_kernels").fetchall()
-    print(f"{'kernel':70s} {'calls':>6s} {'total_us':>12s} {'avg_us':>10s} {'%':>6s}")
+    print(f"{'kernel':60s} {'calls':>6s} {'total_us':>12s} {'avg_us':>10s} {'%':>6s}")
     for name, calls, tot, avg, pct in rows:
-        print(f"{short(name):70s} {calls:6d} {tot / 1e3:12.1f} {avg / 1e3:10.2f} {pct:6.2f}")
+        print(f"{short(name):60s} {calls:6d} {tot:12.1f} {avg:10.2f} {pct:6.2f}")
     return rows
 
 
@@ -42,38 +42,45 @@ def _counter(db: str, counter: str):
 
 
 def traffic(db_fetch: str, db_write: str):
+    """Per-kernel HBM bytes per dispatch, each counter scaled by its reading on the copy kernel."""
     f = _counter(db_fetch, "FETCH_SIZE")
     w = _counter(db_write, "WRITE_SIZE")
-    # calibration from the copy kernel: bytes = 16 * work-items, read once and written once
-    cf = cw = None
-    for (name, grid), vals in f.items():
-        if "k_copy_stream" in name:
-            vals = sorted(vals)[1:] or vals  # first dispatch is a warm-up
-            known = 16.0 * 0  # filled below
-            cf = (name, grid, sum(vals) / len(vals))
-    for (name, grid), vals in w.items():
-        if "k_copy_stream" in name:
-            vals = sorted(vals)[1:] or vals
-            cw = (name, grid, sum(vals) / len(vals))
-    if cf is None or cw is None:
+    cal = {}
+    for name, per in (("fetch", f), ("write", w)):
+        for (kname, grid), vals in per.items():
+            if "k_copy_stream" in kname:
+                cal[name] = (grid, sum(vals) / len(vals))
+    if "fetch" not in cal or "write" not in cal:
         raise SystemExit("no k_copy_stream dispatch in the counter databases: cannot calibrate")
-    out = {}
-    print("copy-kernel calibration: FETCH_SIZE %.0f KiB, WRITE_SIZE %.0f KiB per dispatch" % (cf[2], cw[2]))
-    return f, w, cf, cw, out
+    return f, w, cal
+
+
+def traffic_table(db_fetch: str, db_write: str, copy_bytes: float = float(1 << 30)):
+    """copy_bytes: what k_copy_stream reads and writes per dispatch (bench.py copies 1 GiB)."""
+    f, w, cal = traffic(db_fetch, db_write)
+    kf = copy_bytes / (cal["fetch"][1] * 1024.0)
+    kw = copy_bytes / (cal["write"][1] * 1024.0)
+    rows = []
+    for (name, grid), vals in f.items():
+        wv = w.get((name, grid))
+        if not wv:
+            continue
+        fb = sum(vals) / len(vals) * 1024.0 * kf
+        wb = sum(wv) / len(wv) * 1024.0 * kw
+        rows.append((short(name), grid, fb, wb, len(vals)))
+    rows.sort(key=lambda r: -(r[2] + r[3]))
+    return kf, kw, rows
 
 
 def main(argv):
     if len(argv) >= 2 and argv[0] == "stats":
         stats(argv[1])
     elif len(argv) >= 3 and argv[0] == "traffic":
-        f, w, cf, cw, _ = traffic(argv[1], argv[2])
-        copy_bytes = None
-        for (name, grid), vals in f.items():
-            pass
-        print(f"{'kernel':70s} {'grid':>9s} {'fetch_KiB':>12s} {'write_KiB':>12s}")
-        for (name, grid), vals in sorted(f.items(), key=lambda kv: -max(kv[1])):
-            wv = w.get((name, grid), [float("nan")])
-            print(f"{short(name):70s} {grid:9d} {sum(vals) / len(vals):12.0f} {sum(wv) / len(wv):12.0f}")
+        kf, kw, rows = traffic_table(argv[1], argv[2])
+        print(f"calibration on the 1 GiB copy kernel: FETCH_SIZE x {kf:.4f}, WRITE_SIZE x {kw:.4f}")
+        print(f"{'kernel':60s} {'grid':>9s} {'read_MB':>10s} {'write_MB':>10s} {'n':>4s}")
+        for name, grid, fb, wb, n in rows:
+            print(f"{name:60s} {grid:9d} {fb / 1e6:10.1f} {wb / 1e6:10.1f} {n:4d}")
     else:
         print(__doc__)
         return 2
