@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(64 * TY)
         cplx_t<T> *next = d.buf[half == 0 ? 1 : 0];
         const cplx_t<T> *prev = d.buf[half == 0 ? 0 : 1];
         gather_tile<T, FORM, POT_TRAP, MODE_REAL, TY, true>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, nullptr, bx,
-                                                           by, bz);
+                                                           by, bz, d.gg.lsh);
     }
 }
 
@@ -110,7 +110,8 @@ __global__ void __launch_bounds__(64 * kImagTY)
         const int bx = lt % d.zg.gx, by = (lt / d.zg.gx) % d.zg.gy, bz = lt / (d.zg.gx * d.zg.gy);
         double part[1];
         part[0] = zmarch_tile<T, FORM, POT_TRAP, MODE_IMAG, kImagTY>(d.buf[1 - p], d.buf[p], nullptr, d.L, d.H,
-                                                                     d.trap, d.dt, d.scal + 1, bx, by, bz, d.zcz);
+                                                                     d.trap, d.dt, d.scal + 1, bx, by, bz, d.zcz,
+                                                                     d.zg.lsh);
         block_sum<1, kImagTY>(part, lds);
         if (threadIdx.x == 0 && threadIdx.y == 0) d.partials[lt] = part[0];
         __syncthreads();  // lds is reused by the next work item
@@ -303,8 +304,9 @@ size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, La
     int tiles = 0;
     if (c->mode == GPE_REAL_TIME) {
         Grid3 G;
-        G.gx = (L.nx + 63) / 64;
-        G.gy = (L.ny + real_ty - 1) / real_ty;
+        G.lsh = row_lanes_log2(L.nx);
+        G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
+        G.gy = (L.ny + real_ty * (64 >> G.lsh) - 1) / (real_ty * (64 >> G.lsh));
         G.gz = L.nz;
         G.remap = 0;
         tiles = G.gx * G.gy * G.gz;
@@ -312,8 +314,9 @@ size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, La
     } else {
         const int cz = c->zlayers ? (int)c->zlayers : kDefaultImagCZ;
         Grid3 G;
-        G.gx = (L.nx + 63) / 64;
-        G.gy = (L.ny + kImagTY - 1) / kImagTY;
+        G.lsh = row_lanes_log2(L.nx);
+        G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
+        G.gy = (L.ny + kImagTY * (64 >> G.lsh) - 1) / (kImagTY * (64 >> G.lsh));
         G.gz = (L.nz + cz - 1) / cz;
         G.remap = 0;
         tiles = G.gx * G.gy * G.gz;

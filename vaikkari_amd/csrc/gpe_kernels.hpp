@@ -146,7 +146,22 @@ __device__ __forceinline__ cplx_t<T> cscale(cplx_t<T> a, T s) {
 // block count.
 struct Grid3 {
     int gx, gy, gz, remap;
+    int lsh;  // log2 of the lanes one x-row of cells takes (6: a wave per row; 5, 4: 2 or 4 rows per wave)
 };
+
+// Lanes per x-row for an nx-cell grid: the width in {64, 32, 16} with the
+// least padding (ties to the wider row), so narrow grids keep lanes busy.
+__host__ __device__ inline int row_lanes_log2(int nx) {
+    int best = 6, waste = ((nx + 63) / 64) * 64 - nx;
+    for (int s = 5; s >= 4; s--) {
+        const int w = 1 << s, pad = ((nx + w - 1) / w) * w - nx;
+        if (pad < waste) {
+            waste = pad;
+            best = s;
+        }
+    }
+    return best;
+}
 __device__ __forceinline__ void block_coords(const Grid3 &G, int &bx, int &by, int &bz) {
     const int nb = G.gx * G.gy * G.gz;
     const int lin = blockIdx.x;
@@ -178,11 +193,13 @@ template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
 __device__ __forceinline__ double gather_tile(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                               const T *__restrict__ pot, const Layout &L,
                                               const Hamiltonian<T, FORM> &H, const Trap &trap, double dt,
-                                              const double *__restrict__ scale, int bx, int by, int bz) {
+                                              const double *__restrict__ scale, int bx, int by, int bz,
+                                              int lsh = 6) {
 #pragma clang fp contract(off)
     using C = cplx_t<T>;
-    const int i = bx * 64 + threadIdx.x;
-    const int j = by * TY + threadIdx.y;
+    // a wave covers 64 >> lsh rows of (1 << lsh) cells
+    const int i = (bx << lsh) + (threadIdx.x & ((1 << lsh) - 1));
+    const int j = ((by * TY + threadIdx.y) << (6 - lsh)) + (threadIdx.x >> lsh);
     const int k = bz;
     const bool act = i < L.nx && j < L.ny;
     double part[1] = {0.0};
@@ -253,7 +270,7 @@ __global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next
     int bx, by, bz;
     block_coords(G, bx, by, bz);
     double part[1];
-    part[0] = gather_tile<T, FORM, POT, MODE, TY, NT>(next, prev, pot, L, H, trap, dt, scale, bx, by, bz);
+    part[0] = gather_tile<T, FORM, POT, MODE, TY, NT>(next, prev, pot, L, H, trap, dt, scale, bx, by, bz, G.lsh);
     if (MODE == MODE_IMAG) {
         // partials in logical tile order (independent of the block-to-XCD order)
         block_sum<1, TY>(part, lds);
@@ -275,11 +292,12 @@ template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
 __device__ __forceinline__ double zmarch_tile(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                               const T *__restrict__ pot, const Layout &L,
                                               const Hamiltonian<T, FORM> &H, const Trap &trap, double dt,
-                                              const double *__restrict__ scale, int bx, int by, int bz, int cz) {
+                                              const double *__restrict__ scale, int bx, int by, int bz, int cz,
+                                              int lsh = 6) {
 #pragma clang fp contract(off)
     using C = cplx_t<T>;
-    const int i = bx * 64 + threadIdx.x;
-    const int j = by * TY + threadIdx.y;
+    const int i = (bx << lsh) + (threadIdx.x & ((1 << lsh) - 1));
+    const int j = ((by * TY + threadIdx.y) << (6 - lsh)) + (threadIdx.x >> lsh);
     const int k0 = bz * cz;
     const int k1 = min(k0 + cz, L.nz);
     const bool act = i < L.nx && j < L.ny;
@@ -386,7 +404,7 @@ __global__ void __launch_bounds__(64 * TY) k_zmarch(cplx_t<T> *__restrict__ next
     int bx, by, bz;
     block_coords(G, bx, by, bz);
     double part[1];
-    part[0] = zmarch_tile<T, FORM, POT, MODE, TY, NT>(next, prev, pot, L, H, trap, dt, scale, bx, by, bz, CZ);
+    part[0] = zmarch_tile<T, FORM, POT, MODE, TY, NT>(next, prev, pot, L, H, trap, dt, scale, bx, by, bz, CZ, G.lsh);
     if (MODE == MODE_IMAG) {
         // partials in logical tile order, so the norm's summation order does
         // not depend on the block-to-XCD order

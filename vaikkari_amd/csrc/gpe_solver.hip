@@ -140,8 +140,10 @@ hipError_t launch_fused(int tile, cplx_t<T> *eo, cplx_t<T> *oo, const cplx_t<T> 
 Grid3 grid_of(int kernel, int tile, const Layout &L) {
     const TileShape t = tile_shape(kernel, tile);
     Grid3 G;
-    G.gx = (L.nx + 63) / 64;
-    G.gy = (L.ny + t.ty - 1) / t.ty;
+    G.lsh = row_lanes_log2(L.nx);
+    const int lw = 1 << G.lsh, rows = t.ty * (64 >> G.lsh);
+    G.gx = (L.nx + lw - 1) / lw;
+    G.gy = (L.ny + rows - 1) / rows;
     G.gz = kernel == GPE_KERNEL_ZMARCH ? (L.nz + t.cz - 1) / t.cz : L.nz;
     G.remap = t.remap;
     return G;
