@@ -167,6 +167,25 @@ def serving_workload(seconds, log=log):
     return res
 
 
+def output_timing(s, dims, reps=5):
+    """Per-iteration output of the reference (picture + volume map, GPUGrossPitaevskii_BCC.cu:424-468) done
+    on the device: median wall time of one gpe_volume_map / gpe_slice_picture call (kernel + copy of the
+    result), beside one download of the full even field, which is what the reference copies each time."""
+    def median_ms(fn):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(ts))
+
+    k = dims[2] // 2 + 1
+    return {"volume_map_call_ms": median_ms(lambda: s.volume_map(1.0)),
+            "slice_picture_call_ms": median_ms(lambda: s.slice_picture(k)),
+            "field_download_ms": median_ms(lambda: s.get_state(odd=False)),
+            "volume_bytes": 2 * (dims[0] + 2) * (dims[1] + 2) * (dims[2] + 2)}
+
+
 def precision_study(args):
     """256^3 cells (201,326,592 nodes), h = 0.06, g = 5000, isotropic trap: Thomas-Fermi profile
     with a seeded vortex ring, dt = 5e-5; FP32 bulk vs the FP64 path (bit-identical to the CPU
@@ -297,6 +316,9 @@ def main():
             s.close()
     err = float(np.linalg.norm(results["fp32"]["state"] - results["fp64"]["state"])
                 / np.linalg.norm(results["fp64"]["state"]))
+    output = output_timing(s64, dims)
+    log(f"output: volume map {output['volume_map_call_ms']:.2f} ms, slice {output['slice_picture_call_ms']:.2f} ms "
+        f"per call vs {output['field_download_ms']:.1f} ms to download the even field")
     s64.close()
 
     batched = None
@@ -347,6 +369,7 @@ def main():
         "hbm_copy_GBs": copy_gbs,
         "batched": batched,
         "serving": serving,
+        "output": output,
         "avg_launch_us": {"fp64": r64["avg_launch_us"], "fp32": r32["avg_launch_us"]},
     }
     print(json.dumps(line), flush=True)

@@ -188,6 +188,65 @@ double gpe_vortex_state_search_max_z(const gpe_vortex_state *vs, double eps);
 /* integrateCurvature (VortexState.cpp:313-340). */
 double gpe_vortex_state_curvature(const gpe_vortex_state *vs);
 
+/* ---- generating a VortexState (host, FP64, serial) ---------------------
+ * The reference's VortexState.cpp:5-47, 98-196, 250-257, 282-311, 342-361
+ * and the state recipe of GPUGrossPitaevskii_BCC.cu:516-535.  Results are
+ * bit-identical to the reference's host code built by gcc for x86-64 with
+ * FMA contraction off. */
+/* VortexState() (VortexState.cpp:5-10): kappa 2, g 0, 100 x 1 table, step 0.1. */
+gpe_status gpe_vortex_state_create(gpe_vortex_state **out);
+/* setKappa, setG (VortexState.hpp:23-27). */
+void gpe_vortex_state_set_kappa(gpe_vortex_state *vs, uint32_t kappa);
+void gpe_vortex_state_set_g(gpe_vortex_state *vs, double g);
+/* setDimensions (VortexState.cpp:12-40): table and normalised start shape, mu = 1. */
+gpe_status gpe_vortex_state_set_dimensions(gpe_vortex_state *vs, uint32_t rsize, double rstep, double rplug,
+                                           uint32_t zsize, double zstep);
+/* setRange (VortexState.cpp:42-47).  GPE_EINVAL for an empty range. */
+gpe_status gpe_vortex_state_set_range(gpe_vortex_state *vs, double rmin, double rmax, double zmax, double rstep,
+                                      double zstep);
+/* initialize (VortexState.cpp:98-107): mu and f sampled from src. */
+gpe_status gpe_vortex_state_initialize(gpe_vortex_state *vs, const gpe_vortex_state *src);
+
+/* Potential V(r, z) of the relaxation, the reference's
+ * ddouble (*potential)(ddouble r, ddouble z).  Where an entry point takes
+ * one, NULL selects the trap 0.5(r^2 + trap_ratio^2 z^2) of potentialRZ
+ * (GPUGrossPitaevskii_BCC.cu:28-31). */
+typedef double (*gpe_rz_potential)(double r, double z, void *user);
+/* Called where iterateSolution prints (VortexState.cpp:173). */
+typedef void (*gpe_relax_progress)(uint32_t iteration, double diff, double prob, void *user);
+typedef struct gpe_relax_report {
+    uint32_t iterations;  /* sweeps run */
+    uint32_t converged;   /* 1 when the last sweep's change fell below eps */
+    double diff;          /* change of the last sweep */
+    double prob;          /* <f,f> before the last normalisation */
+    double mu;            /* final mu */
+    double error;         /* integrateFunctionError of the result */
+} gpe_relax_report;
+/* iterateSolution (VortexState.cpp:109-196), without printing.  report may
+ * be NULL.  GPE_EUNSTABLE when mu is not finite at the end. */
+gpe_status gpe_vortex_state_iterate(gpe_vortex_state *vs, gpe_rz_potential V, double trap_ratio, void *user,
+                                    uint32_t maxiters, double eps, gpe_relax_progress progress, void *progress_user,
+                                    gpe_relax_report *report);
+/* integrateFunctionError (VortexState.cpp:282-311). */
+double gpe_vortex_state_function_error(const gpe_vortex_state *vs, gpe_rz_potential V, double trap_ratio,
+                                       void *user);
+/* getFunction (VortexState.cpp:220-238). */
+double gpe_vortex_state_function(const gpe_vortex_state *vs, double r, double z);
+/* searchMinR (VortexState.cpp:250-257). */
+double gpe_vortex_state_search_min_r(const gpe_vortex_state *vs, double eps);
+/* integrateZ (VortexState.cpp:342-361). */
+double gpe_vortex_state_integrate_z(const gpe_vortex_state *vs, double vol);
+/* save (VortexState.cpp:74-96), the format gpe_vortex_state_load reads. */
+gpe_status gpe_vortex_state_save(const gpe_vortex_state *vs, const char *path);
+/* main's state recipe with LOAD_STATE_FROM_DISK 0
+ * (GPUGrossPitaevskii_BCC.cu:516-535): coarse state at step 0.2, support at
+ * eps = 1e-5 max f, fine state at step 0.03 over that support, each relaxed
+ * with `iters` iterations to eps_relax (the reference: 10000, 1e-29).
+ * extent (may be NULL) = {minr, maxr, maxz} as main computes them. */
+gpe_status gpe_vortex_state_reference_recipe(uint32_t kappa, double g, double trap_ratio, int32_t is_3d,
+                                             uint32_t iters, double eps_relax, gpe_relax_progress progress,
+                                             void *progress_user, gpe_vortex_state **out, double extent[3]);
+
 /* Grid and origin of integrateInTime (GPUGrossPitaevskii_BCC.cu:223-227). */
 void gpe_reference_dims(double block_scale, const double minp[3], const double maxp[3], uint32_t dims[3],
                         double origin[3]);

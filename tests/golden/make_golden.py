@@ -15,11 +15,20 @@ Fixtures written (data only: inputs and the reference's outputs):
   vortex_state_ref.json  outputs of the reference's own VortexState.cpp and
                          Random.cpp (compiled from the reference tree into
                          oracle/_ref) on state.dat at seeded points.
+  state_files.json       size, SHA-256 and header of the shipped state.dat and
+                         state2.dat, and what main's state recipe
+                         (GPUGrossPitaevskii_BCC.cu:516-535), run with the
+                         reference's own VortexState.cpp, produces for each:
+                         the SHA-256 of the saved state and the extent
+                         (minr, maxr, maxz) main passes on.
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import os
+import struct
+import tempfile
 import re
 import sys
 
@@ -33,6 +42,7 @@ REF = "/root/reference"
 BCC = os.path.join(REF, "koodit/GrossPitaevskii/GPUGrossPitaevskii_BCC.cu")
 MULTI = os.path.join(REF, "koodit/GrossPitaevskii/GPUGrossPitaevskii_BCC_multi.cu")
 STATE = os.path.join(REF, "koodit/GrossPitaevskii/state.dat")
+STATE2 = os.path.join(REF, "koodit/GrossPitaevskii/state2.dat")
 
 
 def parse_geometry():
@@ -119,6 +129,54 @@ def vortex_state_values():
     return res
 
 
+def ref_recipe(ref, kappa, g, ratio, path):
+    """main's state recipe (GPUGrossPitaevskii_BCC.cu:516-535, 3-D branch) driven through the
+    reference's own VortexState.cpp; saves the fine state to path, returns (minr, maxr, maxz)."""
+    s0 = ref.ref_vs_create()
+    ref.ref_vs_set_physics(s0, kappa, g)
+    ref.ref_vs_set_range(s0, 0.0, 15.0, 35.0, 0.2, 0.2)
+    ref.ref_vs_iterate(s0, ratio, 10000, 1e-29)
+    eps = 1e-5 * ref.ref_vs_search_max(s0)
+    minr, maxr, maxz = ref.ref_vs_search_min_r(s0, eps), ref.ref_vs_search_max_r(s0, eps), ref.ref_vs_search_max_z(s0, eps)
+    s = ref.ref_vs_create()
+    ref.ref_vs_set_physics(s, kappa, g)
+    ref.ref_vs_set_range(s, minr, maxr, maxz, 0.03, 0.03)
+    ref.ref_vs_initialize(s, s0)
+    ref.ref_vs_iterate(s, ratio, 10000, 1e-29)
+    assert ref.ref_vs_save(s, path.encode()) == 1
+    ext = (minr, ref.ref_vs_search_max_r(s, eps), ref.ref_vs_search_max_z(s, eps))
+    ref.ref_vs_free(s0)
+    ref.ref_vs_free(s)
+    return ext
+
+
+def state_files():
+    from oracle_lib import load_ref
+
+    ref = load_ref()
+    assert ref is not None, "build oracle/_ref first: make -C oracle"
+    out = {"source": "koodit/GrossPitaevskii/state.dat, state2.dat; recipe GPUGrossPitaevskii_BCC.cu:516-535 "
+                     "run with the reference's VortexState.cpp (oracle/_ref)"}
+    # (file, kappa, g, RATIO): state.dat is main's configuration (BCC.cu:13-15); state2.dat's header
+    # gives kappa 10 and g 5000, RATIO 0.1 as in every GrossPitaevskii source.
+    for path, kappa, g, ratio in ((STATE, 20, 5000.0, 0.1), (STATE2, 10, 5000.0, 0.1)):
+        raw = open(path, "rb").read()
+        hdr = struct.unpack("<IdIIdddd", raw[:52])
+        with tempfile.TemporaryDirectory() as d:
+            regen = os.path.join(d, "state.dat")
+            ext = ref_recipe(ref, kappa, g, ratio, regen)
+            rraw = open(regen, "rb").read()
+        out[os.path.basename(path)] = {
+            "bytes": len(raw), "sha256": hashlib.sha256(raw).hexdigest(),
+            "header": dict(zip(["kappa", "g", "rsize", "zsize", "rplug", "rstep", "zstep", "mu"], hdr)),
+            "recipe": {"kappa": kappa, "g": g, "ratio": ratio},
+            "recipe_sha256": hashlib.sha256(rraw).hexdigest(), "recipe_bytes": len(rraw),
+            "recipe_extent": list(ext),
+            "recipe_reproduces_file": rraw == raw,
+        }
+    return out
+
+
 def main():
     geo = parse_geometry()
     with open(os.path.join(HERE, "bcc_geometry_ref.json"), "w") as f:
@@ -126,7 +184,10 @@ def main():
     vs = vortex_state_values()
     with open(os.path.join(HERE, "vortex_state_ref.json"), "w") as f:
         json.dump(vs, f)
-    print("wrote bcc_geometry_ref.json, vortex_state_ref.json")
+    sf = state_files()
+    with open(os.path.join(HERE, "state_files.json"), "w") as f:
+        json.dump(sf, f, indent=1)
+    print("wrote bcc_geometry_ref.json, vortex_state_ref.json, state_files.json")
 
 
 if __name__ == "__main__":

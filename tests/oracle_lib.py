@@ -57,7 +57,7 @@ def load_oracle():
 
 
 def load_ref():
-    """The reference's VortexState.cpp + Random.cpp, or None when not built."""
+    """The reference's VortexState.cpp, Random.cpp, Picture.cpp and Text.cpp, or None when not built."""
     if not os.path.exists(REF_SO):
         return None
     r = C.CDLL(REF_SO)
@@ -74,6 +74,19 @@ def load_ref():
         "ref_vs_mu": (_D, [_P]),
         "ref_vs_g": (_D, [_P]),
         "ref_vs_kappa": (_D, [_P]),
+        "ref_vs_create": (_P, []),
+        "ref_vs_set_physics": (None, [_P, C.c_uint, _D]),
+        "ref_vs_set_dimensions": (None, [_P, C.c_uint, _D, _D, C.c_uint, _D]),
+        "ref_vs_set_range": (None, [_P, _D, _D, _D, _D, _D]),
+        "ref_vs_initialize": (None, [_P, _P]),
+        "ref_vs_iterate": (None, [_P, _D, C.c_uint, _D]),
+        "ref_vs_error": (_D, [_P, _D]),
+        "ref_vs_search_min_r": (_D, [_P, _D]),
+        "ref_vs_integrate_z": (_D, [_P, _D]),
+        "ref_vs_save": (C.c_int, [_P, C.c_char_p]),
+        "ref_picture_save": (C.c_int, [C.c_uint, C.c_uint, _DP, C.c_char_p, C.c_int]),
+        "ref_save_volume_map": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint16), C.c_uint, C.c_uint, C.c_uint,
+                                          _D, _D, _D]),
         "ref_random_uints": (None, [_U32, _LL, C.POINTER(C.c_uint32)]),
         "ref_random_circles": (None, [_U32, _LL, _DP]),
     }

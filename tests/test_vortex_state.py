@@ -1,0 +1,182 @@
+"""Generating a VortexState (host, CPU): the start shape, resampling, the relaxation, its residual
+and the state recipe, against the reference's own VortexState.cpp compiled from the reference tree
+(oracle/_ref) and against the state files the reference ships.
+
+Pins:
+  * byte equality of saved states with the reference's own code on the same calls
+    (setDimensions, setRange, initialize, iterateSolution incl. the early stop, 2-D tables, rplug > 0,
+    a caller-supplied potential), and equality of integrateFunctionError / searchMinR / integrateZ;
+  * the shipped state.dat and state2.dat regenerated byte for byte from scratch by the recipe of
+    GPUGrossPitaevskii_BCC.cu:516-535 (hashes in tests/golden/state_files.json, so this pin holds
+    without the reference tree);
+  * closed forms: the start shape's norm is 1, getMaxR/getMaxZ span the table, and for g = 0 in the
+    isotropic trap the relaxed kappa-vortex has mu -> kappa + 3/2 (the 3-D oscillator level).
+"""
+import hashlib
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from vaikkari_amd.solver import TrapRZ, VortexState, reference_state_recipe
+from oracle_lib import load_ref
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "state_files.json")))
+REF = load_ref()
+needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (reference tree absent)")
+
+
+def saved(vs, tmp_path, name):
+    p = str(tmp_path / name)
+    assert vs.save(p)
+    return open(p, "rb").read()
+
+
+def ref_saved(h, tmp_path, name):
+    p = str(tmp_path / name)
+    assert REF.ref_vs_save(h, p.encode()) == 1
+    return open(p, "rb").read()
+
+
+def ref_state(kappa, g):
+    h = REF.ref_vs_create()
+    REF.ref_vs_set_physics(h, kappa, g)
+    return h
+
+
+@needs_ref
+def test_default_and_dimensions_match_reference(tmp_path):
+    h = REF.ref_vs_create()
+    v = VortexState()
+    assert saved(v, tmp_path, "a") == ref_saved(h, tmp_path, "b")
+    for dims in [(37, 0.11, 0.0, 1, 1.0), (12, 0.3, 0.45, 29, 0.07), (1, 0.5, 0.0, 5, 0.2), (0, 0.0, -1.0, 0, 0.0)]:
+        REF.ref_vs_set_dimensions(h, *dims)
+        v.setDimensions(*dims)
+        assert saved(v, tmp_path, "a") == ref_saved(h, tmp_path, "b"), dims
+    for rng in [(0.0, 15.0, 35.0, 0.2, 0.2), (1.6, 8.6, 30.0, 0.03, 0.03), (0.3, 4.0, 1.0, 0.07, 1.0)]:
+        REF.ref_vs_set_range(h, *rng)
+        v.setRange(*rng)
+        assert saved(v, tmp_path, "a") == ref_saved(h, tmp_path, "b"), rng
+    REF.ref_vs_free(h)
+
+
+def test_start_shape_closed_forms():
+    v = VortexState()
+    v.setRange(0.5, 6.5, 4.0, 0.1, 0.05)
+    _, rs, zs = v._params()[1:4]
+    rs, zs = int(rs), int(zs)
+    p = v._params()
+    assert (rs, zs) == (60, 80)
+    assert v.getMinR() == 0.5 and v.getMaxR() == pytest.approx(6.5, abs=1e-12)
+    assert v.getMaxZ() == pytest.approx(4.0, abs=1e-12)
+    # norm in the ring-volume measure (z = 0 row once, the others twice) is 1
+    f = np.array([[v.getFunction(p[4] + i * p[5], j * p[6]) for j in range(zs)] for i in range(rs)])
+    r = p[4] + np.arange(rs) * p[5]
+    w = np.full(zs, 2.0)
+    w[0] = 1.0
+    norm = 2 * math.pi * p[5] * p[6] * np.sum((f ** 2) * r[:, None] * w[None, :])
+    assert norm == pytest.approx(1.0, rel=1e-12)
+    with pytest.raises(Exception):
+        v.setRange(2.0, 1.0, 1.0, 0.1, 0.1)
+
+
+@needs_ref
+@pytest.mark.parametrize("case", [
+    dict(kappa=7, g=800.0, rng=(0.0, 8.0, 10.0, 0.2, 0.25), ratio=0.5, iters=300, eps=1e-29),
+    dict(kappa=1, g=0.0, rng=(0.0, 6.0, 6.0, 0.15, 0.15), ratio=1.0, iters=200, eps=1e-29),
+    dict(kappa=3, g=300.0, rng=(0.4, 7.0, 1.0, 0.1, 1.0), ratio=0.3, iters=400, eps=1e-29),   # 2-D, rplug > 0
+    dict(kappa=5, g=2000.0, rng=(0.0, 9.0, 12.0, 0.3, 0.3), ratio=0.2, iters=5000, eps=1e-12),  # early stop
+])
+def test_relaxation_matches_reference(tmp_path, case):
+    h = ref_state(case["kappa"], case["g"])
+    REF.ref_vs_set_range(h, *case["rng"])
+    REF.ref_vs_iterate(h, case["ratio"], case["iters"], case["eps"])
+    v = VortexState()
+    v.setKappa(case["kappa"])
+    v.setG(case["g"])
+    v.setRange(*case["rng"])
+    rep = v.iterateSolution(TrapRZ(case["ratio"]), case["iters"], case["eps"])
+    assert saved(v, tmp_path, "a") == ref_saved(h, tmp_path, "b")
+    assert rep["error"] == REF.ref_vs_error(h, case["ratio"])
+    assert rep["mu"] == v.getMu()
+    if case["eps"] > 1e-20:
+        assert rep["converged"] == 1 and rep["iterations"] < case["iters"] and rep["diff"] < case["eps"]
+    REF.ref_vs_free(h)
+
+
+@needs_ref
+def test_callable_potential_and_queries_match_reference(tmp_path, state_dat):
+    """A caller-supplied V(r,z) equal to the trap gives the trap's bits; initialize from state.dat
+    onto another table, then searchMinR / integrateZ / integrateFunctionError agree."""
+    h = ref_state(4, 150.0)
+    REF.ref_vs_set_range(h, 0.0, 5.0, 5.0, 0.25, 0.25)
+    REF.ref_vs_iterate(h, 0.7, 150, 1e-29)
+    v = VortexState()
+    v.setKappa(4)
+    v.setG(150.0)
+    v.setRange(0.0, 5.0, 5.0, 0.25, 0.25)
+    v.iterateSolution(lambda r, z: 0.5 * (r * r + 0.7 * 0.7 * z * z), 150, 1e-29)
+    assert saved(v, tmp_path, "a") == ref_saved(h, tmp_path, "b")
+    REF.ref_vs_free(h)
+
+    src_ref = REF.ref_vs_load(state_dat.encode())
+    src = VortexState()
+    assert src.load(state_dat)
+    h = ref_state(20, 5000.0)
+    REF.ref_vs_set_range(h, 1.3, 9.1, 31.0, 0.05, 0.04)
+    REF.ref_vs_initialize(h, src_ref)
+    v = VortexState()
+    v.setKappa(20)
+    v.setG(5000.0)
+    v.setRange(1.3, 9.1, 31.0, 0.05, 0.04)
+    v.initialize(src)
+    assert saved(v, tmp_path, "a") == ref_saved(h, tmp_path, "b")
+    for eps in (1e-9, 1e-4, 0.02, 1.0):
+        assert v.searchMinR(eps) == REF.ref_vs_search_min_r(h, eps)
+        assert src.searchMinR(eps) == REF.ref_vs_search_min_r(src_ref, eps)
+    for vol in (-1.0, 0.0, 1e-6, 0.1, 0.5, 0.9, 0.999, 2.0):
+        assert src.integrateZ(vol) == REF.ref_vs_integrate_z(src_ref, vol)
+        assert v.integrateZ(vol) == REF.ref_vs_integrate_z(h, vol)
+    for ratio in (0.1, 0.35):
+        assert src.integrateFunctionError(TrapRZ(ratio)) == REF.ref_vs_error(src_ref, ratio)
+    REF.ref_vs_free(h)
+    REF.ref_vs_free(src_ref)
+
+
+def test_integrate_z_is_monotone_and_bounded(state_dat):
+    src = VortexState()
+    assert src.load(state_dat)
+    zs = [src.integrateZ(v) for v in np.linspace(0.01, 0.99, 25)]
+    assert all(b > a for a, b in zip(zs, zs[1:]))
+    assert 0.0 < zs[0] and zs[-1] < src.getMaxZ()
+    assert src.integrateZ(0.0) == 0.0 and src.integrateZ(5.0) == src.getMaxZ()
+
+
+def test_harmonic_oscillator_level():
+    """g = 0, isotropic trap: the lowest kappa-vortex state is the oscillator level kappa + 3/2."""
+    for kappa in (1, 2):
+        v = VortexState()
+        v.setKappa(kappa)
+        v.setG(0.0)
+        v.setRange(0.0, 7.0, 7.0, 0.05, 0.05)
+        rep = v.iterateSolution(TrapRZ(1.0), 20000, 1e-26)
+        assert rep["mu"] == pytest.approx(kappa + 1.5, abs=5e-3)
+        assert rep["prob"] == pytest.approx(1.0, abs=1e-9)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("name", ["state.dat", "state2.dat"])
+def test_recipe_regenerates_shipped_state_files(tmp_path, name):
+    """main's recipe (g 5000, RATIO 0.1, 3-D; kappa 20 for state.dat, 10 for state2.dat) rebuilds
+    the shipped state file bit for bit, and hands on the extent the reference's own code computes."""
+    ref = GOLDEN[name]
+    rc = ref["recipe"]
+    assert ref["recipe_reproduces_file"]
+    vs, extent = reference_state_recipe(rc["kappa"], rc["g"], rc["ratio"], True, 10000, 1e-29)
+    raw = saved(vs, tmp_path, name)
+    assert len(raw) == ref["bytes"]
+    assert hashlib.sha256(raw).hexdigest() == ref["sha256"]
+    assert list(extent) == ref["recipe_extent"]

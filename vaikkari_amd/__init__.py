@@ -6,6 +6,7 @@ reference's solver interface over that ABI.
 """
 from .solver import (  # noqa: F401
     Solver,
+    TrapRZ,
     VortexState,
     as_complex,
     bcc_positions,
@@ -17,6 +18,7 @@ from .solver import (  # noqa: F401
     gaussian,
     integrate_in_time,
     integrateInTime,
+    integrateInTimeWithOutput,
     leapfrog_coefs,
     node_measure,
     node_positions,
@@ -24,9 +26,12 @@ from .solver import (  # noqa: F401
     phase_shift,
     reference_dims,
     reference_setup,
+    reference_state_recipe,
     thomas_fermi,
     trap_potential,
     variants,
     vortex_line,
     vortex_ring,
+    write_bmp,
+    write_volume_map,
 )

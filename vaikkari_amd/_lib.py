@@ -127,6 +127,31 @@ class GpeServerStats(C.Structure):
     ]
 
 
+class GpeRelaxReport(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_uint32),
+        ("converged", C.c_uint32),
+        ("diff", C.c_double),
+        ("prob", C.c_double),
+        ("mu", C.c_double),
+        ("error", C.c_double),
+    ]
+
+
+RZ_POTENTIAL = C.CFUNCTYPE(C.c_double, C.c_double, C.c_double, C.c_void_p)
+RELAX_PROGRESS = C.CFUNCTYPE(None, C.c_uint32, C.c_double, C.c_double, C.c_void_p)
+
+
+class GpeOutputOptions(C.Structure):
+    _fields_ = [
+        ("save_picture", C.c_uint32),
+        ("save_volume", C.c_uint32),
+        ("volume_flags", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("prefix", C.c_char_p),
+    ]
+
+
 class GpeError(RuntimeError):
     def __init__(self, status: int, where: str, detail: str):
         super().__init__(f"{where}: status {status} ({detail})")
@@ -196,6 +221,27 @@ SIGNATURES = [
     ("gpe_vortex_state_search_max_r", _D, [_P, _D]),
     ("gpe_vortex_state_search_max_z", _D, [_P, _D]),
     ("gpe_vortex_state_curvature", _D, [_P]),
+    ("gpe_vortex_state_create", _I, [C.POINTER(_P)]),
+    ("gpe_vortex_state_set_kappa", None, [_P, _U32]),
+    ("gpe_vortex_state_set_g", None, [_P, _D]),
+    ("gpe_vortex_state_set_dimensions", _I, [_P, _U32, _D, _D, _U32, _D]),
+    ("gpe_vortex_state_set_range", _I, [_P, _D, _D, _D, _D, _D]),
+    ("gpe_vortex_state_initialize", _I, [_P, _P]),
+    ("gpe_vortex_state_iterate", _I, [_P, RZ_POTENTIAL, _D, _P, _U32, _D, RELAX_PROGRESS, _P,
+                                      C.POINTER(GpeRelaxReport)]),
+    ("gpe_vortex_state_function_error", _D, [_P, RZ_POTENTIAL, _D, _P]),
+    ("gpe_vortex_state_function", _D, [_P, _D, _D]),
+    ("gpe_vortex_state_search_min_r", _D, [_P, _D]),
+    ("gpe_vortex_state_integrate_z", _D, [_P, _D]),
+    ("gpe_vortex_state_save", _I, [_P, C.c_char_p]),
+    ("gpe_vortex_state_reference_recipe", _I, [_U32, _D, _D, C.c_int32, _U32, _D, RELAX_PROGRESS, _P,
+                                               C.POINTER(_P), _DP]),
+    ("gpe_volume_map", _I, [_P, _D, _U32, C.POINTER(C.c_uint16)]),
+    ("gpe_slice_picture", _I, [_P, _U32, _D, C.c_int32, C.POINTER(C.c_uint8)]),
+    ("gpe_write_volume_map", _I, [C.c_char_p, C.POINTER(C.c_uint16), _U32P, _DP]),
+    ("gpe_write_bmp", _I, [C.c_char_p, _U32, _U32, C.POINTER(C.c_uint8)]),
+    ("gpe_integrate_in_time_output", _I, [_P, _D, _DP, _DP, _D, _U32, _D, _U32, C.POINTER(GpeOutputOptions),
+                                          C.POINTER(GpeReferenceRun)]),
     ("gpe_reference_dims", None, [_D, _DP, _DP, _U32P, _DP]),
     ("gpe_reference_setup", _I, [_P, _D, _DP, _DP, _D, _D, _U32, C.POINTER(GpeReferenceRun), _DP, _DP, _DP]),
     ("gpe_integrate_in_time", _I, [_P, _D, _DP, _DP, _D, _U32, _D, _U32, C.POINTER(GpeReferenceRun), _DP, _DP]),

@@ -21,6 +21,7 @@
 
 #include "gpe_fused.hpp"  // remap_block
 #include "gpe_internal.hpp"
+#include "gpe_hip_check.hpp"
 #include "gpe_kernels.hpp"
 #include "gpe_solver.hpp"
 #include "../../include/vaikkari_gpe_batch.h"
@@ -211,15 +212,6 @@ __global__ void k_init_gaussian(cplx_t<T> *__restrict__ even, cplx_t<T> *__restr
 }  // namespace gpe
 
 namespace {
-
-#define GPE_HIP_TRY(expr)                                                                         \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess) {                                                                   \
-            set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
-            return GPE_EDEVICE;                                                                   \
-        }                                                                                         \
-    } while (0)
 
 constexpr int kDefaultImagCZ = 8;
 

@@ -12,21 +12,13 @@
 
 #include "gpe_internal.hpp"
 #include "gpe_fused.hpp"
+#include "gpe_hip_check.hpp"
 #include "gpe_kernels.hpp"
 #include "gpe_solver.hpp"
 
 using namespace gpe;
 
 namespace gpe {
-
-#define GPE_HIP_TRY(expr)                                                                   \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess) {                                                             \
-            set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
-            return GPE_EDEVICE;                                                             \
-        }                                                                                   \
-    } while (0)
 
 Layout make_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes) {
     Layout L{};
@@ -236,6 +228,7 @@ gpe_solver::~gpe_solver() {
     if (pot) (void)hipFree(pot);
     if (scal) (void)hipFree(scal);
     if (partials) (void)hipFree(partials);
+    if (out_buf) (void)hipFree(out_buf);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);

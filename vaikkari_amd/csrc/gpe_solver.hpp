@@ -34,6 +34,8 @@ struct gpe_solver {
     double *scal = nullptr;        // [0] last N, [1] read scale, [4..7] observables
     double *partials = nullptr;    // per-block partial sums
     long long npartials = 0;
+    void *out_buf = nullptr;       // device staging of pictures / volume maps (grown on demand)
+    size_t out_bytes = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t steps_done = 0;

@@ -3,11 +3,14 @@
 The reference drives its stepper through ``integrateInTime`` and a
 ``VortexState`` (koodit/GrossPitaevskii/GPUGrossPitaevskii_BCC.cu:218,
 koodit/GrossPitaevskii/VortexState.hpp:16-76).  This module exposes the same
-names with the same argument meaning (``VortexState.load``, ``getPsi``,
-``getPhase``, ``searchFunctionMax``, ``searchMaxR``, ``searchMaxZ``,
-``integrateCurvature``, ``integrateInTime``) plus ``Solver``, the object form of
-the stepper (gpe_create ... gpe_destroy).  Fields are flat float64 arrays of
-2*nodes values in the host order [k][j][i][l], complex interleaved.
+names with the same argument meaning (``VortexState`` with ``load``/``save``,
+``setKappa``, ``setG``, ``setDimensions``, ``setRange``, ``initialize``,
+``iterateSolution``, ``getPsi``, ``getPhase``, ``getFunction``, the searches and
+integrals; ``integrateInTime``) plus ``Solver``, the object form of the stepper
+(gpe_create ... gpe_destroy), and the reference's picture / volume-map output
+(computed on the device, written in the reference's BMP and MetaImage formats).
+Fields are flat float64 arrays of 2*nodes values in the host order
+[k][j][i][l], complex interleaved.
 """
 from __future__ import annotations
 
@@ -108,6 +111,24 @@ class Solver:
         check(lib.gpe_get_state(self._h, dptr(e), dptr(o)), "gpe_get_state")
         return (e, o) if odd else e
 
+    # -- output (GPUGrossPitaevskii_BCC.cu:424-468), reduced on the device --
+    def volume_map(self, unit: float, reference_class0: bool = False) -> np.ndarray:
+        """uint16 [nz+2, ny+2, nx+2]: min(65535, trunc(unit * sum_l |psi_l|^2)) of the even field over
+        the padded extent; reference_class0 reproduces the reference's |psi_0|^2 added 12 times."""
+        nx, ny, nz = self.dims
+        out = np.empty((nz + 2, ny + 2, nx + 2), np.uint16)
+        check(lib.gpe_volume_map(self._h, float(unit), 1 if reference_class0 else 0,
+                                 out.ctypes.data_as(C.POINTER(C.c_uint16))), "gpe_volume_map")
+        return out
+
+    def slice_picture(self, k: int, gain: float = 5.0, middle: bool = False) -> np.ndarray:
+        """uint8 [ny+2, nx+2, 3] (blue, green, red) of padded z layer k, as Picture::save stores it."""
+        nx, ny, _ = self.dims
+        out = np.empty((ny + 2, nx + 2, 3), np.uint8)
+        check(lib.gpe_slice_picture(self._h, int(k), float(gain), 1 if middle else 0,
+                                    out.ctypes.data_as(C.POINTER(C.c_uint8))), "gpe_slice_picture")
+        return out
+
     def set_mode(self, mode: str, dt: float) -> None:
         check(lib.gpe_set_mode(self._h, MODES[mode], float(dt)), "gpe_set_mode")
         self.cfg.mode = MODES[mode]
@@ -205,7 +226,9 @@ class VortexState:
     loaded from its binary file format (VortexState.cpp:49-72)."""
 
     def __init__(self):
+        # VortexState() (VortexState.cpp:5-10): kappa 2, g 0, a 100 x 1 table.
         self._h = C.c_void_p()
+        check(lib.gpe_vortex_state_create(C.byref(self._h)), "gpe_vortex_state_create")
 
     def load(self, path: str) -> bool:
         h = C.c_void_p()
@@ -262,6 +285,97 @@ class VortexState:
 
     def integrateCurvature(self) -> float:
         return lib.gpe_vortex_state_curvature(self._h)
+
+    # -- generating a state (VortexState.cpp:12-47, 74-196, 250-257, 282-311, 342-361) --
+    def setKappa(self, kappa: int):
+        lib.gpe_vortex_state_set_kappa(self._h, int(kappa))
+
+    def setG(self, g: float):
+        lib.gpe_vortex_state_set_g(self._h, float(g))
+
+    def setDimensions(self, rsize: int, rstep: float, rplug: float = 0.0, zsize: int = 1, zstep: float = 1.0):
+        check(lib.gpe_vortex_state_set_dimensions(self._h, int(rsize), float(rstep), float(rplug), int(zsize),
+                                                  float(zstep)), "gpe_vortex_state_set_dimensions")
+
+    def setRange(self, rmin: float, rmax: float, zmax: float, rstep: float, zstep: float):
+        check(lib.gpe_vortex_state_set_range(self._h, float(rmin), float(rmax), float(zmax), float(rstep),
+                                             float(zstep)), "gpe_vortex_state_set_range")
+
+    def initialize(self, state: "VortexState"):
+        check(lib.gpe_vortex_state_initialize(self._h, state._h), "gpe_vortex_state_initialize")
+
+    def iterateSolution(self, potential, maxiters: int = 100000, eps: float = 1e-28, progress=None) -> dict:
+        """Relaxes f to a stationary state of the potential (a TrapRZ, evaluated natively, or any
+        callable V(r, z)).  progress(iteration, diff, prob) is called every 1000 iterations, where
+        the reference prints.  Returns the run's report (iterations, converged, diff, prob, mu, error)."""
+        fn, ratio = _rz_potential(potential)
+        cb = _lib.RELAX_PROGRESS(lambda it, d, p, _u: progress(it, d, p)) if progress else _lib.RELAX_PROGRESS()
+        rep = _lib.GpeRelaxReport()
+        check(lib.gpe_vortex_state_iterate(self._h, fn, ratio, None, int(maxiters), float(eps), cb, None,
+                                           C.byref(rep)), "gpe_vortex_state_iterate")
+        return {k: getattr(rep, k) for k, _ in rep._fields_}
+
+    def integrateFunctionError(self, potential) -> float:
+        fn, ratio = _rz_potential(potential)
+        return lib.gpe_vortex_state_function_error(self._h, fn, ratio, None)
+
+    def save(self, path: str) -> bool:
+        return lib.gpe_vortex_state_save(self._h, str(path).encode()) == _lib.OK
+
+    def getFunction(self, r: float, z: float) -> float:
+        return lib.gpe_vortex_state_function(self._h, float(r), float(z))
+
+    def getMinR(self) -> float:
+        return float(self._params()[4])
+
+    def getMaxR(self) -> float:
+        p = self._params()
+        return float(p[4] + int(p[2]) * p[5])
+
+    def getMaxZ(self) -> float:
+        p = self._params()
+        return float(int(p[3]) * p[6])
+
+    def searchMinR(self, eps: float) -> float:
+        return lib.gpe_vortex_state_search_min_r(self._h, float(eps))
+
+    def integrateZ(self, vol: float) -> float:
+        return lib.gpe_vortex_state_integrate_z(self._h, float(vol))
+
+
+class TrapRZ:
+    """The trap 0.5(r^2 + ratio^2 z^2) of potentialRZ (GPUGrossPitaevskii_BCC.cu:28-31); passed to
+    VortexState.iterateSolution it is evaluated inside the library."""
+
+    def __init__(self, ratio: float):
+        self.ratio = float(ratio)
+
+    def __call__(self, r: float, z: float) -> float:
+        return 0.5 * (r * r + self.ratio * self.ratio * z * z)
+
+
+def _rz_potential(potential):
+    if isinstance(potential, TrapRZ):
+        return _lib.RZ_POTENTIAL(), potential.ratio
+    if not callable(potential):
+        raise TypeError("potential must be a TrapRZ or a callable V(r, z)")
+    return _lib.RZ_POTENTIAL(lambda r, z, _u: float(potential(r, z))), 0.0
+
+
+def reference_state_recipe(kappa: int, g: float, trap_ratio: float, is_3d: bool = True, iters: int = 10000,
+                           eps: float = 1e-29, progress=None):
+    """The state recipe of GPUGrossPitaevskii_BCC.cu:516-535 (LOAD_STATE_FROM_DISK 0).
+
+    Returns (state, (minr, maxr, maxz)) with the extents main passes to integrateInTime."""
+    cb = _lib.RELAX_PROGRESS(lambda it, d, p, _u: progress(it, d, p)) if progress else _lib.RELAX_PROGRESS()
+    h = C.c_void_p()
+    ext = np.zeros(3)
+    check(lib.gpe_vortex_state_reference_recipe(int(kappa), float(g), float(trap_ratio), 1 if is_3d else 0,
+                                                int(iters), float(eps), cb, None, C.byref(h), dptr(ext)),
+          "gpe_vortex_state_reference_recipe")
+    vs = VortexState.__new__(VortexState)
+    vs._h = h
+    return vs, (float(ext[0]), float(ext[1]), float(ext[2]))
 
 
 def reference_dims(block_scale, minp, maxp):
@@ -320,6 +434,41 @@ def integrateInTime(state: VortexState, block_scale: float, minp, maxp, iteratio
 
 
 integrate_in_time = integrateInTime
+
+
+def integrateInTimeWithOutput(state: VortexState, block_scale: float, minp, maxp, iteration_period: float,
+                              number_of_iterations: int, trap_ratio: float = 0.1, precision: str = "fp64",
+                              save_picture: bool = True, save_volume: bool = True, prefix: str = "",
+                              reference_class0: bool = False) -> dict:
+    """integrateInTime with the reference's SAVE_PICTURE / SAVE_VOLUME output
+    (GPUGrossPitaevskii_BCC.cu:17-18, 424-468): <prefix>kuva<iter>.bmp and <prefix>volume<iter>.mhd
+    (+ .raw) before each iteration and after the last.  Returns the run's setup scalars."""
+    run = _lib.GpeReferenceRun()
+    opt = _lib.GpeOutputOptions()
+    opt.save_picture = 1 if save_picture else 0
+    opt.save_volume = 1 if save_volume else 0
+    opt.volume_flags = 1 if reference_class0 else 0
+    opt.prefix = str(prefix).encode()
+    check(lib.gpe_integrate_in_time_output(state._h, float(block_scale), _lib.d3(minp), _lib.d3(maxp),
+                                           float(iteration_period), int(number_of_iterations), float(trap_ratio),
+                                           PRECISIONS[precision], C.byref(opt), C.byref(run)),
+          "gpe_integrate_in_time_output")
+    return _run_dict(run)
+
+
+def write_volume_map(path: str, vol: np.ndarray, spacing) -> None:
+    """saveVolumeMap (GPUGrossPitaevskii_BCC.cu:38-67) of a uint16 [z][y][x] volume."""
+    vol = np.ascontiguousarray(vol, np.uint16)
+    dims = (C.c_uint32 * 3)(vol.shape[2], vol.shape[1], vol.shape[0])
+    check(lib.gpe_write_volume_map(str(path).encode(), vol.ctypes.data_as(C.POINTER(C.c_uint16)), dims,
+                                   _lib.d3(spacing)), "gpe_write_volume_map")
+
+
+def write_bmp(path: str, bgr: np.ndarray) -> None:
+    """Picture::save (Output/Picture.cpp:81-137) of a uint8 [height][width][3] (blue, green, red) image."""
+    bgr = np.ascontiguousarray(bgr, np.uint8)
+    check(lib.gpe_write_bmp(str(path).encode(), bgr.shape[1], bgr.shape[0], bgr.ctypes.data_as(C.POINTER(C.c_uint8))),
+          "gpe_write_bmp")
 
 
 # ---------------------------------------------------------------------------
