@@ -48,8 +48,11 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson serving sample (0 = skip)")
-    ap.add_argument("--suite", default="main", choices=["main", "precision"],
-                    help="main: the JSON line of record; precision: 256^3 FP32-vs-FP64 error study")
+    ap.add_argument("--suite", default="main", choices=["main", "precision", "serving"],
+                    help="main: the JSON line of record; precision: 256^3 FP32-vs-FP64 error study; "
+                         "serving: the Poisson serving stream alone (use --serve-seconds 600)")
+    ap.add_argument("--serve-policy", default="shortest_first", choices=["shortest_first", "fifo"],
+                    help="admission policy of --suite serving (shortest_first is capped at 64 M resident nodes)")
     ap.add_argument("--precision-cells", type=int, default=256)
     ap.add_argument("--precision-steps", type=int, default=10000)
     ap.add_argument("--precision-every", type=int, default=1000)
@@ -155,16 +158,30 @@ def batched_workload(n_inst, steps, seed=4, log=log):
             "dtype": "fp32 bulk", "sizes": sorted(set(int(c.nx) for c in cfgs)), "h": 0.25}
 
 
-def serving_workload(seconds, log=log):
+SERVE_CAP_NODES = 64 * 10 ** 6  # resident nodes that keep the batched launches saturated
+
+
+def serving_workload(seconds, policy="shortest_first", cap=SERVE_CAP_NODES, log=log):
+    """Poisson stream (rate 20/s, seed 5; n log-uniform in [16, 160], steps in [500, 20000], 30 % ground
+    states), FP32 bulk, 200 GB HBM budget, 50-step chunks, replayed against the wall clock."""
     from vaikkari_amd.serve import poisson_workload, run_serving
 
     w = poisson_workload(seed=5, rate=20.0, duration=seconds)
     res = run_serving(w, duration=seconds, precision="fp32", hbm_budget_bytes=200 * 10 ** 9, chunk_steps=50,
-                      log=log)
+                      log=log, policy=policy, max_active_nodes=cap)
     res["rate_per_s"] = 20.0
     res["hbm_budget_GB"] = 200
-    res["policy"] = "first-fit admission by resident bytes, FIFO order, 50-step chunks"
+    res["chunk_steps"] = 50
     return res
+
+
+def serving_suite(args):
+    """The full serving configuration: 600 s of the Poisson stream (about 12,000 requests) under one
+    admission policy; prints one JSON line with latency percentiles, offered vs served load and the
+    queue-depth time series."""
+    cap = SERVE_CAP_NODES if args.serve_policy == "shortest_first" else 0
+    res = serving_workload(args.serve_seconds, policy=args.serve_policy, cap=cap)
+    print(json.dumps({"suite": "serving", **res}), flush=True)
 
 
 def output_timing(s, dims, reps=5):
@@ -241,10 +258,13 @@ def precision_study(args):
 
 def main():
     args = parse()
-    if args.suite == "precision":
+    if args.suite in ("precision", "serving"):
         if args.gpus != 1:
             sys.exit(2)
-        precision_study(args)
+        if args.suite == "precision":
+            precision_study(args)
+        else:
+            serving_suite(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -329,8 +349,12 @@ def main():
     serving = None
     if args.serve_seconds > 0:
         serving = serving_workload(args.serve_seconds)
-        log(f"serving: {serving['completed']} done, p50 {serving['p50_s']} s, p99 {serving['p99_s']} s, "
-            f"served {serving['served_site_steps_per_s'] / 1e9:.1f} G site-steps/s")
+        fifo = serving_workload(args.serve_seconds, policy="fifo", cap=0)
+        serving["fifo_uncapped"] = {k: fifo[k] for k in ("completed", "p50_s", "p99_s", "served_site_steps_per_s",
+                                                          "active_at_end", "queued_at_end")}
+        for name, r in (("shortest-first, capped", serving), ("fifo, uncapped", fifo)):
+            log(f"serving ({name}): {r['completed']} done, p50 {r['p50_s']} s, p99 {r['p99_s']} s, "
+                f"served {r['served_site_steps_per_s'] / 1e9:.1f} G site-steps/s")
 
     cpu = None
     if not args.no_cpu:

@@ -7,7 +7,8 @@
  * persistent kernel launches per step: the two in-place leapfrog
  * half-updates for real-time instances, the z-march Euler update plus a
  * per-instance FP64 renormalisation for imaginary-time instances.  A chunk of steps is
- * captured once into a hipGraph and replayed.  Each instance's arithmetic
+ * captured once into a hipGraph and replayed (or enqueued directly, see
+ * gpe_batch_set_launch_mode).  Each instance's arithmetic
  * (tile decomposition, reduction order) depends only on that instance, so
  * its result is bit-identical whatever else shares the batch.
  *
@@ -83,6 +84,11 @@ uint64_t gpe_batch_chunk_node_steps(const gpe_batch *b, uint32_t chunk_steps);
 /* Waves per block (2, 4 or 8) of the real-time pass; only while the batch
  * is empty.  The autotuner picks it. */
 gpe_status gpe_batch_set_tile(gpe_batch *b, uint32_t waves);
+/* How gpe_batch_run issues a chunk: GPE_BATCH_GRAPH replays a captured
+ * hipGraph of the chunk (the default), GPE_BATCH_STREAM enqueues the chunk's
+ * kernels onto the batch's stream directly.  Results are identical. */
+typedef enum gpe_batch_launch_mode { GPE_BATCH_GRAPH = 0, GPE_BATCH_STREAM = 1 } gpe_batch_launch_mode;
+gpe_status gpe_batch_set_launch_mode(gpe_batch *b, uint32_t mode);
 /* Number of active instances; bytes resident. */
 int32_t gpe_batch_active(const gpe_batch *b);
 uint64_t gpe_batch_resident_bytes(const gpe_batch *b);

@@ -3,12 +3,24 @@
  *
  * No counterpart in the reference (one solve per process, its state never
  * leaves the GPU until the run ends).  Requests enter a queue; every tick the
- * server admits queued requests while their resident bytes fit the HBM
- * budget (first fit: a request that does not fit does not block smaller
- * ones behind it), advances every resident solve by one chunk of steps in
+ * server admits queued requests (policy below) while their resident bytes
+ * fit the HBM budget, advances every resident solve by one chunk of steps in
  * shared persistent launches (vaikkari_gpe_batch.h), and retires finished
  * solves, recording time-to-result.  A solve's result does not depend on
  * what else is resident.
+ *
+ * Admission policy (gpe_server_set_policy):
+ *   GPE_SERVE_FIFO            queued requests in arrival order, first fit (a
+ *                             request that does not fit does not block smaller
+ *                             ones behind it); everything that fits runs at once.
+ *   GPE_SERVE_SHORTEST_FIRST  queued requests in order of their work
+ *                             (12 n^3 steps, ties by arrival), first fit.
+ * With max_active_nodes > 0 a request is admitted only while the resident
+ * solves hold fewer than that many nodes in total (always when none is
+ * resident).  Every resident solve advances each tick, so an uncapped FIFO
+ * server under overload shares the GPU among all of them and finishes few;
+ * a cap near the node count that saturates the GPU keeps per-solve progress
+ * high.
  *
  * Times are seconds on the server's clock (gpe_server_now).
  */
@@ -66,6 +78,15 @@ gpe_status gpe_server_poll(gpe_server *s, gpe_completion *out, uint32_t max, uin
 /* Final state of a finished keep_result solve (odd may be NULL); frees it. */
 gpe_status gpe_server_fetch(gpe_server *s, uint64_t ticket, double *even, double *odd);
 gpe_status gpe_server_get_stats(const gpe_server *s, gpe_server_stats *st);
+
+typedef enum gpe_serve_policy {
+    GPE_SERVE_FIFO = 0,
+    GPE_SERVE_SHORTEST_FIRST = 1
+} gpe_serve_policy;
+/* Admission policy and cap (see above); the default is GPE_SERVE_FIFO, no cap. */
+gpe_status gpe_server_set_policy(gpe_server *s, uint32_t policy, uint64_t max_active_nodes);
+/* Arrival time of the oldest queued request (server clock), or -1 when the queue is empty. */
+double gpe_server_oldest_queued(const gpe_server *s);
 
 #ifdef __cplusplus
 }

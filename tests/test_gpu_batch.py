@@ -116,6 +116,8 @@ def test_alone_equals_in_mix(precision):
     announce(f"alone vs mix {precision}")
     cfgs = _mixed_configs()
     mix = Batch(precision)
+    if precision == "fp64":
+        mix.set_launch_mode("stream")  # the mix without graphs against graph-launched instances alone
     ids = [mix.add(instance_config(c["dims"], c["h"], c["g"], c["dt"], c["steps"], c["mode"], c["trap_ratio"],
                                    seed=c["seed"], mu0=5.0)) for c in cfgs]
     mix.run(7, 10)
@@ -155,12 +157,15 @@ def test_device_gaussian_init_and_budget():
 
 
 def test_graph_replays_match_eager_chunks():
-    """One captured chunk replayed several times == the same steps in other chunkings."""
+    """One captured chunk replayed several times == the same steps in other chunkings, and ==
+    the chunks enqueued directly on the stream without a graph."""
     announce("graph replay consistency")
     cfg = instance_config((30, 20, 25), 0.3, 800.0, 6e-4, 60, "real", 1.3, seed=3, mu0=4.0)
     outs = []
-    for chunk, chunks in ((10, 6), (60, 1), (7, 9), (1, 60)):
+    for chunk, chunks, mode in ((10, 6, "graph"), (60, 1, "graph"), (7, 9, "graph"), (1, 60, "graph"),
+                                (10, 6, "stream"), (7, 9, "stream")):
         B = Batch("fp64")
+        B.set_launch_mode(mode)
         i = B.add(cfg)
         for _ in range(chunks):
             B.run(chunk, 1)

@@ -77,3 +77,34 @@ def test_run_serving_short():
     res = run_serving(w, duration=3.0, precision="fp32", hbm_budget_bytes=8 * 10 ** 9, chunk_steps=25)
     assert res["submitted"] >= 1 and res["served_site_steps_per_s"] > 0
     assert res["completed"] >= 1 and res["p50_s"] is not None and res["p99_s"] >= res["p50_s"]
+
+
+@pytest.mark.parametrize("policy", ["shortest_first", "fifo"])
+def test_policy_order_and_node_cap(policy):
+    """Long request A arrives before short B and C; the cap fits B + C but not A beside them.
+    shortest_first admits B and C first and A only after both finish; fifo admits A alone first."""
+    announce(f"serve policy {policy}")
+    a = Request(0.0, 24, 400, 500.0, 1.0, "real", 1)
+    b = Request(0.0, 16, 30, 500.0, 1.0, "real", 2)
+    c = Request(0.0, 20, 60, 500.0, 1.0, "imag", 3)
+    cap = 12 * 24 ** 3
+    srv = Server("fp32", 0, chunk_steps=10, policy=policy, max_active_nodes=cap)
+    ta, tb, tc = (srv.submit(r, arrival=0.0) for r in (a, b, c))
+    assert srv.oldest_queued() == 0.0
+    srv.tick()
+    st = srv.stats()
+    assert st["active"] == (2 if policy == "shortest_first" else 1)
+    done = {}
+    while len(done) < 3:
+        srv.tick()
+        for d in srv.poll():
+            done[d["ticket"]] = d
+        assert srv.stats()["active"] <= 2
+    assert srv.oldest_queued() is None
+    if policy == "shortest_first":
+        assert done[ta]["admitted"] >= max(done[tb]["finished"], done[tc]["finished"])
+    else:
+        assert done[ta]["finished"] <= min(done[tb]["admitted"], done[tc]["admitted"])
+    with pytest.raises(Exception):
+        srv.set_policy("lottery")
+    srv.close()

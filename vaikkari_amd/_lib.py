@@ -199,6 +199,7 @@ SIGNATURES = [
     ("gpe_batch_active", C.c_int32, [_P]),
     ("gpe_batch_resident_bytes", _U64, [_P]),
     ("gpe_batch_set_tile", _I, [_P, _U32]),
+    ("gpe_batch_set_launch_mode", _I, [_P, _U32]),
     ("gpe_server_create", _I, [_U32, _U64, _U32, C.POINTER(_P)]),
     ("gpe_server_destroy", None, [_P]),
     ("gpe_server_now", _D, [_P]),
@@ -207,6 +208,8 @@ SIGNATURES = [
     ("gpe_server_poll", _I, [_P, C.POINTER(GpeCompletion), _U32, C.POINTER(_U32)]),
     ("gpe_server_fetch", _I, [_P, _U64, _DP, _DP]),
     ("gpe_server_get_stats", _I, [_P, C.POINTER(GpeServerStats)]),
+    ("gpe_server_set_policy", _I, [_P, _U32, _U64]),
+    ("gpe_server_oldest_queued", _D, [_P]),
     ("gpe_status_string", C.c_char_p, [_I]),
     ("gpe_last_error", C.c_char_p, []),
     ("gpe_bcc_positions", None, [_I32P]),

@@ -59,6 +59,10 @@ class Batch:
     def set_tile(self, waves: int) -> None:
         check(lib.gpe_batch_set_tile(self._h, int(waves)), "gpe_batch_set_tile")
 
+    def set_launch_mode(self, mode: str) -> None:
+        """"graph": replay a captured hipGraph per chunk (default); "stream": enqueue directly."""
+        check(lib.gpe_batch_set_launch_mode(self._h, {"graph": 0, "stream": 1}[mode]), "gpe_batch_set_launch_mode")
+
     def instance_bytes(self, cfg: _lib.GpeInstanceConfig) -> int:
         return int(lib.gpe_batch_instance_bytes(self._h, C.byref(cfg)))
 

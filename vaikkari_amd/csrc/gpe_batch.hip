@@ -1,16 +1,17 @@
-// gpe_batch.hip -- many independent condensates stepped by shared persistent
-// launches (include/vaikkari_gpe_batch.h).
+// gpe_batch.hip -- many independent condensates stepped by shared launches
+// (include/vaikkari_gpe_batch.h).
 //
 // Device side: every active instance has a descriptor (layout, trap,
 // coefficients, tile grid, field pointers, steps left, ping-pong parity).
-// Per step persistent kernels run over fixed grids: the two in-place
-// leapfrog half-updates (gather pass) over all tiles of all real-time
-// instances, the z-march Euler update over all tiles of all imaginary-time
-// instances, and the per-instance FP64 norm reduction of the latter.  Work items (instance,
-// tile) are laid out instance-major; each round of the persistent loop hands
-// each XCD a contiguous range of them.  An "advance" kernel at the end of a
-// chunk updates steps-left and parity on the device, so a captured chunk
-// graph can be replayed without host traffic.
+// Per step, shared kernels run over the work items (instance, tile) of all
+// instances: the two in-place leapfrog half-updates (gather pass) over the
+// real-time instances, the z-march Euler update over the imaginary-time
+// ones, and the per-instance FP64 norm reduction of the latter.  Work items
+// are laid out instance-major, one block each (the kernels loop when there
+// are more than the grid cap); a table maps each work item to its instance,
+// so a block finds its descriptor with one load.  An "advance" kernel at the
+// end of a chunk updates steps-left and parity on the device, so a captured
+// chunk graph can be replayed without host traffic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,7 +20,6 @@
 #include <new>
 #include <vector>
 
-#include "gpe_fused.hpp"  // remap_block
 #include "gpe_internal.hpp"
 #include "gpe_hip_check.hpp"
 #include "gpe_kernels.hpp"
@@ -39,9 +39,11 @@ struct InstDesc {
     Grid3 gg;      // real time (gather tiles, in-place leapfrog)
     Grid3 zg;      // imaginary time (z-march tiles)
     int zcz;
-    cplx_t<T> *buf[4];
-    double *partials;
-    double *scal;  // [0] N, [1] read scale
+    // device addresses (see global_ptr): the fields, the per-tile norm
+    // partials and the scalars [0] N, [1] read scale
+    unsigned long long buf[4];
+    unsigned long long partials;
+    unsigned long long scal;
     double w;
     int tile0;     // first work item in its kernel's list
     int ntiles;
@@ -53,41 +55,46 @@ struct BatchHeader {
     int n_real, total_real, n_imag, total_imag;
 };
 
-constexpr int kBatchGrid = 2048;  // persistent blocks per kernel
+// Blocks per batched launch: one per work item, up to this many (the kernels
+// loop over the rest).
+constexpr int kMaxBatchGrid = 1 << 20;
+
+// A pointer read from memory is a generic pointer: the compiler cannot tell
+// that it addresses global memory and accesses it with flat instructions.
+// The descriptors therefore hold the fields' addresses as integers, made into
+// global-address-space pointers here, so the accesses compile to global loads
+// and stores as they do for kernel arguments.
+template <typename P>
+__device__ __forceinline__ P *global_ptr(unsigned long long a) {
+    return (P *)(__attribute__((address_space(1))) P *)a;
+}
 constexpr int kImagTY = 4;
 constexpr int kRealTY = 4;  // default waves per block of the real-time pass
 constexpr int kFinalizeThreads = 256;
 
-template <typename T>
-__device__ __forceinline__ int find_instance(const InstDesc<T> *__restrict__ D, int n, int w) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (D[mid].tile0 <= w) lo = mid;
-        else hi = mid - 1;
-    }
-    return __builtin_amdgcn_readfirstlane(lo);
-}
 
 // One half of a leapfrog step over all tiles of all real-time instances:
 // half 0 updates odd from even, half 1 even from odd (in place, as
 // GPUGrossPitaevskii_BCC.cu:478,480).
 template <typename T, int TY>
 __global__ void __launch_bounds__(64 * TY)
-    k_batch_real(const InstDesc<T> *__restrict__ D, const BatchHeader *__restrict__ hdr, int s, int chunk, int half) {
+    k_batch_real(const InstDesc<T> *__restrict__ D, const int *__restrict__ items, const BatchHeader *__restrict__ hdr,
+                 int s, int chunk, int half) {
     constexpr int FORM = sizeof(T) == 8 ? FORM_REF : FORM_DIFF;
-    const int n = hdr->n_real, total = hdr->total_real;
+    const int total = hdr->total_real;
     for (int base = 0; base < total; base += gridDim.x) {
-        const int w = base + remap_block(blockIdx.x, gridDim.x, 1);
-        if (w >= total) continue;
-        const int q = find_instance(D, n, w);
+        // half 1 walks the work items backwards, starting where half 0 ended
+        // (those fields are still in the caches)
+        const int w = half ? total - 1 - (base + (int)blockIdx.x) : base + (int)blockIdx.x;
+        if (w < 0 || w >= total) continue;
+        const int q = __builtin_amdgcn_readfirstlane(items[w]);
         const InstDesc<T> &d = D[q];
         const long long steps = d.steps_left < chunk ? d.steps_left : chunk;
         if (s >= steps) continue;
         const int lt = w - d.tile0;
         const int bx = lt % d.gg.gx, by = (lt / d.gg.gx) % d.gg.gy, bz = lt / (d.gg.gx * d.gg.gy);
-        cplx_t<T> *next = d.buf[half == 0 ? 1 : 0];
-        const cplx_t<T> *prev = d.buf[half == 0 ? 0 : 1];
+        cplx_t<T> *next = global_ptr<cplx_t<T>>(d.buf[half == 0 ? 1 : 0]);
+        const cplx_t<T> *prev = global_ptr<cplx_t<T>>(d.buf[half == 0 ? 0 : 1]);
         gather_tile<T, FORM, POT_TRAP, MODE_REAL, TY, true>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, nullptr, bx,
                                                            by, bz, d.gg.lsh);
     }
@@ -95,14 +102,16 @@ __global__ void __launch_bounds__(64 * TY)
 
 template <typename T>
 __global__ void __launch_bounds__(64 * kImagTY)
-    k_batch_imag(const InstDesc<T> *__restrict__ D, const BatchHeader *__restrict__ hdr, int s, int chunk) {
+    k_batch_imag(const InstDesc<T> *__restrict__ D, const int *__restrict__ items, const BatchHeader *__restrict__ hdr,
+                 int s, int chunk) {
     constexpr int FORM = sizeof(T) == 8 ? FORM_REF : FORM_DIFF;
     __shared__ double lds[kImagTY];
-    const int n = hdr->n_imag, total = hdr->total_imag;
+    const int total = hdr->total_imag;
     for (int base = 0; base < total; base += gridDim.x) {
-        const int w = base + remap_block(blockIdx.x, gridDim.x, 1);
-        if (w >= total) continue;
-        const int q = find_instance(D, n, w);
+        // odd steps of a chunk walk the work items backwards (start where the last step ended)
+        const int w = (s & 1) ? total - 1 - (base + (int)blockIdx.x) : base + (int)blockIdx.x;
+        if (w < 0 || w >= total) continue;
+        const int q = __builtin_amdgcn_readfirstlane(items[w]);
         const InstDesc<T> &d = D[q];
         const long long steps = d.steps_left < chunk ? d.steps_left : chunk;
         if (s >= steps) continue;
@@ -110,11 +119,13 @@ __global__ void __launch_bounds__(64 * kImagTY)
         const int lt = w - d.tile0;
         const int bx = lt % d.zg.gx, by = (lt / d.zg.gx) % d.zg.gy, bz = lt / (d.zg.gx * d.zg.gy);
         double part[1];
-        part[0] = zmarch_tile<T, FORM, POT_TRAP, MODE_IMAG, kImagTY>(d.buf[1 - p], d.buf[p], nullptr, d.L, d.H,
-                                                                     d.trap, d.dt, d.scal + 1, bx, by, bz, d.zcz,
-                                                                     d.zg.lsh);
+        cplx_t<T> *out = global_ptr<cplx_t<T>>(d.buf[1 - p]);
+        const cplx_t<T> *in = global_ptr<cplx_t<T>>(d.buf[p]);
+        part[0] = zmarch_tile<T, FORM, POT_TRAP, MODE_IMAG, kImagTY>(out, in, nullptr, d.L, d.H, d.trap, d.dt,
+                                                                     global_ptr<double>(d.scal) + 1, bx, by, bz,
+                                                                     d.zcz, d.zg.lsh);
         block_sum<1, kImagTY>(part, lds);
-        if (threadIdx.x == 0 && threadIdx.y == 0) d.partials[lt] = part[0];
+        if (threadIdx.x == 0 && threadIdx.y == 0) global_ptr<double>(d.partials)[lt] = part[0];
         __syncthreads();  // lds is reused by the next work item
     }
 }
@@ -133,7 +144,7 @@ __global__ void __launch_bounds__(kFinalizeThreads)
         if (s >= steps) continue;
         double sum = 0.0, c = 0.0;
         for (int b = t; b < d.ntiles; b += kFinalizeThreads) {
-            const double x = d.partials[b];
+            const double x = global_ptr<double>(d.partials)[b];
             const double u = sum + x;
             c += (fabs(sum) >= fabs(x)) ? ((sum - u) + x) : ((x - u) + sum);
             sum = u;
@@ -146,8 +157,9 @@ __global__ void __launch_bounds__(kFinalizeThreads)
         }
         if (t == 0) {
             const double N = d.w * red[0];
-            d.scal[0] = N;
-            d.scal[1] = 1.0 / sqrt(N);
+            double *sc = global_ptr<double>(d.scal);
+            sc[0] = N;
+            sc[1] = 1.0 / sqrt(N);
         }
         __syncthreads();
     }
@@ -250,11 +262,18 @@ struct gpe_batch {
     void *h_stage = nullptr;  // pinned staging for descriptor uploads
     size_t stage_bytes = 0;
     int n_real = 0, n_imag = 0;
+    int grid_real = 1, grid_imag = 1;  // blocks of the real / imaginary-time launches
+    int grid_real_items = 0;           // real-time work items (offset of the imaginary-time ones)
+    // instance of every work item: real-time items [0, total_real), then imaginary-time ones
+    int *d_items = nullptr;
+    size_t items_cap = 0;
+    std::vector<int> h_items;
     bool desc_dirty = true;
     // captured chunk graph
     hipGraphExec_t gexec = nullptr;
     uint32_t graph_chunk = 0;
     bool graph_dirty = true;
+    bool use_graph = true;  // GPE_BATCH_GRAPH; false: GPE_BATCH_STREAM
     double *obs_partials = nullptr;
     long long obs_cap = 0;
 
@@ -274,6 +293,7 @@ gpe_batch::~gpe_batch() {
     if (d_real) (void)hipFree(d_real);
     if (d_imag) (void)hipFree(d_imag);
     if (d_hdr) (void)hipFree(d_hdr);
+    if (d_items) (void)hipFree(d_items);
     if (h_stage) (void)hipHostFree(h_stage);
     if (obs_partials) (void)hipFree(obs_partials);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -295,7 +315,7 @@ size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, La
     size_t bytes = 2 * (size_t)L.elems * elem;  // even and odd (real time) or ping-pong pair (imaginary time)
     int tiles = 0;
     if (c->mode == GPE_REAL_TIME) {
-        Grid3 G;
+        Grid3 G{};
         G.lsh = row_lanes_log2(L.nx);
         G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
         G.gy = (L.ny + real_ty * (64 >> G.lsh) - 1) / (real_ty * (64 >> G.lsh));
@@ -305,7 +325,7 @@ size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, La
         if (gg) *gg = G;
     } else {
         const int cz = c->zlayers ? (int)c->zlayers : kDefaultImagCZ;
-        Grid3 G;
+        Grid3 G{};
         G.lsh = row_lanes_log2(L.nx);
         G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
         G.gy = (L.ny + kImagTY * (64 >> G.lsh) - 1) / (kImagTY * (64 >> G.lsh));
@@ -333,9 +353,9 @@ void fill_desc(InstDesc<T> &d, const BInst &b) {
     d.gg = b.gg;
     d.zg = b.zg;
     d.zcz = b.zcz;
-    for (int f = 0; f < 4; f++) d.buf[f] = (cplx_t<T> *)b.buf[f];
-    d.partials = b.partials;
-    d.scal = b.scal;
+    for (int f = 0; f < 4; f++) d.buf[f] = (unsigned long long)(uintptr_t)b.buf[f];
+    d.partials = (unsigned long long)(uintptr_t)b.partials;
+    d.scal = (unsigned long long)(uintptr_t)b.scal;
     d.w = b.co.w;
     d.ntiles = b.ntiles;
     d.steps_left = (long long)b.steps_left;
@@ -388,12 +408,30 @@ gpe_status upload_desc(gpe_batch *B) {
     H->total_real = tr;
     H->n_imag = ni;
     H->total_imag = ti;
+    B->h_items.assign((size_t)std::max(tr + ti, 1), 0);
+    for (int q = 0; q < nr; q++)
+        std::fill(B->h_items.begin() + R[q].tile0, B->h_items.begin() + R[q].tile0 + R[q].ntiles, q);
+    for (int q = 0; q < ni; q++)
+        std::fill(B->h_items.begin() + tr + I[q].tile0, B->h_items.begin() + tr + I[q].tile0 + I[q].ntiles, q);
+    if (B->h_items.size() > B->items_cap) {
+        if (B->d_items) GPE_HIP_TRY(hipFree(B->d_items));
+        B->d_items = nullptr;
+        B->items_cap = 0;
+        const size_t cap = std::max<size_t>(B->h_items.size() * 2, 4096);
+        GPE_HIP_TRY(hipMalloc(&B->d_items, cap * sizeof(int)));
+        B->items_cap = cap;
+    }
+    GPE_HIP_TRY(hipMemcpyAsync(B->d_items, B->h_items.data(), B->h_items.size() * sizeof(int),
+                               hipMemcpyHostToDevice, B->stream));
     GPE_HIP_TRY(hipMemcpyAsync(B->d_real, R, (size_t)std::max(nr, 1) * dsz, hipMemcpyHostToDevice, B->stream));
     GPE_HIP_TRY(hipMemcpyAsync(B->d_imag, I, (size_t)std::max(ni, 1) * dsz, hipMemcpyHostToDevice, B->stream));
     GPE_HIP_TRY(hipMemcpyAsync(B->d_hdr, H, sizeof(BatchHeader), hipMemcpyHostToDevice, B->stream));
     GPE_HIP_TRY(hipStreamSynchronize(B->stream));
     B->n_real = nr;
     B->n_imag = ni;
+    B->grid_real_items = tr;
+    B->grid_real = std::max(1, std::min(tr, kMaxBatchGrid));
+    B->grid_imag = std::max(1, std::min(ti, kMaxBatchGrid));
     B->desc_dirty = false;
     return GPE_OK;
 }
@@ -407,22 +445,22 @@ void enqueue_chunk(gpe_batch *B, uint32_t chunk, hipStream_t st) {
             for (int half = 0; half < 2; half++) {
                 switch (B->real_ty) {
                     case 2:
-                        hipLaunchKernelGGL((k_batch_real<T, 2>), dim3(kBatchGrid), dim3(64, 2), 0, st, R, B->d_hdr,
+                        hipLaunchKernelGGL((k_batch_real<T, 2>), dim3(B->grid_real), dim3(64, 2), 0, st, R, B->d_items, B->d_hdr,
                                            (int)s, (int)chunk, half);
                         break;
                     case 8:
-                        hipLaunchKernelGGL((k_batch_real<T, 8>), dim3(kBatchGrid), dim3(64, 8), 0, st, R, B->d_hdr,
+                        hipLaunchKernelGGL((k_batch_real<T, 8>), dim3(B->grid_real), dim3(64, 8), 0, st, R, B->d_items, B->d_hdr,
                                            (int)s, (int)chunk, half);
                         break;
                     default:
-                        hipLaunchKernelGGL((k_batch_real<T, 4>), dim3(kBatchGrid), dim3(64, 4), 0, st, R, B->d_hdr,
+                        hipLaunchKernelGGL((k_batch_real<T, 4>), dim3(B->grid_real), dim3(64, 4), 0, st, R, B->d_items, B->d_hdr,
                                            (int)s, (int)chunk, half);
                 }
             }
         }
         if (B->n_imag > 0) {
-            hipLaunchKernelGGL((k_batch_imag<T>), dim3(kBatchGrid), dim3(64, kImagTY), 0, st, I, B->d_hdr, (int)s,
-                               (int)chunk);
+            hipLaunchKernelGGL((k_batch_imag<T>), dim3(B->grid_imag), dim3(64, kImagTY), 0, st, I,
+                               B->d_items + B->grid_real_items, B->d_hdr, (int)s, (int)chunk);
             hipLaunchKernelGGL((k_batch_finalize<T>), dim3(std::min(B->n_imag, 1024)), dim3(kFinalizeThreads), 0, st,
                                I, B->d_hdr, (int)s, (int)chunk);
         }
@@ -443,7 +481,12 @@ gpe_status run_t(gpe_batch *B, uint32_t chunk, uint32_t chunks, double *elapsed_
         if (elapsed_ms) *elapsed_ms = 0.0;
         return GPE_OK;
     }
-    if (B->graph_dirty || B->graph_chunk != chunk || !B->gexec) {
+    if (!B->use_graph) {
+        if (elapsed_ms) GPE_HIP_TRY(hipEventRecord(B->ev0, B->stream));
+        for (uint32_t c = 0; c < chunks; c++) enqueue_chunk<T>(B, chunk, B->stream);
+        GPE_HIP_TRY(hipGetLastError());
+        if (elapsed_ms) GPE_HIP_TRY(hipEventRecord(B->ev1, B->stream));
+    } else if (B->graph_dirty || B->graph_chunk != chunk || !B->gexec) {
         if (B->gexec) {
             (void)hipGraphExecDestroy(B->gexec);
             B->gexec = nullptr;
@@ -462,9 +505,11 @@ gpe_status run_t(gpe_batch *B, uint32_t chunk, uint32_t chunks, double *elapsed_
         B->graph_chunk = chunk;
         B->graph_dirty = false;
     }
-    if (elapsed_ms) GPE_HIP_TRY(hipEventRecord(B->ev0, B->stream));
-    for (uint32_t c = 0; c < chunks; c++) GPE_HIP_TRY(hipGraphLaunch(B->gexec, B->stream));
-    if (elapsed_ms) GPE_HIP_TRY(hipEventRecord(B->ev1, B->stream));
+    if (B->use_graph) {
+        if (elapsed_ms) GPE_HIP_TRY(hipEventRecord(B->ev0, B->stream));
+        for (uint32_t c = 0; c < chunks; c++) GPE_HIP_TRY(hipGraphLaunch(B->gexec, B->stream));
+        if (elapsed_ms) GPE_HIP_TRY(hipEventRecord(B->ev1, B->stream));
+    }
     GPE_HIP_TRY(hipStreamSynchronize(B->stream));
     if (elapsed_ms) {
         float ms = 0.0f;
@@ -787,6 +832,16 @@ int32_t gpe_batch_active(const gpe_batch *B) {
 }
 
 uint64_t gpe_batch_resident_bytes(const gpe_batch *B) { return B ? B->resident : 0; }
+
+gpe_status gpe_batch_set_launch_mode(gpe_batch *B, uint32_t mode) {
+    if (!B || mode > GPE_BATCH_STREAM) {
+        set_error("gpe_batch_set_launch_mode: unknown mode");
+        return GPE_EINVAL;
+    }
+    B->use_graph = mode == GPE_BATCH_GRAPH;
+    B->graph_dirty = true;
+    return GPE_OK;
+}
 
 gpe_status gpe_batch_set_tile(gpe_batch *B, uint32_t waves) {
     if (!B || (waves != 2 && waves != 4 && waves != 8)) {

@@ -147,6 +147,7 @@ __device__ __forceinline__ cplx_t<T> cscale(cplx_t<T> a, T s) {
 struct Grid3 {
     int gx, gy, gz, remap;
     int lsh;  // log2 of the lanes one x-row of cells takes (6: a wave per row; 5, 4: 2 or 4 rows per wave)
+    int rev;  // 1: tiles in descending order (the pass starts where the previous one ended)
 };
 
 // Lanes per x-row for an nx-cell grid: the width in {64, 32, 16} with the
@@ -170,6 +171,7 @@ __device__ __forceinline__ void block_coords(const Grid3 &G, int &bx, int &by, i
         const int q = nb >> 3, r = nb & 7, xcd = lin & 7, slot = lin >> 3;
         w = (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
     }
+    if (G.rev) w = nb - 1 - w;
     bx = w % G.gx;
     const int t = w / G.gx;
     by = t % G.gy;

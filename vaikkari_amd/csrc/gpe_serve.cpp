@@ -1,6 +1,7 @@
 // gpe_serve.cpp -- continuous-batching solve server over the batch engine
 // (include/vaikkari_gpe_serve.h).  Host-side scheduling only; all device
 // work goes through gpe_batch_*.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <deque>
@@ -37,6 +38,9 @@ struct Finished {
 struct gpe_server {
     gpe_batch *batch = nullptr;
     uint32_t chunk = 50;
+    uint32_t policy = GPE_SERVE_FIFO;
+    uint64_t max_active_nodes = 0;  // 0: no cap
+    uint64_t active_nodes = 0;
     std::chrono::steady_clock::time_point t0;
     uint64_t next_ticket = 1;
     std::deque<Pending> queue;
@@ -111,21 +115,39 @@ gpe_status gpe_server_submit(gpe_server *s, const gpe_request *r, double arrival
     return GPE_OK;
 }
 
+static uint64_t request_nodes(const gpe_request &r) { return 12ull * r.n * r.n * r.n; }
+static double request_work(const gpe_request &r) { return (double)request_nodes(r) * (double)r.steps; }
+
 gpe_status gpe_server_tick(gpe_server *s, uint32_t *completed) {
     if (!s) return GPE_EINVAL;
-    // admission, first fit in arrival order
-    for (auto it = s->queue.begin(); it != s->queue.end();) {
-        const gpe_instance_config c = to_instance(it->req);
+    // Admission: candidates in policy order, first fit within the HBM budget
+    // and the node cap.
+    std::vector<size_t> order(s->queue.size());
+    for (size_t q = 0; q < order.size(); q++) order[q] = q;
+    if (s->policy == GPE_SERVE_SHORTEST_FIRST)
+        std::stable_sort(order.begin(), order.end(), [s](size_t a, size_t b) {
+            return request_work(s->queue[a].req) < request_work(s->queue[b].req);
+        });
+    std::vector<char> taken(s->queue.size(), 0);
+    for (size_t q : order) {
+        const Pending &p = s->queue[q];
+        const uint64_t nodes = request_nodes(p.req);
+        if (s->max_active_nodes && s->active_nodes > 0 && s->active_nodes + nodes > s->max_active_nodes) continue;
+        const gpe_instance_config c = to_instance(p.req);
         int32_t id = -1;
         const gpe_status st = gpe_batch_add(s->batch, &c, &id);
-        if (st == GPE_EBUSY) {
-            ++it;
-            continue;
-        }
+        if (st == GPE_EBUSY) continue;
         if (st != GPE_OK) return st;
-        s->running.push_back({it->ticket, it->req, it->arrival, gpe_server_now(s), id});
+        s->running.push_back({p.ticket, p.req, p.arrival, gpe_server_now(s), id});
+        s->active_nodes += nodes;
         s->st.admitted++;
-        it = s->queue.erase(it);
+        taken[q] = 1;
+    }
+    if (!s->queue.empty()) {
+        std::deque<Pending> rest;
+        for (size_t q = 0; q < s->queue.size(); q++)
+            if (!taken[q]) rest.push_back(s->queue[q]);
+        s->queue.swap(rest);
     }
     uint32_t ndone = 0;
     if (!s->running.empty()) {
@@ -158,6 +180,7 @@ gpe_status gpe_server_tick(gpe_server *s, uint32_t *completed) {
             cpl.norm = obs[0];
             cpl.energy = obs[1] + obs[2] + 0.5 * obs[3];
             s->done.push_back(cpl);
+            s->active_nodes -= request_nodes(r.req);
             if (r.req.keep_result) {
                 s->kept[r.ticket] = {r.id, r.req};
             } else {
@@ -195,6 +218,23 @@ gpe_status gpe_server_fetch(gpe_server *s, uint64_t ticket, double *even, double
     st = gpe_batch_remove(s->batch, it->second.id);
     s->kept.erase(it);
     return st;
+}
+
+gpe_status gpe_server_set_policy(gpe_server *s, uint32_t policy, uint64_t max_active_nodes) {
+    if (!s || policy > GPE_SERVE_SHORTEST_FIRST) {
+        gpe::set_error("gpe_server_set_policy: unknown policy");
+        return GPE_EINVAL;
+    }
+    s->policy = policy;
+    s->max_active_nodes = max_active_nodes;
+    return GPE_OK;
+}
+
+double gpe_server_oldest_queued(const gpe_server *s) {
+    if (!s || s->queue.empty()) return -1.0;
+    double t = s->queue.front().arrival;
+    for (const Pending &p : s->queue) t = std::min(t, p.arrival);
+    return t;
 }
 
 gpe_status gpe_server_get_stats(const gpe_server *s, gpe_server_stats *out) {

@@ -131,7 +131,7 @@ hipError_t launch_fused(int tile, cplx_t<T> *eo, cplx_t<T> *oo, const cplx_t<T> 
 
 Grid3 grid_of(int kernel, int tile, const Layout &L) {
     const TileShape t = tile_shape(kernel, tile);
-    Grid3 G;
+    Grid3 G{};
     G.lsh = row_lanes_log2(L.nx);
     const int lw = 1 << G.lsh, rows = t.ty * (64 >> G.lsh);
     G.gx = (L.nx + lw - 1) / lw;
@@ -160,8 +160,9 @@ void launch_zmarch(cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const L
 template <typename T, int FORM, int POT, int MODE>
 hipError_t launch_update(int kernel, int tile, cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const Layout &L,
                          const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, const double *scale,
-                         double *partials, hipStream_t st) {
-    const Grid3 G = grid_of(kernel, tile, L);
+                         double *partials, hipStream_t st, bool reverse = false) {
+    Grid3 G = grid_of(kernel, tile, L);
+    G.rev = reverse ? 1 : 0;
     const TileShape t = tile_shape(kernel, tile);
 #define GPE_ARGS next, prev, pot, L, H, trap, dt, scale, partials, G, st
     if (kernel == GPE_KERNEL_ZMARCH) {
@@ -457,14 +458,16 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
     } else if (cfg.mode == GPE_REAL_TIME) {
         C *ev = (C *)field[ie], *od = (C *)field[io];
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
+            // The even update walks the tiles backwards: it starts on the layers
+            // the odd update touched last, which are still in the caches.
             if (arr) {
                 e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(kernel, tile, od, ev, p, L, H, trap, co.dt, nullptr, nullptr, stream);
                 if (e == hipSuccess)
-                    e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(kernel, tile, ev, od, p, L, H, trap, co.dt, nullptr, nullptr, stream);
+                    e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(kernel, tile, ev, od, p, L, H, trap, co.dt, nullptr, nullptr, stream, true);
             } else {
                 e = launch_update<T, FORM, POT_TRAP, MODE_REAL>(kernel, tile, od, ev, p, L, H, trap, co.dt, nullptr, nullptr, stream);
                 if (e == hipSuccess)
-                    e = launch_update<T, FORM, POT_TRAP, MODE_REAL>(kernel, tile, ev, od, p, L, H, trap, co.dt, nullptr, nullptr, stream);
+                    e = launch_update<T, FORM, POT_TRAP, MODE_REAL>(kernel, tile, ev, od, p, L, H, trap, co.dt, nullptr, nullptr, stream, true);
             }
             nl += 2;
         }
@@ -476,10 +479,11 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
         const int tile = this->kernel == GPE_KERNEL_FUSED ? 0 : this->tile;
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
             C *in = (C *)field[ie], *out = (C *)field[io];
+            const bool rev = (steps_done + s) & 1;  // alternate directions: start where the last step ended
             if (arr)
-                e = launch_update<T, FORM, POT_ARRAY, MODE_IMAG>(kernel, tile, out, in, p, L, H, trap, co.dt, scal + 1, partials, stream);
+                e = launch_update<T, FORM, POT_ARRAY, MODE_IMAG>(kernel, tile, out, in, p, L, H, trap, co.dt, scal + 1, partials, stream, rev);
             else
-                e = launch_update<T, FORM, POT_TRAP, MODE_IMAG>(kernel, tile, out, in, p, L, H, trap, co.dt, scal + 1, partials, stream);
+                e = launch_update<T, FORM, POT_TRAP, MODE_IMAG>(kernel, tile, out, in, p, L, H, trap, co.dt, scal + 1, partials, stream, rev);
             if (e != hipSuccess) break;
             hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(256), 0, stream, partials, update_blocks(kernel, tile, L), co.w, 1, scal);
             e = hipGetLastError();

@@ -2,6 +2,7 @@
 
     python -m vaikkari_amd.profile_summary stats <kernel-trace db>
     python -m vaikkari_amd.profile_summary traffic <FETCH_SIZE db> <WRITE_SIZE db>
+    python -m vaikkari_amd.profile_summary traffic-json <FETCH_SIZE db> <WRITE_SIZE db> <out.json> [workload]
 
 ``stats`` prints calls / total / average duration per kernel.  ``traffic``
 prints HBM bytes per dispatch for every kernel from the FETCH_SIZE and
@@ -72,9 +73,31 @@ def traffic_table(db_fetch: str, db_write: str, copy_bytes: float = float(1 << 3
     return kf, kw, rows
 
 
+def key(name: str) -> str:
+    """Kernel name without spaces, e.g. k_gather<double,0,1,0,4,true>."""
+    return short(name).replace(" ", "")
+
+
+def traffic_json(db_fetch: str, db_write: str, out: str, workload: str = ""):
+    """The traffic table as JSON (bytes per dispatch per kernel), the form bench.py reads."""
+    import json
+
+    kf, kw, rows = traffic_table(db_fetch, db_write)
+    doc = {"workload": workload,
+           "calibration": {"kernel": "k_copy_stream 1 GiB", "fetch_scale": kf, "write_scale": kw},
+           "kernels": {}}
+    for name, grid, fb, wb, n in rows:
+        doc["kernels"].setdefault(key(name), {"grid": grid, "read_bytes": fb, "write_bytes": wb, "dispatches": n})
+    with open(out, "w") as fh:
+        json.dump(doc, fh, indent=1)
+    return doc
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "stats":
         stats(argv[1])
+    elif len(argv) >= 4 and argv[0] == "traffic-json":
+        traffic_json(argv[1], argv[2], argv[3], " ".join(argv[4:]))
     elif len(argv) >= 3 and argv[0] == "traffic":
         kf, kw, rows = traffic_table(argv[1], argv[2])
         print(f"calibration on the 1 GiB copy kernel: FETCH_SIZE x {kf:.4f}, WRITE_SIZE x {kw:.4f}")

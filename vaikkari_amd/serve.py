@@ -70,11 +70,23 @@ def poisson_workload(seed: int, rate: float, duration: float, n_min: int = 16, n
     return out
 
 
+POLICIES = {"fifo": 0, "shortest_first": 1}
+
+
 class Server:
-    def __init__(self, precision: str = "fp32", hbm_budget_bytes: int = 200 * 10 ** 9, chunk_steps: int = 50):
+    def __init__(self, precision: str = "fp32", hbm_budget_bytes: int = 200 * 10 ** 9, chunk_steps: int = 50,
+                 policy: str = "fifo", max_active_nodes: int = 0):
         self._h = C.c_void_p()
         check(lib.gpe_server_create(PRECISIONS[precision], int(hbm_budget_bytes), int(chunk_steps),
                                     C.byref(self._h)), "gpe_server_create")
+        self.set_policy(policy, max_active_nodes)
+
+    def set_policy(self, policy: str, max_active_nodes: int = 0) -> None:
+        check(lib.gpe_server_set_policy(self._h, POLICIES[policy], int(max_active_nodes)), "gpe_server_set_policy")
+
+    def oldest_queued(self) -> Optional[float]:
+        t = lib.gpe_server_oldest_queued(self._h)
+        return None if t < 0 else t
 
     def close(self):
         if self._h:
@@ -133,15 +145,19 @@ class Server:
 
 
 def run_serving(workload: List[Request], duration: float, precision: str = "fp32",
-                hbm_budget_bytes: int = 200 * 10 ** 9, chunk_steps: int = 50, log=None) -> dict:
+                hbm_budget_bytes: int = 200 * 10 ** 9, chunk_steps: int = 50, log=None, policy: str = "fifo",
+                max_active_nodes: int = 0, sample_every: float = 10.0) -> dict:
     """Replays the workload against the wall clock for `duration` seconds: requests are
-    submitted when their arrival time has passed, the server ticks continuously."""
-    srv = Server(precision, hbm_budget_bytes, chunk_steps)
+    submitted when their arrival time has passed, the server ticks continuously.  Every
+    `sample_every` seconds the queue depth, resident solves and bytes, completions and the age
+    of the oldest queued request are recorded (and logged)."""
+    srv = Server(precision, hbm_budget_bytes, chunk_steps, policy, max_active_nodes)
     t_start = srv.now()
     pending = sorted(workload, key=lambda r: r.arrival)
     k = 0
     done = []
-    last_log = t_start
+    samples = []
+    last_sample = t_start
     while True:
         now = srv.now() - t_start
         if now >= duration:
@@ -151,11 +167,16 @@ def run_serving(workload: List[Request], duration: float, precision: str = "fp32
             k += 1
         srv.tick()
         done.extend(srv.poll())
-        if log and srv.now() - last_log > 10.0:
+        if srv.now() - last_sample > sample_every:
             st = srv.stats()
-            log(f"serve t={now:.0f}s completed={st['completed']} active={st['active']} queued={st['queued']} "
-                f"resident={st['resident_bytes'] / 1e9:.1f} GB")
-            last_log = srv.now()
+            oldest = srv.oldest_queued()
+            samples.append({"t": round(now, 2), "queued": int(st["queued"]), "active": int(st["active"]),
+                            "completed": int(st["completed"]), "resident_GB": st["resident_bytes"] / 1e9,
+                            "oldest_queued_age_s": None if oldest is None else srv.now() - oldest})
+            if log:
+                log(f"serve t={now:.0f}s completed={st['completed']} active={st['active']} queued={st['queued']} "
+                    f"resident={st['resident_bytes'] / 1e9:.1f} GB")
+            last_sample = srv.now()
     wall = srv.now() - t_start
     st = srv.stats()
     lat = np.array([d["finished"] - d["arrival"] for d in done]) if done else np.array([])
@@ -172,6 +193,10 @@ def run_serving(workload: List[Request], duration: float, precision: str = "fp32
         "completed_norm_range": [float(min(d["norm"] for d in done)), float(max(d["norm"] for d in done))]
         if done else None,
         "ticks": int(st["ticks"]),
+        "policy": policy, "max_active_nodes": int(max_active_nodes),
+        "mean_s": float(lat.mean()) if lat.size else None,
+        "completed_work_site_steps": float(sum(d["node_steps"] for d in done)),
+        "samples": samples,
     }
     srv.close()
     return res
