@@ -64,7 +64,7 @@ def measured_traffic(nodes, precision, info):
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic_128cubed.md), when the
     workload and kernel variant are the ones profiled; else None."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_128cubed.json")
-    if nodes != 25165824 or info["kernel"] != "gather" or info["tile"] != 1 or not os.path.exists(path):
+    if nodes != 25165824 or info["kernel"] != "gather" or info["tile"] != 11 or not os.path.exists(path):
         return None
     key = "k_gather<double,0,1,0,4,true>" if precision == "fp64" else "k_gather<float,1,1,0,4,true>"
     try:

@@ -148,7 +148,9 @@ gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info);
  * (numbered 1..n; 0 selects the family's default), gpe_variant_shape the
  * shape {waves per block, z layers per block, flags} of tile 1..n with
  * flags bit 0 = XCD-contiguous block order, bit 1 = streaming
- * (non-temporal) access to the updated field.
+ * (non-temporal) access to the updated field, bit 2 = each XCD owns a
+ * y-band of every z layer (grids whose block rows divide by 8; others run
+ * in plain order).
  * GPE_KERNEL_FUSED needs two more field buffers (allocated on selection);
  * imaginary-time steps of a fused solver run the z-march kernel. */
 gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile);

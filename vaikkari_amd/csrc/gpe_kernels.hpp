@@ -139,11 +139,14 @@ __device__ __forceinline__ cplx_t<T> cscale(cplx_t<T> a, T s) {
 }
 
 // Logical block coordinates from a 1-D launch.  Workgroups are dispatched
-// round-robin over the 8 XCDs (each with its own L2); with remap on, block
-// lin runs on XCD lin%8 and is given the (lin/8)-th block of that XCD's
-// contiguous share of the (bx fastest, then by, then bz) order, so blocks
-// sharing halo rows and layers share one L2.  The map is a bijection for any
-// block count.
+// round-robin over the 8 XCDs (each with its own L2): block lin runs on XCD
+// lin%8.  remap 0: block lin is tile lin of the (bx fastest, then by, then
+// bz) order.  remap 1: XCD x takes the x-th contiguous eighth of that order,
+// so blocks sharing halo rows and layers share one L2 (a bijection for any
+// block count).  remap 2 (gy a multiple of 8): XCD x owns the y-band
+// [x gy/8, (x+1) gy/8) of every layer and walks it layer by layer, so y-halo
+// rows and z-neighbours stay in one L2 while all XCDs sweep z together.
+// rev walks the layers (remap 2) or the whole order (remap 0, 1) backwards.
 struct Grid3 {
     int gx, gy, gz, remap;
     int lsh;  // log2 of the lanes one x-row of cells takes (6: a wave per row; 5, 4: 2 or 4 rows per wave)
@@ -166,6 +169,14 @@ __host__ __device__ inline int row_lanes_log2(int nx) {
 __device__ __forceinline__ void block_coords(const Grid3 &G, int &bx, int &by, int &bz) {
     const int nb = G.gx * G.gy * G.gz;
     const int lin = blockIdx.x;
+    if (G.remap == 2) {
+        const int br = G.gy >> 3, bs = G.gx * br, xcd = lin & 7, slot = lin >> 3;
+        const int layer = slot / bs, within = slot - layer * bs;
+        bz = G.rev ? G.gz - 1 - layer : layer;
+        by = xcd * br + within / G.gx;
+        bx = within % G.gx;
+        return;
+    }
     int w = lin;
     if (G.remap) {
         const int q = nb >> 3, r = nb & 7, xcd = lin & 7, slot = lin >> 3;

@@ -58,14 +58,17 @@ constexpr int kTY = 4;  // waves per block of the observables kernel
 
 // Tile shapes selectable per solver (gpe_config.tile; the autotuner sweeps
 // them).  ty = waves (rows of 64 cells) per block, cz = layers per z-march
-// block, remap = XCD-contiguous block order.
+// block, remap = block order over the XCDs (see block_coords: 0 plain,
+// 1 contiguous eighths, 2 y-bands).
 struct TileShape {
     int ty, cz, remap, nt;  // nt: streaming (non-temporal) read-modify-write of the updated field
 };
 constexpr TileShape kGatherTiles[] = {{4, 1, 0, 1}, {4, 1, 0, 0}, {4, 1, 1, 0}, {1, 1, 1, 0}, {2, 1, 1, 0},
-                                      {8, 1, 1, 0}, {4, 1, 1, 1}, {2, 1, 0, 1}, {8, 1, 0, 1}, {1, 1, 0, 1}};
+                                      {8, 1, 1, 0}, {4, 1, 1, 1}, {2, 1, 0, 1}, {8, 1, 0, 1}, {1, 1, 0, 1},
+                                      {4, 1, 2, 1}, {2, 1, 2, 1}, {8, 1, 2, 1}, {1, 1, 2, 1}};
 constexpr TileShape kZmarchTiles[] = {{4, 8, 1, 0}, {4, 8, 0, 1}, {1, 8, 1, 0}, {2, 8, 1, 1}, {8, 8, 1, 1},
-                                      {4, 4, 1, 1}, {4, 16, 1, 1}, {2, 16, 1, 0}, {1, 16, 1, 1}, {2, 4, 1, 1}};
+                                      {4, 4, 1, 1}, {4, 16, 1, 1}, {2, 16, 1, 0}, {1, 16, 1, 1}, {2, 4, 1, 1},
+                                      {4, 4, 2, 1}, {8, 8, 2, 1}};
 // fused: ty = waves per block (footprint rows, interior ty-2), cz = layers per block
 constexpr TileShape kFusedTiles[] = {{8, 16, 1, 0}, {4, 16, 1, 0}, {4, 32, 1, 0}, {8, 32, 1, 0}, {8, 8, 1, 0},
                                      {16, 16, 1, 0}, {4, 8, 1, 0}, {8, 16, 0, 0}, {6, 16, 1, 0}, {6, 32, 1, 0}};
@@ -80,6 +83,7 @@ int num_tiles(int kernel) {
 // Tile index used when the caller asks for tile 0.
 int default_tile(int kernel, uint32_t precision) {
     if (kernel == GPE_KERNEL_FUSED) return precision == GPE_FP64 ? 1 : 0;
+    if (kernel == GPE_KERNEL_GATHER) return 10;  // 4 waves, streaming RMW, y-bands per XCD
     return 0;
 }
 
@@ -138,6 +142,7 @@ Grid3 grid_of(int kernel, int tile, const Layout &L) {
     G.gy = (L.ny + rows - 1) / rows;
     G.gz = kernel == GPE_KERNEL_ZMARCH ? (L.nz + t.cz - 1) / t.cz : L.nz;
     G.remap = t.remap;
+    if (G.remap == 2 && G.gy % 8 != 0) G.remap = 0;  // y-bands need gy divisible by the 8 XCDs
     return G;
 }
 
@@ -739,7 +744,7 @@ void gpe_variant_shape(uint32_t kernel, uint32_t tile, int32_t out[3]) {
         const TileShape t = tile_shape((int)kernel, (int)tile - 1);
         out[0] = t.ty;
         out[1] = t.cz;
-        out[2] = t.remap | (t.nt << 1);
+        out[2] = (t.remap == 1 ? 1 : 0) | (t.nt << 1) | (t.remap == 2 ? 4 : 0);
     }
 }
 
