@@ -68,6 +68,28 @@ template <typename P>
 __device__ __forceinline__ P *global_ptr(unsigned long long a) {
     return (P *)(__attribute__((address_space(1))) P *)a;
 }
+
+// Tile of an instance's work item lt.  Every instance's items start at a
+// multiple of 8 and work items go to the 8 XCDs round-robin, so item lt runs
+// on XCD (lt mod 8) (or on XCD 7 - (lt mod 8) when the items are walked
+// backwards).  With remap 2 (block rows divisible by 8) band (lt mod 8) is a
+// y-band of every layer, as in block_coords: each XCD keeps the halo rows of
+// its band in its own L2.  Otherwise tiles go in (bx, by, bz) order.
+__device__ __forceinline__ void item_tile(const Grid3 &G, int lt, int &bx, int &by, int &bz) {
+    if (G.remap == 2) {
+        const int br = G.gy >> 3, bs = G.gx * br, slot = lt >> 3;
+        const int layer = slot / bs, within = slot - layer * bs;
+        bz = layer;
+        by = (lt & 7) * br + within / G.gx;
+        bx = within % G.gx;
+    } else {
+        bx = lt % G.gx;
+        by = (lt / G.gx) % G.gy;
+        bz = lt / (G.gx * G.gy);
+    }
+}
+
+constexpr int round8(int n) { return (n + 7) & ~7; }
 constexpr int kImagTY = 4;
 constexpr int kRealTY = 4;  // default waves per block of the real-time pass
 constexpr int kFinalizeThreads = 256;
@@ -92,7 +114,9 @@ __global__ void __launch_bounds__(64 * TY)
         const long long steps = d.steps_left < chunk ? d.steps_left : chunk;
         if (s >= steps) continue;
         const int lt = w - d.tile0;
-        const int bx = lt % d.gg.gx, by = (lt / d.gg.gx) % d.gg.gy, bz = lt / (d.gg.gx * d.gg.gy);
+        if (lt >= d.ntiles) continue;  // padding to the next multiple of 8
+        int bx, by, bz;
+        item_tile(d.gg, lt, bx, by, bz);
         cplx_t<T> *next = global_ptr<cplx_t<T>>(d.buf[half == 0 ? 1 : 0]);
         const cplx_t<T> *prev = global_ptr<cplx_t<T>>(d.buf[half == 0 ? 0 : 1]);
         gather_tile<T, FORM, POT_TRAP, MODE_REAL, TY, true>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, nullptr, bx,
@@ -117,7 +141,9 @@ __global__ void __launch_bounds__(64 * kImagTY)
         if (s >= steps) continue;
         const int p = (d.parity + s) & 1;
         const int lt = w - d.tile0;
-        const int bx = lt % d.zg.gx, by = (lt / d.zg.gx) % d.zg.gy, bz = lt / (d.zg.gx * d.zg.gy);
+        if (lt >= d.ntiles) continue;  // padding to the next multiple of 8
+        int bx, by, bz;
+        item_tile(d.zg, lt, bx, by, bz);
         double part[1];
         cplx_t<T> *out = global_ptr<cplx_t<T>>(d.buf[1 - p]);
         const cplx_t<T> *in = global_ptr<cplx_t<T>>(d.buf[p]);
@@ -125,7 +151,9 @@ __global__ void __launch_bounds__(64 * kImagTY)
                                                                      global_ptr<double>(d.scal) + 1, bx, by, bz,
                                                                      d.zcz, d.zg.lsh);
         block_sum<1, kImagTY>(part, lds);
-        if (threadIdx.x == 0 && threadIdx.y == 0) global_ptr<double>(d.partials)[lt] = part[0];
+        // partials in logical tile order, as the single solver stores them
+        if (threadIdx.x == 0 && threadIdx.y == 0)
+            global_ptr<double>(d.partials)[bx + d.zg.gx * (by + d.zg.gy * bz)] = part[0];
         __syncthreads();  // lds is reused by the next work item
     }
 }
@@ -320,7 +348,7 @@ size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, La
         G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
         G.gy = (L.ny + real_ty * (64 >> G.lsh) - 1) / (real_ty * (64 >> G.lsh));
         G.gz = L.nz;
-        G.remap = 0;
+        G.remap = G.gy % 8 == 0 ? 2 : 0;
         tiles = G.gx * G.gy * G.gz;
         if (gg) *gg = G;
     } else {
@@ -330,7 +358,7 @@ size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, La
         G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
         G.gy = (L.ny + kImagTY * (64 >> G.lsh) - 1) / (kImagTY * (64 >> G.lsh));
         G.gz = (L.nz + cz - 1) / cz;
-        G.remap = 0;
+        G.remap = G.gy % 8 == 0 ? 2 : 0;
         tiles = G.gx * G.gy * G.gz;
         bytes += (size_t)tiles * sizeof(double) + 8 * sizeof(double);
         if (zg) *zg = G;
@@ -395,12 +423,12 @@ gpe_status upload_desc(gpe_batch *B) {
         if (b.cfg.mode == GPE_REAL_TIME) {
             fill_desc(R[nr], b);
             R[nr].tile0 = tr;
-            tr += b.ntiles;
+            tr += round8(b.ntiles);
             nr++;
         } else {
             fill_desc(I[ni], b);
             I[ni].tile0 = ti;
-            ti += b.ntiles;
+            ti += round8(b.ntiles);
             ni++;
         }
     }
@@ -410,9 +438,9 @@ gpe_status upload_desc(gpe_batch *B) {
     H->total_imag = ti;
     B->h_items.assign((size_t)std::max(tr + ti, 1), 0);
     for (int q = 0; q < nr; q++)
-        std::fill(B->h_items.begin() + R[q].tile0, B->h_items.begin() + R[q].tile0 + R[q].ntiles, q);
+        std::fill(B->h_items.begin() + R[q].tile0, B->h_items.begin() + R[q].tile0 + round8(R[q].ntiles), q);
     for (int q = 0; q < ni; q++)
-        std::fill(B->h_items.begin() + tr + I[q].tile0, B->h_items.begin() + tr + I[q].tile0 + I[q].ntiles, q);
+        std::fill(B->h_items.begin() + tr + I[q].tile0, B->h_items.begin() + tr + I[q].tile0 + round8(I[q].ntiles), q);
     if (B->h_items.size() > B->items_cap) {
         if (B->d_items) GPE_HIP_TRY(hipFree(B->d_items));
         B->d_items = nullptr;
