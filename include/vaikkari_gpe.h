@@ -224,6 +224,13 @@ typedef struct gpe_relax_report {
     double mu;            /* final mu */
     double error;         /* integrateFunctionError of the result */
 } gpe_relax_report;
+/* Threads of gpe_vortex_state_iterate: 0 = automatic (one per hardware
+ * thread, at most 16, for tables of 40000 entries or more, and serial when a
+ * potential function is given), 1 = serial, n = n threads (the potential
+ * function, if any, is then called from all of them).  The result does not
+ * depend on it: rows run as a pipelined wavefront and every sum keeps the
+ * serial order. */
+void gpe_vortex_state_set_threads(gpe_vortex_state *vs, uint32_t threads);
 /* iterateSolution (VortexState.cpp:109-196), without printing.  report may
  * be NULL.  GPE_EUNSTABLE when mu is not finite at the end. */
 gpe_status gpe_vortex_state_iterate(gpe_vortex_state *vs, gpe_rz_potential V, double trap_ratio, void *user,

@@ -108,6 +108,29 @@ def test_relaxation_matches_reference(tmp_path, case):
 
 
 @needs_ref
+@pytest.mark.parametrize("threads", [2, 3, 7])
+def test_parallel_relaxation_matches_reference(tmp_path, threads):
+    """The threaded relaxation (pipelined rows, serial-order sums) gives the reference's bytes,
+    with and without the early stop, for row counts that do not divide by the thread count."""
+    for kappa, g, rng, ratio, iters, eps in ((6, 900.0, (0.0, 8.3, 9.0, 0.1, 0.1), 0.4, 150, 1e-29),
+                                             (3, 2000.0, (0.5, 6.0, 7.0, 0.11, 0.13), 0.3, 4000, 1e-13)):
+        h = ref_state(kappa, g)
+        REF.ref_vs_set_range(h, *rng)
+        REF.ref_vs_iterate(h, ratio, iters, eps)
+        v = VortexState()
+        v.setKappa(kappa)
+        v.setG(g)
+        v.setRange(*rng)
+        v.setThreads(threads)
+        rep = v.iterateSolution(TrapRZ(ratio), iters, eps)
+        assert saved(v, tmp_path, "a") == ref_saved(h, tmp_path, "b")
+        assert rep["error"] == REF.ref_vs_error(h, ratio)
+        if eps > 1e-20:
+            assert rep["converged"] == 1 and rep["iterations"] < iters
+        REF.ref_vs_free(h)
+
+
+@needs_ref
 def test_callable_potential_and_queries_match_reference(tmp_path, state_dat):
     """A caller-supplied V(r,z) equal to the trap gives the trap's bits; initialize from state.dat
     onto another table, then searchMinR / integrateZ / integrateFunctionError agree."""

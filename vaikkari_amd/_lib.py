@@ -227,6 +227,7 @@ SIGNATURES = [
     ("gpe_vortex_state_create", _I, [C.POINTER(_P)]),
     ("gpe_vortex_state_set_kappa", None, [_P, _U32]),
     ("gpe_vortex_state_set_g", None, [_P, _D]),
+    ("gpe_vortex_state_set_threads", None, [_P, _U32]),
     ("gpe_vortex_state_set_dimensions", _I, [_P, _U32, _D, _D, _U32, _D]),
     ("gpe_vortex_state_set_range", _I, [_P, _D, _D, _D, _D, _D]),
     ("gpe_vortex_state_initialize", _I, [_P, _P]),

@@ -11,17 +11,27 @@
 // with second-order differences in r and z, f = 0 on the r = rplug row and
 // beyond the table, and the mirror f(r,-dz) = f(r,+dz) at z = 0.
 //
-// It stays on one host core, serial, on purpose: each update of a sweep
-// reads the values the same sweep has just produced at (r - dr, z) and
-// (r, z - dz) (Gauss-Seidel), and mu and the normalisation of every sweep
-// come from sums whose order fixes the last bits of the result, so the
-// sweep order is the algorithm.  The table holds a few hundred thousand
-// doubles (cache resident); the 3-D field it seeds is what runs on the GPU.
+// Each update of a sweep reads the values the same sweep has just produced
+// at (r - dr, z) and (r, z - dz) (Gauss-Seidel), and mu and the
+// normalisation of every sweep come from sums whose order fixes the last
+// bits of the result, so the sweep order is part of the algorithm.  The
+// table holds a few hundred thousand doubles (cache resident) and stays on
+// the host; the 3-D field it seeds is what runs on the GPU.  Large tables
+// are relaxed by a team of threads without changing a bit of the result:
+// rows go round-robin to the threads, and row i updates column j only after
+// row i-1 has passed it (a pipelined wavefront), so every update reads the
+// same values as in the serial sweep; the terms of the three sums are
+// computed in parallel and added by one thread in the serial order.
 // Compiled with -ffp-contract=off, every expression below is evaluated in
 // the grouping written, which is the reference's.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <memory>
+#include <thread>
+#include <vector>
 
 #include "gpe_internal.hpp"
 #include "vortex_state.hpp"
@@ -133,6 +143,128 @@ void rayleigh_sums(const VortexState &v, const RzOperator &op, const RzPotential
     }
 }
 
+// Relaxation by a team of threads, bit-identical to the serial loop in
+// gpe_vortex_state_iterate (see the file comment).  The calling thread is
+// worker 0; it sums, calls `progress` and decides when to stop.
+struct RelaxTeam {
+    VortexState &v;
+    const RzOperator &op;
+    const RzPotential &V;
+    const int T;                 // workers
+    static constexpr int kChunk = 32;  // columns between progress updates
+    std::unique_ptr<std::atomic<int>[]> done;  // per row: columns finished this sweep
+    std::vector<double> dd, t1, t2;             // per entry: sweep change, <f,f> and <f,Hf> terms
+    std::atomic<int> arrived{0}, phase{0};
+    std::atomic<int> stop{0};
+    double diff = 0.0, prob = 0.0, prob2 = 0.0;
+
+    RelaxTeam(VortexState &vs, const RzOperator &o, const RzPotential &pot, int workers)
+        : v(vs), op(o), V(pot), T(workers), done(new std::atomic<int>[vs.rsize]),
+          dd(vs.f.size()), t1(vs.f.size()), t2(vs.f.size()) {
+        for (uint32_t i = 0; i < v.rsize; i++) done[i].store(i == 0 ? (int)v.zsize : 0);
+    }
+
+    // Sense-reversing barrier over the T workers.
+    void barrier() {
+        const int ph = phase.load(std::memory_order_acquire);
+        if (arrived.fetch_add(1, std::memory_order_acq_rel) == T - 1) {
+            arrived.store(0, std::memory_order_relaxed);
+            phase.store(ph + 1, std::memory_order_release);
+        } else {
+            int spins = 0;
+            while (phase.load(std::memory_order_acquire) == ph)
+                if (++spins > 64) std::this_thread::yield();
+        }
+    }
+
+    void sweep_rows(int t) {
+        for (uint32_t i = 1 + t; i < v.rsize; i += T) {
+            const RzRow q = op.row(i);
+            const size_t base = (size_t)i * v.zsize;
+            for (int j0 = 0; j0 < (int)v.zsize; j0 += kChunk) {
+                const int j1 = std::min((int)v.zsize, j0 + kChunk);
+                int spins = 0;
+                while (done[i - 1].load(std::memory_order_acquire) < j1)
+                    if (++spins > 64) std::this_thread::yield();
+                for (int j = j0; j < j1; j++) {
+                    const size_t ii = base + j;
+                    const double pot = V(q.r, j * v.zstep) + v.g * v.f[ii] * v.f[ii];
+                    const RzNeighbours n = neighbours(v, i, (uint32_t)j, ii);
+                    const double num =
+                        (q.grad * (n.dn_r - n.up_r) - op.cr * (n.dn_r + n.up_r)) - op.cz * (n.dn_z + n.up_z);
+                    const double fresh = num / (v.mu - q.diag - pot);
+                    const double d = v.f[ii] - fresh;
+                    v.f[ii] = fresh;
+                    dd[ii] = d * d * (j == 0 ? q.dv : 2.0 * q.dv);
+                }
+                done[i].store(j1, std::memory_order_release);
+            }
+        }
+    }
+
+    void term_rows(int t) {
+        for (uint32_t i = 1 + t; i < v.rsize; i += T) {
+            const RzRow q = op.row(i);
+            for (uint32_t j = 0; j < v.zsize; j++) {
+                const size_t ii = (size_t)i * v.zsize + j;
+                const double dv = j == 0 ? q.dv : 2.0 * q.dv;
+                const double fij = v.f[ii];
+                const double pot = V(q.r, j * v.zstep) + v.g * fij * fij;
+                const RzNeighbours n = neighbours(v, i, j, ii);
+                t1[ii] = fij * fij * dv;
+                t2[ii] = off_plus((pot + q.diag) * fij, op, q, n) * fij * dv;
+            }
+        }
+    }
+
+    void scale_rows(int t, double scale) {
+        for (uint32_t i = 1 + t; i < v.rsize; i += T) {
+            double *row = v.f.data() + (size_t)i * v.zsize;
+            for (uint32_t j = 0; j < v.zsize; j++) row[j] *= scale;
+            done[i].store(0, std::memory_order_relaxed);
+        }
+    }
+
+    // Returns the iterations run (the last one converged when stop == 2).
+    uint32_t run(int t, uint32_t maxiters, double eps, gpe_relax_progress progress, void *puser) {
+        uint32_t it = 0;
+        for (it = 0; it < maxiters; it++) {
+            sweep_rows(t);
+            barrier();
+            term_rows(t);
+            barrier();
+            if (t == 0) {
+                double a = 0.0, b = 0.0, c = 0.0;
+                for (size_t ii = v.zsize; ii < v.f.size(); ii++) {
+                    a += dd[ii];
+                    b += t1[ii];
+                    c += t2[ii];
+                }
+                diff = a;
+                prob = b;
+                prob2 = c;
+                v.mu = prob2 / prob;
+                if (progress && it % 1000 == 0) progress(it, diff, prob, puser);
+                stop.store(diff < eps ? 2 : 0, std::memory_order_relaxed);
+            }
+            barrier();
+            scale_rows(t, 1.0 / std::sqrt(prob));
+            barrier();
+            if (stop.load(std::memory_order_relaxed)) return it + 1;
+        }
+        return it;
+    }
+};
+
+// Workers for a table of this size: one per hardware thread (at most 16 and
+// one per 4 rows); small tables stay serial.
+int relax_workers(const VortexState &v, uint32_t requested) {
+    if (requested) return (int)std::max<uint32_t>(1, std::min<uint32_t>(requested, std::max<uint32_t>(1, v.rsize - 1)));
+    if ((size_t)v.rsize * v.zsize < 40000) return 1;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    return (int)std::max<uint32_t>(1, std::min<uint32_t>({hw, 16u, (v.rsize - 1) / 4}));
+}
+
 // || (H - mu) f ||^2 (VortexState.cpp:282-311).
 double residual(const VortexState &v, const RzPotential &V) {
     const RzOperator op(v);
@@ -204,6 +336,7 @@ gpe_status gpe_vortex_state_create(gpe_vortex_state **out) {
 }
 
 void gpe_vortex_state_set_kappa(gpe_vortex_state *h, uint32_t kappa) { h->vs.kappa = kappa; }
+void gpe_vortex_state_set_threads(gpe_vortex_state *h, uint32_t threads) { h->threads = threads; }
 void gpe_vortex_state_set_g(gpe_vortex_state *h, double g) { h->vs.g = g; }
 
 gpe_status gpe_vortex_state_set_dimensions(gpe_vortex_state *h, uint32_t rsize, double rstep, double rplug,
@@ -272,16 +405,33 @@ gpe_status gpe_vortex_state_iterate(gpe_vortex_state *h, gpe_rz_potential V, dou
     uint32_t it = 0;
     double diff = 0.0, prob = 0.0, prob2 = 0.0;
     bool converged = false;
-    for (it = 0; it < maxiters; it++) {
-        diff = relax_sweep(v, op, pot);
-        rayleigh_sums(v, op, pot, prob, prob2);
-        v.mu = prob2 / prob;
-        const double scale = 1.0 / std::sqrt(prob);
-        for (size_t ii = v.zsize; ii < v.f.size(); ii++) v.f[ii] *= scale;
-        if (progress && it % 1000 == 0) progress(it, diff, prob, progress_user);
-        if (diff < eps) {
-            converged = true;
-            break;
+    // A caller's potential runs on every worker; in automatic mode it is only
+    // called from this thread.
+    const int workers = (V && h->threads == 0) ? 1 : relax_workers(v, h->threads);
+    if (workers > 1) {
+        RelaxTeam team(v, op, pot, workers);
+        std::vector<std::thread> pool;
+        for (int t = 1; t < workers; t++)
+            pool.emplace_back([&team, t, maxiters, eps] { team.run(t, maxiters, eps, nullptr, nullptr); });
+        it = team.run(0, maxiters, eps, progress, progress_user);
+        for (auto &th : pool) th.join();
+        converged = team.stop.load() == 2;
+        if (converged) it -= 1;  // index of the last iteration, as in the serial loop
+        diff = team.diff;
+        prob = team.prob;
+        prob2 = team.prob2;
+    } else {
+        for (it = 0; it < maxiters; it++) {
+            diff = relax_sweep(v, op, pot);
+            rayleigh_sums(v, op, pot, prob, prob2);
+            v.mu = prob2 / prob;
+            const double scale = 1.0 / std::sqrt(prob);
+            for (size_t ii = v.zsize; ii < v.f.size(); ii++) v.f[ii] *= scale;
+            if (progress && it % 1000 == 0) progress(it, diff, prob, progress_user);
+            if (diff < eps) {
+                converged = true;
+                break;
+            }
         }
     }
     double sum = 0.0;

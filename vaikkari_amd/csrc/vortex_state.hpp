@@ -65,4 +65,5 @@ struct VortexState {
 
 struct gpe_vortex_state {
     gpe::VortexState vs;
+    uint32_t threads = 0;  // relaxation threads: 0 = automatic, 1 = serial
 };

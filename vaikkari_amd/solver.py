@@ -293,6 +293,10 @@ class VortexState:
     def setG(self, g: float):
         lib.gpe_vortex_state_set_g(self._h, float(g))
 
+    def setThreads(self, threads: int):
+        """Threads of iterateSolution: 0 automatic, 1 serial, n threads (same result for any n)."""
+        lib.gpe_vortex_state_set_threads(self._h, int(threads))
+
     def setDimensions(self, rsize: int, rstep: float, rplug: float = 0.0, zsize: int = 1, zstep: float = 1.0):
         check(lib.gpe_vortex_state_set_dimensions(self._h, int(rsize), float(rstep), float(rplug), int(zsize),
                                                   float(zstep)), "gpe_vortex_state_set_dimensions")
