@@ -123,7 +123,7 @@ def batched_workload(n_inst, steps, seed=4, log=log):
         g = float(np.exp(rng.uniform(np.log(100.0), np.log(5000.0))))
         ratio = float(rng.uniform(0.5, 4.0))
         dt = stable_dt(n, 0.25, g, ratio)
-        cfgs.append(instance_config((n, n, n), 0.25, g, dt, steps, "real", ratio, sigma=1.5, noise=1e-3, mu0=0.0,
+        cfgs.append(instance_config((n, n, n), 0.25, g, dt, steps, "real", ratio, sigma=1.5, noise=1e-3,
                                     seed=seed * 100003 + q))
 
     def make(waves, cfg_list=cfgs):

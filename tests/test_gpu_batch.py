@@ -175,3 +175,33 @@ def test_graph_replays_match_eager_chunks():
     for e, o in outs[1:]:
         np.testing.assert_array_equal(e, outs[0][0])
         np.testing.assert_array_equal(o, outs[0][1])
+
+
+def test_automatic_start_phase():
+    """mu0 = NaN: odd = even exp(i mu dt/2) with mu = (K + P + I)/N of the initial state, exactly;
+    a strongly interacting Gaussian then keeps its norm better than with mu0 = 0."""
+    announce("automatic start phase")
+    dims, h, g, ratio = (24, 24, 24), 0.25, 3000.0, 2.0
+    dt = stable_dt(24, h, g, ratio)
+    B = Batch("fp64")
+    i = B.add(instance_config(dims, h, g, dt, 400, "real", ratio, seed=9))
+    e, o = B.get_state(i)
+    obs = B.observables(i)
+    mu = (obs["K"] + obs["P"] + obs["I"]) / obs["N"]
+    assert mu > 30.0  # strongly interacting: the phase matters
+    cr, ci = math.cos(0.5 * mu * dt), math.sin(0.5 * mu * dt)
+    np.testing.assert_array_equal(o[0::2], cr * e[0::2] - ci * e[1::2])
+    np.testing.assert_array_equal(o[1::2], ci * e[0::2] + cr * e[1::2])
+    drift = {}
+    for name, m0 in (("auto", float("nan")), ("zero", 0.0)):
+        C = Batch("fp64")
+        j = C.add(instance_config(dims, h, g, dt, 400, "real", ratio, seed=9, mu0=m0))
+        n0 = C.observables(j)["N"]
+        worst = 0.0
+        for _ in range(8):
+            C.run(50, 1)
+            worst = max(worst, abs(C.observables(j)["N"] / n0 - 1.0))
+        drift[name] = worst
+        C.close()
+    assert drift["auto"] < 0.8 * drift["zero"], drift
+    B.close()

@@ -19,9 +19,11 @@ from .solver import MODES, PRECISIONS, nodes_of
 
 def instance_config(dims: Sequence[int], h: float, g: float, dt: float, steps: int, mode: str = "real",
                     trap_ratio: float = 1.0, init: str = "gaussian", sigma: float = 1.5, noise: float = 1e-3,
-                    mu0: float = 0.0, seed: int = 0, zlayers: int = 0,
+                    mu0: float = float("nan"), seed: int = 0, zlayers: int = 0,
                     origin: Optional[Sequence[float]] = None) -> _lib.GpeInstanceConfig:
-    """Instance of cube edge h (block_scale h/sqrt(8)), harmonic trap 0.5(x^2+y^2+(ratio z)^2)."""
+    """Instance of cube edge h (block_scale h/sqrt(8)), harmonic trap 0.5(x^2+y^2+(ratio z)^2).
+    With init "gaussian", mu0 NaN (the default) starts the leapfrog with the state's own chemical
+    potential, odd = even exp(i mu dt/2), mu = <H>/N computed on the device."""
     c = _lib.GpeInstanceConfig()
     c.nx, c.ny, c.nz = (int(d) for d in dims)
     c.mode = MODES[mode]

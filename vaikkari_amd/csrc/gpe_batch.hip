@@ -566,11 +566,36 @@ BInst *get_inst(gpe_batch *B, int32_t id) {
     return &B->inst[id];
 }
 
+// odd = even exp(i mu dt/2) with mu = (K + P + I)/N of the instance's even
+// field (FP64 observables), the phase of the reference's start of the
+// leapfrog (GPUGrossPitaevskii_BCC.cu:325,342-343) with the state's own mu.
+gpe_status auto_odd(gpe_batch *B, int32_t id) {
+    double obs[4];
+    gpe_status st = gpe_batch_observables(B, id, obs);
+    if (st != GPE_OK) return st;
+    BInst &b = B->inst[id];
+    const double mu = (obs[1] + obs[2] + obs[3]) / obs[0];
+    if (!std::isfinite(mu)) {
+        set_error("gpe_batch_add: initial state has no finite chemical potential");
+        return GPE_EUNSTABLE;
+    }
+    const double ph = 0.5 * mu * b.cfg.dt;
+    if (B->precision == GPE_FP64)
+        hipLaunchKernelGGL((k_phase_copy<double>), dim3(1024), dim3(256), 0, B->stream, (double2 *)b.buf[1],
+                           (const double2 *)b.buf[0], b.L.elems, std::cos(ph), std::sin(ph));
+    else
+        hipLaunchKernelGGL((k_phase_copy<float>), dim3(1024), dim3(256), 0, B->stream, (float2 *)b.buf[1],
+                           (const float2 *)b.buf[0], b.L.elems, std::cos(ph), std::sin(ph));
+    GPE_HIP_TRY(hipGetLastError());
+    GPE_HIP_TRY(hipStreamSynchronize(B->stream));
+    return GPE_OK;
+}
+
 template <typename T>
 gpe_status init_gaussian(gpe_batch *B, BInst &b) {
     const double s = b.cfg.sigma > 0 ? b.cfg.sigma : 1.0;
     const double amp = std::pow(3.14159265358979323846 * s * s, -0.75);
-    const double ph = 0.5 * b.cfg.mu0 * b.cfg.dt;
+    const double ph = std::isnan(b.cfg.mu0) ? 0.0 : 0.5 * b.cfg.mu0 * b.cfg.dt;  // NaN: set after init
     cplx_t<T> *odd = b.cfg.mode == GPE_REAL_TIME ? (cplx_t<T> *)b.buf[1] : nullptr;
     hipLaunchKernelGGL((k_init_gaussian<T>), dim3(1024), dim3(256), 0, B->stream, (cplx_t<T> *)b.buf[0], odd, b.L,
                        b.trap, 1.0 / (2.0 * s * s), amp, b.cfg.noise, (unsigned long long)b.cfg.seed, std::cos(ph),
@@ -702,6 +727,15 @@ gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id
     B->resident += b.bytes;
     B->desc_dirty = true;
     *id = slot;
+    if (c->init == GPE_INIT_GAUSSIAN && c->mode == GPE_REAL_TIME && std::isnan(c->mu0)) {
+        // start the leapfrog with the state's own chemical potential
+        const gpe_status st = auto_odd(B, slot);
+        if (st != GPE_OK) {
+            (void)gpe_batch_remove(B, slot);
+            *id = -1;
+            return st;
+        }
+    }
     return GPE_OK;
 }
 

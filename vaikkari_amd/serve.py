@@ -111,7 +111,7 @@ class Server:
         q.dt = stable_dt(r.n, r.h, r.g, r.ratio, mode=r.mode)
         q.g = r.g
         q.trap_ratio = r.ratio
-        q.sigma, q.noise, q.mu0 = 1.5, 1e-3, 0.0
+        q.sigma, q.noise, q.mu0 = 1.5, 1e-3, float("nan")  # leapfrog started with the state's own mu
         q.seed = r.seed
         q.keep_result = 1 if keep_result else 0
         t = C.c_uint64()
