@@ -102,7 +102,18 @@ def cpu_baseline(dims, b, g, dt, ratio, seconds):
     t3 = time.perf_counter()
     rate = n * (n2 - 1) / ((t3 - t2) - (t1 - t0))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model = "unknown CPU"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    # the oracle streams prev, next (read + write) and V per node-update, 16 + 32 + 8 bytes
+    host_gbs = rate * 2 * 56 / 1e9
     return {"value": rate, "unit": "site-steps/s", "cores": threads, "kind": "port",
+            "cpu": model, "host_GBs": host_gbs,
             "sample": f"oracle FP64 leapfrog on the same {dims[0]}^3-cell grid ({n} nodes), "
                       f"{n2} steps timed ({t3 - t2:.1f} s) minus a 1-step run"}
 
