@@ -292,6 +292,9 @@ def main():
     log(f"grid {dims} cells, {nodes} nodes")
     copy_gbs = va.copy_bandwidth(1 << 30, 10)
     log(f"plain streaming copy: {copy_gbs:.0f} GB/s ({copy_gbs / HBM_PEAK_GBS * 100:.1f}% of the 8 TB/s peak)")
+    rmw_gbs = va.stream_bandwidth("rmw", 1 << 30, 10)
+    read_gbs = va.stream_bandwidth("read", 1 << 30, 10)
+    log(f"stencil-pattern stream (read + non-temporal RMW, no stencil): {rmw_gbs:.0f} GB/s; read-only: {read_gbs:.0f} GB/s")
 
     # ---- phase 1: imaginary time, FP64 -----------------------------------
     psi0 = va.gaussian(dims, origin, b, 1.5, 1e-3, 1)
@@ -402,6 +405,8 @@ def main():
                         "E": obs["E"], "mu": obs["mu"], "N": obs["N"]},
         "bytes_per_node_step": {"fp64": r64["bytes_per_node_step"], "fp32": r32["bytes_per_node_step"]},
         "hbm_copy_GBs": copy_gbs,
+        "hbm_rmw_stream_GBs": rmw_gbs,
+        "hbm_read_stream_GBs": read_gbs,
         "batched": batched,
         "serving": serving,
         "output": output,

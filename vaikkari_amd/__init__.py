@@ -27,6 +27,7 @@ from .solver import (  # noqa: F401
     reference_dims,
     reference_setup,
     reference_state_recipe,
+    stream_bandwidth,
     thomas_fermi,
     trap_potential,
     variants,

@@ -185,6 +185,7 @@ SIGNATURES = [
     ("gpe_variant_tiles", _I, [_U32]),
     ("gpe_variant_shape", None, [_U32, _U32, _I32P]),
     ("gpe_diag_copy_bandwidth", _I, [_U64, _U32, _DP]),
+    ("gpe_diag_stream_bandwidth", _I, [_U32, _U64, _U32, _DP]),
     ("gpe_batch_create", _I, [_U32, _U64, C.POINTER(_P)]),
     ("gpe_batch_destroy", None, [_P]),
     ("gpe_batch_add", _I, [_P, C.POINTER(GpeInstanceConfig), C.POINTER(C.c_int32)]),

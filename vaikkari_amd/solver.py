@@ -190,6 +190,16 @@ def variants():
     return out
 
 
+def stream_bandwidth(kind: str, nbytes: int = 1 << 30, iters: int = 20) -> float:
+    """GB/s of a streaming probe over buffers of nbytes: "copy" (read + write), "rmw" (read prev,
+    non-temporal read-modify-write of next: the half-update's traffic) or "read"."""
+    kinds = {"copy": (0, 2), "rmw": (1, 3), "read": (2, 1)}
+    k, moved = kinds[kind]
+    ms = C.c_double()
+    check(lib.gpe_diag_stream_bandwidth(k, int(nbytes), int(iters), C.byref(ms)), "gpe_diag_stream_bandwidth")
+    return moved * nbytes / (ms.value * 1e-3) / 1e9
+
+
 def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20) -> float:
     """GB/s of a plain device-to-device streaming copy (read + write bytes)."""
     ms = C.c_double()
