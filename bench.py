@@ -53,6 +53,9 @@ def parse():
                          "serving: the Poisson serving stream alone (use --serve-seconds 600)")
     ap.add_argument("--serve-policy", default="shortest_first", choices=["shortest_first", "fifo"],
                     help="admission policy of --suite serving (shortest_first is capped at 64 M resident nodes)")
+    ap.add_argument("--primal-cells", default="128,256",
+                    help="cube sizes of the 14-point primal stepper sweep (comma list; empty = skip)")
+    ap.add_argument("--primal-steps", type=int, default=200)
     ap.add_argument("--precision-cells", type=int, default=256)
     ap.add_argument("--precision-steps", type=int, default=10000)
     ap.add_argument("--precision-every", type=int, default=1000)
@@ -193,6 +196,60 @@ def serving_suite(args):
     cap = SERVE_CAP_NODES if args.serve_policy == "shortest_first" else 0
     res = serving_workload(args.serve_seconds, policy=args.serve_policy, cap=cap)
     print(json.dumps({"suite": "serving", **res}), flush=True)
+
+
+def primal_workload(sizes, steps, warmup=5, log=log):
+    """The 14-point primal BCC stepper (2 sites per cubic cell): FP64 and FP32
+    leapfrog steps of a Gaussian condensate (sigma 2, kick e^{0.3ix}) in the trap
+    0.5 r^2 on a box of edge 16, g = 100, dt = 0.4 / (7.5/a^2 + max V), over every
+    tile shape.  One launch updates every site once: FP64 reads prev (16 B) and
+    reads + writes next (32 B), 48 B per site; FP32 24 B."""
+    import vaikkari_amd as va
+
+    out = []
+    for n in sizes:
+        dims = (n, n, n)
+        a = 16.0 / n
+        g = 100.0
+        origin = va.primal_centered_origin(dims, a)
+        vmax = 0.5 * 3 * (8.0 ** 2)
+        dt = 0.4 / (7.5 / a ** 2 + vmax)
+        xyz = va.primal_positions(dims, origin, a)
+        psi = np.exp(-(xyz ** 2).sum(axis=1) / 8.0 + 0.3j * xyz[:, 0])
+        del xyz
+        psi /= math.sqrt(va.site_measure(a) * float(np.sum(np.abs(psi) ** 2)))
+        psi = psi.view(np.float64)
+        sites = va.sites_of(dims)
+        entry = {"cells": n, "sites": sites, "a": a, "dt": dt, "g": g}
+        mu = None
+        for prec, bps in (("fp64", 48), ("fp32", 24)):
+            tiles = {}
+            for tile in range(1, va.primal_tiles() + 1):
+                with va.PrimalSolver(dims, a, dt, g, origin, 1.0, prec, "real", tile) as s:
+                    s.set_state(psi)
+                    if mu is None:
+                        o = s.observables(0)
+                        mu = o["mu"]
+                    s.init_odd(mu)
+                    s.step(warmup)
+                    s.synchronize()
+                    ms, launches = s.time_steps(steps)
+                    avg_s = ms * 1e-3 / launches
+                    gbs = sites * bps / avg_s / 1e9
+                    tiles[tile] = {"site_steps_per_s": sites * steps / (ms * 1e-3), "avg_launch_us": avg_s * 1e6,
+                                   "achieved_GBs": gbs}
+            best = max(tiles, key=lambda t: tiles[t]["site_steps_per_s"])
+            b = tiles[best]
+            entry[prec] = {"best_tile": best, "site_steps_per_s": b["site_steps_per_s"],
+                           "avg_launch_us": b["avg_launch_us"], "bytes_per_site_update": bps,
+                           "achieved_GBs": b["achieved_GBs"], "frac_of_peak": b["achieved_GBs"] / HBM_PEAK_GBS,
+                           "default_tile_site_steps_per_s": tiles[1]["site_steps_per_s"],
+                           "tiles": {str(t): round(v["site_steps_per_s"] / 1e9, 3) for t, v in tiles.items()}}
+            log(f"primal14 {n}^3 {prec}: best tile {best}: {b['site_steps_per_s'] / 1e9:.2f} G site-steps/s, "
+                f"{b['avg_launch_us']:.1f} us/launch, {b['achieved_GBs']:.0f} GB/s "
+                f"({b['achieved_GBs'] / HBM_PEAK_GBS * 100:.1f}% of peak); per tile (G/s) {entry[prec]['tiles']}")
+        out.append(entry)
+    return out
 
 
 def output_timing(s, dims, reps=5):
@@ -370,6 +427,10 @@ def main():
             log(f"serving ({name}): {r['completed']} done, p50 {r['p50_s']} s, p99 {r['p99_s']} s, "
                 f"served {r['served_site_steps_per_s'] / 1e9:.1f} G site-steps/s")
 
+    primal = None
+    if args.primal_cells.strip():
+        primal = primal_workload([int(x) for x in args.primal_cells.split(",")], args.primal_steps)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(dims, b, g, dt, ratio, args.cpu_seconds)
@@ -409,6 +470,7 @@ def main():
         "hbm_read_stream_GBs": read_gbs,
         "batched": batched,
         "serving": serving,
+        "primal14": primal,
         "output": output,
         "avg_launch_us": {"fp64": r64["avg_launch_us"], "fp32": r32["avg_launch_us"]},
     }

@@ -145,3 +145,91 @@ class Oracle:
         out = (C.c_int * 36)()
         self.o.or_positions(out)
         return np.array(out, dtype=np.int32).reshape(12, 3)
+
+
+MESH_SO = os.path.join(ROOT, "oracle", "_ref", "libmeshref.so")
+_IP = C.POINTER(C.c_int)
+
+
+def load_mesh_ref():
+    """The reference's mesh library with oracle/mesh_shim.cpp, or None when not built."""
+    if not os.path.exists(MESH_SO):
+        return None
+    m = C.CDLL(MESH_SO)
+    m.ref_bcc_stencils.restype = _I
+    m.ref_bcc_stencils.argtypes = [_D, _D, _I, _I, _DP, _IP, _DP]
+    return m
+
+
+def mesh_stencils(m, h, ext, max_nodes=20000, max_nb=32):
+    """(positions [n,3], counts [n], neighbours [n, max_nb, 4] = (dx, dy, dz, weight))."""
+    pos = np.zeros((max_nodes, 3))
+    cnt = np.zeros(max_nodes, np.int32)
+    nb = np.zeros((max_nodes, max_nb, 4))
+    n = m.ref_bcc_stencils(float(h), float(ext), max_nodes, max_nb, _dp(pos), cnt.ctypes.data_as(_IP), _dp(nb))
+    assert n > 0, f"mesh has {-n} nodes, more than {max_nodes}"
+    return pos[:n], cnt[:n], nb[:n]
+
+
+class PrimalOracle:
+    """numpy wrappers over the primal 14-point functions of liboracle.so
+    (fields [k][j][i][s] complex interleaved, dims = (nx, ny, nz))."""
+
+    def __init__(self):
+        o = C.CDLL(ORACLE_SO) if os.path.exists(ORACLE_SO) else load_oracle()
+        sig = {
+            "or_primal_half_update": (_D, [_DP, _DP, _DP, _IP, _DP, _I, _D]),
+            "or_primal_leapfrog": (None, [_DP, _DP, _DP, _IP, _DP, _LL]),
+            "or_primal_imag_steps": (_D, [_DP, _DP, _DP, _IP, _DP, _D, _LL]),
+            "or_primal_observables": (None, [_DP, _DP, _IP, _D, _D, _DP]),
+            "or_primal_laplacian": (None, [_DP, _DP, _IP, _D]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(o, name)
+            f.restype, f.argtypes = res, args
+        self.o = o
+
+    @staticmethod
+    def _dims(dims):
+        return (C.c_int * 3)(*[int(d) for d in dims])
+
+    @staticmethod
+    def _coef(coef):
+        return np.ascontiguousarray(coef, np.float64)
+
+    def half_update(self, nxt, prev, pot_dt, dims, coef, imag=False, scale=1.0):
+        n = np.array(nxt, np.float64, copy=True)
+        p = np.ascontiguousarray(prev, np.float64)
+        v = np.ascontiguousarray(pot_dt, np.float64)
+        c = self._coef(coef)
+        s = self.o.or_primal_half_update(_dp(n), _dp(p), _dp(v), self._dims(dims), _dp(c), int(imag), float(scale))
+        return n, s
+
+    def leapfrog(self, even, odd, pot_dt, dims, coef, nsteps):
+        e = np.array(even, np.float64, copy=True)
+        d = np.array(odd, np.float64, copy=True)
+        v = np.ascontiguousarray(pot_dt, np.float64)
+        c = self._coef(coef)
+        self.o.or_primal_leapfrog(_dp(e), _dp(d), _dp(v), self._dims(dims), _dp(c), int(nsteps))
+        return e, d
+
+    def imag_steps(self, psi, pot_dt, dims, coef, w, nsteps):
+        p = np.array(psi, np.float64, copy=True)
+        work = np.zeros_like(p)
+        v = np.ascontiguousarray(pot_dt, np.float64)
+        c = self._coef(coef)
+        N = self.o.or_primal_imag_steps(_dp(p), _dp(work), _dp(v), self._dims(dims), _dp(c), float(w), int(nsteps))
+        return p, N
+
+    def observables(self, psi, V, dims, a, g):
+        out = np.zeros(4)
+        p = np.ascontiguousarray(psi, np.float64)
+        v = np.ascontiguousarray(V, np.float64)
+        self.o.or_primal_observables(_dp(p), _dp(v), self._dims(dims), float(a), float(g), _dp(out))
+        return out
+
+    def laplacian(self, f, dims, a):
+        f = np.ascontiguousarray(f, np.float64)
+        out = np.zeros_like(f)
+        self.o.or_primal_laplacian(_dp(f), _dp(out), self._dims(dims), float(a))
+        return out

@@ -36,3 +36,13 @@ from .solver import (  # noqa: F401
     write_bmp,
     write_volume_map,
 )
+from .primal import (  # noqa: F401
+    PrimalSolver,
+    primal_centered_origin,
+    primal_coefficients,
+    primal_positions,
+    primal_tiles,
+    primal_trap_potential,
+    site_measure,
+    sites_of,
+)

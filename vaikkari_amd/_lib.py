@@ -152,6 +152,20 @@ class GpeOutputOptions(C.Structure):
     ]
 
 
+class GpePrimalConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("a", C.c_double),
+        ("dt", C.c_double),
+        ("g", C.c_double),
+        ("origin", C.c_double * 3),
+        ("trap_ratio", C.c_double),
+        ("mode", C.c_uint32),
+        ("tile", C.c_uint32),
+    ]
+
+
 class GpeError(RuntimeError):
     def __init__(self, status: int, where: str, detail: str):
         super().__init__(f"{where}: status {status} ({detail})")
@@ -167,7 +181,7 @@ _DP = C.POINTER(C.c_double)
 _U32P = C.POINTER(C.c_uint32)
 _I32P = C.POINTER(C.c_int32)
 
-# (name, restype, argtypes) of every exported symbol of the two headers.
+# (name, restype, argtypes) of every exported symbol of include/*.h.
 SIGNATURES = [
     ("gpe_create", _I, [C.POINTER(GpeConfig), C.POINTER(_P)]),
     ("gpe_destroy", None, [_P]),
@@ -211,6 +225,21 @@ SIGNATURES = [
     ("gpe_server_get_stats", _I, [_P, C.POINTER(GpeServerStats)]),
     ("gpe_server_set_policy", _I, [_P, _U32, _U64]),
     ("gpe_server_oldest_queued", _D, [_P]),
+    ("gpe_primal_create", _I, [C.POINTER(GpePrimalConfig), C.POINTER(_P)]),
+    ("gpe_primal_destroy", None, [_P]),
+    ("gpe_primal_set_state", _I, [_P, _DP, _DP]),
+    ("gpe_primal_get_state", _I, [_P, _DP, _DP]),
+    ("gpe_primal_set_mode", _I, [_P, _U32, _D]),
+    ("gpe_primal_init_odd", _I, [_P, _D]),
+    ("gpe_primal_step", _I, [_P, _U64]),
+    ("gpe_primal_time_steps", _I, [_P, _U64, _DP, C.POINTER(_U64)]),
+    ("gpe_primal_observables", _I, [_P, _I, _DP]),
+    ("gpe_primal_synchronize", _I, [_P]),
+    ("gpe_primal_tiles", _I, []),
+    ("gpe_primal_coefficients", None, [_D, _D, _D, _DP]),
+    ("gpe_primal_centered_origin", None, [_U32P, _D, _DP]),
+    ("gpe_primal_positions", None, [_U32P, _DP, _D, _DP]),
+    ("gpe_primal_trap_potential", None, [_U32P, _DP, _D, _D, _DP]),
     ("gpe_status_string", C.c_char_p, [_I]),
     ("gpe_last_error", C.c_char_p, []),
     ("gpe_bcc_positions", None, [_I32P]),

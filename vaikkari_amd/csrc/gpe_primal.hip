@@ -1,0 +1,833 @@
+// gpe_primal.hip -- Gross-Pitaevskii stepper on the BCC sites with the
+// 14-point primal operator (include/vaikkari_gpe_primal.h).
+//
+// The Laplacian is the DEC Laplacian of 0-forms on the BCC lattice built by
+// the reference's mesh library (DelaunayMesh::createBccGrid,
+// jukan_koodit/Mesh/DelaunayMesh.cpp:178; Hodge weights Mesh::getEdgeHodge /
+// getNodeHodge, Mesh/Mesh.cpp:1479-1489): with cube edge a, every site has 8
+// diagonal neighbours (the other sublattice, weight 3/(4a^2)) and 6 axial
+// neighbours (its own sublattice, weight 1/(4a^2)).  The time integrators and
+// the order of the Hamiltonian's terms follow the reference stepper
+// (GPUGrossPitaevskii_BCC.cu:205-215, 475-481).
+//
+// Field layout in HBM, the class-row SoA of the dual stepper with 2 classes:
+//   site (i, j, k, s) at origin + k*sz + j*sy + s*xp + i,  sy = 2*xp,
+// s = 0 the corner sublattice, s = 1 the centre sublattice.  A wave reads 64
+// consecutive cells of one class row: 1 KiB (FP64) or 512 B (FP32) per
+// access, and every neighbour is a class row shifted by dx + dy*sy + dz*sz.
+// One zero halo cell on every side (the corner site's diagonal neighbours
+// reach back one cell, the centre site's forward one cell).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "../../include/vaikkari_gpe_primal.h"
+#include "gpe_hip_check.hpp"
+#include "gpe_internal.hpp"
+#include "gpe_fused.hpp"
+#include "gpe_kernels.hpp"
+
+using namespace gpe;
+
+namespace {
+
+// Trap V = 0.5 (x^2 + y^2 + rr z^2) at site positions p0 + a (i + off),
+// off = 0 (corner) or 0.5 (centre); the host helpers evaluate the same
+// expression, so V*dt has the same bits on both sides.
+struct PTrap {
+    double p0[3];
+    double a;
+    double rr;
+};
+
+__host__ __device__ inline double site_v(const PTrap &t, int i, int j, int k, double off) {
+#pragma clang fp contract(off)
+    const double x = t.p0[0] + t.a * ((double)i + off);
+    const double y = t.p0[1] + t.a * ((double)j + off);
+    const double z = t.p0[2] + t.a * ((double)k + off);
+    return 0.5 * (x * x + y * y + t.rr * z * z);
+}
+
+// {cD, cA, c0, g} times dt in the arithmetic type.
+template <typename T>
+struct PCoef {
+    T cD, cA, c0, g;
+};
+
+// s = dt H psi at one site from its 8 diagonal (d) and 6 axial (x)
+// neighbour values.
+//  FORM_REF  (FP64): cD*sum_8 d + cA*sum_6 x + ((c0 + V) + g|psi|^2) psi,
+//            sums left to right from the first element.
+//  FORM_DIFF (FP32): cD*sum_8 (d - psi) + cA*sum_6 (x - psi) + (V + g|psi|^2) psi,
+//            algebraically equal (c0 = -(8 cD + 6 cA)) and free of the
+//            cancellation between the neighbour sums and c0 psi.
+template <typename T, int FORM>
+__device__ __forceinline__ cplx_t<T> primal_apply(const cplx_t<T> (&d)[8], const cplx_t<T> (&x)[6], cplx_t<T> p,
+                                                  T pot, const PCoef<T> &H) {
+#pragma clang fp contract(off)
+    cplx_t<T> s;
+    const T nsq = p.x * p.x + p.y * p.y;
+    T dr, di, ar, ai, c;
+    if (FORM == FORM_REF) {
+        dr = d[0].x;
+        di = d[0].y;
+#pragma unroll
+        for (int q = 1; q < 8; q++) {
+            dr = dr + d[q].x;
+            di = di + d[q].y;
+        }
+        ar = x[0].x;
+        ai = x[0].y;
+#pragma unroll
+        for (int q = 1; q < 6; q++) {
+            ar = ar + x[q].x;
+            ai = ai + x[q].y;
+        }
+        c = (H.c0 + pot) + H.g * nsq;
+    } else {
+        dr = d[0].x - p.x;
+        di = d[0].y - p.y;
+#pragma unroll
+        for (int q = 1; q < 8; q++) {
+            dr = dr + (d[q].x - p.x);
+            di = di + (d[q].y - p.y);
+        }
+        ar = x[0].x - p.x;
+        ai = x[0].y - p.y;
+#pragma unroll
+        for (int q = 1; q < 6; q++) {
+            ar = ar + (x[q].x - p.x);
+            ai = ai + (x[q].y - p.y);
+        }
+        c = pot + H.g * nsq;
+    }
+    s.x = (H.cD * dr + H.cA * ar) + c * p.x;
+    s.y = (H.cD * di + H.cA * ai) + c * p.y;
+    return s;
+}
+
+// x-neighbour exchange between adjacent lanes (XSM: 0 none, the neighbour
+// is loaded; 1 ds_bpermute shuffle; 2 DPP wave shift, a VALU move with no
+// memory or LDS traffic).  Lanes at a row end replace the result by a load.
+enum { XS_LOAD = 0, XS_SHFL = 1, XS_DPP = 2 };
+template <int XSM, typename T>
+__device__ __forceinline__ cplx_t<T> from_left(cplx_t<T> v) {
+    if (XSM == XS_DPP) return shift_lane_c<true>(v);
+    v.x = __shfl_up(v.x, 1, 64);
+    v.y = __shfl_up(v.y, 1, 64);
+    return v;
+}
+template <int XSM, typename T>
+__device__ __forceinline__ cplx_t<T> from_right(cplx_t<T> v) {
+    if (XSM == XS_DPP) return shift_lane_c<false>(v);
+    v.x = __shfl_down(v.x, 1, 64);
+    v.y = __shfl_down(v.y, 1, 64);
+    return v;
+}
+
+template <typename T, bool NT>
+__device__ __forceinline__ void rmw_site(cplx_t<T> *__restrict__ next, long long at, cplx_t<T> s) {
+#pragma clang fp contract(off)
+    cplx_t<T> n;
+    if (NT) {
+        n.x = __builtin_nontemporal_load(&next[at].x);
+        n.y = __builtin_nontemporal_load(&next[at].y);
+    } else {
+        n = next[at];
+    }
+    n.x = n.x + s.y;
+    n.y = n.y + (-s.x);
+    if (NT) {
+        __builtin_nontemporal_store(n.x, &next[at].x);
+        __builtin_nontemporal_store(n.y, &next[at].y);
+    } else {
+        next[at] = n;
+    }
+}
+
+// One cell (both sites) of an update pass; returns its sum of |out|^2
+// (imaginary time).  Real time: next[site] += (s.y, -s.x) in place
+// (GPUGrossPitaevskii_BCC.cu:215), non-temporal when NT.  Imaginary time:
+// out = psi - s with psi = scale*prev, streamed to next.
+// Neighbour order (the oracle's): diagonal dz, dy, dx over {-1, 0} (corner
+// site) or {0, +1} (centre site); axial -x, +x, -y, +y, -z, +z.
+// The two sites need 12 distinct class rows of prev -- corner rows (j,k),
+// (j+-1,k), (j,k+-1), (j+1,k+1) and centre rows (j,k), (j-1,k), (j,k-1),
+// (j-1,k-1), (j+1,k), (j,k+1) -- each loaded once, plus 10 of them shifted
+// by one cell in x (from the adjacent lane, or loaded when XSM == XS_LOAD).
+// The corner site is finished before the centre site's three remaining
+// rows are read, which keeps fewer values live.
+template <typename T, int FORM, int MODE, int TY, bool NT, int XSM>
+__device__ __forceinline__ double primal_cell(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                              const Layout &L, const PCoef<T> &H, const PTrap &trap, double dt,
+                                              const double *__restrict__ scale, int bx, int by, int bz, int lsh) {
+#pragma clang fp contract(off)
+    using C = cplx_t<T>;
+    const int lw = 1 << lsh;
+    const int xl = threadIdx.x & (lw - 1);
+    const int i = (bx << lsh) + xl;
+    const int j = ((by * TY + threadIdx.y) << (6 - lsh)) + (threadIdx.x >> lsh);
+    const int k = bz;
+    const bool act = i < L.nx && j < L.ny;
+    // a lane at a row end takes its x-neighbour from memory (the halo at the grid's edge)
+    const bool left_end = act && xl == 0, right_end = act && xl == lw - 1;
+    const long long sy = L.sy, sz = L.sz;
+    const long long b0 = L.origin + (long long)k * sz + (long long)j * sy + i;  // corner site
+    const long long b1 = b0 + L.xp;                                               // centre site
+    T sc = (T)1;
+    if (MODE == MODE_IMAG) sc = (T)(*scale);
+    auto ld = [&](long long at) {
+        C v = prev[at];
+        if (MODE == MODE_IMAG) v = cscale<T>(v, sc);
+        return v;
+    };
+    C zero;
+    zero.x = (T)0;
+    zero.y = (T)0;
+    // lanes beyond the grid hold zeros: the halo value a neighbouring lane needs
+    C c0 = zero, c1 = zero, cym = zero, cyp = zero, czm = zero, czp = zero, u0 = zero, u1 = zero, u2 = zero;
+    if (act) {
+        c0 = ld(b0);
+        c1 = ld(b1);
+        u0 = ld(b1 - sz - sy);  // centre rows (j-1,k-1), (j,k-1), (j-1,k)
+        u1 = ld(b1 - sz);
+        u2 = ld(b1 - sy);
+        cym = ld(b0 - sy);  // corner rows (j-1,k), (j+1,k), (j,k-1), (j,k+1)
+        cyp = ld(b0 + sy);
+        czm = ld(b0 - sz);
+        czp = ld(b0 + sz);
+    }
+    double part = 0.0;
+    // ---- corner site: diagonal centres at x = i-1, i of rows u0, u1, u2, c1
+    C w0, w1, w2, w3, xm, xq;  // w*: rows at i-1; xm, xq: c0 at i-1, i+1
+    if (XSM != XS_LOAD) {
+        w0 = from_left<XSM, T>(u0);
+        w1 = from_left<XSM, T>(u1);
+        w2 = from_left<XSM, T>(u2);
+        w3 = from_left<XSM, T>(c1);
+        xm = from_left<XSM, T>(c0);
+        xq = from_right<XSM, T>(c0);
+        if (left_end) {
+            w0 = ld(b1 - sz - sy - 1);
+            w1 = ld(b1 - sz - 1);
+            w2 = ld(b1 - sy - 1);
+            w3 = ld(b1 - 1);
+            xm = ld(b0 - 1);
+        }
+        if (right_end) xq = ld(b0 + 1);
+    } else if (act) {
+        w0 = ld(b1 - sz - sy - 1);
+        w1 = ld(b1 - sz - 1);
+        w2 = ld(b1 - sy - 1);
+        w3 = ld(b1 - 1);
+        xm = ld(b0 - 1);
+        xq = ld(b0 + 1);
+    }
+    if (act) {
+        const C dC[8] = {w0, u0, w1, u1, w2, u2, w3, c1};
+        const C aC[6] = {xm, xq, cym, cyp, czm, czp};
+        const C s0 = primal_apply<T, FORM>(dC, aC, c0, (T)(site_v(trap, i, j, k, 0.0) * dt), H);
+        if (MODE == MODE_REAL) {
+            rmw_site<T, NT>(next, b0, s0);
+        } else {
+            C o;
+            o.x = c0.x - s0.x;
+            o.y = c0.y - s0.y;
+            __builtin_nontemporal_store(o.x, &next[b0].x);
+            __builtin_nontemporal_store(o.y, &next[b0].y);
+            part = (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+        }
+    }
+    // ---- centre site: diagonal corners at x = i, i+1 of rows c0, cyp, czp, v3;
+    // axial centres c1 at i-1 (= w3), i+1, and rows u2, zyp, u1, zzp
+    C v3 = zero, zyp = zero, zzp = zero;
+    if (act) {
+        v3 = ld(b0 + sz + sy);  // corner row (j+1,k+1)
+        zyp = ld(b1 + sy);      // centre rows (j+1,k), (j,k+1)
+        zzp = ld(b1 + sz);
+    }
+    C e1, e2, e3, zxp;  // rows at i+1
+    if (XSM != XS_LOAD) {
+        e1 = from_right<XSM, T>(cyp);
+        e2 = from_right<XSM, T>(czp);
+        e3 = from_right<XSM, T>(v3);
+        zxp = from_right<XSM, T>(c1);
+        if (right_end) {
+            e1 = ld(b0 + sy + 1);
+            e2 = ld(b0 + sz + 1);
+            e3 = ld(b0 + sz + sy + 1);
+            zxp = ld(b1 + 1);
+        }
+    } else if (act) {
+        e1 = ld(b0 + sy + 1);
+        e2 = ld(b0 + sz + 1);
+        e3 = ld(b0 + sz + sy + 1);
+        zxp = ld(b1 + 1);
+    }
+    if (act) {
+        const C dZ[8] = {c0, xq, cyp, e1, czp, e2, v3, e3};
+        const C aZ[6] = {w3, zxp, u2, zyp, u1, zzp};
+        const C s1 = primal_apply<T, FORM>(dZ, aZ, c1, (T)(site_v(trap, i, j, k, 0.5) * dt), H);
+        if (MODE == MODE_REAL) {
+            rmw_site<T, NT>(next, b1, s1);
+        } else {
+            C o;
+            o.x = c1.x - s1.x;
+            o.y = c1.y - s1.y;
+            __builtin_nontemporal_store(o.x, &next[b1].x);
+            __builtin_nontemporal_store(o.y, &next[b1].y);
+            part += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+        }
+    }
+    return part;
+}
+
+// Update pass: one thread per cell, block (64, TY), logical grid
+// (gx, gy, nz) ordered over the XCDs by block_coords.
+template <typename T, int FORM, int MODE, int TY, bool NT, int XSM>
+__global__ void __launch_bounds__(64 * TY) k_primal(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                                    Layout L, PCoef<T> H, PTrap trap, double dt,
+                                                    const double *__restrict__ scale, double *__restrict__ partials,
+                                                    Grid3 G) {
+    __shared__ double lds[TY];
+    int bx, by, bz;
+    block_coords(G, bx, by, bz);
+    double part[1];
+    part[0] = primal_cell<T, FORM, MODE, TY, NT, XSM>(next, prev, L, H, trap, dt, scale, bx, by, bz, G.lsh);
+    if (MODE == MODE_IMAG) {
+        // partials in logical tile order (independent of the block-to-XCD order)
+        block_sum<1, TY>(part, lds);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partials[bx + G.gx * (by + G.gy * bz)] = part[0];
+    }
+}
+
+// Block partials of {sum|psi|^2, sum 1/2 Re(conj(psi)(-L psi)), sum V|psi|^2,
+// sum g|psi|^4} over both sites of every cell, FP64, unscaled weights.
+constexpr int kObsTY = 4;
+template <typename T>
+__global__ void __launch_bounds__(64 * kObsTY) k_primal_observables(const cplx_t<T> *__restrict__ psi, Layout L,
+                                                                    PTrap trap, double lD, double lA, double g,
+                                                                    double *__restrict__ partials) {
+    __shared__ double lds[4 * kObsTY];
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * kObsTY + threadIdx.y;
+    const int k = blockIdx.z;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < L.nx && j < L.ny) {
+        const long long base = L.origin + (long long)k * L.sz + (long long)j * L.sy + i;
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const long long at = base + s * L.xp;
+            const cplx_t<T> c = psi[at];
+            const double pr = (double)c.x, pi = (double)c.y;
+            const int o = s == 0 ? -1 : 0;
+            const long long other = base + (1 - s) * L.xp;
+            double lr = 0.0, li = 0.0;
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+                    for (int dx = 0; dx < 2; dx++) {
+                        const cplx_t<T> n = psi[other + (o + dx) + (o + dy) * L.sy + (o + dz) * L.sz];
+                        lr += lD * ((double)n.x - pr);
+                        li += lD * ((double)n.y - pi);
+                    }
+            const long long ax[6] = {-1, 1, -L.sy, L.sy, -L.sz, L.sz};
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const cplx_t<T> n = psi[at + ax[q]];
+                lr += lA * ((double)n.x - pr);
+                li += lA * ((double)n.y - pi);
+            }
+            const double nsq = pr * pr + pi * pi;
+            acc[0] += nsq;
+            acc[1] += 0.5 * -(pr * lr + pi * li);
+            acc[2] += site_v(trap, i, j, k, s ? 0.5 : 0.0) * nsq;
+            acc[3] += g * nsq * nsq;
+        }
+    }
+    block_sum<4, kObsTY>(acc, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+#pragma unroll
+        for (int q = 0; q < 4; q++) partials[4 * b + q] = acc[q];
+    }
+}
+
+// Tile shapes: waves per block, block order over the XCDs (block_coords),
+// non-temporal read-modify-write, x-neighbour exchange (XS_*).
+struct PTile {
+    int ty, remap, nt, xs;
+};
+constexpr PTile kPrimalTiles[] = {{4, 2, 1, XS_DPP}, {4, 2, 1, XS_LOAD}, {2, 2, 1, XS_DPP}, {8, 2, 1, XS_DPP},
+                                  {2, 2, 1, XS_LOAD}, {8, 2, 1, XS_LOAD}, {4, 0, 1, XS_DPP}, {4, 1, 1, XS_DPP},
+                                  {4, 2, 0, XS_DPP}, {4, 2, 1, XS_SHFL}, {4, 2, 0, XS_LOAD}};
+constexpr int kNumPrimalTiles = sizeof(kPrimalTiles) / sizeof(kPrimalTiles[0]);
+
+Layout primal_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes) {
+    Layout L{};
+    L.nx = (int)nx;
+    L.ny = (int)ny;
+    L.nz = (int)nz;
+    const long long line = (long long)(128 / elem_bytes);  // elements per 128-byte line
+    L.padl = (int)line;
+    L.xp = ((L.padl + (long long)nx + 1 + line - 1) / line) * line;
+    L.sy = 2LL * L.xp;
+    L.sz = (long long)(ny + 2) * L.sy;
+    L.origin = L.sz + L.sy + L.padl;
+    L.elems = (long long)(nz + 2) * L.sz;
+    return L;
+}
+
+Grid3 primal_grid(const PTile &t, const Layout &L) {
+    Grid3 G{};
+    G.lsh = row_lanes_log2(L.nx);
+    const int lw = 1 << G.lsh, rows = t.ty * (64 >> G.lsh);
+    G.gx = (L.nx + lw - 1) / lw;
+    G.gy = (L.ny + rows - 1) / rows;
+    G.gz = L.nz;
+    G.remap = t.remap;
+    if (G.remap == 2 && G.gy % 8 != 0) G.remap = 0;  // y-bands need gy divisible by the 8 XCDs
+    return G;
+}
+
+template <typename T, int FORM, int MODE, int TY, bool NT, int XSM>
+void launch_t(cplx_t<T> *next, const cplx_t<T> *prev, const Layout &L, const PCoef<T> &H, const PTrap &trap,
+              double dt, const double *scale, double *partials, const Grid3 &G, hipStream_t st) {
+    hipLaunchKernelGGL((k_primal<T, FORM, MODE, TY, NT, XSM>), dim3(G.gx * G.gy * G.gz), dim3(64, TY), 0, st, next,
+                       prev, L, H, trap, dt, scale, partials, G);
+}
+
+// Instantiates one kernel per tile shape (imaginary time always stores
+// streaming, so its shapes differ only in waves, order and exchange).
+template <typename T, int FORM, int MODE, int IDX = 0>
+hipError_t launch_primal(int tile, cplx_t<T> *next, const cplx_t<T> *prev, const Layout &L, const PCoef<T> &H,
+                         const PTrap &trap, double dt, const double *scale, double *partials, hipStream_t st,
+                         bool reverse) {
+    if constexpr (IDX < kNumPrimalTiles) {
+        if (tile != IDX)
+            return launch_primal<T, FORM, MODE, IDX + 1>(tile, next, prev, L, H, trap, dt, scale, partials, st,
+                                                         reverse);
+        constexpr PTile t = kPrimalTiles[IDX];
+        Grid3 G = primal_grid(t, L);
+        G.rev = reverse ? 1 : 0;
+        launch_t<T, FORM, MODE, t.ty, MODE == MODE_REAL && t.nt, t.xs>(next, prev, L, H, trap, dt, scale, partials,
+                                                                         G, st);
+        return hipGetLastError();
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
+long long primal_blocks(int tile, const Layout &L) {
+    const Grid3 G = primal_grid(kPrimalTiles[tile], L);
+    return (long long)G.gx * G.gy * G.gz;
+}
+
+long long primal_obs_blocks(const Layout &L) {
+    return (long long)((L.nx + 63) / 64) * ((L.ny + kObsTY - 1) / kObsTY) * L.nz;
+}
+
+double primal_vmax(const uint32_t dims[3], const double origin[3], double a, double ratio) {
+    // sites span p0 + a [0, n - 1/2] per axis; the extreme |coordinate| bounds V
+    double m[3];
+    for (int d = 0; d < 3; d++) m[d] = std::max(std::fabs(origin[d]), std::fabs(origin[d] + a * (dims[d] - 0.5)));
+    return 0.5 * (m[0] * m[0] + m[1] * m[1] + ratio * ratio * m[2] * m[2]);
+}
+
+// -L/2 has spectrum in [0, 7.5/a^2] (Gershgorin: diagonal 15/(8a^2), off-diagonal
+// row sum 15/(8a^2)), so both integrators need dt (7.5/a^2 + max V) < 2.
+bool primal_stable(double dt, double a, double vmax) { return dt * (7.5 / (a * a) + vmax) < 2.0; }
+
+}  // namespace
+
+struct gpe_primal {
+    gpe_primal_config cfg{};
+    Layout L{};
+    PTrap trap{};
+    double coef[4] = {0, 0, 0, 0};
+    double w = 0.0;  // site measure a^3/2
+    int tile = 0;
+    void *field[2] = {nullptr, nullptr};  // field[ie] even level, field[io] odd level / imaginary-time scratch
+    int ie = 0, io = 1;
+    double *scal = nullptr;      // [0] last N, [1] read scale, [4..7] observables
+    double *partials = nullptr;  // per-block partial sums
+    long long npartials = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t steps_done = 0;
+    bool odd_valid = false;
+
+    ~gpe_primal() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto *p : field)
+            if (p) (void)hipFree(p);
+        if (scal) (void)hipFree(scal);
+        if (partials) (void)hipFree(partials);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    size_t elem_bytes() const { return cfg.precision == GPE_FP64 ? 16 : 8; }
+    size_t sites() const { return 2ull * cfg.nx * cfg.ny * cfg.nz; }
+    void set_coefs() {
+        gpe_primal_coefficients(cfg.a, cfg.dt, cfg.g, coef);
+        w = 0.5 * cfg.a * cfg.a * cfg.a;
+    }
+    gpe_status init();
+    template <typename T> gpe_status upload(int f, const double *h);
+    template <typename T> gpe_status download(int f, double *h);
+    template <typename T, int FORM> gpe_status enqueue(uint64_t nsteps, uint64_t *launches);
+    template <typename T> gpe_status observables_t(int f, double out[4]);
+};
+
+gpe_status gpe_primal::init() {
+    L = primal_layout(cfg.nx, cfg.ny, cfg.nz, elem_bytes());
+    for (int d = 0; d < 3; d++) trap.p0[d] = cfg.origin[d];
+    trap.a = cfg.a;
+    trap.rr = cfg.trap_ratio * cfg.trap_ratio;
+    // default shape per precision, the fastest measured on MI355X at 256^3 cells:
+    // FP64 DPP lane shifts (the cache path bounds it), FP32 loads (the VALU does)
+    tile = cfg.tile != 0 ? (int)cfg.tile - 1 : cfg.precision == GPE_FP64 ? 0 : 1;
+    set_coefs();
+    GPE_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    GPE_HIP_TRY(hipEventCreate(&ev0));
+    GPE_HIP_TRY(hipEventCreate(&ev1));
+    const size_t fbytes = (size_t)L.elems * elem_bytes();
+    for (int f = 0; f < 2; f++) {
+        if (hipMalloc(&field[f], fbytes) != hipSuccess) {
+            field[f] = nullptr;
+            set_error("gpe_primal_create: cannot allocate %zu bytes of device memory", fbytes);
+            return GPE_ENOMEM;
+        }
+        // the zero halo is never written afterwards
+        GPE_HIP_TRY(hipMemsetAsync(field[f], 0, fbytes, stream));
+    }
+    npartials = primal_obs_blocks(L);
+    for (int t = 0; t < kNumPrimalTiles; t++) npartials = std::max(npartials, primal_blocks(t, L));
+    GPE_HIP_TRY(hipMalloc(&partials, (size_t)npartials * 4 * sizeof(double)));
+    GPE_HIP_TRY(hipMalloc(&scal, 8 * sizeof(double)));
+    const double init_scal[8] = {1.0, 1.0, 0, 0, 0, 0, 0, 0};
+    GPE_HIP_TRY(hipMemcpyAsync(scal, init_scal, sizeof(init_scal), hipMemcpyHostToDevice, stream));
+    GPE_HIP_TRY(hipStreamSynchronize(stream));
+    return GPE_OK;
+}
+
+// Host [k][j][i][s] complex double <-> device class rows.
+template <typename T>
+gpe_status gpe_primal::upload(int f, const double *h) {
+    std::vector<T> d((size_t)L.elems * 2, T(0));
+    size_t n = 0;
+    for (int k = 0; k < L.nz; k++)
+        for (int j = 0; j < L.ny; j++)
+            for (int i = 0; i < L.nx; i++)
+                for (int s = 0; s < 2; s++, n++) {
+                    const long long e = L.origin + (long long)k * L.sz + (long long)j * L.sy + s * L.xp + i;
+                    d[2 * e] = (T)h[2 * n];
+                    d[2 * e + 1] = (T)h[2 * n + 1];
+                }
+    GPE_HIP_TRY(hipMemcpyAsync(field[f], d.data(), d.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    GPE_HIP_TRY(hipStreamSynchronize(stream));
+    return GPE_OK;
+}
+
+template <typename T>
+gpe_status gpe_primal::download(int f, double *h) {
+    std::vector<T> d((size_t)L.elems * 2);
+    GPE_HIP_TRY(hipMemcpyAsync(d.data(), field[f], d.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+    GPE_HIP_TRY(hipStreamSynchronize(stream));
+    size_t n = 0;
+    for (int k = 0; k < L.nz; k++)
+        for (int j = 0; j < L.ny; j++)
+            for (int i = 0; i < L.nx; i++)
+                for (int s = 0; s < 2; s++, n++) {
+                    const long long e = L.origin + (long long)k * L.sz + (long long)j * L.sy + s * L.xp + i;
+                    h[2 * n] = (double)d[2 * e];
+                    h[2 * n + 1] = (double)d[2 * e + 1];
+                }
+    return GPE_OK;
+}
+
+template <typename T, int FORM>
+gpe_status gpe_primal::enqueue(uint64_t nsteps, uint64_t *launches) {
+    using C = cplx_t<T>;
+    PCoef<T> H;
+    H.cD = (T)coef[0];
+    H.cA = (T)coef[1];
+    H.c0 = (T)coef[2];
+    H.g = (T)coef[3];
+    const double dt = cfg.dt;
+    uint64_t nl = 0;
+    hipError_t e = hipSuccess;
+    if (cfg.mode == GPE_REAL_TIME) {
+        C *ev = (C *)field[ie], *od = (C *)field[io];
+        for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
+            // the even update walks the tiles backwards, starting on the layers
+            // the odd update touched last
+            e = launch_primal<T, FORM, MODE_REAL>(tile, od, ev, L, H, trap, dt, nullptr, nullptr, stream, false);
+            if (e == hipSuccess)
+                e = launch_primal<T, FORM, MODE_REAL>(tile, ev, od, L, H, trap, dt, nullptr, nullptr, stream, true);
+            nl += 2;
+        }
+    } else {
+        // Euler step into the other buffer; the renormalisation is folded into
+        // the next read through scal[1] = 1/sqrt(N).
+        const long long nb = primal_blocks(tile, L);
+        for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
+            C *in = (C *)field[ie], *out = (C *)field[io];
+            const bool rev = (steps_done + s) & 1;
+            e = launch_primal<T, FORM, MODE_IMAG>(tile, out, in, L, H, trap, dt, scal + 1, partials, stream, rev);
+            if (e != hipSuccess) break;
+            hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(256), 0, stream, partials, nb, w, 1, scal);
+            e = hipGetLastError();
+            std::swap(ie, io);
+            nl += 1;
+        }
+        if (e == hipSuccess && nsteps > 0) {
+            // leave a normalised field behind: field[ie] *= scal[1], scal[1] = 1
+            hipLaunchKernelGGL((k_scale_field<T>), dim3(1024), dim3(256), 0, stream, (C *)field[ie], L.elems, scal + 1);
+            e = hipGetLastError();
+            const double one = 1.0;
+            if (e == hipSuccess) e = hipMemcpyAsync(scal + 1, &one, sizeof(double), hipMemcpyHostToDevice, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);  // `one` lives on this stack frame
+        }
+    }
+    if (e != hipSuccess) {
+        set_error("gpe_primal_step: launch failed: %s", hipGetErrorString(e));
+        return GPE_EDEVICE;
+    }
+    steps_done += nsteps;
+    if (launches) *launches = nl;
+    return GPE_OK;
+}
+
+template <typename T>
+gpe_status gpe_primal::observables_t(int f, double out[4]) {
+    const dim3 grid((L.nx + 63) / 64, (L.ny + kObsTY - 1) / kObsTY, L.nz), block(64, kObsTY);
+    const double a2 = cfg.a * cfg.a;
+    hipLaunchKernelGGL((k_primal_observables<T>), grid, block, 0, stream, (const cplx_t<T> *)field[f], L, trap,
+                       0.75 / a2, 0.25 / a2, cfg.g, partials);
+    GPE_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_finalize<4>), dim3(1), dim3(256), 0, stream, partials, primal_obs_blocks(L), w, 0, scal + 4);
+    GPE_HIP_TRY(hipGetLastError());
+    GPE_HIP_TRY(hipMemcpyAsync(out, scal + 4, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    GPE_HIP_TRY(hipStreamSynchronize(stream));
+    return GPE_OK;
+}
+
+extern "C" {
+
+void gpe_primal_coefficients(double a, double dt, double g, double out[4]) {
+    const double a2 = a * a;
+    out[0] = (-0.375 / a2) * dt;
+    out[1] = (-0.125 / a2) * dt;
+    out[2] = (3.75 / a2) * dt;
+    out[3] = g * dt;
+}
+
+void gpe_primal_centered_origin(const uint32_t dims[3], double a, double origin[3]) {
+    for (int d = 0; d < 3; d++) origin[d] = -0.5 * a * ((double)dims[d] - 0.5);
+}
+
+void gpe_primal_positions(const uint32_t dims[3], const double origin[3], double a, double *xyz) {
+    size_t n = 0;
+    for (uint32_t k = 0; k < dims[2]; k++)
+        for (uint32_t j = 0; j < dims[1]; j++)
+            for (uint32_t i = 0; i < dims[0]; i++)
+                for (int s = 0; s < 2; s++, n++) {
+                    const double off = s ? 0.5 : 0.0;
+                    xyz[3 * n] = origin[0] + a * ((double)i + off);
+                    xyz[3 * n + 1] = origin[1] + a * ((double)j + off);
+                    xyz[3 * n + 2] = origin[2] + a * ((double)k + off);
+                }
+}
+
+void gpe_primal_trap_potential(const uint32_t dims[3], const double origin[3], double a, double ratio, double *V) {
+    PTrap t{};
+    for (int d = 0; d < 3; d++) t.p0[d] = origin[d];
+    t.a = a;
+    t.rr = ratio * ratio;
+    size_t n = 0;
+    for (uint32_t k = 0; k < dims[2]; k++)
+        for (uint32_t j = 0; j < dims[1]; j++)
+            for (uint32_t i = 0; i < dims[0]; i++)
+                for (int s = 0; s < 2; s++, n++) V[n] = site_v(t, (int)i, (int)j, (int)k, s ? 0.5 : 0.0);
+}
+
+int gpe_primal_tiles(void) { return kNumPrimalTiles; }
+
+gpe_status gpe_primal_create(const gpe_primal_config *cfg, gpe_primal **out) {
+    if (!cfg || !out) {
+        set_error("gpe_primal_create: null argument");
+        return GPE_EINVAL;
+    }
+    *out = nullptr;
+    if (cfg->nx == 0 || cfg->ny == 0 || cfg->nz == 0 || cfg->nx > 65536 || cfg->ny > 65536 || cfg->nz > 65536) {
+        set_error("gpe_primal_create: grid %u x %u x %u out of range", cfg->nx, cfg->ny, cfg->nz);
+        return GPE_EINVAL;
+    }
+    if (cfg->precision > GPE_FP32_BULK || cfg->mode > GPE_IMAG_TIME || cfg->tile > (uint32_t)kNumPrimalTiles) {
+        set_error("gpe_primal_create: bad precision, mode or tile");
+        return GPE_EINVAL;
+    }
+    if (!(cfg->a > 0) || !(cfg->dt > 0) || !std::isfinite(cfg->a) || !std::isfinite(cfg->dt) ||
+        !std::isfinite(cfg->g) || !std::isfinite(cfg->trap_ratio)) {
+        set_error("gpe_primal_create: a and dt must be positive, g and trap_ratio finite");
+        return GPE_EINVAL;
+    }
+    gpe_primal_config c = *cfg;
+    const uint32_t dims[3] = {c.nx, c.ny, c.nz};
+    if (!std::isfinite(c.origin[0]) || !std::isfinite(c.origin[1]) || !std::isfinite(c.origin[2]))
+        gpe_primal_centered_origin(dims, c.a, c.origin);
+    const double vmax = primal_vmax(dims, c.origin, c.a, c.trap_ratio);
+    if (!primal_stable(c.dt, c.a, vmax)) {
+        set_error("gpe_primal_create: dt = %g violates dt*(7.5/a^2 + max V) < 2 (max V = %g)", c.dt, vmax);
+        return GPE_EUNSTABLE;
+    }
+    gpe_primal *p = new (std::nothrow) gpe_primal;
+    if (!p) {
+        set_error("gpe_primal_create: out of host memory");
+        return GPE_ENOMEM;
+    }
+    p->cfg = c;
+    const gpe_status st = p->init();
+    if (st != GPE_OK) {
+        delete p;
+        return st;
+    }
+    *out = p;
+    return GPE_OK;
+}
+
+void gpe_primal_destroy(gpe_primal *p) { delete p; }
+
+gpe_status gpe_primal_set_state(gpe_primal *p, const double *even, const double *odd) {
+    if (!p || !even) {
+        set_error("gpe_primal_set_state: null argument");
+        return GPE_EINVAL;
+    }
+    const size_t n = 2 * p->sites();
+    for (size_t e = 0; e < n; e++)
+        if (!std::isfinite(even[e]) || (odd && !std::isfinite(odd[e]))) {
+            set_error("gpe_primal_set_state: non-finite value at %zu", e);
+            return GPE_EUNSTABLE;
+        }
+    GPE_HIP_TRY(hipStreamSynchronize(p->stream));
+    const bool fp64 = p->cfg.precision == GPE_FP64;
+    gpe_status st = fp64 ? p->upload<double>(p->ie, even) : p->upload<float>(p->ie, even);
+    if (st != GPE_OK) return st;
+    p->odd_valid = false;
+    if (odd) {
+        st = fp64 ? p->upload<double>(p->io, odd) : p->upload<float>(p->io, odd);
+        if (st != GPE_OK) return st;
+        p->odd_valid = true;
+    }
+    return GPE_OK;
+}
+
+gpe_status gpe_primal_get_state(gpe_primal *p, double *even, double *odd) {
+    if (!p || !even) {
+        set_error("gpe_primal_get_state: null argument");
+        return GPE_EINVAL;
+    }
+    const bool fp64 = p->cfg.precision == GPE_FP64;
+    gpe_status st = fp64 ? p->download<double>(p->ie, even) : p->download<float>(p->ie, even);
+    if (st == GPE_OK && odd) st = fp64 ? p->download<double>(p->io, odd) : p->download<float>(p->io, odd);
+    return st;
+}
+
+gpe_status gpe_primal_set_mode(gpe_primal *p, uint32_t mode, double dt) {
+    if (!p || mode > GPE_IMAG_TIME || !(dt > 0) || !std::isfinite(dt)) {
+        set_error("gpe_primal_set_mode: invalid argument");
+        return GPE_EINVAL;
+    }
+    const uint32_t dims[3] = {p->cfg.nx, p->cfg.ny, p->cfg.nz};
+    if (!primal_stable(dt, p->cfg.a, primal_vmax(dims, p->cfg.origin, p->cfg.a, p->cfg.trap_ratio))) {
+        set_error("gpe_primal_set_mode: dt = %g violates dt*(7.5/a^2 + max V) < 2", dt);
+        return GPE_EUNSTABLE;
+    }
+    GPE_HIP_TRY(hipStreamSynchronize(p->stream));
+    if (mode != p->cfg.mode) p->odd_valid = false;
+    p->cfg.mode = mode;
+    p->cfg.dt = dt;
+    p->set_coefs();
+    return GPE_OK;
+}
+
+gpe_status gpe_primal_init_odd(gpe_primal *p, double mu) {
+    if (!p || !std::isfinite(mu)) {
+        set_error("gpe_primal_init_odd: invalid argument");
+        return GPE_EINVAL;
+    }
+    // getPhase(-dt/2) = (cos(-mu dt/2), -sin(-mu dt/2)) (VortexState.cpp:198-202)
+    const double ph = (-0.5 * p->cfg.dt) * mu;
+    const double cr = std::cos(ph), ci = -std::sin(ph);
+    if (p->cfg.precision == GPE_FP64)
+        hipLaunchKernelGGL((k_phase_copy<double>), dim3(1024), dim3(256), 0, p->stream, (double2 *)p->field[p->io],
+                           (const double2 *)p->field[p->ie], p->L.elems, cr, ci);
+    else
+        hipLaunchKernelGGL((k_phase_copy<float>), dim3(1024), dim3(256), 0, p->stream, (float2 *)p->field[p->io],
+                           (const float2 *)p->field[p->ie], p->L.elems, cr, ci);
+    GPE_HIP_TRY(hipGetLastError());
+    GPE_HIP_TRY(hipStreamSynchronize(p->stream));
+    p->odd_valid = true;
+    return GPE_OK;
+}
+
+static gpe_status primal_step(gpe_primal *p, uint64_t nsteps, uint64_t *launches) {
+    if (p->cfg.mode == GPE_REAL_TIME && !p->odd_valid) {
+        set_error("gpe_primal_step: real-time stepping needs an odd field (set_state or init_odd)");
+        return GPE_EINVAL;
+    }
+    if (p->cfg.precision == GPE_FP64) return p->enqueue<double, FORM_REF>(nsteps, launches);
+    return p->enqueue<float, FORM_DIFF>(nsteps, launches);
+}
+
+gpe_status gpe_primal_step(gpe_primal *p, uint64_t nsteps) {
+    if (!p) {
+        set_error("gpe_primal_step: null solver");
+        return GPE_EINVAL;
+    }
+    return primal_step(p, nsteps, nullptr);
+}
+
+gpe_status gpe_primal_time_steps(gpe_primal *p, uint64_t nsteps, double *elapsed_ms, uint64_t *launches) {
+    if (!p || !elapsed_ms) {
+        set_error("gpe_primal_time_steps: null argument");
+        return GPE_EINVAL;
+    }
+    GPE_HIP_TRY(hipEventRecord(p->ev0, p->stream));
+    const gpe_status st = primal_step(p, nsteps, launches);
+    if (st != GPE_OK) return st;
+    GPE_HIP_TRY(hipEventRecord(p->ev1, p->stream));
+    GPE_HIP_TRY(hipEventSynchronize(p->ev1));
+    float ms = 0.0f;
+    GPE_HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+    *elapsed_ms = ms;
+    return GPE_OK;
+}
+
+gpe_status gpe_primal_observables(gpe_primal *p, int field, double out[4]) {
+    if (!p || !out || (field != 0 && field != 1)) {
+        set_error("gpe_primal_observables: invalid argument");
+        return GPE_EINVAL;
+    }
+    const int f = field == 0 ? p->ie : p->io;
+    if (p->cfg.precision == GPE_FP64) return p->observables_t<double>(f, out);
+    return p->observables_t<float>(f, out);
+}
+
+gpe_status gpe_primal_synchronize(gpe_primal *p) {
+    if (!p) {
+        set_error("gpe_primal_synchronize: null solver");
+        return GPE_EINVAL;
+    }
+    GPE_HIP_TRY(hipStreamSynchronize(p->stream));
+    return GPE_OK;
+}
+
+}  // extern "C"
