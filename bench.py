@@ -48,9 +48,11 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson serving sample (0 = skip)")
-    ap.add_argument("--suite", default="main", choices=["main", "precision", "serving"],
+    ap.add_argument("--suite", default="main", choices=["main", "precision", "serving", "primal"],
                     help="main: the JSON line of record; precision: 256^3 FP32-vs-FP64 error study; "
-                         "serving: the Poisson serving stream alone (use --serve-seconds 600)")
+                         "serving: the Poisson serving stream alone (use --serve-seconds 600); "
+                         "primal: the 14-point primal stepper's tile sweep alone (with the 1 GiB copy "
+                         "probe that calibrates rocprof traffic counters)")
     ap.add_argument("--serve-policy", default="shortest_first", choices=["shortest_first", "fifo"],
                     help="admission policy of --suite serving (shortest_first is capped at 64 M resident nodes)")
     ap.add_argument("--primal-cells", default="128,256",
@@ -326,11 +328,17 @@ def precision_study(args):
 
 def main():
     args = parse()
-    if args.suite in ("precision", "serving"):
+    if args.suite in ("precision", "serving", "primal"):
         if args.gpus != 1:
             sys.exit(2)
         if args.suite == "precision":
             precision_study(args)
+        elif args.suite == "primal":
+            import vaikkari_amd as va
+
+            copy_gbs = va.copy_bandwidth(1 << 30, 3)
+            res = primal_workload([int(x) for x in args.primal_cells.split(",")], args.primal_steps)
+            print(json.dumps({"suite": "primal", "hbm_copy_GBs": copy_gbs, "primal14": res}), flush=True)
         else:
             serving_suite(args)
         return

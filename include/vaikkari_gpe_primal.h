@@ -63,10 +63,11 @@ gpe_status gpe_primal_time_steps(gpe_primal *p, uint64_t nsteps, double *elapsed
  * I = w sum g|psi|^4. */
 gpe_status gpe_primal_observables(gpe_primal *p, int field, double out[4]);
 gpe_status gpe_primal_synchronize(gpe_primal *p);
-/* Number of tile shapes: waves per block (2, 4, 8), block order over the
- * XCDs, streaming read-modify-write, and how a lane gets its x-neighbours
- * (DPP lane shift, cross-lane shuffle or a second load).  Every shape gives
- * the same bits. */
+/* Number of tile shapes: one layer per block or a z-march of 4 to 16
+ * layers, waves per block (2, 4, 8), block order over the XCDs, streaming
+ * read-modify-write, and how a lane gets its x-neighbours (DPP lane shift,
+ * cross-lane shuffle or a second load).  Every shape gives the same bits.
+ * Tile 0 in the config picks the z-march of 16 layers (tile 13). */
 int gpe_primal_tiles(void);
 
 /* Host helpers, the arithmetic the kernels use. */

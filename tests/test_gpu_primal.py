@@ -56,9 +56,9 @@ def setup(dims, a, dt, g, ratio):
 
 
 @pytest.mark.parametrize("dims", SHAPES)
-@pytest.mark.parametrize("tile", range(1, 12))
+@pytest.mark.parametrize("tile", range(1, 19))
 def test_fp64_leapfrog_bit_exact(po, dims, tile):
-    assert va.primal_tiles() == 11
+    assert va.primal_tiles() == 18
     a, dt, g, ratio = 0.35, 0.004, 1.3, 1.3
     origin, V, coef = setup(dims, a, dt, g, ratio)
     even, odd = random_fields(dims, seed=11 + tile)
@@ -73,7 +73,7 @@ def test_fp64_leapfrog_bit_exact(po, dims, tile):
 
 
 @pytest.mark.parametrize("dims", SHAPES[:3])
-@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("tile", [1, 2, 13])
 def test_fp64_imaginary_time(po, dims, tile):
     a, dt, g, ratio = 0.35, 0.004, 1.3, 1.3
     origin, V, coef = setup(dims, a, dt, g, ratio)
@@ -119,7 +119,7 @@ def test_init_odd_phase():
     np.testing.assert_allclose(odd.view(np.complex128), want, rtol=1e-15, atol=1e-18)
 
 
-@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("tile", [1, 2, 13])
 def test_fp32_leapfrog_tolerance(po, tile):
     """FP32 storage and stencil (difference form) against the FP64 oracle:
     relative L2 error <= 1e-5 after 3 leapfrog steps."""
@@ -141,8 +141,8 @@ def test_fp32_leapfrog_tolerance(po, tile):
 
 def test_fp64_bench_size_bit_exact(po):
     """128^3 cells (4.19M sites), the bench workload's shape: two leapfrog steps
-    equal the oracle bit for bit under the default tile and the
-    plain-order tile, and the imaginary-time norm stays 1."""
+    equal the oracle bit for bit under the default tile, the plain-order
+    tile and a z-march tile, and the imaginary-time norm stays 1."""
     dims = (128, 128, 128)
     a, dt, g, ratio = 0.25, 0.002, 1.0, 1.0
     origin, V, coef = setup(dims, a, dt, g, ratio)
@@ -150,7 +150,7 @@ def test_fp64_bench_size_bit_exact(po):
     odd = (psi.view(np.complex128) * complex(math.cos(-0.5 * dt * 5.0), -math.sin(-0.5 * dt * 5.0))).view(
         np.float64).copy()
     re, ro = po.leapfrog(psi, odd, V * dt, dims, coef, 2)
-    for tile in (1, 7):
+    for tile in (1, 7, 13):
         announce(f"primal fp64 128^3 leapfrog tile={tile}")
         with va.PrimalSolver(dims, a, dt, g, origin, ratio, "fp64", "real", tile) as s:
             s.set_state(psi, odd)

@@ -128,9 +128,10 @@ __device__ __forceinline__ cplx_t<T> from_right(cplx_t<T> v) {
     return v;
 }
 
+// Reads of the updated field for the in-place read-modify-write, issued
+// together with the neighbour loads (non-temporal when NT).
 template <typename T, bool NT>
-__device__ __forceinline__ void rmw_site(cplx_t<T> *__restrict__ next, long long at, cplx_t<T> s) {
-#pragma clang fp contract(off)
+__device__ __forceinline__ cplx_t<T> load_next(const cplx_t<T> *__restrict__ next, long long at) {
     cplx_t<T> n;
     if (NT) {
         n.x = __builtin_nontemporal_load(&next[at].x);
@@ -138,14 +139,33 @@ __device__ __forceinline__ void rmw_site(cplx_t<T> *__restrict__ next, long long
     } else {
         n = next[at];
     }
-    n.x = n.x + s.y;
-    n.y = n.y + (-s.x);
-    if (NT) {
-        __builtin_nontemporal_store(n.x, &next[at].x);
-        __builtin_nontemporal_store(n.y, &next[at].y);
-    } else {
-        next[at] = n;
+    return n;
+}
+
+// Finishes one site: real time next = n + (s.y, -s.x) with n the value read
+// from next (GPUGrossPitaevskii_BCC.cu:215); imaginary time out = psi - s,
+// streamed.  Returns |out|^2 (imaginary time) or 0.
+template <typename T, int MODE, bool NT>
+__device__ __forceinline__ double finish_site(cplx_t<T> *__restrict__ next, long long at, cplx_t<T> n, cplx_t<T> psi,
+                                              cplx_t<T> s) {
+#pragma clang fp contract(off)
+    cplx_t<T> o;
+    if (MODE == MODE_REAL) {
+        o.x = n.x + s.y;
+        o.y = n.y + (-s.x);
+        if (NT) {
+            __builtin_nontemporal_store(o.x, &next[at].x);
+            __builtin_nontemporal_store(o.y, &next[at].y);
+        } else {
+            next[at] = o;
+        }
+        return 0.0;
     }
+    o.x = psi.x - s.x;
+    o.y = psi.y - s.y;
+    __builtin_nontemporal_store(o.x, &next[at].x);
+    __builtin_nontemporal_store(o.y, &next[at].y);
+    return (double)o.x * (double)o.x + (double)o.y * (double)o.y;
 }
 
 // One cell (both sites) of an update pass; returns its sum of |out|^2
@@ -157,9 +177,11 @@ __device__ __forceinline__ void rmw_site(cplx_t<T> *__restrict__ next, long long
 // The two sites need 12 distinct class rows of prev -- corner rows (j,k),
 // (j+-1,k), (j,k+-1), (j+1,k+1) and centre rows (j,k), (j-1,k), (j,k-1),
 // (j-1,k-1), (j+1,k), (j,k+1) -- each loaded once, plus 10 of them shifted
-// by one cell in x (from the adjacent lane, or loaded when XSM == XS_LOAD).
-// The corner site is finished before the centre site's three remaining
-// rows are read, which keeps fewer values live.
+// by one cell in x: from the adjacent lane (XS_SHFL, XS_DPP; the lanes at
+// the two row ends load theirs, one instruction per left/right pair), or
+// loaded again (XS_LOAD).  Every load of the cell, the two reads of next
+// included, is issued before the first use of any of them, so a wave waits
+// for memory once per cell rather than once per dependent step.
 template <typename T, int FORM, int MODE, int TY, bool NT, int XSM>
 __device__ __forceinline__ double primal_cell(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                               const Layout &L, const PCoef<T> &H, const PTrap &trap, double dt,
@@ -189,6 +211,9 @@ __device__ __forceinline__ double primal_cell(cplx_t<T> *__restrict__ next, cons
     zero.y = (T)0;
     // lanes beyond the grid hold zeros: the halo value a neighbouring lane needs
     C c0 = zero, c1 = zero, cym = zero, cyp = zero, czm = zero, czp = zero, u0 = zero, u1 = zero, u2 = zero;
+    C v3 = zero, zyp = zero, zzp = zero;
+    C w0, w1, w2, w3, xm, xq, e1, e2, e3, zxp;  // x-shifted rows: w* centre rows at i-1, e* corner rows at i+1
+    C t[5], n0, n1;
     if (act) {
         c0 = ld(b0);
         c1 = ld(b1);
@@ -199,10 +224,33 @@ __device__ __forceinline__ double primal_cell(cplx_t<T> *__restrict__ next, cons
         cyp = ld(b0 + sy);
         czm = ld(b0 - sz);
         czp = ld(b0 + sz);
+        v3 = ld(b0 + sz + sy);  // corner row (j+1,k+1)
+        zyp = ld(b1 + sy);      // centre rows (j+1,k), (j,k+1)
+        zzp = ld(b1 + sz);
+        if (XSM == XS_LOAD) {
+            w0 = ld(b1 - sz - sy - 1);
+            w1 = ld(b1 - sz - 1);
+            w2 = ld(b1 - sy - 1);
+            w3 = ld(b1 - 1);
+            xm = ld(b0 - 1);
+            xq = ld(b0 + 1);
+            e1 = ld(b0 + sy + 1);
+            e2 = ld(b0 + sz + 1);
+            e3 = ld(b0 + sz + sy + 1);
+            zxp = ld(b1 + 1);
+        }
     }
-    double part = 0.0;
-    // ---- corner site: diagonal centres at x = i-1, i of rows u0, u1, u2, c1
-    C w0, w1, w2, w3, xm, xq;  // w*: rows at i-1; xm, xq: c0 at i-1, i+1
+    if (XSM != XS_LOAD && (left_end || right_end)) {
+        t[0] = ld(left_end ? b1 - sz - sy - 1 : b0 + 1);  // w0 | xq
+        t[1] = ld(left_end ? b1 - sz - 1 : b0 + sy + 1);  // w1 | e1
+        t[2] = ld(left_end ? b1 - sy - 1 : b0 + sz + 1);  // w2 | e2
+        t[3] = ld(left_end ? b1 - 1 : b0 + sz + sy + 1);  // w3 | e3
+        t[4] = ld(left_end ? b0 - 1 : b1 + 1);            // xm | zxp
+    }
+    if (MODE == MODE_REAL && act) {
+        n0 = load_next<T, NT>(next, b0);
+        n1 = load_next<T, NT>(next, b1);
+    }
     if (XSM != XS_LOAD) {
         w0 = from_left<XSM, T>(u0);
         w1 = from_left<XSM, T>(u1);
@@ -210,77 +258,35 @@ __device__ __forceinline__ double primal_cell(cplx_t<T> *__restrict__ next, cons
         w3 = from_left<XSM, T>(c1);
         xm = from_left<XSM, T>(c0);
         xq = from_right<XSM, T>(c0);
-        if (left_end) {
-            w0 = ld(b1 - sz - sy - 1);
-            w1 = ld(b1 - sz - 1);
-            w2 = ld(b1 - sy - 1);
-            w3 = ld(b1 - 1);
-            xm = ld(b0 - 1);
-        }
-        if (right_end) xq = ld(b0 + 1);
-    } else if (act) {
-        w0 = ld(b1 - sz - sy - 1);
-        w1 = ld(b1 - sz - 1);
-        w2 = ld(b1 - sy - 1);
-        w3 = ld(b1 - 1);
-        xm = ld(b0 - 1);
-        xq = ld(b0 + 1);
-    }
-    if (act) {
-        const C dC[8] = {w0, u0, w1, u1, w2, u2, w3, c1};
-        const C aC[6] = {xm, xq, cym, cyp, czm, czp};
-        const C s0 = primal_apply<T, FORM>(dC, aC, c0, (T)(site_v(trap, i, j, k, 0.0) * dt), H);
-        if (MODE == MODE_REAL) {
-            rmw_site<T, NT>(next, b0, s0);
-        } else {
-            C o;
-            o.x = c0.x - s0.x;
-            o.y = c0.y - s0.y;
-            __builtin_nontemporal_store(o.x, &next[b0].x);
-            __builtin_nontemporal_store(o.y, &next[b0].y);
-            part = (double)o.x * (double)o.x + (double)o.y * (double)o.y;
-        }
-    }
-    // ---- centre site: diagonal corners at x = i, i+1 of rows c0, cyp, czp, v3;
-    // axial centres c1 at i-1 (= w3), i+1, and rows u2, zyp, u1, zzp
-    C v3 = zero, zyp = zero, zzp = zero;
-    if (act) {
-        v3 = ld(b0 + sz + sy);  // corner row (j+1,k+1)
-        zyp = ld(b1 + sy);      // centre rows (j+1,k), (j,k+1)
-        zzp = ld(b1 + sz);
-    }
-    C e1, e2, e3, zxp;  // rows at i+1
-    if (XSM != XS_LOAD) {
         e1 = from_right<XSM, T>(cyp);
         e2 = from_right<XSM, T>(czp);
         e3 = from_right<XSM, T>(v3);
         zxp = from_right<XSM, T>(c1);
-        if (right_end) {
-            e1 = ld(b0 + sy + 1);
-            e2 = ld(b0 + sz + 1);
-            e3 = ld(b0 + sz + sy + 1);
-            zxp = ld(b1 + 1);
+        if (left_end) {
+            w0 = t[0];
+            w1 = t[1];
+            w2 = t[2];
+            w3 = t[3];
+            xm = t[4];
         }
-    } else if (act) {
-        e1 = ld(b0 + sy + 1);
-        e2 = ld(b0 + sz + 1);
-        e3 = ld(b0 + sz + sy + 1);
-        zxp = ld(b1 + 1);
+        if (right_end) {
+            xq = t[0];
+            e1 = t[1];
+            e2 = t[2];
+            e3 = t[3];
+            zxp = t[4];
+        }
     }
+    double part = 0.0;
     if (act) {
+        const C dC[8] = {w0, u0, w1, u1, w2, u2, w3, c1};
+        const C aC[6] = {xm, xq, cym, cyp, czm, czp};
+        const C s0 = primal_apply<T, FORM>(dC, aC, c0, (T)(site_v(trap, i, j, k, 0.0) * dt), H);
         const C dZ[8] = {c0, xq, cyp, e1, czp, e2, v3, e3};
         const C aZ[6] = {w3, zxp, u2, zyp, u1, zzp};
         const C s1 = primal_apply<T, FORM>(dZ, aZ, c1, (T)(site_v(trap, i, j, k, 0.5) * dt), H);
-        if (MODE == MODE_REAL) {
-            rmw_site<T, NT>(next, b1, s1);
-        } else {
-            C o;
-            o.x = c1.x - s1.x;
-            o.y = c1.y - s1.y;
-            __builtin_nontemporal_store(o.x, &next[b1].x);
-            __builtin_nontemporal_store(o.y, &next[b1].y);
-            part += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
-        }
+        part = finish_site<T, MODE, NT>(next, b0, n0, c0, s0);
+        part += finish_site<T, MODE, NT>(next, b1, n1, c1, s1);
     }
     return part;
 }
@@ -299,6 +305,159 @@ __global__ void __launch_bounds__(64 * TY) k_primal(cplx_t<T> *__restrict__ next
     part[0] = primal_cell<T, FORM, MODE, TY, NT, XSM>(next, prev, L, H, trap, dt, scale, bx, by, bz, G.lsh);
     if (MODE == MODE_IMAG) {
         // partials in logical tile order (independent of the block-to-XCD order)
+        block_sum<1, TY>(part, lds);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partials[bx + G.gx * (by + G.gy * bz)] = part[0];
+    }
+}
+
+// Z-MARCH: one thread per (i, j) column walking CZ layers upwards.  Of the
+// 12 class rows a cell needs, the previous layer's rows stay in registers
+// (own corner and centre, the centre rows (j, k-1) and (j-1, k-1), the corner
+// row (j+1, k) fetched as (j+1, k+1) one layer earlier), and so do 4 of the
+// 10 x-shifted copies; per layer a lane issues 6 row loads -- the new layer's
+// corner rows (j, k+1), (j+1, k+1) and centre row (j, k+1), and the
+// y-neighbours corner (j-1, k), centre (j-+1, k), which the neighbouring
+// waves of the block read too -- and takes the 6 new x-shifts by DPP.  The
+// lanes at the two row ends fetch their x-neighbours with 3 shared loads
+// (left end: x-1 values, right end: x+1 values, one instruction per pair).
+// All loads of layer k+1, the reads of next included, are issued before
+// layer k is computed (one layer of software prefetch), so memory latency
+// overlaps the arithmetic.  CZ is small (4 to 16) because the layer loop is
+// fully unrolled.  Same arithmetic and neighbour order as
+// primal_cell: bit-identical.
+template <typename T>
+struct ZLayer {
+    cplx_t<T> czp, v3, zzp, cym, u2, zyp;  // rows of prev
+    cplx_t<T> t0, t1, t2;                  // row-end x-neighbours
+    cplx_t<T> n0, n1;                      // next, for the read-modify-write
+};
+
+template <typename T, int FORM, int MODE, int TY, int CZ, bool NT>
+__device__ __forceinline__ double primal_column(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                                const Layout &L, const PCoef<T> &H, const PTrap &trap, double dt,
+                                                const double *__restrict__ scale, int bx, int by, int bz, int lsh) {
+#pragma clang fp contract(off)
+    using C = cplx_t<T>;
+    const int lw = 1 << lsh;
+    const int xl = threadIdx.x & (lw - 1);
+    const int i = (bx << lsh) + xl;
+    const int j = ((by * TY + threadIdx.y) << (6 - lsh)) + (threadIdx.x >> lsh);
+    const int k0 = bz * CZ;
+    const int k1 = min(k0 + CZ, L.nz);
+    const bool act = i < L.nx && j < L.ny;
+    const bool left_end = act && xl == 0, right_end = act && xl == lw - 1;
+    const long long xp = L.xp, sy = L.sy, sz = L.sz;
+    // Loads are unconditional, so the layer code is straight-line and the
+    // compiler's wait counts stay exact: a lane beyond the grid reads the
+    // zero halo column x = -1 of an existing row instead (its row loads then
+    // give the zeros its neighbours' x-shifts need), and the lanes not at a
+    // row end point their row-end load at their own corner site.
+    const int ie = act ? i : -1;
+    const int je = act ? j : min(j, L.ny - 1);
+    T sc = (T)1;
+    if (MODE == MODE_IMAG) sc = (T)(*scale);
+    auto ld = [&](long long at) {
+        C v = prev[at];
+        if (MODE == MODE_IMAG) v = cscale<T>(v, sc);
+        return v;
+    };
+    // loads of one layer (b0: corner site of (ie, je, k))
+    auto fetch = [&](long long b0, ZLayer<T> &f) {
+        const long long b1 = b0 + xp;
+        f.czp = ld(b0 + sz);      // corner (j, k+1)
+        f.v3 = ld(b0 + sz + sy);  // corner (j+1, k+1)
+        f.zzp = ld(b1 + sz);      // centre (j, k+1)
+        f.cym = ld(b0 - sy);      // corner (j-1, k)
+        f.u2 = ld(b1 - sy);       // centre (j-1, k)
+        f.zyp = ld(b1 + sy);      // centre (j+1, k)
+        f.t0 = ld(left_end ? b1 - sy - 1 : right_end ? b0 + sz + 1 : b0);       // w2 | e2
+        f.t1 = ld(left_end ? b1 - 1 : right_end ? b0 + sz + sy + 1 : b0);       // w3 | e3
+        f.t2 = ld(left_end ? b0 - 1 : right_end ? b1 + 1 : b0);                 // xm | zxp
+        if (MODE == MODE_REAL) {
+            f.n0 = load_next<T, NT>(next, b0);
+            f.n1 = load_next<T, NT>(next, b1);
+        }
+    };
+    long long b0 = L.origin + (long long)k0 * sz + (long long)je * sy + ie;
+    // carried state at layer k
+    C c0 = ld(b0), c1 = ld(b0 + xp), czm = ld(b0 - sz), u1 = ld(b0 + xp - sz), u0 = ld(b0 + xp - sz - sy);
+    C cyp = ld(b0 + sy);
+    const C t0 = ld(left_end ? b0 + xp - sz - sy - 1 : right_end ? b0 + sy + 1 : b0);  // w0 | e1
+    const C t1 = ld(left_end ? b0 + xp - sz - 1 : right_end ? b0 + 1 : b0);            // w1 | xq
+    // two layer buffers used in turn (the loop is unrolled, so buf[kk & 1]
+    // names fixed registers and no copy waits for a load in flight)
+    ZLayer<T> buf[2];
+    fetch(b0, buf[0]);
+    C w0 = shift_lane_c<true>(u0), w1 = shift_lane_c<true>(u1);
+    C e1 = shift_lane_c<false>(cyp), xq = shift_lane_c<false>(c0);
+    if (left_end) {
+        w0 = t0;
+        w1 = t1;
+    }
+    if (right_end) {
+        e1 = t0;
+        xq = t1;
+    }
+    double part = 0.0;
+    // unrolled: with no loop back edge the compiler's wait counts track each
+    // load, so the wait for layer k leaves the prefetch of layer k+1 in flight
+#pragma unroll
+    for (int kk = 0; kk < CZ; kk++, b0 += sz) {
+        const int k = k0 + kk;
+        if (k >= k1) break;
+        ZLayer<T> &cur = buf[kk & 1];
+        if (k + 1 < k1) fetch(b0 + sz, buf[(kk + 1) & 1]);
+        // keep the scheduler from hoisting layer k's first uses (and their
+        // wait) above the prefetch loads
+        __builtin_amdgcn_sched_barrier(0);
+        C w2 = shift_lane_c<true>(cur.u2), w3 = shift_lane_c<true>(c1), xm = shift_lane_c<true>(c0);
+        C e2 = shift_lane_c<false>(cur.czp), e3 = shift_lane_c<false>(cur.v3), zxp = shift_lane_c<false>(c1);
+        if (left_end) {
+            w2 = cur.t0;
+            w3 = cur.t1;
+            xm = cur.t2;
+        }
+        if (right_end) {
+            e2 = cur.t0;
+            e3 = cur.t1;
+            zxp = cur.t2;
+        }
+        const C dC[8] = {w0, u0, w1, u1, w2, cur.u2, w3, c1};
+        const C aC[6] = {xm, xq, cur.cym, cyp, czm, cur.czp};
+        const C s0 = primal_apply<T, FORM>(dC, aC, c0, (T)(site_v(trap, i, j, k, 0.0) * dt), H);
+        const C dZ[8] = {c0, xq, cyp, e1, cur.czp, e2, cur.v3, e3};
+        const C aZ[6] = {w3, zxp, cur.u2, cur.zyp, u1, cur.zzp};
+        const C s1 = primal_apply<T, FORM>(dZ, aZ, c1, (T)(site_v(trap, i, j, k, 0.5) * dt), H);
+        if (act) {
+            part += finish_site<T, MODE, NT>(next, b0, cur.n0, c0, s0);
+            part += finish_site<T, MODE, NT>(next, b0 + xp, cur.n1, c1, s1);
+        }
+        // layer k+1 from what layer k read
+        czm = c0;
+        c0 = cur.czp;
+        u1 = c1;
+        c1 = cur.zzp;
+        u0 = cur.u2;
+        w0 = w2;
+        w1 = w3;
+        cyp = cur.v3;
+        e1 = e3;
+        xq = e2;
+    }
+    return part;
+}
+
+template <typename T, int FORM, int MODE, int TY, int CZ, bool NT>
+__global__ void __launch_bounds__(64 * TY) k_primal_zm(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                                       Layout L, PCoef<T> H, PTrap trap, double dt,
+                                                       const double *__restrict__ scale, double *__restrict__ partials,
+                                                       Grid3 G) {
+    __shared__ double lds[TY];
+    int bx, by, bz;
+    block_coords(G, bx, by, bz);
+    double part[1];
+    part[0] = primal_column<T, FORM, MODE, TY, CZ, NT>(next, prev, L, H, trap, dt, scale, bx, by, bz, G.lsh);
+    if (MODE == MODE_IMAG) {
         block_sum<1, TY>(part, lds);
         if (threadIdx.x == 0 && threadIdx.y == 0) partials[bx + G.gx * (by + G.gy * bz)] = part[0];
     }
@@ -359,14 +518,20 @@ __global__ void __launch_bounds__(64 * kObsTY) k_primal_observables(const cplx_t
 }
 
 // Tile shapes: waves per block, block order over the XCDs (block_coords),
-// non-temporal read-modify-write, x-neighbour exchange (XS_*).
+// non-temporal read-modify-write, x-neighbour exchange (XS_*), and layers per
+// block (0: one layer per block, primal_cell; > 0: a z-march, primal_column,
+// which always exchanges by DPP).
 struct PTile {
-    int ty, remap, nt, xs;
+    int ty, remap, nt, xs, cz;
 };
-constexpr PTile kPrimalTiles[] = {{4, 2, 1, XS_DPP}, {4, 2, 1, XS_LOAD}, {2, 2, 1, XS_DPP}, {8, 2, 1, XS_DPP},
-                                  {2, 2, 1, XS_LOAD}, {8, 2, 1, XS_LOAD}, {4, 0, 1, XS_DPP}, {4, 1, 1, XS_DPP},
-                                  {4, 2, 0, XS_DPP}, {4, 2, 1, XS_SHFL}, {4, 2, 0, XS_LOAD}};
+constexpr PTile kPrimalTiles[] = {
+    {4, 2, 1, XS_DPP, 0},  {4, 2, 1, XS_LOAD, 0}, {2, 2, 1, XS_DPP, 0},  {8, 2, 1, XS_DPP, 0},
+    {2, 2, 1, XS_LOAD, 0}, {8, 2, 1, XS_LOAD, 0}, {4, 0, 1, XS_DPP, 0},  {4, 1, 1, XS_DPP, 0},
+    {4, 2, 0, XS_DPP, 0},  {4, 2, 1, XS_SHFL, 0}, {4, 2, 0, XS_LOAD, 0}, {4, 2, 1, XS_DPP, 8},
+    {4, 2, 1, XS_DPP, 16}, {4, 2, 1, XS_DPP, 4},  {2, 2, 1, XS_DPP, 8},  {8, 2, 1, XS_DPP, 8},
+    {4, 0, 1, XS_DPP, 8},  {4, 2, 0, XS_DPP, 8}};
 constexpr int kNumPrimalTiles = sizeof(kPrimalTiles) / sizeof(kPrimalTiles[0]);
+constexpr int kDefaultPrimalTile = 12;  // {4, 2, 1, XS_DPP, 16}
 
 Layout primal_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes) {
     Layout L{};
@@ -389,7 +554,7 @@ Grid3 primal_grid(const PTile &t, const Layout &L) {
     const int lw = 1 << G.lsh, rows = t.ty * (64 >> G.lsh);
     G.gx = (L.nx + lw - 1) / lw;
     G.gy = (L.ny + rows - 1) / rows;
-    G.gz = L.nz;
+    G.gz = t.cz > 0 ? (L.nz + t.cz - 1) / t.cz : L.nz;
     G.remap = t.remap;
     if (G.remap == 2 && G.gy % 8 != 0) G.remap = 0;  // y-bands need gy divisible by the 8 XCDs
     return G;
@@ -415,8 +580,12 @@ hipError_t launch_primal(int tile, cplx_t<T> *next, const cplx_t<T> *prev, const
         constexpr PTile t = kPrimalTiles[IDX];
         Grid3 G = primal_grid(t, L);
         G.rev = reverse ? 1 : 0;
-        launch_t<T, FORM, MODE, t.ty, MODE == MODE_REAL && t.nt, t.xs>(next, prev, L, H, trap, dt, scale, partials,
-                                                                         G, st);
+        constexpr bool nt = MODE == MODE_REAL && t.nt;
+        if constexpr (t.cz > 0)
+            hipLaunchKernelGGL((k_primal_zm<T, FORM, MODE, t.ty, t.cz, nt>), dim3(G.gx * G.gy * G.gz), dim3(64, t.ty),
+                               0, st, next, prev, L, H, trap, dt, scale, partials, G);
+        else
+            launch_t<T, FORM, MODE, t.ty, nt, t.xs>(next, prev, L, H, trap, dt, scale, partials, G, st);
         return hipGetLastError();
     } else {
         return hipErrorInvalidValue;
@@ -490,9 +659,10 @@ gpe_status gpe_primal::init() {
     for (int d = 0; d < 3; d++) trap.p0[d] = cfg.origin[d];
     trap.a = cfg.a;
     trap.rr = cfg.trap_ratio * cfg.trap_ratio;
-    // default shape per precision, the fastest measured on MI355X at 256^3 cells:
-    // FP64 DPP lane shifts (the cache path bounds it), FP32 loads (the VALU does)
-    tile = cfg.tile != 0 ? (int)cfg.tile - 1 : cfg.precision == GPE_FP64 ? 0 : 1;
+    // default: the z-march of 16 layers with one layer of prefetch, 4 waves,
+    // y-bands per XCD, the fastest shape measured on MI355X at 256^3 cells in
+    // both precisions
+    tile = cfg.tile != 0 ? (int)cfg.tile - 1 : kDefaultPrimalTile;
     set_coefs();
     GPE_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     GPE_HIP_TRY(hipEventCreate(&ev0));

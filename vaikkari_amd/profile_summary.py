@@ -3,8 +3,10 @@
     python -m vaikkari_amd.profile_summary stats <kernel-trace db>
     python -m vaikkari_amd.profile_summary traffic <FETCH_SIZE db> <WRITE_SIZE db>
     python -m vaikkari_amd.profile_summary traffic-json <FETCH_SIZE db> <WRITE_SIZE db> <out.json> [workload]
+    python -m vaikkari_amd.profile_summary counters <pmc db> [kernel-name substring]
 
-``stats`` prints calls / total / average duration per kernel.  ``traffic``
+``stats`` prints calls / total / average duration per kernel; ``counters``
+prints the mean per dispatch of every counter of a --pmc run, per kernel.  ``traffic``
 prints HBM bytes per dispatch for every kernel from the FETCH_SIZE and
 WRITE_SIZE counters (KiB per dispatch), scaled by what the same counters read
 for the plain copy kernel k_copy_stream, whose true traffic is known (it reads
@@ -93,9 +95,28 @@ def traffic_json(db_fetch: str, db_write: str, out: str, workload: str = ""):
     return doc
 
 
+def counters(db: str, match: str = ""):
+    """Mean value per dispatch of every collected counter, per kernel whose name contains `match`."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, counter_name, value from counters_collection").fetchall()
+    per = defaultdict(lambda: defaultdict(list))
+    for name, cname, val in rows:
+        if match in name:
+            per[short(name)][cname].append(val)
+    names = sorted({c for k in per.values() for c in k})
+    print("kernel," + ",".join(names) + ",dispatches")
+    for k, cs in sorted(per.items()):
+        n = max(len(v) for v in cs.values())
+        print(k.replace(",", ";") + "," + ",".join(f"{sum(cs[c]) / len(cs[c]):.6g}" if cs.get(c) else "" for c in names)
+              + f",{n}")
+    return per
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "stats":
         stats(argv[1])
+    elif len(argv) >= 2 and argv[0] == "counters":
+        counters(argv[1], argv[2] if len(argv) > 2 else "")
     elif len(argv) >= 4 and argv[0] == "traffic-json":
         traffic_json(argv[1], argv[2], argv[3], " ".join(argv[4:]))
     elif len(argv) >= 3 and argv[0] == "traffic":
