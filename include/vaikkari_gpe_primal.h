@@ -67,7 +67,10 @@ gpe_status gpe_primal_synchronize(gpe_primal *p);
  * layers, waves per block (2, 4, 8), block order over the XCDs, streaming
  * read-modify-write, and how a lane gets its x-neighbours (DPP lane shift,
  * cross-lane shuffle or a second load).  Every shape gives the same bits.
- * Tile 0 in the config picks the z-march of 16 layers (tile 13). */
+ * Tile 0 in the config picks by size: when both fields fit in 192 MB (the
+ * Infinity Cache holds them), one layer per block with a cached
+ * read-modify-write (tile 9 FP64, tile 11 FP32); otherwise the z-march of
+ * 16 layers (tile 13). */
 int gpe_primal_tiles(void);
 
 /* Host helpers, the arithmetic the kernels use. */

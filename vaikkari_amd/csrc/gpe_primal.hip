@@ -532,6 +532,8 @@ constexpr PTile kPrimalTiles[] = {
     {4, 0, 1, XS_DPP, 8},  {4, 2, 0, XS_DPP, 8}};
 constexpr int kNumPrimalTiles = sizeof(kPrimalTiles) / sizeof(kPrimalTiles[0]);
 constexpr int kDefaultPrimalTile = 12;  // {4, 2, 1, XS_DPP, 16}
+constexpr int kCachedTileFP64 = 8;      // {4, 2, 0, XS_DPP, 0}
+constexpr int kCachedTileFP32 = 10;     // {4, 2, 0, XS_LOAD, 0}
 
 Layout primal_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes) {
     Layout L{};
@@ -659,10 +661,14 @@ gpe_status gpe_primal::init() {
     for (int d = 0; d < 3; d++) trap.p0[d] = cfg.origin[d];
     trap.a = cfg.a;
     trap.rr = cfg.trap_ratio * cfg.trap_ratio;
-    // default: the z-march of 16 layers with one layer of prefetch, 4 waves,
-    // y-bands per XCD, the fastest shape measured on MI355X at 256^3 cells in
-    // both precisions
-    tile = cfg.tile != 0 ? (int)cfg.tile - 1 : kDefaultPrimalTile;
+    // default by size, the fastest shapes measured on MI355X: when both fields
+    // fit the 256 MB Infinity Cache (128^3 cells) one layer per block with a
+    // cached read-modify-write (FP64 DPP exchange, FP32 loads); larger grids
+    // (256^3) the z-march of 16 layers with one layer of prefetch
+    const bool cached = 2.0 * (double)L.elems * (double)elem_bytes() < 192e6;
+    tile = cfg.tile != 0 ? (int)cfg.tile - 1
+           : cached      ? (cfg.precision == GPE_FP64 ? kCachedTileFP64 : kCachedTileFP32)
+                         : kDefaultPrimalTile;
     set_coefs();
     GPE_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     GPE_HIP_TRY(hipEventCreate(&ev0));
