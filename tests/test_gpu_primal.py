@@ -20,6 +20,8 @@ from oracle_lib import PrimalOracle
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(37, 19, 11), (64, 32, 6), (96, 8, 5), (5, 3, 2)]
+# degenerate grids: one cell, single rows and columns
+TINY = [(1, 1, 1), (2, 1, 3), (17, 1, 1), (1, 9, 2)]
 
 
 def announce(msg):
@@ -70,6 +72,22 @@ def test_fp64_leapfrog_bit_exact(po, dims, tile):
     re, ro = po.leapfrog(even, odd, V * dt, dims, coef, 3)
     assert np.array_equal(ge, re), f"max diff {np.max(np.abs(ge - re))}"
     assert np.array_equal(go, ro), f"max diff {np.max(np.abs(go - ro))}"
+
+
+@pytest.mark.parametrize("dims", TINY)
+@pytest.mark.parametrize("tile", [0, 1, 2, 10, 13, 14])
+def test_fp64_degenerate_grids_bit_exact(po, dims, tile):
+    """Grids of one cell or one row: every neighbour of the edge lanes is halo."""
+    a, dt, g, ratio = 0.35, 0.004, 1.3, 1.3
+    origin, V, coef = setup(dims, a, dt, g, ratio)
+    even, odd = random_fields(dims, seed=31 + tile)
+    announce(f"primal fp64 degenerate dims={dims} tile={tile}")
+    with va.PrimalSolver(dims, a, dt, g, origin, ratio, "fp64", "real", tile) as s:
+        s.set_state(even, odd)
+        s.step(4)
+        ge, go = s.get_state()
+    re, ro = po.leapfrog(even, odd, V * dt, dims, coef, 4)
+    assert np.array_equal(ge, re) and np.array_equal(go, ro)
 
 
 @pytest.mark.parametrize("dims", SHAPES[:3])
