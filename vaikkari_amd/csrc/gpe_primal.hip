@@ -51,6 +51,20 @@ __host__ __device__ inline double site_v(const PTrap &t, int i, int j, int k, do
     return 0.5 * (x * x + y * y + t.rr * z * z);
 }
 
+// site_v split for a column of fixed (i, j): the x^2 + y^2 part once, then
+// per layer the z part, in site_v's operation order (same bits).
+__device__ __forceinline__ double site_xy(const PTrap &t, int i, int j, double off) {
+#pragma clang fp contract(off)
+    const double x = t.p0[0] + t.a * ((double)i + off);
+    const double y = t.p0[1] + t.a * ((double)j + off);
+    return x * x + y * y;
+}
+__device__ __forceinline__ double site_v_xy(const PTrap &t, double xy, int k, double off) {
+#pragma clang fp contract(off)
+    const double z = t.p0[2] + t.a * ((double)k + off);
+    return 0.5 * (xy + t.rr * z * z);
+}
+
 // {cD, cA, c0, g} times dt in the arithmetic type.
 template <typename T>
 struct PCoef {
@@ -398,6 +412,7 @@ __device__ __forceinline__ double primal_column(cplx_t<T> *__restrict__ next, co
         e1 = t0;
         xq = t1;
     }
+    const double xy0 = site_xy(trap, i, j, 0.0), xy1 = site_xy(trap, i, j, 0.5);
     double part = 0.0;
     // unrolled: with no loop back edge the compiler's wait counts track each
     // load, so the wait for layer k leaves the prefetch of layer k+1 in flight
@@ -424,10 +439,10 @@ __device__ __forceinline__ double primal_column(cplx_t<T> *__restrict__ next, co
         }
         const C dC[8] = {w0, u0, w1, u1, w2, cur.u2, w3, c1};
         const C aC[6] = {xm, xq, cur.cym, cyp, czm, cur.czp};
-        const C s0 = primal_apply<T, FORM>(dC, aC, c0, (T)(site_v(trap, i, j, k, 0.0) * dt), H);
+        const C s0 = primal_apply<T, FORM>(dC, aC, c0, (T)(site_v_xy(trap, xy0, k, 0.0) * dt), H);
         const C dZ[8] = {c0, xq, cyp, e1, cur.czp, e2, cur.v3, e3};
         const C aZ[6] = {w3, zxp, cur.u2, cur.zyp, u1, cur.zzp};
-        const C s1 = primal_apply<T, FORM>(dZ, aZ, c1, (T)(site_v(trap, i, j, k, 0.5) * dt), H);
+        const C s1 = primal_apply<T, FORM>(dZ, aZ, c1, (T)(site_v_xy(trap, xy1, k, 0.5) * dt), H);
         if (act) {
             part += finish_site<T, MODE, NT>(next, b0, cur.n0, c0, s0);
             part += finish_site<T, MODE, NT>(next, b0 + xp, cur.n1, c1, s1);
