@@ -193,9 +193,11 @@ __device__ __forceinline__ double finish_site(cplx_t<T> *__restrict__ next, long
 // (j-1,k-1), (j+1,k), (j,k+1) -- each loaded once, plus 10 of them shifted
 // by one cell in x: from the adjacent lane (XS_SHFL, XS_DPP; the lanes at
 // the two row ends load theirs, one instruction per left/right pair), or
-// loaded again (XS_LOAD).  Every load of the cell, the two reads of next
-// included, is issued before the first use of any of them, so a wave waits
-// for memory once per cell rather than once per dependent step.
+// loaded again (XS_LOAD).  The source issues every load of the cell, the two
+// reads of next included, before the first use of any of them, so a wave
+// waits for memory about once per cell rather than once per dependent step
+// (with XS_LOAD's 24 loads the compiler interleaves some waits to save
+// registers).
 template <typename T, int FORM, int MODE, int TY, bool NT, int XSM>
 __device__ __forceinline__ double primal_cell(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                               const Layout &L, const PCoef<T> &H, const PTrap &trap, double dt,
