@@ -2,7 +2,8 @@
  * primal_oracle.c -- CPU restatement of the 14-point primal BCC stepper.
  *
  * TEST INFRASTRUCTURE ONLY.  Part of liboracle.so (oracle/Makefile): only
- * tests/ and bench.py's cpu_baseline may load it, and only as the checker.
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it, and only as the checker.
  * The product library never links or calls anything in oracle/.
  *
  * No reference stepper uses this operator.  It is the DEC Laplacian of
