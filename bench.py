@@ -262,6 +262,29 @@ def primal_workload(sizes, steps, warmup=5, log=log):
     return out
 
 
+def monitor_timing(s, reps=20):
+    """Cost of energy monitoring on the FP64 solver: gpe_step_observe (N, K, P, I
+    reduced inside the last step's odd half-update) against a plain step, and the
+    separate observables pass (k_observables + k_finalize) it replaces."""
+    def ms(fn):
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        s.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / reps
+    s.step(5)
+    # wall time of an observed step (one synchronisation: it returns the values)
+    # against a plain step followed by a synchronisation
+    plain_ms, _ = s.time_steps(reps)
+    plain_ms /= reps
+    observed = ms(lambda: s.step_observe(1))
+    separate = ms(lambda: s.observables(0))
+    stepsync = ms(lambda: (s.step(1), s.synchronize()))
+    return {"plain_step_ms": plain_ms, "observed_step_ms": observed, "plain_step_with_sync_ms": stepsync,
+            "observed_step_extra_us": (observed - stepsync) * 1e3, "separate_pass_us": separate * 1e3}
+
+
 def output_timing(s, dims, reps=5):
     """Per-iteration output of the reference (picture + volume map, GPUGrossPitaevskii_BCC.cu:424-468) done
     on the device: median wall time of one gpe_volume_map / gpe_slice_picture call (kernel + copy of the
@@ -424,6 +447,9 @@ def main():
     err = float(np.linalg.norm(results["fp32"]["state"] - results["fp64"]["state"])
                 / np.linalg.norm(results["fp64"]["state"]))
     output = output_timing(s64, dims)
+    monitor = monitor_timing(s64)
+    log(f"monitoring: a step with fused observables costs {monitor['observed_step_extra_us']:.1f} us more than a "
+        f"plain step; a separate observables pass {monitor['separate_pass_us']:.1f} us")
     log(f"output: volume map {output['volume_map_call_ms']:.2f} ms, slice {output['slice_picture_call_ms']:.2f} ms "
         f"per call vs {output['field_download_ms']:.1f} ms to download the even field")
     s64.close()
@@ -488,6 +514,7 @@ def main():
         "serving": serving,
         "primal14": primal,
         "output": output,
+        "monitor": monitor,
         "avg_launch_us": {"fp64": r64["avg_launch_us"], "fp32": r32["avg_launch_us"]},
     }
     print(json.dumps(line), flush=True)

@@ -138,6 +138,13 @@ gpe_status gpe_time_steps(gpe_solver *s, uint64_t nsteps, double *elapsed_ms, ui
  * Energy E = K + P + I/2; chemical potential mu = (K + P + I)/N. */
 gpe_status gpe_observables(gpe_solver *s, int field, double out[4]);
 
+/* Real time: nsteps (>= 1) leapfrog steps, with out = the observables of
+ * gpe_observables of the even field as the last step reads it (the state
+ * after nsteps - 1 steps), accumulated inside that step's odd half-update
+ * from the values it already holds: no extra pass over the field.  The
+ * steps are bit-identical to gpe_step.  GPE_ENOTSUP in imaginary time. */
+gpe_status gpe_step_observe(gpe_solver *s, uint64_t nsteps, double out[4]);
+
 gpe_status gpe_synchronize(gpe_solver *s);
 gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info);
 

@@ -197,6 +197,49 @@ def test_observables(orc, precision, potential):
     s.close()
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("potential", ["trap", "array"])
+def test_step_observe_fused_reduction(orc, kernel, precision, potential):
+    """gpe_step_observe: the observables reduced inside the last step's odd
+    half-update equal gpe_observables of the even field after nsteps - 1
+    steps (FP64 reductions in another block order: <= 1e-12 relative), and
+    the steps equal gpe_step bit for bit."""
+    dims, h, g, dt = (9, 8, 10), 0.45, 700.0, 1e-3
+    announce(f"step_observe {kernel} {precision} {potential}")
+    b, origin, even, odd, V = case(dims, h, g, dt, 1.3, seed=13, sigma=1.1)
+
+    def solver():
+        s = va.Solver(dims, b, dt, g, origin, precision, "real", potential, 1.3, kernel)
+        if potential == "array":
+            s.set_potential(V)
+        s.set_state(even, odd)
+        return s
+
+    a, c = solver(), solver()
+    got = a.step_observe(7)
+    ea, oa = a.get_state()
+    c.step(6)
+    ref = c.observables(0)
+    c.step(1)
+    ec, oc = c.get_state()
+    for k in "NKPI":
+        assert got[k] == pytest.approx(ref[k], rel=1e-12), k
+    assert np.array_equal(ea, ec) and np.array_equal(oa, oc)
+    if precision == "fp64" and kernel == "gather" and potential == "trap":
+        # and against the oracle: N, K, P, I of the oracle's field after 6 steps
+        oe, _ = orc.leapfrog(even, odd, V * dt, dims, *va.leapfrog_coefs(b, dt, g), 6)
+        want = orc.observables(oe, V, dims, b, g)
+        for k, r in zip("NKPI", want):
+            assert got[k] == pytest.approx(r, rel=1e-12), k
+        a.set_mode("imag", dt)
+        with pytest.raises(va._lib.GpeError) as ei:
+            a.step_observe(1)
+        assert ei.value.status == va._lib.ENOTSUP
+    a.close()
+    c.close()
+
+
 def test_init_odd_matches_host_phase(orc):
     dims = (5, 6, 7)
     b, origin, even, _, V = case(dims, 0.5, 100.0, 1e-3, 1.0, seed=2)

@@ -193,6 +193,7 @@ SIGNATURES = [
     ("gpe_step", _I, [_P, _U64]),
     ("gpe_time_steps", _I, [_P, _U64, _DP, C.POINTER(_U64)]),
     ("gpe_observables", _I, [_P, _I, _DP]),
+    ("gpe_step_observe", _I, [_P, _U64, _DP]),
     ("gpe_synchronize", _I, [_P]),
     ("gpe_get_info", _I, [_P, C.POINTER(GpeSolverInfo)]),
     ("gpe_set_variant", _I, [_P, _U32, _U32]),

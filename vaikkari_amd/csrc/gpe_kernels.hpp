@@ -202,12 +202,21 @@ __device__ __forceinline__ void block_coords(const Grid3 &G, int &bx, int &by, i
 // the calling thread; returns its sum of |out|^2 (imaginary time).  NT
 // marks the read-modify-write operand as streaming (read once, written
 // once) so it does not displace neighbour rows from the caches.
-template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
+// Unscaled weights of the observables (k_observables): 1/dt turns a
+// streamed V*dt back into V, lap = 1.5/b^2, g.
+struct ObsCoefs {
+    double inv_dt, lap, g;
+};
+
+// OBS (real time): also accumulate into obs[4] the observables N, K, P, I
+// (k_observables' per-node terms, FP64) of the field being read, from the
+// values the update already holds -- no extra memory traffic.
+template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false, bool OBS = false>
 __device__ __forceinline__ double gather_tile(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                               const T *__restrict__ pot, const Layout &L,
                                               const Hamiltonian<T, FORM> &H, const Trap &trap, double dt,
                                               const double *__restrict__ scale, int bx, int by, int bz,
-                                              int lsh = 6) {
+                                              int lsh = 6, double *obs = nullptr, ObsCoefs oc = {}) {
 #pragma clang fp contract(off)
     using C = cplx_t<T>;
     // a wave covers 64 >> lsh rows of (1 << lsh) cells
@@ -242,6 +251,21 @@ __device__ __forceinline__ double gather_tile(cplx_t<T> *__restrict__ next, cons
             T pv;
             if (POT == POT_TRAP) pv = (T)(trap_v(trap, i, j, k, l) * dt);
             else pv = pot[base + l * L.xp];
+            if (OBS) {
+                const double pr = (double)own[l].x, pi = (double)own[l].y;
+                double dxs = 0.0, dys = 0.0;
+#pragma unroll
+                for (int f = 0; f < 4; f++) {
+                    dxs += (double)n[f].x - pr;
+                    dys += (double)n[f].y - pi;
+                }
+                const double n2 = pr * pr + pi * pi;
+                const double v = POT == POT_TRAP ? trap_v(trap, i, j, k, l) : (double)pv * oc.inv_dt;
+                obs[0] += n2;
+                obs[1] += -0.5 * oc.lap * (pr * dxs + pi * dys);
+                obs[2] += v * n2;
+                obs[3] += oc.g * n2 * n2;
+            }
             const C sres = H.apply(n, own[l], pv);
             const long long at = base + l * L.xp;
             if (MODE == MODE_REAL) {
@@ -288,6 +312,27 @@ __global__ void __launch_bounds__(64 * TY) k_gather(cplx_t<T> *__restrict__ next
         // partials in logical tile order (independent of the block-to-XCD order)
         block_sum<1, TY>(part, lds);
         if (threadIdx.x == 0 && threadIdx.y == 0) partials[bx + G.gx * (by + G.gy * bz)] = part[0];
+    }
+}
+
+// Real-time gather pass that also writes block partials of the observables
+// of `prev` (4 per block, logical tile order) for k_finalize<4>.
+template <typename T, int FORM, int POT, int TY, bool NT>
+__global__ void __launch_bounds__(64 * TY)
+    k_gather_obs(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev, const T *__restrict__ pot,
+                 Layout L, Hamiltonian<T, FORM> H, Trap trap, double dt, ObsCoefs oc,
+                 double *__restrict__ partials, Grid3 G) {
+    __shared__ double lds[4 * TY];
+    int bx, by, bz;
+    block_coords(G, bx, by, bz);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    gather_tile<T, FORM, POT, MODE_REAL, TY, NT, true>(next, prev, pot, L, H, trap, dt, nullptr, bx, by, bz, G.lsh,
+                                                       acc, oc);
+    block_sum<4, TY>(acc, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const long long b = bx + (long long)G.gx * (by + (long long)G.gy * bz);
+#pragma unroll
+        for (int q = 0; q < 4; q++) partials[4 * b + q] = acc[q];
     }
 }
 

@@ -513,6 +513,60 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
     return GPE_OK;
 }
 
+// One real-time step whose odd half-update (gather, `tile`) also reduces the
+// observables of the even field into scal[4..7].
+template <typename T, int FORM>
+gpe_status gpe_solver::observe_step(int tile) {
+    using C = cplx_t<T>;
+    Hamiltonian<T, FORM> H;
+    H.lapfac = (T)co.lapfac_dt;
+    H.lapfac0 = (T)co.lapfac0_dt;
+    H.g = (T)co.g_dt;
+    const T *p = (const T *)pot;
+    const bool arr = cfg.potential == GPE_POT_ARRAY;
+    C *ev = (C *)field[ie], *od = (C *)field[io];
+    const TileShape t = tile_shape(GPE_KERNEL_GATHER, tile);
+    const Grid3 G = grid_of(GPE_KERNEL_GATHER, tile, L);
+    const ObsCoefs oc{1.0 / cfg.dt, co.lap, co.g};
+    const dim3 grid(G.gx * G.gy * G.gz);
+#define GPE_OBS(POTK, TYV, NTV)                                                                                  \
+    hipLaunchKernelGGL((k_gather_obs<T, FORM, POTK, TYV, NTV>), grid, dim3(64, TYV), 0, stream, od, ev, p, L, H, \
+                       trap, co.dt, oc, partials, G)
+#define GPE_OBS_TY(POTK)                                 \
+    switch (t.ty * 10 + t.nt) {                          \
+        case 10: GPE_OBS(POTK, 1, false); break;         \
+        case 20: GPE_OBS(POTK, 2, false); break;         \
+        case 40: GPE_OBS(POTK, 4, false); break;         \
+        case 80: GPE_OBS(POTK, 8, false); break;         \
+        case 11: GPE_OBS(POTK, 1, true); break;          \
+        case 21: GPE_OBS(POTK, 2, true); break;          \
+        case 41: GPE_OBS(POTK, 4, true); break;          \
+        case 81: GPE_OBS(POTK, 8, true); break;          \
+        default: set_error("gpe_step_observe: bad tile"); return GPE_EINVAL; \
+    }
+    if (arr) {
+        GPE_OBS_TY(POT_ARRAY)
+    } else {
+        GPE_OBS_TY(POT_TRAP)
+    }
+#undef GPE_OBS_TY
+#undef GPE_OBS
+    GPE_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_finalize<4>), dim3(1), dim3(256), 0, stream, partials, (long long)G.gx * G.gy * G.gz, co.w,
+                       0, scal + 4);
+    GPE_HIP_TRY(hipGetLastError());
+    hipError_t e;
+    if (arr)
+        e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(GPE_KERNEL_GATHER, tile, ev, od, p, L, H, trap, co.dt,
+                                                          nullptr, nullptr, stream, true);
+    else
+        e = launch_update<T, FORM, POT_TRAP, MODE_REAL>(GPE_KERNEL_GATHER, tile, ev, od, p, L, H, trap, co.dt, nullptr,
+                                                         nullptr, stream, true);
+    GPE_HIP_TRY(e);
+    steps_done += 1;
+    return GPE_OK;
+}
+
 gpe_status gpe_solver::step(uint64_t nsteps, uint64_t *launches) {
     if (cfg.mode == GPE_REAL_TIME && !odd_valid) {
         set_error("gpe_step: real-time stepping needs an odd field (gpe_set_state or gpe_init_odd)");
@@ -704,6 +758,37 @@ gpe_status gpe_observables(gpe_solver *s, int which, double out[4]) {
     const int f = which == 0 ? s->ie : s->io;
     if (s->cfg.precision == GPE_FP64) return s->observables_t<double>(f, out);
     return s->observables_t<float>(f, out);
+}
+
+gpe_status gpe_step_observe(gpe_solver *s, uint64_t nsteps, double out[4]) {
+    if (!s || !out || nsteps == 0) {
+        set_error("gpe_step_observe: invalid argument");
+        return GPE_EINVAL;
+    }
+    if (s->cfg.mode != GPE_REAL_TIME) {
+        set_error("gpe_step_observe: real time only (imaginary time renormalises every step)");
+        return GPE_ENOTSUP;
+    }
+    if (nsteps > 1) {
+        const gpe_status st = s->step(nsteps - 1, nullptr);
+        if (st != GPE_OK) return st;
+    } else if (!s->odd_valid) {
+        set_error("gpe_step_observe: real-time stepping needs an odd field (gpe_set_state or gpe_init_odd)");
+        return GPE_EINVAL;
+    }
+    if (s->cfg.potential == GPE_POT_ARRAY && !s->pot) {
+        set_error("gpe_step_observe: potential kind is ARRAY but no potential was set");
+        return GPE_EINVAL;
+    }
+    // the last step as two gather passes (any family gives the same bits),
+    // the odd one accumulating the observables of the even field it reads
+    const int tile = s->kernel == GPE_KERNEL_GATHER ? s->tile : default_tile(GPE_KERNEL_GATHER, s->cfg.precision);
+    const gpe_status st = s->cfg.precision == GPE_FP64 ? s->observe_step<double, FORM_REF>(tile)
+                                                       : s->observe_step<float, FORM_DIFF>(tile);
+    if (st != GPE_OK) return st;
+    GPE_HIP_TRY(hipMemcpyAsync(out, s->scal + 4, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    GPE_HIP_TRY(hipStreamSynchronize(s->stream));
+    return GPE_OK;
 }
 
 gpe_status gpe_synchronize(gpe_solver *s) {

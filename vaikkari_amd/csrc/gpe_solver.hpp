@@ -54,4 +54,5 @@ struct gpe_solver {
     template <typename T, int FORM> gpe_status enqueue_steps(uint64_t nsteps, uint64_t *launches);
     gpe_status step(uint64_t nsteps, uint64_t *launches);
     template <typename T> gpe_status observables_t(int which, double out[4]);
+    template <typename T, int FORM> gpe_status observe_step(int tile);
 };

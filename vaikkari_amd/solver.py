@@ -157,6 +157,14 @@ class Solver:
         N, K, P, I = (float(x) for x in out)
         return {"N": N, "K": K, "P": P, "I": I, "E": K + P + 0.5 * I, "mu": (K + P + I) / N if N else float("nan")}
 
+    def step_observe(self, nsteps: int) -> dict:
+        """nsteps real-time steps; returns the observables of the even field as the last step read
+        it (after nsteps - 1 steps), reduced inside that step's first half-update."""
+        out = np.zeros(4)
+        check(lib.gpe_step_observe(self._h, int(nsteps), dptr(out)), "gpe_step_observe")
+        N, K, P, I = (float(x) for x in out)
+        return {"N": N, "K": K, "P": P, "I": I, "E": K + P + 0.5 * I, "mu": (K + P + I) / N if N else float("nan")}
+
     def set_variant(self, kernel: str, tile: int) -> None:
         check(lib.gpe_set_variant(self._h, KERNELS[kernel], int(tile)), "gpe_set_variant")
 
