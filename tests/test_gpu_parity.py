@@ -287,6 +287,23 @@ def test_deterministic(precision, kernel):
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
+def test_config2_full_size_bit_exact(orc):
+    """The bench workload's grid and physics (128^3 cells, 25,165,824 nodes, h = 0.1,
+    g = 2000, trap 0.5(x^2+y^2+4z^2), dt = 2e-4): 100 FP64 leapfrog steps equal the
+    oracle bit for bit (the OpenMP oracle takes a few seconds)."""
+    dims, h, g, dt = (128, 128, 128), 0.1, 2000.0, 2e-4
+    announce("fp64 128^3 x 100 vs oracle")
+    b, origin, even, odd, V = case(dims, h, g, dt, 2.0, seed=1, sigma=1.5)
+    s = va.Solver(dims, b, dt, g, origin, "fp64", "real", "trap", 2.0)
+    s.set_state(even, odd)
+    s.step(100)
+    ge, go = s.get_state()
+    s.close()
+    lf, lf0, gdt = va.leapfrog_coefs(b, dt, g)
+    oe, oo = orc.leapfrog(even, odd, V * dt, dims, lf, lf0, gdt, 100)
+    assert np.array_equal(ge, oe) and np.array_equal(go, oo)
+
+
 def test_full_size_fp64_invariant():
     """BCC 128^3 cells (25,165,824 nodes), h = 0.1, g = 2000, trap 0.5(x^2+y^2+4z^2),
     dt = 2e-4: the exact leapfrog invariant Re<even,odd> holds to 1e-12 relative over 200 steps
