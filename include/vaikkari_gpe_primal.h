@@ -70,7 +70,7 @@ gpe_status gpe_primal_synchronize(gpe_primal *p);
  * Tile 0 in the config picks by size: when both fields fit in 192 MB (the
  * Infinity Cache holds them), one layer per block with a cached
  * read-modify-write (tile 9 FP64, tile 11 FP32); otherwise the z-march of
- * 16 layers (tile 13). */
+ * 16 layers (tile 13 FP64, tile 19 FP32). */
 int gpe_primal_tiles(void);
 
 /* Host helpers, the arithmetic the kernels use. */

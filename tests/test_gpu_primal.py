@@ -58,9 +58,9 @@ def setup(dims, a, dt, g, ratio):
 
 
 @pytest.mark.parametrize("dims", SHAPES)
-@pytest.mark.parametrize("tile", range(1, 19))
+@pytest.mark.parametrize("tile", range(1, 21))
 def test_fp64_leapfrog_bit_exact(po, dims, tile):
-    assert va.primal_tiles() == 18
+    assert va.primal_tiles() == 20
     a, dt, g, ratio = 0.35, 0.004, 1.3, 1.3
     origin, V, coef = setup(dims, a, dt, g, ratio)
     even, odd = random_fields(dims, seed=11 + tile)
@@ -75,7 +75,7 @@ def test_fp64_leapfrog_bit_exact(po, dims, tile):
 
 
 @pytest.mark.parametrize("dims", TINY)
-@pytest.mark.parametrize("tile", [0, 1, 2, 10, 13, 14])
+@pytest.mark.parametrize("tile", [0, 1, 2, 10, 13, 14, 19])
 def test_fp64_degenerate_grids_bit_exact(po, dims, tile):
     """Grids of one cell or one row: every neighbour of the edge lanes is halo."""
     a, dt, g, ratio = 0.35, 0.004, 1.3, 1.3

@@ -348,7 +348,7 @@ struct ZLayer {
     cplx_t<T> n0, n1;                      // next, for the read-modify-write
 };
 
-template <typename T, int FORM, int MODE, int TY, int CZ, bool NT>
+template <typename T, int FORM, int MODE, int TY, int CZ, bool NT, bool LY>
 __device__ __forceinline__ double primal_column(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                                 const Layout &L, const PCoef<T> &H, const PTrap &trap, double dt,
                                                 const double *__restrict__ scale, int bx, int by, int bz, int lsh) {
@@ -383,9 +383,11 @@ __device__ __forceinline__ double primal_column(cplx_t<T> *__restrict__ next, co
         f.czp = ld(b0 + sz);      // corner (j, k+1)
         f.v3 = ld(b0 + sz + sy);  // corner (j+1, k+1)
         f.zzp = ld(b1 + sz);      // centre (j, k+1)
-        f.cym = ld(b0 - sy);      // corner (j-1, k)
-        f.u2 = ld(b1 - sy);       // centre (j-1, k)
-        f.zyp = ld(b1 + sy);      // centre (j+1, k)
+        if (!LY) {
+            f.cym = ld(b0 - sy);  // corner (j-1, k)
+            f.u2 = ld(b1 - sy);   // centre (j-1, k)
+            f.zyp = ld(b1 + sy);  // centre (j+1, k)
+        }
         f.t0 = ld(left_end ? b1 - sy - 1 : right_end ? b0 + sz + 1 : b0);       // w2 | e2
         f.t1 = ld(left_end ? b1 - 1 : right_end ? b0 + sz + sy + 1 : b0);       // w3 | e3
         f.t2 = ld(left_end ? b0 - 1 : right_end ? b1 + 1 : b0);                 // xm | zxp
@@ -423,6 +425,15 @@ __device__ __forceinline__ double primal_column(cplx_t<T> *__restrict__ next, co
         const int k = k0 + kk;
         if (k >= k1) break;
         ZLayer<T> &cur = buf[kk & 1];
+        if (LY) {
+            // the y-neighbour rows of layer k (rows the block's other waves
+            // read too) are loaded in the layer itself, before the prefetch so
+            // that waiting for them leaves the prefetch in flight: three fewer
+            // values held across a layer
+            cur.cym = ld(b0 - sy);
+            cur.u2 = ld(b0 + xp - sy);
+            cur.zyp = ld(b0 + xp + sy);
+        }
         if (k + 1 < k1) fetch(b0 + sz, buf[(kk + 1) & 1]);
         // keep the scheduler from hoisting layer k's first uses (and their
         // wait) above the prefetch loads
@@ -464,7 +475,7 @@ __device__ __forceinline__ double primal_column(cplx_t<T> *__restrict__ next, co
     return part;
 }
 
-template <typename T, int FORM, int MODE, int TY, int CZ, bool NT>
+template <typename T, int FORM, int MODE, int TY, int CZ, bool NT, bool LY>
 __global__ void __launch_bounds__(64 * TY) k_primal_zm(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                                        Layout L, PCoef<T> H, PTrap trap, double dt,
                                                        const double *__restrict__ scale, double *__restrict__ partials,
@@ -473,7 +484,7 @@ __global__ void __launch_bounds__(64 * TY) k_primal_zm(cplx_t<T> *__restrict__ n
     int bx, by, bz;
     block_coords(G, bx, by, bz);
     double part[1];
-    part[0] = primal_column<T, FORM, MODE, TY, CZ, NT>(next, prev, L, H, trap, dt, scale, bx, by, bz, G.lsh);
+    part[0] = primal_column<T, FORM, MODE, TY, CZ, NT, LY>(next, prev, L, H, trap, dt, scale, bx, by, bz, G.lsh);
     if (MODE == MODE_IMAG) {
         block_sum<1, TY>(part, lds);
         if (threadIdx.x == 0 && threadIdx.y == 0) partials[bx + G.gx * (by + G.gy * bz)] = part[0];
@@ -540,17 +551,19 @@ __global__ void __launch_bounds__(64 * kObsTY) k_primal_observables(const cplx_t
 // which always exchanges by DPP).
 struct PTile {
     int ty, remap, nt, xs, cz;
+    int ly;  // z-march: y-neighbour rows loaded in the layer instead of prefetched
 };
 constexpr PTile kPrimalTiles[] = {
     {4, 2, 1, XS_DPP, 0},  {4, 2, 1, XS_LOAD, 0}, {2, 2, 1, XS_DPP, 0},  {8, 2, 1, XS_DPP, 0},
     {2, 2, 1, XS_LOAD, 0}, {8, 2, 1, XS_LOAD, 0}, {4, 0, 1, XS_DPP, 0},  {4, 1, 1, XS_DPP, 0},
     {4, 2, 0, XS_DPP, 0},  {4, 2, 1, XS_SHFL, 0}, {4, 2, 0, XS_LOAD, 0}, {4, 2, 1, XS_DPP, 8},
     {4, 2, 1, XS_DPP, 16}, {4, 2, 1, XS_DPP, 4},  {2, 2, 1, XS_DPP, 8},  {8, 2, 1, XS_DPP, 8},
-    {4, 0, 1, XS_DPP, 8},  {4, 2, 0, XS_DPP, 8}};
+    {4, 0, 1, XS_DPP, 8},  {4, 2, 0, XS_DPP, 8},  {4, 2, 1, XS_DPP, 16, 1}, {4, 2, 1, XS_DPP, 8, 1}};
 constexpr int kNumPrimalTiles = sizeof(kPrimalTiles) / sizeof(kPrimalTiles[0]);
 constexpr int kDefaultPrimalTile = 12;  // {4, 2, 1, XS_DPP, 16}
 constexpr int kCachedTileFP64 = 8;      // {4, 2, 0, XS_DPP, 0}
 constexpr int kCachedTileFP32 = 10;     // {4, 2, 0, XS_LOAD, 0}
+constexpr int kMarchTileFP32 = 18;      // {4, 2, 1, XS_DPP, 16, 1}
 
 Layout primal_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes) {
     Layout L{};
@@ -601,7 +614,7 @@ hipError_t launch_primal(int tile, cplx_t<T> *next, const cplx_t<T> *prev, const
         G.rev = reverse ? 1 : 0;
         constexpr bool nt = MODE == MODE_REAL && t.nt;
         if constexpr (t.cz > 0)
-            hipLaunchKernelGGL((k_primal_zm<T, FORM, MODE, t.ty, t.cz, nt>), dim3(G.gx * G.gy * G.gz), dim3(64, t.ty),
+            hipLaunchKernelGGL((k_primal_zm<T, FORM, MODE, t.ty, t.cz, nt, t.ly != 0>), dim3(G.gx * G.gy * G.gz), dim3(64, t.ty),
                                0, st, next, prev, L, H, trap, dt, scale, partials, G);
         else
             launch_t<T, FORM, MODE, t.ty, nt, t.xs>(next, prev, L, H, trap, dt, scale, partials, G, st);
@@ -681,11 +694,13 @@ gpe_status gpe_primal::init() {
     // default by size, the fastest shapes measured on MI355X: when both fields
     // fit the 256 MB Infinity Cache (128^3 cells) one layer per block with a
     // cached read-modify-write (FP64 DPP exchange, FP32 loads); larger grids
-    // (256^3) the z-march of 16 layers with one layer of prefetch
+    // (256^3) the z-march of 16 layers with one layer of prefetch (FP32: with
+    // the y-neighbour rows loaded in the layer, which leaves room for 5 waves)
     const bool cached = 2.0 * (double)L.elems * (double)elem_bytes() < 192e6;
+    const bool fp64 = cfg.precision == GPE_FP64;
     tile = cfg.tile != 0 ? (int)cfg.tile - 1
-           : cached      ? (cfg.precision == GPE_FP64 ? kCachedTileFP64 : kCachedTileFP32)
-                         : kDefaultPrimalTile;
+           : cached      ? (fp64 ? kCachedTileFP64 : kCachedTileFP32)
+                         : (fp64 ? kDefaultPrimalTile : kMarchTileFP32);
     set_coefs();
     GPE_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     GPE_HIP_TRY(hipEventCreate(&ev0));
