@@ -66,7 +66,9 @@ gpe_status gpe_primal_synchronize(gpe_primal *p);
 /* Number of tile shapes: one layer per block or a z-march of 4 to 16
  * layers, waves per block (2, 4, 8), block order over the XCDs, streaming
  * read-modify-write, and how a lane gets its x-neighbours (DPP lane shift,
- * cross-lane shuffle or a second load).  Every shape gives the same bits.
+ * cross-lane shuffle or a second load); tiles 21-25 fuse both half-updates
+ * of a real-time step into one out-of-place pass (temporal blocking, two
+ * more field buffers).  Every shape gives the same bits.
  * Tile 0 in the config picks by size: when both fields fit in 192 MB (the
  * Infinity Cache holds them), one layer per block with a cached
  * read-modify-write (tile 9 FP64, tile 11 FP32); otherwise the z-march of

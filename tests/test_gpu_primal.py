@@ -58,9 +58,9 @@ def setup(dims, a, dt, g, ratio):
 
 
 @pytest.mark.parametrize("dims", SHAPES)
-@pytest.mark.parametrize("tile", range(1, 21))
+@pytest.mark.parametrize("tile", range(1, 26))
 def test_fp64_leapfrog_bit_exact(po, dims, tile):
-    assert va.primal_tiles() == 20
+    assert va.primal_tiles() == 25
     a, dt, g, ratio = 0.35, 0.004, 1.3, 1.3
     origin, V, coef = setup(dims, a, dt, g, ratio)
     even, odd = random_fields(dims, seed=11 + tile)
@@ -75,7 +75,7 @@ def test_fp64_leapfrog_bit_exact(po, dims, tile):
 
 
 @pytest.mark.parametrize("dims", TINY)
-@pytest.mark.parametrize("tile", [0, 1, 2, 10, 13, 14, 19])
+@pytest.mark.parametrize("tile", [0, 1, 2, 10, 13, 14, 19, 21, 25])
 def test_fp64_degenerate_grids_bit_exact(po, dims, tile):
     """Grids of one cell or one row: every neighbour of the edge lanes is halo."""
     a, dt, g, ratio = 0.35, 0.004, 1.3, 1.3
@@ -91,7 +91,7 @@ def test_fp64_degenerate_grids_bit_exact(po, dims, tile):
 
 
 @pytest.mark.parametrize("dims", SHAPES[:3])
-@pytest.mark.parametrize("tile", [1, 2, 13])
+@pytest.mark.parametrize("tile", [1, 2, 13, 21])
 def test_fp64_imaginary_time(po, dims, tile):
     a, dt, g, ratio = 0.35, 0.004, 1.3, 1.3
     origin, V, coef = setup(dims, a, dt, g, ratio)
@@ -159,7 +159,7 @@ def test_init_odd_phase():
     np.testing.assert_allclose(odd.view(np.complex128), want, rtol=1e-15, atol=1e-18)
 
 
-@pytest.mark.parametrize("tile", [1, 2, 13])
+@pytest.mark.parametrize("tile", [1, 2, 13, 21])
 def test_fp32_leapfrog_tolerance(po, tile):
     """FP32 storage and stencil (difference form) against the FP64 oracle:
     relative L2 error <= 1e-5 after 3 leapfrog steps."""
@@ -190,7 +190,7 @@ def test_fp64_bench_size_bit_exact(po):
     odd = (psi.view(np.complex128) * complex(math.cos(-0.5 * dt * 5.0), -math.sin(-0.5 * dt * 5.0))).view(
         np.float64).copy()
     re, ro = po.leapfrog(psi, odd, V * dt, dims, coef, 2)
-    for tile in (1, 7, 13):
+    for tile in (1, 7, 13, 21):
         announce(f"primal fp64 128^3 leapfrog tile={tile}")
         with va.PrimalSolver(dims, a, dt, g, origin, ratio, "fp64", "real", tile) as s:
             s.set_state(psi, odd)

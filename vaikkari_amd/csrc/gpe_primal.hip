@@ -491,6 +491,193 @@ __global__ void __launch_bounds__(64 * TY) k_primal_zm(cplx_t<T> *__restrict__ n
     }
 }
 
+// FUSED (real time): one pass per leapfrog step, out of place,
+//   O' = O - i dt H(E) E,   E' = E - i dt H(O') O',
+// reading E and O once and writing E' and O' once (64 B per site-step in
+// FP64 instead of 96 B for two passes).  A block of TY waves marches a
+// z-chunk of CZ layers over a footprint of 64 x TY columns whose interior
+// 62 x (TY - 2) columns it owns.  At layer t every footprint lane computes
+// O'(t) exactly as primal_column's layer step does (the E rows carried and
+// prefetched, x-shifts by DPP), stores it if it owns the column, and puts it
+// in a 4-slot LDS ring; after one barrier each interior lane computes
+// E'(t - 1) from O' at layers t-2..t of its own and the 8 surrounding columns
+// (all read from LDS) and E(t - 1) still in registers.  The ring columns and
+// the layers k0 - 1 and k1 of O' are computed again by the neighbouring
+// block; O' values outside the grid are zero (the halo).  Same arithmetic and
+// neighbour order as the two passes: bit-identical.
+template <typename T, int FORM, int TY, int CZ>
+__global__ void __launch_bounds__(64 * TY)
+    k_primal_fused(cplx_t<T> *__restrict__ eout, cplx_t<T> *__restrict__ oout, const cplx_t<T> *__restrict__ ein,
+                   const cplx_t<T> *__restrict__ oin, Layout L, PCoef<T> H, PTrap trap, double dt, Grid3 G) {
+#pragma clang fp contract(off)
+    using C = cplx_t<T>;
+    __shared__ C ring[4][TY][64][2];
+    int bx, by, bz;
+    block_coords(G, bx, by, bz);
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int i = bx * 62 - 1 + lane, j = by * (TY - 2) - 1 + w;
+    const int k0 = bz * CZ, k1 = min(k0 + CZ, L.nz);
+    const bool ingrid = i >= 0 && i < L.nx && j >= 0 && j < L.ny;
+    const bool owner = ingrid && lane >= 1 && lane <= 62 && w >= 1 && w <= TY - 2;
+    const bool left_end = ingrid && lane == 0, right_end = ingrid && lane == 63;
+    const long long xp = L.xp, sy = L.sy, sz = L.sz;
+    // lanes outside the grid read the zero halo column x = -1 of an existing row
+    const int ie = ingrid ? i : -1;
+    const int je = ingrid ? j : min(max(j, 0), L.ny - 1);
+    const long long lane0 = L.origin + (long long)je * sy + ie;  // corner site of layer 0
+    C zero;
+    zero.x = (T)0;
+    zero.y = (T)0;
+    // E at element lane0 + u of layer `layer`; layers beyond the halo are zero
+    // (the zero is built here, not captured: a captured local would live in
+    // scratch memory)
+    auto ldE = [&](long long u, int layer) -> C {
+        C v;
+        v.x = (T)0;
+        v.y = (T)0;
+        if (layer >= -1 && layer <= L.nz) v = ein[lane0 + u];
+        return v;
+    };
+    auto ldEe = [&](long long uL, long long uR, int layer) -> C {  // row-end x-neighbours
+        C v;
+        v.x = (T)0;
+        v.y = (T)0;
+        if (layer >= -1 && layer <= L.nz) v = ein[lane0 + (left_end ? uL : right_end ? uR : 0)];
+        return v;
+    };
+    auto fetch = [&](int t, ZLayer<T> &f) {  // loads for the O' step of layer t
+        const long long u = (long long)t * sz, v = u + xp;
+        f.czp = ldE(u + sz, t + 1);
+        f.v3 = ldE(u + sz + sy, t + 1);
+        f.zzp = ldE(v + sz, t + 1);
+        f.cym = ldE(u - sy, t);
+        f.u2 = ldE(v - sy, t);
+        f.zyp = ldE(v + sy, t);
+        f.t0 = left_end ? ldE(v - sy - 1, t) : ldE(u + sz + 1, t + 1);
+        f.t1 = left_end ? ldE(v - 1, t) : ldE(u + sz + sy + 1, t + 1);
+        f.t2 = ldEe(u - 1, v + 1, t);
+        f.n0.x = (T)0;
+        f.n0.y = (T)0;
+        f.n1 = f.n0;
+        if (ingrid && t >= 0 && t < L.nz) {
+            f.n0 = oin[lane0 + u];
+            f.n1 = oin[lane0 + v];
+        }
+    };
+    int t = k0 - 1;
+    long long u = (long long)t * sz;
+    C c0 = ldE(u, t), c1 = ldE(u + xp, t), czm = ldE(u - sz, t - 1), u1 = ldE(u + xp - sz, t - 1);
+    C u0 = ldE(u + xp - sz - sy, t - 1), cyp = ldE(u + sy, t);
+    const C q0 = left_end ? ldE(u + xp - sz - sy - 1, t - 1) : ldE(u + sy + 1, t);  // w0 | e1
+    const C q1 = left_end ? ldE(u + xp - sz - 1, t - 1) : ldE(u + 1, t);            // w1 | xq
+    ZLayer<T> buf[2];
+    fetch(t, buf[0]);
+    C w0 = shift_lane_c<true>(u0), w1 = shift_lane_c<true>(u1);
+    C e1 = shift_lane_c<false>(cyp), xq = shift_lane_c<false>(c0);
+    if (left_end) {
+        w0 = q0;
+        w1 = q1;
+    }
+    if (right_end) {
+        e1 = q0;
+        xq = q1;
+    }
+    const double xy0 = site_xy(trap, i, j, 0.0), xy1 = site_xy(trap, i, j, 0.5);
+    // a fixed trip count with a block-uniform guard (no break), so the loop
+    // unrolls despite the barrier and buf[kk & 1] names fixed registers
+#pragma unroll
+    for (int kk = 0; kk < CZ + 2; kk++, t++, u += sz) {
+      if (t <= k1) {
+        ZLayer<T> &cur = buf[kk & 1];
+        if (t + 1 <= k1) fetch(t + 1, buf[(kk + 1) & 1]);
+        __builtin_amdgcn_sched_barrier(0);
+        // ---- O'(t) on the footprint ----
+        C w2 = shift_lane_c<true>(cur.u2), w3 = shift_lane_c<true>(c1), xm = shift_lane_c<true>(c0);
+        C e2 = shift_lane_c<false>(cur.czp), e3 = shift_lane_c<false>(cur.v3), zxp = shift_lane_c<false>(c1);
+        if (left_end) {
+            w2 = cur.t0;
+            w3 = cur.t1;
+            xm = cur.t2;
+        }
+        if (right_end) {
+            e2 = cur.t0;
+            e3 = cur.t1;
+            zxp = cur.t2;
+        }
+        C o0 = zero, o1 = zero;
+        if (ingrid && t >= 0 && t < L.nz) {
+            const C dC[8] = {w0, u0, w1, u1, w2, cur.u2, w3, c1};
+            const C aC[6] = {xm, xq, cur.cym, cyp, czm, cur.czp};
+            const C s0 = primal_apply<T, FORM>(dC, aC, c0, (T)(site_v_xy(trap, xy0, t, 0.0) * dt), H);
+            const C dZ[8] = {c0, xq, cyp, e1, cur.czp, e2, cur.v3, e3};
+            const C aZ[6] = {w3, zxp, cur.u2, cur.zyp, u1, cur.zzp};
+            const C s1 = primal_apply<T, FORM>(dZ, aZ, c1, (T)(site_v_xy(trap, xy1, t, 0.5) * dt), H);
+            o0.x = cur.n0.x + s0.y;
+            o0.y = cur.n0.y + (-s0.x);
+            o1.x = cur.n1.x + s1.y;
+            o1.y = cur.n1.y + (-s1.x);
+            if (owner && t >= k0 && t < k1) {
+                oout[lane0 + u] = o0;
+                oout[lane0 + u + xp] = o1;
+            }
+        }
+        ring[t & 3][w][lane][0] = o0;
+        ring[t & 3][w][lane][1] = o1;
+        __syncthreads();
+        // ---- E'(t-1) on the interior, O' from the ring ----
+        const int te = t - 1;
+        if (owner && te >= k0 && te < k1) {
+            const int sm = (te - 1) & 3, s0i = te & 3, sp = (te + 1) & 3;
+            C dC[8], aC[6], dZ[8], aZ[6];
+            // corner site: centres at dz, dy, dx in {-1, 0}; centre site: corners at {0, +1}
+#pragma unroll
+            for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+                    for (int dx = 0; dx < 2; dx++) {
+                        dC[4 * dz + 2 * dy + dx] = ring[dz ? s0i : sm][w - 1 + dy][lane - 1 + dx][1];
+                        dZ[4 * dz + 2 * dy + dx] = ring[dz ? sp : s0i][w + dy][lane + dx][0];
+                    }
+            aC[0] = ring[s0i][w][lane - 1][0];
+            aC[1] = ring[s0i][w][lane + 1][0];
+            aC[2] = ring[s0i][w - 1][lane][0];
+            aC[3] = ring[s0i][w + 1][lane][0];
+            aC[4] = ring[sm][w][lane][0];
+            aC[5] = ring[sp][w][lane][0];
+            aZ[0] = ring[s0i][w][lane - 1][1];
+            aZ[1] = ring[s0i][w][lane + 1][1];
+            aZ[2] = ring[s0i][w - 1][lane][1];
+            aZ[3] = ring[s0i][w + 1][lane][1];
+            aZ[4] = ring[sm][w][lane][1];
+            aZ[5] = ring[sp][w][lane][1];
+            const C p0 = ring[s0i][w][lane][0], p1 = ring[s0i][w][lane][1];
+            const C s0 = primal_apply<T, FORM>(dC, aC, p0, (T)(site_v_xy(trap, xy0, te, 0.0) * dt), H);
+            const C s1 = primal_apply<T, FORM>(dZ, aZ, p1, (T)(site_v_xy(trap, xy1, te, 0.5) * dt), H);
+            // E(te) is the layer below this step's: czm (corner) and u1 (centre)
+            C r0, r1;
+            r0.x = czm.x + s0.y;
+            r0.y = czm.y + (-s0.x);
+            r1.x = u1.x + s1.y;
+            r1.y = u1.y + (-s1.x);
+            eout[lane0 + u - sz] = r0;
+            eout[lane0 + u - sz + xp] = r1;
+        }
+        // layer t+1 from what layer t read
+        czm = c0;
+        c0 = cur.czp;
+        u1 = c1;
+        c1 = cur.zzp;
+        u0 = cur.u2;
+        w0 = w2;
+        w1 = w3;
+        cyp = cur.v3;
+        e1 = e3;
+        xq = e2;
+      }
+    }
+}
+
 // Block partials of {sum|psi|^2, sum 1/2 Re(conj(psi)(-L psi)), sum V|psi|^2,
 // sum g|psi|^4} over both sites of every cell, FP64, unscaled weights.
 constexpr int kObsTY = 4;
@@ -551,14 +738,17 @@ __global__ void __launch_bounds__(64 * kObsTY) k_primal_observables(const cplx_t
 // which always exchanges by DPP).
 struct PTile {
     int ty, remap, nt, xs, cz;
-    int ly;  // z-march: y-neighbour rows loaded in the layer instead of prefetched
+    int ly;     // z-march: y-neighbour rows loaded in the layer instead of prefetched
+    int fused;  // real time: both half-updates in one z-march pass (k_primal_fused)
 };
 constexpr PTile kPrimalTiles[] = {
     {4, 2, 1, XS_DPP, 0},  {4, 2, 1, XS_LOAD, 0}, {2, 2, 1, XS_DPP, 0},  {8, 2, 1, XS_DPP, 0},
     {2, 2, 1, XS_LOAD, 0}, {8, 2, 1, XS_LOAD, 0}, {4, 0, 1, XS_DPP, 0},  {4, 1, 1, XS_DPP, 0},
     {4, 2, 0, XS_DPP, 0},  {4, 2, 1, XS_SHFL, 0}, {4, 2, 0, XS_LOAD, 0}, {4, 2, 1, XS_DPP, 8},
     {4, 2, 1, XS_DPP, 16}, {4, 2, 1, XS_DPP, 4},  {2, 2, 1, XS_DPP, 8},  {8, 2, 1, XS_DPP, 8},
-    {4, 0, 1, XS_DPP, 8},  {4, 2, 0, XS_DPP, 8},  {4, 2, 1, XS_DPP, 16, 1}, {4, 2, 1, XS_DPP, 8, 1}};
+    {4, 0, 1, XS_DPP, 8},  {4, 2, 0, XS_DPP, 8},  {4, 2, 1, XS_DPP, 16, 1}, {4, 2, 1, XS_DPP, 8, 1},
+    {8, 2, 0, XS_DPP, 16, 0, 1}, {8, 2, 0, XS_DPP, 8, 0, 1}, {6, 2, 0, XS_DPP, 16, 0, 1},
+    {8, 0, 0, XS_DPP, 16, 0, 1}, {4, 2, 0, XS_DPP, 16, 0, 1}};
 constexpr int kNumPrimalTiles = sizeof(kPrimalTiles) / sizeof(kPrimalTiles[0]);
 constexpr int kDefaultPrimalTile = 12;  // {4, 2, 1, XS_DPP, 16}
 constexpr int kCachedTileFP64 = 8;      // {4, 2, 0, XS_DPP, 0}
@@ -582,6 +772,16 @@ Layout primal_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes) {
 
 Grid3 primal_grid(const PTile &t, const Layout &L) {
     Grid3 G{};
+    if (t.fused) {
+        // 64-lane rows, interior 62 x (ty - 2) columns per block
+        G.lsh = 6;
+        G.gx = (L.nx + 61) / 62;
+        G.gy = (L.ny + t.ty - 3) / (t.ty - 2);
+        G.gz = (L.nz + t.cz - 1) / t.cz;
+        G.remap = t.remap;
+        if (G.remap == 2 && G.gy % 8 != 0) G.remap = 0;
+        return G;
+    }
     G.lsh = row_lanes_log2(L.nx);
     const int lw = 1 << G.lsh, rows = t.ty * (64 >> G.lsh);
     G.gx = (L.nx + lw - 1) / lw;
@@ -610,6 +810,9 @@ hipError_t launch_primal(int tile, cplx_t<T> *next, const cplx_t<T> *prev, const
             return launch_primal<T, FORM, MODE, IDX + 1>(tile, next, prev, L, H, trap, dt, scale, partials, st,
                                                          reverse);
         constexpr PTile t = kPrimalTiles[IDX];
+        if constexpr (t.fused) {
+            return hipErrorInvalidValue;  // launched by launch_primal_fused
+        }
         Grid3 G = primal_grid(t, L);
         G.rev = reverse ? 1 : 0;
         constexpr bool nt = MODE == MODE_REAL && t.nt;
@@ -619,6 +822,28 @@ hipError_t launch_primal(int tile, cplx_t<T> *next, const cplx_t<T> *prev, const
         else
             launch_t<T, FORM, MODE, t.ty, nt, t.xs>(next, prev, L, H, trap, dt, scale, partials, G, st);
         return hipGetLastError();
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
+// One fused leapfrog step (real time) with a fused tile shape.
+template <typename T, int FORM, int IDX = 0>
+hipError_t launch_primal_fused(int tile, cplx_t<T> *eout, cplx_t<T> *oout, const cplx_t<T> *ein,
+                               const cplx_t<T> *oin, const Layout &L, const PCoef<T> &H, const PTrap &trap, double dt,
+                               hipStream_t st) {
+    if constexpr (IDX < kNumPrimalTiles) {
+        if (tile != IDX)
+            return launch_primal_fused<T, FORM, IDX + 1>(tile, eout, oout, ein, oin, L, H, trap, dt, st);
+        constexpr PTile t = kPrimalTiles[IDX];
+        if constexpr (!t.fused) {
+            return hipErrorInvalidValue;
+        } else {
+            const Grid3 G = primal_grid(t, L);
+            hipLaunchKernelGGL((k_primal_fused<T, FORM, t.ty, t.cz>), dim3(G.gx * G.gy * G.gz), dim3(64, t.ty), 0, st,
+                               eout, oout, ein, oin, L, H, trap, dt, G);
+            return hipGetLastError();
+        }
     } else {
         return hipErrorInvalidValue;
     }
@@ -653,7 +878,10 @@ struct gpe_primal {
     double coef[4] = {0, 0, 0, 0};
     double w = 0.0;  // site measure a^3/2
     int tile = 0;
-    void *field[2] = {nullptr, nullptr};  // field[ie] even level, field[io] odd level / imaginary-time scratch
+    int imag_tile = 0;  // the size's default shape, for imaginary time under a fused tile
+    // field[ie] even level, field[io] odd level / imaginary-time scratch; a
+    // fused tile writes a step into the other two and swaps
+    void *field[4] = {nullptr, nullptr, nullptr, nullptr};
     int ie = 0, io = 1;
     double *scal = nullptr;      // [0] last N, [1] read scale, [4..7] observables
     double *partials = nullptr;  // per-block partial sums
@@ -698,15 +926,15 @@ gpe_status gpe_primal::init() {
     // the y-neighbour rows loaded in the layer, which leaves room for 5 waves)
     const bool cached = 2.0 * (double)L.elems * (double)elem_bytes() < 192e6;
     const bool fp64 = cfg.precision == GPE_FP64;
-    tile = cfg.tile != 0 ? (int)cfg.tile - 1
-           : cached      ? (fp64 ? kCachedTileFP64 : kCachedTileFP32)
-                         : (fp64 ? kDefaultPrimalTile : kMarchTileFP32);
+    imag_tile = cached ? (fp64 ? kCachedTileFP64 : kCachedTileFP32) : (fp64 ? kDefaultPrimalTile : kMarchTileFP32);
+    tile = cfg.tile != 0 ? (int)cfg.tile - 1 : imag_tile;
     set_coefs();
     GPE_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     GPE_HIP_TRY(hipEventCreate(&ev0));
     GPE_HIP_TRY(hipEventCreate(&ev1));
     const size_t fbytes = (size_t)L.elems * elem_bytes();
-    for (int f = 0; f < 2; f++) {
+    const int nfields = kPrimalTiles[tile].fused ? 4 : 2;
+    for (int f = 0; f < nfields; f++) {
         if (hipMalloc(&field[f], fbytes) != hipSuccess) {
             field[f] = nullptr;
             set_error("gpe_primal_create: cannot allocate %zu bytes of device memory", fbytes);
@@ -771,7 +999,18 @@ gpe_status gpe_primal::enqueue(uint64_t nsteps, uint64_t *launches) {
     const double dt = cfg.dt;
     uint64_t nl = 0;
     hipError_t e = hipSuccess;
-    if (cfg.mode == GPE_REAL_TIME) {
+    if (cfg.mode == GPE_REAL_TIME && kPrimalTiles[tile].fused) {
+        for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
+            int spare[2], n = 0;
+            for (int f = 0; f < 4; f++)
+                if (f != ie && f != io) spare[n++] = f;
+            e = launch_primal_fused<T, FORM>(tile, (C *)field[spare[0]], (C *)field[spare[1]], (const C *)field[ie],
+                                             (const C *)field[io], L, H, trap, dt, stream);
+            ie = spare[0];
+            io = spare[1];
+            nl += 1;
+        }
+    } else if (cfg.mode == GPE_REAL_TIME) {
         C *ev = (C *)field[ie], *od = (C *)field[io];
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
             // the even update walks the tiles backwards, starting on the layers
@@ -783,12 +1022,14 @@ gpe_status gpe_primal::enqueue(uint64_t nsteps, uint64_t *launches) {
         }
     } else {
         // Euler step into the other buffer; the renormalisation is folded into
-        // the next read through scal[1] = 1/sqrt(N).
-        const long long nb = primal_blocks(tile, L);
+        // the next read through scal[1] = 1/sqrt(N).  (A fused tile is a
+        // leapfrog step; imaginary time runs the size's default shape.)
+        const int itile = kPrimalTiles[tile].fused ? imag_tile : tile;
+        const long long nb = primal_blocks(itile, L);
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
             C *in = (C *)field[ie], *out = (C *)field[io];
             const bool rev = (steps_done + s) & 1;
-            e = launch_primal<T, FORM, MODE_IMAG>(tile, out, in, L, H, trap, dt, scal + 1, partials, stream, rev);
+            e = launch_primal<T, FORM, MODE_IMAG>(itile, out, in, L, H, trap, dt, scal + 1, partials, stream, rev);
             if (e != hipSuccess) break;
             hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(256), 0, stream, partials, nb, w, 1, scal);
             e = hipGetLastError();
