@@ -149,9 +149,10 @@ def batched_workload(n_inst, steps, seed=4, log=log):
             b.add(c)
         return b
 
-    tuned = tune_batch_tile(lambda w: make(w, cfgs[: max(8, n_inst // 4)]), chunk_steps=10, chunks=2)
-    log(f"batched: tile autotuned to {tuned['waves']} waves ({tuned['ms']})")
-    B = make(tuned["waves"])
+    tuned = tune_batch_tile(lambda t: make(t, cfgs[: max(8, n_inst // 4)]), chunk_steps=10, chunks=2)
+    log(f"batched: tile autotuned to {tuned['waves']} waves, {'cached' if tuned['cached'] else 'streamed'} "
+        f"read-modify-write ({tuned['ms']})")
+    B = make(tuned["tile"])
     ids = list(B.dims.keys())
     n0 = {i: B.observables(i)["N"] for i in ids}
     chunk = 100
@@ -170,7 +171,8 @@ def batched_workload(n_inst, steps, seed=4, log=log):
     gbs = rate * 48.0 / 1e9
     return {"instances": n_inst, "steps": steps, "nodes_total": nodes, "site_steps_per_s": rate,
             "timed_ms": ms_total, "timed_steps": rest, "algorithmic_GBs": gbs,
-            "frac_of_peak": gbs / HBM_PEAK_GBS, "tile_waves": tuned["waves"], "max_rel_norm_drift": drift,
+            "frac_of_peak": gbs / HBM_PEAK_GBS, "tile_waves": tuned["waves"], "tile_cached_rmw": tuned["cached"],
+            "max_rel_norm_drift": drift,
             "dtype": "fp32 bulk", "sizes": sorted(set(int(c.nx) for c in cfgs)), "h": 0.25}
 
 

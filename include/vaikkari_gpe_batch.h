@@ -84,9 +84,12 @@ gpe_status gpe_batch_run(gpe_batch *b, uint32_t chunk_steps, uint32_t chunks, do
 /* Sum over active instances of nodes * min(chunk_steps, steps_left), i.e.
  * the node-steps the next chunk will perform. */
 uint64_t gpe_batch_chunk_node_steps(const gpe_batch *b, uint32_t chunk_steps);
-/* Waves per block (2, 4 or 8) of the real-time pass; only while the batch
- * is empty.  The autotuner picks it. */
-gpe_status gpe_batch_set_tile(gpe_batch *b, uint32_t waves);
+/* Tile of the real-time pass: waves per block (1, 2, 4 or 8), optionally
+ * | GPE_BATCH_TILE_CACHED for a cached read-modify-write of the updated field
+ * (by default it streams past the caches, non-temporal); only while the
+ * batch is empty.  Results do not depend on it.  The autotuner picks it. */
+#define GPE_BATCH_TILE_CACHED 0x100u
+gpe_status gpe_batch_set_tile(gpe_batch *b, uint32_t tile);
 /* How gpe_batch_run issues a chunk: GPE_BATCH_GRAPH replays a captured
  * hipGraph of the chunk (the default), GPE_BATCH_STREAM enqueues the chunk's
  * kernels onto the batch's stream directly.  Results are identical. */

@@ -41,7 +41,7 @@ def test_tune_persist_and_parity(tmp_path):
 def test_batch_tile_choice_does_not_change_results():
     cfg = instance_config((33, 21, 27), 0.3, 700.0, 6e-4, 25, "real", 1.6, seed=4, mu0=3.0)
     outs = []
-    for w in (2, 4, 8):
+    for w in (1, 2, 4, 8, (1, True), (2, True), (8, True)):
         B = Batch("fp32")
         B.set_tile(w)
         i = B.add(cfg)
@@ -52,7 +52,7 @@ def test_batch_tile_choice_does_not_change_results():
         np.testing.assert_array_equal(e, outs[0][0])
         np.testing.assert_array_equal(o, outs[0][1])
     res = autotune.tune_batch_tile(lambda w: _mk(w, cfg), chunk_steps=5, chunks=1)
-    assert res["waves"] in (2, 4, 8)
+    assert res["waves"] in (1, 2, 4, 8) and res["cached"] in (False, True)
 
 
 def _mk(w, cfg):

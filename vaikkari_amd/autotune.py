@@ -91,15 +91,22 @@ def tune(s: Solver, steps: int = 20, families=("gather", "zmarch", "fused"), use
     return dict(out, cached=False)
 
 
-def tune_batch_tile(make_batch, chunk_steps: int = 10, chunks: int = 2, candidates=(2, 4, 8)) -> dict:
-    """Picks the real-time tile (waves per block) of a batch: make_batch(waves) must return a
-    populated Batch; each candidate is timed over chunks x chunk_steps steps."""
+BATCH_TILES = tuple((w, c) for c in (False, True) for w in (1, 2, 4, 8))
+
+
+def tune_batch_tile(make_batch, chunk_steps: int = 10, chunks: int = 2, candidates=BATCH_TILES) -> dict:
+    """Picks the real-time tile of a batch -- waves per block and streamed or cached
+    read-modify-write: make_batch(tile) must return a populated Batch whose set_tile was given
+    `tile` (an int number of waves, or a (waves, cached) tuple); each candidate is timed over
+    chunks x chunk_steps steps."""
     timings = {}
-    for w in candidates:
-        b = make_batch(w)
+    for t in candidates:
+        b = make_batch(t)
         b.run(chunk_steps, 1)  # capture + warm-up
         ms = b.run(chunk_steps, chunks)
-        timings[w] = ms
+        timings[t] = ms
         b.close()
     best = min(timings, key=timings.get)
-    return {"waves": best, "ms": timings}
+    waves, cached = best if isinstance(best, tuple) else (best, False)
+    return {"waves": waves, "cached": cached, "tile": best,
+            "ms": {(f"{k[0]}{'c' if k[1] else ''}" if isinstance(k, tuple) else str(k)): v for k, v in timings.items()}}

@@ -16,6 +16,8 @@ from . import _lib
 from ._lib import check, dptr, lib
 from .solver import MODES, PRECISIONS, nodes_of
 
+TILE_CACHED = 0x100  # GPE_BATCH_TILE_CACHED: cached read-modify-write of the real-time pass
+
 
 def instance_config(dims: Sequence[int], h: float, g: float, dt: float, steps: int, mode: str = "real",
                     trap_ratio: float = 1.0, init: str = "gaussian", sigma: float = 1.5, noise: float = 1e-3,
@@ -58,8 +60,13 @@ class Batch:
         except Exception:
             pass
 
-    def set_tile(self, waves: int) -> None:
-        check(lib.gpe_batch_set_tile(self._h, int(waves)), "gpe_batch_set_tile")
+    def set_tile(self, waves, cached: bool = False) -> None:
+        """Real-time tile: waves per block (1, 2, 4, 8) and whether the updated field's
+        read-modify-write is cached (default: streamed).  A (waves, cached) tuple is accepted."""
+        if isinstance(waves, tuple):
+            waves, cached = waves
+        tile = int(waves) | (TILE_CACHED if cached else 0)
+        check(lib.gpe_batch_set_tile(self._h, tile), "gpe_batch_set_tile")
 
     def set_launch_mode(self, mode: str) -> None:
         """"graph": replay a captured hipGraph per chunk (default); "stream": enqueue directly."""

@@ -98,7 +98,7 @@ constexpr int kFinalizeThreads = 256;
 // One half of a leapfrog step over all tiles of all real-time instances:
 // half 0 updates odd from even, half 1 even from odd (in place, as
 // GPUGrossPitaevskii_BCC.cu:478,480).
-template <typename T, int TY>
+template <typename T, int TY, bool NT>
 __global__ void __launch_bounds__(64 * TY)
     k_batch_real(const InstDesc<T> *__restrict__ D, const int *__restrict__ items, const BatchHeader *__restrict__ hdr,
                  int s, int chunk, int half) {
@@ -119,8 +119,8 @@ __global__ void __launch_bounds__(64 * TY)
         item_tile(d.gg, lt, bx, by, bz);
         cplx_t<T> *next = global_ptr<cplx_t<T>>(d.buf[half == 0 ? 1 : 0]);
         const cplx_t<T> *prev = global_ptr<cplx_t<T>>(d.buf[half == 0 ? 0 : 1]);
-        gather_tile<T, FORM, POT_TRAP, MODE_REAL, TY, true>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, nullptr, bx,
-                                                           by, bz, d.gg.lsh);
+        gather_tile<T, FORM, POT_TRAP, MODE_REAL, TY, NT>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, nullptr, bx,
+                                                         by, bz, d.gg.lsh);
     }
 }
 
@@ -279,6 +279,7 @@ struct gpe_batch {
     uint32_t precision = GPE_FP32_BULK;
     uint64_t budget = 0;
     int real_ty = kRealTY;  // waves per block of the real-time pass (fixed while instances exist)
+    bool real_nt = true;    // streaming (non-temporal) read-modify-write of the updated field
     std::vector<BInst> inst;
     uint64_t resident = 0;
     hipStream_t stream = nullptr;
@@ -471,19 +472,20 @@ void enqueue_chunk(gpe_batch *B, uint32_t chunk, hipStream_t st) {
     for (uint32_t s = 0; s < chunk; s++) {
         if (B->n_real > 0) {
             for (int half = 0; half < 2; half++) {
-                switch (B->real_ty) {
-                    case 2:
-                        hipLaunchKernelGGL((k_batch_real<T, 2>), dim3(B->grid_real), dim3(64, 2), 0, st, R, B->d_items, B->d_hdr,
-                                           (int)s, (int)chunk, half);
-                        break;
-                    case 8:
-                        hipLaunchKernelGGL((k_batch_real<T, 8>), dim3(B->grid_real), dim3(64, 8), 0, st, R, B->d_items, B->d_hdr,
-                                           (int)s, (int)chunk, half);
-                        break;
-                    default:
-                        hipLaunchKernelGGL((k_batch_real<T, 4>), dim3(B->grid_real), dim3(64, 4), 0, st, R, B->d_items, B->d_hdr,
-                                           (int)s, (int)chunk, half);
+#define GPE_BATCH_REAL(TY, NT)                                                                            \
+    hipLaunchKernelGGL((k_batch_real<T, TY, NT>), dim3(B->grid_real), dim3(64, TY), 0, st, R, B->d_items, \
+                       B->d_hdr, (int)s, (int)chunk, half)
+                switch (B->real_ty * 2 + (B->real_nt ? 1 : 0)) {
+                    case 2: GPE_BATCH_REAL(1, false); break;
+                    case 3: GPE_BATCH_REAL(1, true); break;
+                    case 4: GPE_BATCH_REAL(2, false); break;
+                    case 5: GPE_BATCH_REAL(2, true); break;
+                    case 8: GPE_BATCH_REAL(4, false); break;
+                    case 16: GPE_BATCH_REAL(8, false); break;
+                    case 17: GPE_BATCH_REAL(8, true); break;
+                    default: GPE_BATCH_REAL(4, true);
                 }
+#undef GPE_BATCH_REAL
             }
         }
         if (B->n_imag > 0) {
@@ -905,9 +907,10 @@ gpe_status gpe_batch_set_launch_mode(gpe_batch *B, uint32_t mode) {
     return GPE_OK;
 }
 
-gpe_status gpe_batch_set_tile(gpe_batch *B, uint32_t waves) {
-    if (!B || (waves != 2 && waves != 4 && waves != 8)) {
-        set_error("gpe_batch_set_tile: waves must be 2, 4 or 8");
+gpe_status gpe_batch_set_tile(gpe_batch *B, uint32_t tile) {
+    const uint32_t waves = tile & 0xff;
+    if (!B || (waves != 1 && waves != 2 && waves != 4 && waves != 8) || (tile & ~(0xffu | GPE_BATCH_TILE_CACHED))) {
+        set_error("gpe_batch_set_tile: waves must be 1, 2, 4 or 8 (optionally | GPE_BATCH_TILE_CACHED)");
         return GPE_EINVAL;
     }
     for (auto &b : B->inst)
@@ -916,6 +919,7 @@ gpe_status gpe_batch_set_tile(gpe_batch *B, uint32_t waves) {
             return GPE_EBUSY;
         }
     B->real_ty = (int)waves;
+    B->real_nt = !(tile & GPE_BATCH_TILE_CACHED);
     B->graph_dirty = true;
     return GPE_OK;
 }
