@@ -48,11 +48,15 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson serving sample (0 = skip)")
-    ap.add_argument("--suite", default="main", choices=["main", "precision", "serving", "primal"],
+    ap.add_argument("--suite", default="main", choices=["main", "precision", "serving", "primal", "batched"],
                     help="main: the JSON line of record; precision: 256^3 FP32-vs-FP64 error study; "
                          "serving: the Poisson serving stream alone (use --serve-seconds 600); "
                          "primal: the 14-point primal stepper's tile sweep alone (with the 1 GiB copy "
-                         "probe that calibrates rocprof traffic counters)")
+                         "probe that calibrates rocprof traffic counters); batched: the 64-instance batched "
+                         "workload alone (with the same probe)")
+    ap.add_argument("--batch-launch", default="graph", choices=["graph", "stream"],
+                    help="batched workload: replay a captured hipGraph per chunk, or enqueue the kernels "
+                         "directly (for per-dispatch counter runs)")
     ap.add_argument("--serve-policy", default="shortest_first", choices=["shortest_first", "fifo"],
                     help="admission policy of --suite serving (shortest_first is capped at 64 M resident nodes)")
     ap.add_argument("--primal-cells", default="128,256",
@@ -123,12 +127,12 @@ def cpu_baseline(dims, b, g, dt, ratio, seconds):
                       f"{n2} steps timed ({t3 - t2:.1f} s) minus a 1-step run"}
 
 
-def batched_workload(n_inst, steps, seed=4, log=log):
+def batched_workload(n_inst, steps, seed=4, log=log, launch="graph"):
     """64 independent condensates: n^3 cells with n uniform in {24,32,48,64,96}, cube edge 0.25,
     g log-uniform in [100, 5000], trap anisotropy uniform in [0.5, 4], dt = 1e-3*32/n capped by
     the reference's stability rule, FP32 bulk; Gaussian + noise initial states (device recipe)."""
     import vaikkari_amd as va
-    from vaikkari_amd.autotune import tune_batch_tile
+    from vaikkari_amd.autotune import tune_batch_groups, tune_batch_tile
     from vaikkari_amd.batch import Batch, instance_config
     from vaikkari_amd.serve import stable_dt
 
@@ -145,14 +149,20 @@ def batched_workload(n_inst, steps, seed=4, log=log):
     def make(waves, cfg_list=cfgs):
         b = Batch("fp32")
         b.set_tile(waves)
+        b.set_launch_mode(launch)
         for c in cfg_list:
             b.add(c)
         return b
 
     tuned = tune_batch_tile(lambda t: make(t, cfgs[: max(8, n_inst // 4)]), chunk_steps=10, chunks=2)
     log(f"batched: tile autotuned to {tuned['waves']} waves, {'cached' if tuned['cached'] else 'streamed'} "
-        f"read-modify-write ({tuned['ms']})")
+        f"read-modify-write, {'slab' if tuned['slabs'] else 'y-band'} order ({tuned['ms']})")
+    T = make(tuned["tile"])  # the whole workload, only to tune the group size of the schedule
+    grouping = tune_batch_groups(T, chunk_steps=10, chunks=2)
+    T.close()
+    log(f"batched: groups of at most {grouping['group_bytes'] >> 20} MiB of fields ({grouping['ms']})")
     B = make(tuned["tile"])
+    B.set_group_bytes(grouping["group_bytes"])
     ids = list(B.dims.keys())
     n0 = {i: B.observables(i)["N"] for i in ids}
     chunk = 100
@@ -172,6 +182,7 @@ def batched_workload(n_inst, steps, seed=4, log=log):
     return {"instances": n_inst, "steps": steps, "nodes_total": nodes, "site_steps_per_s": rate,
             "timed_ms": ms_total, "timed_steps": rest, "algorithmic_GBs": gbs,
             "frac_of_peak": gbs / HBM_PEAK_GBS, "tile_waves": tuned["waves"], "tile_cached_rmw": tuned["cached"],
+            "tile_slabs": tuned["slabs"], "group_MiB": grouping["group_bytes"] >> 20,
             "max_rel_norm_drift": drift,
             "dtype": "fp32 bulk", "sizes": sorted(set(int(c.nx) for c in cfgs)), "h": 0.25}
 
@@ -361,7 +372,7 @@ def precision_study(args):
 
 def main():
     args = parse()
-    if args.suite in ("precision", "serving", "primal"):
+    if args.suite in ("precision", "serving", "primal", "batched"):
         if args.gpus != 1:
             sys.exit(2)
         if args.suite == "precision":
@@ -372,6 +383,12 @@ def main():
             copy_gbs = va.copy_bandwidth(1 << 30, 3)
             res = primal_workload([int(x) for x in args.primal_cells.split(",")], args.primal_steps)
             print(json.dumps({"suite": "primal", "hbm_copy_GBs": copy_gbs, "primal14": res}), flush=True)
+        elif args.suite == "batched":
+            import vaikkari_amd as va
+
+            copy_gbs = va.copy_bandwidth(1 << 30, 3)
+            res = batched_workload(args.batch_instances, args.batch_steps, launch=args.batch_launch)
+            print(json.dumps({"suite": "batched", "hbm_copy_GBs": copy_gbs, "batched": res}), flush=True)
         else:
             serving_suite(args)
         return

@@ -86,15 +86,29 @@ gpe_status gpe_batch_run(gpe_batch *b, uint32_t chunk_steps, uint32_t chunks, do
 uint64_t gpe_batch_chunk_node_steps(const gpe_batch *b, uint32_t chunk_steps);
 /* Tile of the real-time pass: waves per block (1, 2, 4 or 8), optionally
  * | GPE_BATCH_TILE_CACHED for a cached read-modify-write of the updated field
- * (by default it streams past the caches, non-temporal); only while the
+ * (by default it streams past the caches, non-temporal) and
+ * | GPE_BATCH_TILE_SLABS to give each XCD a contiguous eighth (a slab of
+ * layers) of every instance's tiles (by default y-bands of every layer when
+ * an instance's block rows divide by 8, else plain order); only while the
  * batch is empty.  Results do not depend on it.  The autotuner picks it. */
 #define GPE_BATCH_TILE_CACHED 0x100u
+#define GPE_BATCH_TILE_SLABS 0x200u
 gpe_status gpe_batch_set_tile(gpe_batch *b, uint32_t tile);
 /* How gpe_batch_run issues a chunk: GPE_BATCH_GRAPH replays a captured
  * hipGraph of the chunk (the default), GPE_BATCH_STREAM enqueues the chunk's
  * kernels onto the batch's stream directly.  Results are identical. */
 typedef enum gpe_batch_launch_mode { GPE_BATCH_GRAPH = 0, GPE_BATCH_STREAM = 1 } gpe_batch_launch_mode;
 gpe_status gpe_batch_set_launch_mode(gpe_batch *b, uint32_t mode);
+/* Real-time instances advance group by group: contiguous runs of instances
+ * (in the order they were added) whose fields total at most `bytes` each run
+ * the whole chunk before the next group starts, so that much of a group's
+ * data is still in the GPU's last-level cache (256 MB Infinity Cache) when
+ * the next half-update reads it.  0 puts every instance in one group (each
+ * half-update sweeps the whole batch).  Default 640 MiB; may change at any
+ * time (takes effect at the next gpe_batch_run).  Results do not depend on
+ * it: instances are independent. */
+gpe_status gpe_batch_set_group_bytes(gpe_batch *b, uint64_t bytes);
+uint64_t gpe_batch_group_bytes(const gpe_batch *b);
 /* Number of active instances; bytes resident. */
 int32_t gpe_batch_active(const gpe_batch *b);
 uint64_t gpe_batch_resident_bytes(const gpe_batch *b);

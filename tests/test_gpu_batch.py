@@ -118,6 +118,9 @@ def test_alone_equals_in_mix(precision):
     mix = Batch(precision)
     if precision == "fp64":
         mix.set_launch_mode("stream")  # the mix without graphs against graph-launched instances alone
+        mix.set_group_bytes(0)  # every real-time instance in one group
+    else:
+        mix.set_group_bytes(1)  # every real-time instance a group of its own
     ids = [mix.add(instance_config(c["dims"], c["h"], c["g"], c["dt"], c["steps"], c["mode"], c["trap_ratio"],
                                    seed=c["seed"], mu0=5.0)) for c in cfgs]
     mix.run(7, 10)
@@ -134,6 +137,29 @@ def test_alone_equals_in_mix(precision):
             np.testing.assert_array_equal(a_o, m_o)
         alone.close()
     mix.close()
+
+
+def test_group_schedule_changes_nothing():
+    """Real-time instances advanced group by group (group sizes changed between chunks, from one
+    instance per group to all in one) end bit-identical to the whole batch in one group."""
+    announce("batch group schedule")
+    cfgs = _mixed_configs()
+    runs = []
+    for groups in ([0, 0, 0, 0], [1, 3 << 20, 0, 1]):
+        B = Batch("fp32")
+        ids = [B.add(instance_config(c["dims"], c["h"], c["g"], c["dt"], c["steps"], c["mode"], c["trap_ratio"],
+                                     seed=c["seed"], mu0=5.0)) for c in cfgs]
+        for gb in groups:
+            B.set_group_bytes(gb)
+            assert B.group_bytes == gb
+            B.run(9, 2)
+        runs.append([B.get_state(i) for i in ids])
+        assert all(B.info(i)["steps_done"] == c["steps"] for c, i in zip(cfgs, ids))
+        B.close()
+    for (e0, o0), (e1, o1) in zip(*runs):
+        np.testing.assert_array_equal(e1, e0)
+        if o0 is not None:
+            np.testing.assert_array_equal(o1, o0)
 
 
 def test_device_gaussian_init_and_budget():

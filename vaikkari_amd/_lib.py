@@ -215,6 +215,8 @@ SIGNATURES = [
     ("gpe_batch_active", C.c_int32, [_P]),
     ("gpe_batch_resident_bytes", _U64, [_P]),
     ("gpe_batch_set_tile", _I, [_P, _U32]),
+    ("gpe_batch_set_group_bytes", _I, [_P, _U64]),
+    ("gpe_batch_group_bytes", _U64, [_P]),
     ("gpe_batch_set_launch_mode", _I, [_P, _U32]),
     ("gpe_server_create", _I, [_U32, _U64, _U32, C.POINTER(_P)]),
     ("gpe_server_destroy", None, [_P]),

@@ -91,14 +91,18 @@ def tune(s: Solver, steps: int = 20, families=("gather", "zmarch", "fused"), use
     return dict(out, cached=False)
 
 
-BATCH_TILES = tuple((w, c) for c in (False, True) for w in (1, 2, 4, 8))
+BATCH_TILES = tuple((w, c, sl) for sl in (False, True) for c in (False, True) for w in (1, 2, 4, 8))
+
+
+def _tile_triple(t):
+    return (tuple(t) + (False, False))[:3] if isinstance(t, tuple) else (int(t), False, False)
 
 
 def tune_batch_tile(make_batch, chunk_steps: int = 10, chunks: int = 2, candidates=BATCH_TILES) -> dict:
-    """Picks the real-time tile of a batch -- waves per block and streamed or cached
-    read-modify-write: make_batch(tile) must return a populated Batch whose set_tile was given
-    `tile` (an int number of waves, or a (waves, cached) tuple); each candidate is timed over
-    chunks x chunk_steps steps."""
+    """Picks the real-time tile of a batch -- waves per block, streamed or cached
+    read-modify-write, and y-band or slab order of the tiles over the XCDs: make_batch(tile) must
+    return a populated Batch whose set_tile was given `tile` (an int number of waves, or a
+    (waves, cached[, slabs]) tuple); each candidate is timed over chunks x chunk_steps steps."""
     timings = {}
     for t in candidates:
         b = make_batch(t)
@@ -107,6 +111,24 @@ def tune_batch_tile(make_batch, chunk_steps: int = 10, chunks: int = 2, candidat
         timings[t] = ms
         b.close()
     best = min(timings, key=timings.get)
-    waves, cached = best if isinstance(best, tuple) else (best, False)
-    return {"waves": waves, "cached": cached, "tile": best,
-            "ms": {(f"{k[0]}{'c' if k[1] else ''}" if isinstance(k, tuple) else str(k)): v for k, v in timings.items()}}
+    waves, cached, slabs = _tile_triple(best)
+    return {"waves": waves, "cached": cached, "slabs": slabs, "tile": best,
+            "ms": {"%d%s%s" % (w, "c" if c else "", "s" if sl else ""): v
+                   for (w, c, sl), v in ((_tile_triple(k), v) for k, v in timings.items())}}
+
+
+BATCH_GROUP_BYTES = (0, 256 << 20, 512 << 20, 640 << 20, 1 << 30, 2 << 30)
+
+
+def tune_batch_groups(batch, chunk_steps: int = 10, chunks: int = 2, candidates=BATCH_GROUP_BYTES) -> dict:
+    """Picks the group size of a batch's real-time schedule (Batch.set_group_bytes) by timing
+    chunks x chunk_steps steps of `batch` itself under each candidate (every candidate advances
+    the instances: give them steps to spare), and leaves the best one set."""
+    timings = {}
+    for gb in candidates:
+        batch.set_group_bytes(gb)
+        batch.run(chunk_steps, 1)  # re-grouping, capture + warm-up
+        timings[gb] = batch.run(chunk_steps, chunks)
+    best = min(timings, key=timings.get)
+    batch.set_group_bytes(best)
+    return {"group_bytes": best, "ms": {str(k >> 20) + "MiB": v for k, v in timings.items()}}

@@ -17,6 +17,7 @@ from ._lib import check, dptr, lib
 from .solver import MODES, PRECISIONS, nodes_of
 
 TILE_CACHED = 0x100  # GPE_BATCH_TILE_CACHED: cached read-modify-write of the real-time pass
+TILE_SLABS = 0x200  # GPE_BATCH_TILE_SLABS: each XCD takes a contiguous eighth of an instance's tiles
 
 
 def instance_config(dims: Sequence[int], h: float, g: float, dt: float, steps: int, mode: str = "real",
@@ -60,13 +61,24 @@ class Batch:
         except Exception:
             pass
 
-    def set_tile(self, waves, cached: bool = False) -> None:
-        """Real-time tile: waves per block (1, 2, 4, 8) and whether the updated field's
-        read-modify-write is cached (default: streamed).  A (waves, cached) tuple is accepted."""
+    def set_tile(self, waves, cached: bool = False, slabs: bool = False) -> None:
+        """Real-time tile: waves per block (1, 2, 4, 8), whether the updated field's
+        read-modify-write is cached (default: streamed) and whether each XCD takes a contiguous
+        slab of an instance's tiles (default: y-bands or plain order).  A (waves, cached) or
+        (waves, cached, slabs) tuple is accepted."""
         if isinstance(waves, tuple):
-            waves, cached = waves
-        tile = int(waves) | (TILE_CACHED if cached else 0)
+            waves, cached, slabs = (tuple(waves) + (False, False))[:3]
+        tile = int(waves) | (TILE_CACHED if cached else 0) | (TILE_SLABS if slabs else 0)
         check(lib.gpe_batch_set_tile(self._h, tile), "gpe_batch_set_tile")
+
+    def set_group_bytes(self, nbytes: int) -> None:
+        """Real-time instances advance a chunk group by group, groups of at most nbytes of fields
+        (0: one group); results do not depend on it."""
+        check(lib.gpe_batch_set_group_bytes(self._h, int(nbytes)), "gpe_batch_set_group_bytes")
+
+    @property
+    def group_bytes(self) -> int:
+        return int(lib.gpe_batch_group_bytes(self._h))
 
     def set_launch_mode(self, mode: str) -> None:
         """"graph": replay a captured hipGraph per chunk (default); "stream": enqueue directly."""

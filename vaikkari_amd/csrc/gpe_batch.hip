@@ -74,22 +74,35 @@ __device__ __forceinline__ P *global_ptr(unsigned long long a) {
 // on XCD (lt mod 8) (or on XCD 7 - (lt mod 8) when the items are walked
 // backwards).  With remap 2 (block rows divisible by 8) band (lt mod 8) is a
 // y-band of every layer, as in block_coords: each XCD keeps the halo rows of
-// its band in its own L2.  Otherwise tiles go in (bx, by, bz) order.
-__device__ __forceinline__ void item_tile(const Grid3 &G, int lt, int &bx, int &by, int &bz) {
+// its band in its own L2.  With remap 1 XCD (lt mod 8) takes the (lt mod 8)-th
+// contiguous eighth of the tiles in (bx, by, bz) order, a slab of layers, so
+// only the layers at the slab ends are shared between L2s.  Otherwise tiles go
+// in (bx, by, bz) order.  Returns false for padding items (the item count is
+// the tile count rounded up to a multiple of 8).
+__device__ __forceinline__ bool item_tile(const Grid3 &G, int lt, int &bx, int &by, int &bz) {
+    const int n = G.gx * G.gy * G.gz;
+    int t = lt;
     if (G.remap == 2) {
         const int br = G.gy >> 3, bs = G.gx * br, slot = lt >> 3;
         const int layer = slot / bs, within = slot - layer * bs;
         bz = layer;
         by = (lt & 7) * br + within / G.gx;
         bx = within % G.gx;
-    } else {
-        bx = lt % G.gx;
-        by = (lt / G.gx) % G.gy;
-        bz = lt / (G.gx * G.gy);
+        return layer < G.gz;
     }
+    if (G.remap == 1) t = (lt & 7) * ((n + 7) >> 3) + (lt >> 3);
+    bx = t % G.gx;
+    by = (t / G.gx) % G.gy;
+    bz = t / (G.gx * G.gy);
+    return t < n;
 }
 
 constexpr int round8(int n) { return (n + 7) & ~7; }
+// Field bytes per group of real-time instances.  A group's leapfrog reads and
+// writes its fields twice per step; the second half-update walks the tiles
+// backwards, so what the 256 MB Infinity Cache holds of the first is read
+// again from there, a large share for a group a few times the cache's size.
+constexpr uint64_t kDefaultGroupBytes = 640ull << 20;
 constexpr int kImagTY = 4;
 constexpr int kRealTY = 4;  // default waves per block of the real-time pass
 constexpr int kFinalizeThreads = 256;
@@ -98,25 +111,25 @@ constexpr int kFinalizeThreads = 256;
 // One half of a leapfrog step over all tiles of all real-time instances:
 // half 0 updates odd from even, half 1 even from odd (in place, as
 // GPUGrossPitaevskii_BCC.cu:478,480).
+// The launch covers the work items [item0, item0 + nitems) of one group of
+// instances (item0 and nitems are multiples of 8).
 template <typename T, int TY, bool NT>
 __global__ void __launch_bounds__(64 * TY)
-    k_batch_real(const InstDesc<T> *__restrict__ D, const int *__restrict__ items, const BatchHeader *__restrict__ hdr,
-                 int s, int chunk, int half) {
+    k_batch_real(const InstDesc<T> *__restrict__ D, const int *__restrict__ items, int item0, int nitems, int s,
+                 int chunk, int half) {
     constexpr int FORM = sizeof(T) == 8 ? FORM_REF : FORM_DIFF;
-    const int total = hdr->total_real;
-    for (int base = 0; base < total; base += gridDim.x) {
+    for (int base = 0; base < nitems; base += gridDim.x) {
         // half 1 walks the work items backwards, starting where half 0 ended
         // (those fields are still in the caches)
-        const int w = half ? total - 1 - (base + (int)blockIdx.x) : base + (int)blockIdx.x;
-        if (w < 0 || w >= total) continue;
+        const int local = base + (int)blockIdx.x;
+        if (local >= nitems) continue;
+        const int w = item0 + (half ? nitems - 1 - local : local);
         const int q = __builtin_amdgcn_readfirstlane(items[w]);
         const InstDesc<T> &d = D[q];
         const long long steps = d.steps_left < chunk ? d.steps_left : chunk;
         if (s >= steps) continue;
-        const int lt = w - d.tile0;
-        if (lt >= d.ntiles) continue;  // padding to the next multiple of 8
         int bx, by, bz;
-        item_tile(d.gg, lt, bx, by, bz);
+        if (!item_tile(d.gg, w - d.tile0, bx, by, bz)) continue;  // padding to the next multiple of 8
         cplx_t<T> *next = global_ptr<cplx_t<T>>(d.buf[half == 0 ? 1 : 0]);
         const cplx_t<T> *prev = global_ptr<cplx_t<T>>(d.buf[half == 0 ? 0 : 1]);
         gather_tile<T, FORM, POT_TRAP, MODE_REAL, TY, NT>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, nullptr, bx,
@@ -140,10 +153,8 @@ __global__ void __launch_bounds__(64 * kImagTY)
         const long long steps = d.steps_left < chunk ? d.steps_left : chunk;
         if (s >= steps) continue;
         const int p = (d.parity + s) & 1;
-        const int lt = w - d.tile0;
-        if (lt >= d.ntiles) continue;  // padding to the next multiple of 8
         int bx, by, bz;
-        item_tile(d.zg, lt, bx, by, bz);
+        if (!item_tile(d.zg, w - d.tile0, bx, by, bz)) continue;  // padding to the next multiple of 8
         double part[1];
         cplx_t<T> *out = global_ptr<cplx_t<T>>(d.buf[1 - p]);
         const cplx_t<T> *in = global_ptr<cplx_t<T>>(d.buf[p]);
@@ -280,6 +291,11 @@ struct gpe_batch {
     uint64_t budget = 0;
     int real_ty = kRealTY;  // waves per block of the real-time pass (fixed while instances exist)
     bool real_nt = true;    // streaming (non-temporal) read-modify-write of the updated field
+    bool real_slabs = false;  // each XCD takes a contiguous eighth of every instance's tiles
+    // Real-time instances advance a whole chunk group by group: contiguous
+    // runs of instances whose fields total at most group_bytes (0: one group)
+    uint64_t group_bytes = kDefaultGroupBytes;
+    std::vector<std::pair<int, int>> real_groups;  // (first work item, work items) per group
     std::vector<BInst> inst;
     uint64_t resident = 0;
     hipStream_t stream = nullptr;
@@ -291,7 +307,7 @@ struct gpe_batch {
     void *h_stage = nullptr;  // pinned staging for descriptor uploads
     size_t stage_bytes = 0;
     int n_real = 0, n_imag = 0;
-    int grid_real = 1, grid_imag = 1;  // blocks of the real / imaginary-time launches
+    int grid_imag = 1;                 // blocks of the imaginary-time launch
     int grid_real_items = 0;           // real-time work items (offset of the imaginary-time ones)
     // instance of every work item: real-time items [0, total_real), then imaginary-time ones
     int *d_items = nullptr;
@@ -338,8 +354,8 @@ bool valid_instance(const gpe_instance_config *c) {
            std::isfinite(c->trap_ratio) && c->init <= GPE_INIT_GAUSSIAN;
 }
 
-size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, Layout *Lout, Grid3 *gg, Grid3 *zg,
-                      int *zcz, int *ntiles) {
+size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, bool real_slabs, Layout *Lout,
+                      Grid3 *gg, Grid3 *zg, int *zcz, int *ntiles) {
     const Layout L = make_layout(c->nx, c->ny, c->nz, elem);
     size_t bytes = 2 * (size_t)L.elems * elem;  // even and odd (real time) or ping-pong pair (imaginary time)
     int tiles = 0;
@@ -349,7 +365,7 @@ size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, La
         G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
         G.gy = (L.ny + real_ty * (64 >> G.lsh) - 1) / (real_ty * (64 >> G.lsh));
         G.gz = L.nz;
-        G.remap = G.gy % 8 == 0 ? 2 : 0;
+        G.remap = real_slabs ? 1 : G.gy % 8 == 0 ? 2 : 0;
         tiles = G.gx * G.gy * G.gz;
         if (gg) *gg = G;
     } else {
@@ -419,12 +435,20 @@ gpe_status upload_desc(gpe_batch *B) {
     BatchHeader *H = (BatchHeader *)(I + B->capacity);
     int tr = 0, ti = 0;
     nr = ni = 0;
+    B->real_groups.clear();
+    uint64_t group_fill = 0;
     for (auto &b : B->inst) {
         if (!b.active || b.steps_left == 0) continue;
         if (b.cfg.mode == GPE_REAL_TIME) {
+            if (nr == 0 || (B->group_bytes && group_fill + b.bytes > B->group_bytes)) {
+                B->real_groups.emplace_back(tr, 0);
+                group_fill = 0;
+            }
+            group_fill += b.bytes;
             fill_desc(R[nr], b);
             R[nr].tile0 = tr;
             tr += round8(b.ntiles);
+            B->real_groups.back().second = tr - B->real_groups.back().first;
             nr++;
         } else {
             fill_desc(I[ni], b);
@@ -459,7 +483,6 @@ gpe_status upload_desc(gpe_batch *B) {
     B->n_real = nr;
     B->n_imag = ni;
     B->grid_real_items = tr;
-    B->grid_real = std::max(1, std::min(tr, kMaxBatchGrid));
     B->grid_imag = std::max(1, std::min(ti, kMaxBatchGrid));
     B->desc_dirty = false;
     return GPE_OK;
@@ -469,12 +492,16 @@ template <typename T>
 void enqueue_chunk(gpe_batch *B, uint32_t chunk, hipStream_t st) {
     const InstDesc<T> *R = (const InstDesc<T> *)B->d_real;
     const InstDesc<T> *I = (const InstDesc<T> *)B->d_imag;
-    for (uint32_t s = 0; s < chunk; s++) {
-        if (B->n_real > 0) {
+    // Real-time instances: each group runs the whole chunk while its fields
+    // are in the caches, then the next group.  Instances are independent, so
+    // the order does not change any result.
+    for (const auto &grp : B->real_groups) {
+        const int grid = std::max(1, std::min(grp.second, kMaxBatchGrid));
+        for (uint32_t s = 0; s < chunk; s++) {
             for (int half = 0; half < 2; half++) {
-#define GPE_BATCH_REAL(TY, NT)                                                                            \
-    hipLaunchKernelGGL((k_batch_real<T, TY, NT>), dim3(B->grid_real), dim3(64, TY), 0, st, R, B->d_items, \
-                       B->d_hdr, (int)s, (int)chunk, half)
+#define GPE_BATCH_REAL(TY, NT)                                                                          \
+    hipLaunchKernelGGL((k_batch_real<T, TY, NT>), dim3(grid), dim3(64, TY), 0, st, R, B->d_items, grp.first, \
+                       grp.second, (int)s, (int)chunk, half)
                 switch (B->real_ty * 2 + (B->real_nt ? 1 : 0)) {
                     case 2: GPE_BATCH_REAL(1, false); break;
                     case 3: GPE_BATCH_REAL(1, true); break;
@@ -488,12 +515,12 @@ void enqueue_chunk(gpe_batch *B, uint32_t chunk, hipStream_t st) {
 #undef GPE_BATCH_REAL
             }
         }
-        if (B->n_imag > 0) {
-            hipLaunchKernelGGL((k_batch_imag<T>), dim3(B->grid_imag), dim3(64, kImagTY), 0, st, I,
-                               B->d_items + B->grid_real_items, B->d_hdr, (int)s, (int)chunk);
-            hipLaunchKernelGGL((k_batch_finalize<T>), dim3(std::min(B->n_imag, 1024)), dim3(kFinalizeThreads), 0, st,
-                               I, B->d_hdr, (int)s, (int)chunk);
-        }
+    }
+    for (uint32_t s = 0; B->n_imag > 0 && s < chunk; s++) {
+        hipLaunchKernelGGL((k_batch_imag<T>), dim3(B->grid_imag), dim3(64, kImagTY), 0, st, I,
+                           B->d_items + B->grid_real_items, B->d_hdr, (int)s, (int)chunk);
+        hipLaunchKernelGGL((k_batch_finalize<T>), dim3(std::min(B->n_imag, 1024)), dim3(kFinalizeThreads), 0, st, I,
+                           B->d_hdr, (int)s, (int)chunk);
     }
     const int n = B->n_real + B->n_imag;
     hipLaunchKernelGGL((k_batch_advance<T>), dim3((n + 255) / 256), dim3(256), 0, st, (InstDesc<T> *)B->d_real,
@@ -635,7 +662,7 @@ void gpe_batch_destroy(gpe_batch *b) { delete b; }
 
 uint64_t gpe_batch_instance_bytes(const gpe_batch *B, const gpe_instance_config *c) {
     if (!B || !valid_instance(c)) return 0;
-    return instance_bytes(c, B->elem(), B->real_ty, nullptr, nullptr, nullptr, nullptr, nullptr);
+    return instance_bytes(c, B->elem(), B->real_ty, B->real_slabs, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id) {
@@ -645,7 +672,7 @@ gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id
     }
     BInst b;
     b.cfg = *c;
-    b.bytes = instance_bytes(c, B->elem(), B->real_ty, &b.L, &b.gg, &b.zg, &b.zcz, &b.ntiles);
+    b.bytes = instance_bytes(c, B->elem(), B->real_ty, B->real_slabs, &b.L, &b.gg, &b.zg, &b.zcz, &b.ntiles);
     if (B->budget && B->resident + b.bytes > B->budget) {
         set_error("gpe_batch_add: %zu bytes would exceed the budget (%llu of %llu resident)", b.bytes,
                   (unsigned long long)B->resident, (unsigned long long)B->budget);
@@ -907,10 +934,24 @@ gpe_status gpe_batch_set_launch_mode(gpe_batch *B, uint32_t mode) {
     return GPE_OK;
 }
 
+gpe_status gpe_batch_set_group_bytes(gpe_batch *B, uint64_t bytes) {
+    if (!B) {
+        set_error("gpe_batch_set_group_bytes: null batch");
+        return GPE_EINVAL;
+    }
+    B->group_bytes = bytes;
+    B->desc_dirty = true;  // groups are formed with the descriptors
+    return GPE_OK;
+}
+
+uint64_t gpe_batch_group_bytes(const gpe_batch *B) { return B ? B->group_bytes : 0; }
+
 gpe_status gpe_batch_set_tile(gpe_batch *B, uint32_t tile) {
     const uint32_t waves = tile & 0xff;
-    if (!B || (waves != 1 && waves != 2 && waves != 4 && waves != 8) || (tile & ~(0xffu | GPE_BATCH_TILE_CACHED))) {
-        set_error("gpe_batch_set_tile: waves must be 1, 2, 4 or 8 (optionally | GPE_BATCH_TILE_CACHED)");
+    if (!B || (waves != 1 && waves != 2 && waves != 4 && waves != 8) ||
+        (tile & ~(0xffu | GPE_BATCH_TILE_CACHED | GPE_BATCH_TILE_SLABS))) {
+        set_error("gpe_batch_set_tile: waves must be 1, 2, 4 or 8 (optionally | GPE_BATCH_TILE_CACHED, "
+                  "| GPE_BATCH_TILE_SLABS)");
         return GPE_EINVAL;
     }
     for (auto &b : B->inst)
@@ -920,6 +961,7 @@ gpe_status gpe_batch_set_tile(gpe_batch *B, uint32_t tile) {
         }
     B->real_ty = (int)waves;
     B->real_nt = !(tile & GPE_BATCH_TILE_CACHED);
+    B->real_slabs = (tile & GPE_BATCH_TILE_SLABS) != 0;
     B->graph_dirty = true;
     return GPE_OK;
 }
