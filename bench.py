@@ -58,7 +58,7 @@ def parse():
                     help="batched workload: replay a captured hipGraph per chunk, or enqueue the kernels "
                          "directly (for per-dispatch counter runs)")
     ap.add_argument("--serve-policy", default="shortest_first", choices=["shortest_first", "fifo"],
-                    help="admission policy of --suite serving (shortest_first is capped at 64 M resident nodes)")
+                    help="admission policy of --suite serving (shortest_first is capped at 32 M resident nodes)")
     ap.add_argument("--primal-cells", default="128,256",
                     help="cube sizes of the 14-point primal stepper sweep (comma list; empty = skip)")
     ap.add_argument("--primal-steps", type=int, default=200)
@@ -187,7 +187,10 @@ def batched_workload(n_inst, steps, seed=4, log=log, launch="graph"):
             "dtype": "fp32 bulk", "sizes": sorted(set(int(c.nx) for c in cfgs)), "h": 0.25}
 
 
-SERVE_CAP_NODES = 64 * 10 ** 6  # resident nodes that keep the batched launches saturated
+# Resident nodes under shortest-first admission.  On this stream (20 s) 32 M finishes the most work:
+# 223 requests, p50 2.7 s, p99 7.4-8.6 s, 105 G site-steps/s served; 16 M gives p50 1.7 s at 92 G,
+# 64 M p50 4.5 s / p99 13.6 s at 104 G, 128 M p50 5.9 s / p99 17.4 s at 111 G.
+SERVE_CAP_NODES = 32 * 10 ** 6
 
 
 def serving_workload(seconds, policy="shortest_first", cap=SERVE_CAP_NODES, log=log):
