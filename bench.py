@@ -54,6 +54,11 @@ def parse():
                          "primal: the 14-point primal stepper's tile sweep alone (with the 1 GiB copy "
                          "probe that calibrates rocprof traffic counters); batched: the 64-instance batched "
                          "workload alone (with the same probe)")
+    ap.add_argument("--batch-tile", default="",
+                    help="batched suite: fixed real-time tile instead of the autotuner's, waves followed by "
+                         "c (cached read-modify-write) and/or s (slab order), e.g. 2 or 1cs")
+    ap.add_argument("--batch-group-mib", type=int, default=None,
+                    help="batched suite: fixed group size of the real-time schedule in MiB (0 = one group)")
     ap.add_argument("--batch-launch", default="graph", choices=["graph", "stream"],
                     help="batched workload: replay a captured hipGraph per chunk, or enqueue the kernels "
                          "directly (for per-dispatch counter runs)")
@@ -127,10 +132,12 @@ def cpu_baseline(dims, b, g, dt, ratio, seconds):
                       f"{n2} steps timed ({t3 - t2:.1f} s) minus a 1-step run"}
 
 
-def batched_workload(n_inst, steps, seed=4, log=log, launch="graph"):
+def batched_workload(n_inst, steps, seed=4, log=log, launch="graph", tile=None, group_mib=None):
     """64 independent condensates: n^3 cells with n uniform in {24,32,48,64,96}, cube edge 0.25,
     g log-uniform in [100, 5000], trap anisotropy uniform in [0.5, 4], dt = 1e-3*32/n capped by
-    the reference's stability rule, FP32 bulk; Gaussian + noise initial states (device recipe)."""
+    the reference's stability rule, FP32 bulk; Gaussian + noise initial states (device recipe).
+    tile / group_mib fix the real-time tile (waves, cached, slabs) and the group size instead of
+    autotuning them."""
     import vaikkari_amd as va
     from vaikkari_amd.autotune import tune_batch_groups, tune_batch_tile
     from vaikkari_amd.batch import Batch, instance_config
@@ -154,12 +161,18 @@ def batched_workload(n_inst, steps, seed=4, log=log, launch="graph"):
             b.add(c)
         return b
 
-    tuned = tune_batch_tile(lambda t: make(t, cfgs[: max(8, n_inst // 4)]), chunk_steps=10, chunks=2)
-    log(f"batched: tile autotuned to {tuned['waves']} waves, {'cached' if tuned['cached'] else 'streamed'} "
+    if tile is None:
+        tuned = tune_batch_tile(lambda t: make(t, cfgs[: max(8, n_inst // 4)]), chunk_steps=10, chunks=2)
+    else:
+        tuned = {"waves": tile[0], "cached": tile[1], "slabs": tile[2], "tile": tuple(tile), "ms": "fixed"}
+    log(f"batched: tile {tuned['waves']} waves, {'cached' if tuned['cached'] else 'streamed'} "
         f"read-modify-write, {'slab' if tuned['slabs'] else 'y-band'} order ({tuned['ms']})")
-    T = make(tuned["tile"])  # the whole workload, only to tune the group size of the schedule
-    grouping = tune_batch_groups(T, chunk_steps=10, chunks=2)
-    T.close()
+    if group_mib is None:
+        T = make(tuned["tile"])  # the whole workload, only to tune the group size of the schedule
+        grouping = tune_batch_groups(T, chunk_steps=10, chunks=2)
+        T.close()
+    else:
+        grouping = {"group_bytes": int(group_mib) << 20, "ms": "fixed"}
     log(f"batched: groups of at most {grouping['group_bytes'] >> 20} MiB of fields ({grouping['ms']})")
     B = make(tuned["tile"])
     B.set_group_bytes(grouping["group_bytes"])
@@ -390,7 +403,12 @@ def main():
             import vaikkari_amd as va
 
             copy_gbs = va.copy_bandwidth(1 << 30, 3)
-            res = batched_workload(args.batch_instances, args.batch_steps, launch=args.batch_launch)
+            tile = None
+            if args.batch_tile:
+                w, flags = int(args.batch_tile.rstrip("cs")), args.batch_tile.lstrip("0123456789")
+                tile = (w, "c" in flags, "s" in flags)
+            res = batched_workload(args.batch_instances, args.batch_steps, launch=args.batch_launch, tile=tile,
+                                   group_mib=args.batch_group_mib)
             print(json.dumps({"suite": "batched", "hbm_copy_GBs": copy_gbs, "batched": res}), flush=True)
         else:
             serving_suite(args)
