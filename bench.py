@@ -417,7 +417,7 @@ def main():
             serving_suite(args)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+        print(f"bench.py: this stepper runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the stepper shards by z-slabs across GPUs by design, not built here)", file=sys.stderr)
         sys.exit(2)
     import vaikkari_amd as va

@@ -129,7 +129,7 @@ static double *or_pad_real(const double *h, const or_grid *g) {
  *   sum = ((n0 + n1) + n2) + n3
  *   s   = lapfac*sum + ((lapfac0 + pot) + g*|prev|^2) * prev
  *   next += (s.y, -s.x)                      i.e. next -= i*s
- * mode 1 (imaginary time, defined by this build): next = prev - s. */
+ * mode 1 (imaginary time, not in the reference): next = prev - s. */
 static void or_update_padded(double *next, const double *prev, const double *pot, const or_grid *g,
                              double lapfac, double lapfac0, double gg, int imag) {
     const long long soff[3] = {(long long)g->sx, (long long)g->sy, (long long)g->sz};
@@ -207,7 +207,7 @@ static void or_add(or_acc *a, double x) {
 }
 static double or_val(const or_acc *a) { return a->s + a->c; }
 
-/* Observables of one field (defined by this build; the reference has none):
+/* Observables of one field (the reference has none):
  *   N = w * sum |psi|^2
  *   K = w * sum 1/2 Re(conj(psi) * (-L psi)),  (L psi)_i = (1.5/b^2) sum_f (psi_f - psi_i)
  *   P = w * sum V |psi|^2
@@ -260,7 +260,7 @@ double or_overlap(const double *a, const double *b, long long nvals) {
     return or_val(&q);
 }
 
-/* Imaginary-time Euler steps (defined by this build):
+/* Imaginary-time Euler steps (not in the reference):
  *   psi <- psi - dtau*H(psi)psi      (same s as the real-time update)
  *   N   = w * sum |psi|^2            (Neumaier, sequential)
  *   psi <- psi * (1/sqrt(N))
