@@ -181,14 +181,7 @@ __global__ void __launch_bounds__(kFinalizeThreads)
         const InstDesc<T> &d = D[q];
         const long long steps = d.steps_left < chunk ? d.steps_left : chunk;
         if (s >= steps) continue;
-        double sum = 0.0, c = 0.0;
-        for (int b = t; b < d.ntiles; b += kFinalizeThreads) {
-            const double x = global_ptr<double>(d.partials)[b];
-            const double u = sum + x;
-            c += (fabs(sum) >= fabs(x)) ? ((sum - u) + x) : ((x - u) + sum);
-            sum = u;
-        }
-        red[t] = sum + c;
+        red[t] = strided_neumaier(global_ptr<double>(d.partials), 1, t, kFinalizeThreads, d.ntiles);
         __syncthreads();
         for (int h = kFinalizeThreads / 2; h > 0; h >>= 1) {
             if (t < h) red[t] += red[t + h];

@@ -523,22 +523,39 @@ __global__ void __launch_bounds__(64 * TY) k_observables(const cplx_t<T> *__rest
 // w and writes out[q].  For the imaginary-time norm (NV == 1, renorm != 0)
 // it also writes out[1] = 1/sqrt(N), the scale the next step applies to
 // every value it reads.
+// Neumaier-compensated sum of p[b * pitch] for b = first, first + step, ...
+// below n, added in that order.  The loads go out eight at a time ahead of
+// their additions (one memory latency per eight terms instead of per term);
+// the order of the additions, and so the result, is that of a plain loop.
+__device__ __forceinline__ void neumaier_add(double &s, double &c, double x) {
+#pragma clang fp contract(off)
+    const double u = s + x;
+    c += (fabs(s) >= fabs(x)) ? ((s - u) + x) : ((x - u) + s);
+    s = u;
+}
+__device__ __forceinline__ double strided_neumaier(const double *__restrict__ p, long long pitch, long long first,
+                                                   long long step, long long n) {
+#pragma clang fp contract(off)
+    double s = 0.0, c = 0.0;
+    long long b = first;
+    for (; b + 7 * step < n; b += 8 * step) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = p[(b + u * step) * pitch];
+#pragma unroll
+        for (int u = 0; u < 8; u++) neumaier_add(s, c, x[u]);
+    }
+    for (; b < n; b += step) neumaier_add(s, c, p[b * pitch]);
+    return s + c;
+}
+
 template <int NV>
 __global__ void __launch_bounds__(256) k_finalize(const double *__restrict__ partials, long long nblocks, double w,
                                                   int renorm, double *__restrict__ out) {
     __shared__ double red[NV][256];
     const int t = threadIdx.x;
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
-        double s = 0.0, c = 0.0;
-        for (long long b = t; b < nblocks; b += 256) {
-            const double x = partials[NV * b + q];
-            const double u = s + x;
-            c += (fabs(s) >= fabs(x)) ? ((s - u) + x) : ((x - u) + s);
-            s = u;
-        }
-        red[q][t] = s + c;
-    }
+    for (int q = 0; q < NV; q++) red[q][t] = strided_neumaier(partials + q, NV, t, 256, nblocks);
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
         if (t < h) {
