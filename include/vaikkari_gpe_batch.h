@@ -5,7 +5,7 @@
  * process (GPUGrossPitaevskii_BCC.cu:475-481).  A batch keeps every
  * instance's state resident in HBM and advances all of them with one set of
  * persistent kernel launches per step: the two in-place leapfrog
- * half-updates for real-time instances, the z-march Euler update plus a
+ * half-updates for real-time instances, the Euler update plus a
  * per-instance FP64 renormalisation for imaginary-time instances.  A chunk of steps is
  * captured once into a hipGraph and replayed (or enqueued directly, see
  * gpe_batch_set_launch_mode).  Each instance's arithmetic
@@ -47,7 +47,9 @@ typedef struct gpe_instance_config {
     double origin[3];      /* corner of cell (0,0,0); non-finite -> box centred on 0 */
     uint64_t steps;        /* steps to run */
     uint32_t init;         /* gpe_init_kind */
-    uint32_t zlayers;      /* z layers per tile, 0 = default */
+    uint32_t zlayers;      /* imaginary time: layers per tile; 0 or 1 = one layer
+                            * (the default, as the single solver's gather
+                            * kernel), > 1 = a z-march of that many layers */
     double sigma, noise, mu0; /* GPE_INIT_GAUSSIAN parameters */
     uint64_t seed;
 } gpe_instance_config;

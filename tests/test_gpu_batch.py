@@ -69,17 +69,20 @@ def test_batch_real_time_matches_oracle(orc, precision):
     B.close()
 
 
-def test_batch_imag_time_matches_solver_and_oracle(orc):
+@pytest.mark.parametrize("zlayers,kernel", [(0, "gather"), (8, "zmarch")])
+def test_batch_imag_time_matches_solver_and_oracle(orc, zlayers, kernel):
+    """One-layer tiles (the default) equal the single solver's gather kernel bit for bit, 8-layer
+    z-march tiles its z-march kernel; both match the oracle to 1e-12."""
     dims, h, g, ratio = (20, 14, 17), 0.4, 600.0, 1.7
-    announce("batch imag fp64")
+    announce(f"batch imag fp64 zlayers={zlayers}")
     b, origin, even, _, V = host_case(dims, h, g, 1e-3, ratio, 5)
     dtau = 0.5 / (3.0 / b ** 2 + V.max())
     B = Batch("fp64")
-    i = B.add(instance_config(dims, h, g, dtau, 45, "imag", ratio, init="host", origin=origin))
+    i = B.add(instance_config(dims, h, g, dtau, 45, "imag", ratio, init="host", origin=origin, zlayers=zlayers))
     B.set_state(i, even)
     B.run(16, 3)
     ge, _ = B.get_state(i)
-    s = va.Solver(dims, b, dtau, g, origin, "fp64", "imag", "trap", ratio, "zmarch")
+    s = va.Solver(dims, b, dtau, g, origin, "fp64", "imag", "trap", ratio, kernel)
     s.set_state(even)
     s.step(45)
     se = s.get_state(odd=False)

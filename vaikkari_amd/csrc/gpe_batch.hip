@@ -158,9 +158,14 @@ __global__ void __launch_bounds__(64 * kImagTY)
         double part[1];
         cplx_t<T> *out = global_ptr<cplx_t<T>>(d.buf[1 - p]);
         const cplx_t<T> *in = global_ptr<cplx_t<T>>(d.buf[p]);
-        part[0] = zmarch_tile<T, FORM, POT_TRAP, MODE_IMAG, kImagTY>(out, in, nullptr, d.L, d.H, d.trap, d.dt,
-                                                                     global_ptr<double>(d.scal) + 1, bx, by, bz,
-                                                                     d.zcz, d.zg.lsh);
+        const double *scale = global_ptr<double>(d.scal) + 1;
+        // one layer per tile (the default) or a z-march of zcz layers
+        if (d.zcz == 1)
+            part[0] = gather_tile<T, FORM, POT_TRAP, MODE_IMAG, kImagTY>(out, in, nullptr, d.L, d.H, d.trap, d.dt,
+                                                                         scale, bx, by, bz, d.zg.lsh);
+        else
+            part[0] = zmarch_tile<T, FORM, POT_TRAP, MODE_IMAG, kImagTY>(out, in, nullptr, d.L, d.H, d.trap, d.dt,
+                                                                         scale, bx, by, bz, d.zcz, d.zg.lsh);
         block_sum<1, kImagTY>(part, lds);
         // partials in logical tile order, as the single solver stores them
         if (threadIdx.x == 0 && threadIdx.y == 0)
@@ -257,7 +262,6 @@ __global__ void k_init_gaussian(cplx_t<T> *__restrict__ even, cplx_t<T> *__restr
 
 namespace {
 
-constexpr int kDefaultImagCZ = 8;
 
 struct BInst {
     gpe_instance_config cfg{};
@@ -266,7 +270,7 @@ struct BInst {
     Coefs co{};
     Grid3 gg{};
     Grid3 zg{};
-    int zcz = 8;
+    int zcz = 1;
     void *buf[4] = {nullptr, nullptr, nullptr, nullptr};
     double *partials = nullptr;
     double *scal = nullptr;
@@ -362,7 +366,7 @@ size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, bo
         tiles = G.gx * G.gy * G.gz;
         if (gg) *gg = G;
     } else {
-        const int cz = c->zlayers ? (int)c->zlayers : kDefaultImagCZ;
+        const int cz = c->zlayers ? (int)c->zlayers : 1;
         Grid3 G{};
         G.lsh = row_lanes_log2(L.nx);
         G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
