@@ -88,11 +88,12 @@ gpe_status gpe_batch_run(gpe_batch *b, uint32_t chunk_steps, uint32_t chunks, do
 uint64_t gpe_batch_chunk_node_steps(const gpe_batch *b, uint32_t chunk_steps);
 /* Tile of the real-time pass: waves per block (1, 2, 4 or 8), optionally
  * | GPE_BATCH_TILE_CACHED for a cached read-modify-write of the updated field
- * (by default it streams past the caches, non-temporal) and
+ * (without it the field streams past the caches, non-temporal) and
  * | GPE_BATCH_TILE_SLABS to give each XCD a contiguous eighth (a slab of
- * layers) of every instance's tiles (by default y-bands of every layer when
+ * layers) of every instance's tiles (without it y-bands of every layer when
  * an instance's block rows divide by 8, else plain order); only while the
- * batch is empty.  Results do not depend on it.  The autotuner picks it. */
+ * batch is empty.  A new batch starts with 2 | GPE_BATCH_TILE_SLABS.  Results
+ * do not depend on it.  The autotuner picks it. */
 #define GPE_BATCH_TILE_CACHED 0x100u
 #define GPE_BATCH_TILE_SLABS 0x200u
 gpe_status gpe_batch_set_tile(gpe_batch *b, uint32_t tile);

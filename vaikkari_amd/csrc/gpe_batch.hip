@@ -104,7 +104,10 @@ constexpr int round8(int n) { return (n + 7) & ~7; }
 // again from there, a large share for a group a few times the cache's size.
 constexpr uint64_t kDefaultGroupBytes = 640ull << 20;
 constexpr int kImagTY = 4;
-constexpr int kRealTY = 4;  // default waves per block of the real-time pass
+// Default tile of the real-time pass: 2 waves per block, streamed
+// read-modify-write, slab order (the best measured on mixes of small and
+// mid-size instances, such as a serving stream's resident set).
+constexpr int kRealTY = 2;
 constexpr int kFinalizeThreads = 256;
 
 
@@ -288,7 +291,7 @@ struct gpe_batch {
     uint64_t budget = 0;
     int real_ty = kRealTY;  // waves per block of the real-time pass (fixed while instances exist)
     bool real_nt = true;    // streaming (non-temporal) read-modify-write of the updated field
-    bool real_slabs = false;  // each XCD takes a contiguous eighth of every instance's tiles
+    bool real_slabs = true;   // each XCD takes a contiguous eighth of every instance's tiles
     // Real-time instances advance a whole chunk group by group: contiguous
     // runs of instances whose fields total at most group_bytes (0: one group)
     uint64_t group_bytes = kDefaultGroupBytes;
