@@ -204,8 +204,8 @@ def batched_workload(n_inst, steps, seed=4, log=log, launch="graph", tile=None, 
 
 
 # Resident nodes under shortest-first admission.  On this stream (20 s) 32 M finishes the most work:
-# 223 requests, p50 2.7 s, p99 7.4-8.6 s, 105 G site-steps/s served; 16 M gives p50 1.7 s at 92 G,
-# 64 M p50 4.5 s / p99 13.6 s at 104 G, 128 M p50 5.9 s / p99 17.4 s at 111 G.
+# 228-230 requests, p50 2.5 s, p99 7.0-7.7 s, 118-119 G site-steps/s served; 24 M gives p50 1.9 s at
+# 116 G, 48 M p50 3.4 s / p99 10.0 s at 117 G, 64 M p50 4.2 s / p99 11.4 s at 116 G.
 SERVE_CAP_NODES = 32 * 10 ** 6
 
 
