@@ -80,8 +80,11 @@ gpe_status gpe_batch_remove(gpe_batch *b, int32_t id);
 
 /* Advances every active instance by min(chunk_steps, steps_left) steps,
  * `chunks` times in a row; the chunk is one captured hipGraph (re-captured
- * only when chunk_steps or the instance set changes).  With elapsed_ms
- * non-NULL the device time of the whole call is returned (HIP events). */
+ * only when chunk_steps or the instance set changes).  A chunk that would
+ * hold more than 16384 kernel launches (one per group of real-time instances
+ * and half-step, two per imaginary-time step) runs as shorter chunks; where
+ * chunks end changes no result.  With elapsed_ms non-NULL the device time of
+ * the whole call is returned (HIP events). */
 gpe_status gpe_batch_run(gpe_batch *b, uint32_t chunk_steps, uint32_t chunks, double *elapsed_ms);
 /* Sum over active instances of nodes * min(chunk_steps, steps_left), i.e.
  * the node-steps the next chunk will perform. */

@@ -165,6 +165,28 @@ def test_group_schedule_changes_nothing():
             np.testing.assert_array_equal(o1, o0)
 
 
+def test_long_chunk_split_changes_nothing():
+    """A chunk too long for one captured graph (3 groups x 2 launches x 3000 steps > 16384 launches)
+    runs as shorter pieces and ends bit-identical to 100-step chunks."""
+    announce("batch long chunk split")
+    cfgs = [instance_config((8, 8, 8), 0.3, 300.0, 5e-4, 3000, "real", 1.0, seed=1, mu0=2.0),
+            instance_config((12, 10, 9), 0.3, 700.0, 5e-4, 2950, "real", 1.6, seed=2, mu0=3.0),
+            instance_config((5, 7, 3), 0.3, 100.0, 5e-4, 3000, "real", 2.0, seed=3, mu0=1.0)]
+    runs = []
+    for group_bytes, chunk, chunks in ((1, 3000, 1), (0, 100, 30)):
+        B = Batch("fp32")
+        B.set_group_bytes(group_bytes)
+        ids = [B.add(c) for c in cfgs]
+        ms = B.run(chunk, chunks)
+        assert ms > 0
+        assert [B.info(i)["steps_left"] for i in ids] == [0, 0, 0]
+        runs.append([B.get_state(i) for i in ids])
+        B.close()
+    for (e0, o0), (e1, o1) in zip(*runs):
+        np.testing.assert_array_equal(e1, e0)
+        np.testing.assert_array_equal(o1, o0)
+
+
 def test_device_gaussian_init_and_budget():
     announce("batch init + budget")
     cfg = instance_config((32, 32, 32), 0.25, 500.0, 1e-3, 10, "real", 1.0, sigma=1.5, noise=1e-3, seed=7)

@@ -103,6 +103,8 @@ constexpr int round8(int n) { return (n + 7) & ~7; }
 // backwards, so what the 256 MB Infinity Cache holds of the first is read
 // again from there, a large share for a group a few times the cache's size.
 constexpr uint64_t kDefaultGroupBytes = 640ull << 20;
+// Most kernel launches one captured chunk may hold (longer chunks are split).
+constexpr uint64_t kMaxGraphLaunches = 16384;
 constexpr int kImagTY = 4;
 // Default tile of the real-time pass: 2 waves per block, streamed
 // read-modify-write, slab order (the best measured on mixes of small and
@@ -527,6 +529,26 @@ void enqueue_chunk(gpe_batch *B, uint32_t chunk, hipStream_t st) {
                        (InstDesc<T> *)B->d_imag, B->d_hdr, (int)chunk);
 }
 
+// Kernel launches per step of a chunk under the grouping the next upload
+// forms (the same walk as upload_desc).
+uint64_t launches_per_step(const gpe_batch *B) {
+    uint64_t groups = 0, fill = 0;
+    bool imag = false;
+    for (const auto &b : B->inst) {
+        if (!b.active || b.steps_left == 0) continue;
+        if (b.cfg.mode != GPE_REAL_TIME) {
+            imag = true;
+            continue;
+        }
+        if (groups == 0 || (B->group_bytes && fill + b.bytes > B->group_bytes)) {
+            groups++;
+            fill = 0;
+        }
+        fill += b.bytes;
+    }
+    return 2 * groups + (imag ? 2 : 0);
+}
+
 template <typename T>
 gpe_status run_t(gpe_batch *B, uint32_t chunk, uint32_t chunks, double *elapsed_ms) {
     if (B->desc_dirty) {
@@ -901,8 +923,35 @@ gpe_status gpe_batch_run(gpe_batch *B, uint32_t chunk_steps, uint32_t chunks, do
         set_error("gpe_batch_run: invalid argument");
         return GPE_EINVAL;
     }
-    if (B->precision == GPE_FP64) return run_t<double>(B, chunk_steps, chunks, elapsed_ms);
-    return run_t<float>(B, chunk_steps, chunks, elapsed_ms);
+    auto run = [B](uint32_t c, uint32_t n, double *ms) {
+        return B->precision == GPE_FP64 ? run_t<double>(B, c, n, ms) : run_t<float>(B, c, n, ms);
+    };
+    // A captured chunk holds, for every step, one launch per group and half-step
+    // plus two for the imaginary-time instances.  A long chunk over many groups
+    // runs as shorter chunks of at most kMaxGraphLaunches launches each: where
+    // chunks end changes no result (every instance still takes the same steps).
+    const uint64_t per_step = launches_per_step(B);
+    if (!B->use_graph || per_step * chunk_steps <= kMaxGraphLaunches)
+        return run(chunk_steps, chunks, elapsed_ms);
+    const uint32_t piece = (uint32_t)std::max<uint64_t>(1, kMaxGraphLaunches / per_step);
+    const uint64_t total = (uint64_t)chunk_steps * chunks;
+    uint64_t full = total / piece;
+    const uint32_t rest = (uint32_t)(total % piece);
+    double sum = 0.0, ms = 0.0;
+    while (full > 0) {
+        const uint32_t n = (uint32_t)std::min<uint64_t>(full, 0xffffffffu);
+        const gpe_status st = run(piece, n, elapsed_ms ? &ms : nullptr);
+        if (st != GPE_OK) return st;
+        sum += ms;
+        full -= n;
+    }
+    if (rest) {
+        const gpe_status st = run(rest, 1, elapsed_ms ? &ms : nullptr);
+        if (st != GPE_OK) return st;
+        sum += ms;
+    }
+    if (elapsed_ms) *elapsed_ms = sum;
+    return GPE_OK;
 }
 
 uint64_t gpe_batch_chunk_node_steps(const gpe_batch *B, uint32_t chunk_steps) {
