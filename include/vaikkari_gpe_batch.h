@@ -106,7 +106,8 @@ gpe_status gpe_batch_set_tile(gpe_batch *b, uint32_t tile);
 typedef enum gpe_batch_launch_mode { GPE_BATCH_GRAPH = 0, GPE_BATCH_STREAM = 1 } gpe_batch_launch_mode;
 gpe_status gpe_batch_set_launch_mode(gpe_batch *b, uint32_t mode);
 /* Real-time instances advance group by group: contiguous runs of instances
- * (in the order they were added) whose fields total at most `bytes` each run
+ * (in id order; a removed instance's id is reused) whose fields total at
+ * most `bytes` each run
  * the whole chunk before the next group starts, so that much of a group's
  * data is still in the GPU's last-level cache (256 MB Infinity Cache) when
  * the next half-update reads it.  0 puts every instance in one group (each
