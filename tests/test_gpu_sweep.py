@@ -1,0 +1,124 @@
+"""Seeded random sweep of FP64 parity: grid shapes drawn at random (every axis 1..70, ragged
+against the 64-lane rows, the 4/8-wave blocks and the 8 XCD bands), every gather and z-march tile
+shape and both potential kinds, random batches of real-time instances under random tiles and group
+sizes, and the 14-point primal stepper's 25 tile shapes on random shapes.  Each case is stepped on
+the GPU through the C ABI and compared bit for bit with the CPU oracles (oracle/gpe_oracle.c and
+oracle/primal_oracle.c, pinned by tests/test_oracle_pins.py and tests/test_primal_pins.py) on the
+same inputs."""
+import math
+
+import numpy as np
+import pytest
+
+import vaikkari_amd as va
+from oracle_lib import Oracle, PrimalOracle
+from vaikkari_amd.batch import Batch, instance_config
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def _shapes(seed, count, max_cells=60000):
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        d = tuple(int(x) for x in rng.integers(1, 71, 3))
+        if d[0] * d[1] * d[2] <= max_cells:
+            out.append(d)
+    return out
+
+
+def _fields(dims, h, ratio, seed, amp=0.05):
+    rng = np.random.default_rng(seed)
+    b = h / math.sqrt(8)
+    origin = va.centered_origin(dims, b)
+    n = va.nodes_of(dims)
+    even = amp * rng.uniform(-1, 1, 2 * n)
+    odd = even + 0.01 * amp * rng.uniform(-1, 1, 2 * n)
+    return b, origin, even, odd, va.trap_potential(dims, origin, b, ratio)
+
+
+SHAPES = _shapes(2024, 24)
+TILES = [(v["kernel"], v["tile"]) for v in va.variants() if v["kernel"] in ("gather", "zmarch")]
+
+
+@pytest.mark.parametrize("case", range(len(SHAPES)))
+def test_random_shapes_every_tile_bit_exact(orc, case):
+    dims = SHAPES[case]
+    h, g, dt, ratio = 0.45, 400.0, 8e-4, 1.7
+    b, origin, even, odd, V = _fields(dims, h, ratio, seed=100 + case)
+    lf, lf0, gdt = va.leapfrog_coefs(b, dt, g)
+    oe, oo = orc.leapfrog(even, odd, V * dt, dims, lf, lf0, gdt, 5)
+    for q, (kernel, tile) in enumerate(TILES):
+        potential = "array" if (case + q) % 2 else "trap"
+        announce(f"sweep {dims} {kernel} tile={tile} {potential}")
+        s = va.Solver(dims, b, dt, g, origin, "fp64", "real", potential, ratio, kernel, tile)
+        if potential == "array":
+            s.set_potential(V)
+        s.set_state(even, odd)
+        s.step(5)
+        ge, go = s.get_state()
+        s.close()
+        assert np.array_equal(ge, oe) and np.array_equal(go, oo), (dims, kernel, tile, potential)
+
+
+@pytest.mark.parametrize("seed", [7, 8, 9, 10, 11, 12])
+def test_random_batch_mix_bit_exact(orc, seed):
+    rng = np.random.default_rng(seed)
+    tile = (int(rng.choice([1, 2, 4, 8])), bool(rng.integers(2)), bool(rng.integers(2)))
+    group_bytes = int(rng.choice([0, 1, 1 << 20, 640 << 20]))
+    chunk = int(rng.integers(3, 12))
+    announce(f"sweep batch seed={seed} tile={tile} group_bytes={group_bytes} chunk={chunk}")
+    B = Batch("fp64")
+    B.set_tile(tile)
+    B.set_group_bytes(group_bytes)
+    ids, refs = [], []
+    for q in range(6):
+        dims = tuple(int(x) for x in rng.integers(1, 40, 3))
+        h, ratio = float(rng.uniform(0.3, 0.6)), float(rng.uniform(0.5, 3.0))
+        g, dt, steps = float(rng.uniform(50, 900)), 5e-4, int(rng.integers(5, 30))
+        b, origin, even, odd, V = _fields(dims, h, ratio, seed=1000 * seed + q)
+        i = B.add(instance_config(dims, h, g, dt, steps, "real", ratio, init="host", origin=origin))
+        B.set_state(i, even, odd)
+        ids.append((i, steps))
+        refs.append(orc.leapfrog(even, odd, V * dt, dims, *va.leapfrog_coefs(b, dt, g), steps))
+    B.run(chunk, -(-30 // chunk))
+    for (i, steps), (oe, oo) in zip(ids, refs):
+        assert B.info(i)["steps_done"] == steps
+        ge, go = B.get_state(i)
+        assert np.array_equal(ge, oe) and np.array_equal(go, oo), (seed, i)
+    B.close()
+
+
+PRIMAL_SHAPES = _shapes(77, 8, max_cells=40000)
+
+
+@pytest.mark.parametrize("case", range(len(PRIMAL_SHAPES)))
+def test_random_shapes_primal_every_tile_bit_exact(case):
+    po = PrimalOracle()
+    dims = PRIMAL_SHAPES[case]
+    a, dt, g, ratio = 0.35, 0.004, 1.3, 1.3
+    origin = va.primal_centered_origin(dims, a)
+    V = va.primal_trap_potential(dims, origin, a, ratio)
+    coef = va.primal_coefficients(a, dt, g)
+    rng = np.random.default_rng(300 + case)
+    n = va.sites_of(dims)
+    even = 0.05 * rng.uniform(-1, 1, 2 * n)
+    odd = even + 0.0005 * rng.uniform(-1, 1, 2 * n)
+    re, ro = po.leapfrog(even, odd, V * dt, dims, coef, 3)
+    for tile in range(1, va.primal_tiles() + 1):
+        announce(f"sweep primal {dims} tile={tile}")
+        with va.PrimalSolver(dims, a, dt, g, origin, ratio, "fp64", "real", tile) as s:
+            s.set_state(even, odd)
+            s.step(3)
+            ge, go = s.get_state()
+        assert np.array_equal(ge, re) and np.array_equal(go, ro), (dims, tile)
+
