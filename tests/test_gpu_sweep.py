@@ -70,6 +70,40 @@ def test_random_shapes_every_tile_bit_exact(orc, case):
         assert np.array_equal(ge, oe) and np.array_equal(go, oo), (dims, kernel, tile, potential)
 
 
+def _rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("case", range(0, len(SHAPES), 3))
+def test_random_shapes_imag_and_fp32_every_tile(orc, case):
+    """The same shapes in imaginary time (FP64, 6 Euler steps with renormalisation: <= 1e-12
+    relative L2, the norm is a grid-wide sum in another order) and in FP32 real time (10 steps of a
+    field of full amplitude everywhere: <= 2e-6 relative L2 of the FP64 oracle)."""
+    dims = SHAPES[case]
+    h, g, dt, ratio = 0.45, 400.0, 8e-4, 1.7
+    b, origin, even, odd, V = _fields(dims, h, ratio, seed=200 + case)
+    lf, lf0, gdt = va.leapfrog_coefs(b, dt, g)
+    dtau = 0.5 / (3.0 / b ** 2 + V.max())
+    ilf, ilf0, igdt = va.leapfrog_coefs(b, dtau, g)
+    psi = even / math.sqrt(va.field_norm(dims, b, even))
+    oi, _ = orc.imag_steps(psi, V * dtau, dims, ilf, ilf0, igdt, b, 6)
+    oe, oo = orc.leapfrog(even, odd, V * dt, dims, lf, lf0, gdt, 10)
+    for kernel, tile in TILES:
+        announce(f"sweep imag+fp32 {dims} {kernel} tile={tile}")
+        s = va.Solver(dims, b, dtau, g, origin, "fp64", "imag", "trap", ratio, kernel, tile)
+        s.set_state(psi)
+        s.step(6)
+        gi = s.get_state(odd=False)
+        s.close()
+        assert _rel_l2(gi, oi) <= 1e-12, (dims, kernel, tile)
+        s = va.Solver(dims, b, dt, g, origin, "fp32", "real", "trap", ratio, kernel, tile)
+        s.set_state(even, odd)
+        s.step(10)
+        ge, go = s.get_state()
+        s.close()
+        assert _rel_l2(ge, oe) < 2e-6 and _rel_l2(go, oo) < 2e-6, (dims, kernel, tile)
+
+
 @pytest.mark.parametrize("seed", [7, 8, 9, 10, 11, 12])
 def test_random_batch_mix_bit_exact(orc, seed):
     rng = np.random.default_rng(seed)
