@@ -6,6 +6,8 @@ the GPU through the C ABI and compared bit for bit with the CPU oracles (oracle/
 oracle/primal_oracle.c, pinned by tests/test_oracle_pins.py and tests/test_primal_pins.py) on the
 same inputs."""
 import math
+import os
+import sys
 
 import numpy as np
 import pytest
@@ -13,6 +15,9 @@ import pytest
 import vaikkari_amd as va
 from oracle_lib import Oracle, PrimalOracle
 from vaikkari_amd.batch import Batch, instance_config
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import output_oracle as oo  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -130,6 +135,30 @@ def test_random_batch_mix_bit_exact(orc, seed):
         ge, go = B.get_state(i)
         assert np.array_equal(ge, oe) and np.array_equal(go, oo), (seed, i)
     B.close()
+
+
+@pytest.mark.parametrize("case", range(1, len(SHAPES), 4))
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_random_shapes_output_bit_exact(case, precision):
+    """Volume map (all classes and the reference's class-0 form) and slice pictures of every kind of
+    layer (halo, first, middle, last) on the random shapes, against the output oracle on the
+    downloaded field."""
+    dims = SHAPES[case]
+    announce(f"sweep output {dims} {precision}")
+    rng = np.random.default_rng(500 + case)
+    b = 0.25 / math.sqrt(8)
+    s = va.Solver(dims, b, 1e-4, 100.0, None, precision, "real", "trap", 1.0)
+    even = 0.08 * rng.standard_normal(2 * va.nodes_of(dims))
+    s.set_state(even, even)
+    e = s.get_state(odd=False)
+    for class0 in (False, True):
+        np.testing.assert_array_equal(s.volume_map(3.0e5, reference_class0=class0),
+                                      oo.volume_map(e, dims, 3.0e5, class0))
+    for k in sorted({0, 1, dims[2] // 2 + 1, dims[2], dims[2] + 1}):
+        for middle in (False, True):
+            np.testing.assert_array_equal(s.slice_picture(k, 5.0, middle),
+                                          oo.colour_bytes(oo.slice_colours(e, dims, k, 5.0), middle))
+    s.close()
 
 
 PRIMAL_SHAPES = _shapes(77, 8, max_cells=40000)
