@@ -161,6 +161,57 @@ def test_random_shapes_output_bit_exact(case, precision):
     s.close()
 
 
+# batch ground-state tiles and the single-solver variant with the same logical tiles (4 waves,
+# one layer / 8 / 4 layers per tile), hence the same norm summation order
+IMAG_TILES = {0: ("gather", 0), 8: ("zmarch", 1), 4: ("zmarch", 6)}
+
+
+@pytest.mark.parametrize("seed", [21, 22, 23])
+def test_random_batch_ground_states_match_solver(orc, seed):
+    """Ground states (one-layer, 8-layer and 4-layer tiles) mixed with real-time instances under a
+    random group size: each ground state equals the single solver's run of the same tiles bit for
+    bit, each real-time instance the oracle."""
+    rng = np.random.default_rng(seed)
+    B = Batch("fp64")
+    B.set_group_bytes(int(rng.choice([0, 1, 1 << 20])))
+    cases = []
+    for q in range(6):
+        dims = tuple(int(x) for x in rng.integers(2, 36, 3))
+        h, ratio, g = float(rng.uniform(0.3, 0.6)), float(rng.uniform(0.5, 3.0)), float(rng.uniform(50, 900))
+        b, origin, even, odd, V = _fields(dims, h, ratio, seed=3000 * seed + q)
+        if q % 3 == 2:
+            dt, steps = 5e-4, int(rng.integers(5, 25))
+            i = B.add(instance_config(dims, h, g, dt, steps, "real", ratio, init="host", origin=origin))
+            B.set_state(i, even, odd)
+            cases.append(("real", i, dims, b, origin, even, odd, V, g, dt, steps, ratio, None))
+        else:
+            zl = int(rng.choice([0, 4, 8]))
+            dtau = 0.5 / (3.0 / b ** 2 + V.max())
+            steps = int(rng.integers(5, 25))
+            psi = even / math.sqrt(va.field_norm(dims, b, even))
+            i = B.add(instance_config(dims, h, g, dtau, steps, "imag", ratio, init="host", origin=origin,
+                                      zlayers=zl))
+            B.set_state(i, psi)
+            cases.append(("imag", i, dims, b, origin, psi, None, V, g, dtau, steps, ratio, zl))
+    announce(f"sweep batch ground states seed={seed}")
+    B.run(7, 4)
+    for mode, i, dims, b, origin, e, o, V, g, dt, steps, ratio, zl in cases:
+        assert B.info(i)["steps_done"] == steps
+        ge, go = B.get_state(i)
+        if mode == "real":
+            oe, oo_ = orc.leapfrog(e, o, V * dt, dims, *va.leapfrog_coefs(b, dt, g), steps)
+            assert np.array_equal(ge, oe) and np.array_equal(go, oo_), (seed, i)
+        else:
+            kernel, tile = IMAG_TILES[zl]
+            s = va.Solver(dims, b, dt, g, origin, "fp64", "imag", "trap", ratio, kernel, tile)
+            s.set_state(e)
+            s.step(steps)
+            se = s.get_state(odd=False)
+            s.close()
+            assert np.array_equal(ge, se), (seed, i, zl)
+    B.close()
+
+
 PRIMAL_SHAPES = _shapes(77, 8, max_cells=40000)
 
 
