@@ -93,6 +93,31 @@ def test_random_shapes_fused_every_tile_bit_exact(orc, case):
         assert np.array_equal(ge, oe) and np.array_equal(go, oo), (dims, tile)
 
 
+@pytest.mark.parametrize("case", range(0, len(SHAPES), 4))
+def test_random_shapes_fused_observables(orc, case):
+    """gpe_step_observe on the random shapes under every gather tile: N, K, P, I summed inside the
+    last step's first half-update equal the oracle's observables of its field after nsteps - 1
+    steps to 1e-12 relative (FP64 sums in another order); the steps stay bit-identical."""
+    dims = SHAPES[case]
+    h, g, dt, ratio = 0.45, 400.0, 8e-4, 1.7
+    b, origin, even, odd, V = _fields(dims, h, ratio, seed=600 + case)
+    oe, _ = orc.leapfrog(even, odd, V * dt, dims, *va.leapfrog_coefs(b, dt, g), 4)
+    want = orc.observables(oe, V, dims, b, g)
+    fe, fo = orc.leapfrog(even, odd, V * dt, dims, *va.leapfrog_coefs(b, dt, g), 5)
+    for kernel, tile in TILES:
+        if kernel != "gather":
+            continue
+        announce(f"sweep observe {dims} tile={tile}")
+        s = va.Solver(dims, b, dt, g, origin, "fp64", "real", "trap", ratio, kernel, tile)
+        s.set_state(even, odd)
+        got = s.step_observe(5)
+        ge, go = s.get_state()
+        s.close()
+        for k, r in zip("NKPI", want):
+            assert got[k] == pytest.approx(r, rel=1e-12, abs=1e-300), (dims, tile, k)
+        assert np.array_equal(ge, fe) and np.array_equal(go, fo), (dims, tile)
+
+
 def _rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
