@@ -145,6 +145,15 @@ gpe_status gpe_observables(gpe_solver *s, int field, double out[4]);
  * steps are bit-identical to gpe_step.  GPE_ENOTSUP in imaginary time. */
 gpe_status gpe_step_observe(gpe_solver *s, uint64_t nsteps, double out[4]);
 
+/* Distance of field field_a (0 even, 1 odd) of solver a to field field_b of
+ * solver b on the same cell grid, in any mix of precisions, reduced on the
+ * device in FP64 without copying either field to the host:
+ * out = {sum |a - b|^2, sum |b|^2, sum |a|^2} over the nodes, so the relative
+ * L2 error of a against b is sqrt(out[0] / out[1]).  No counterpart in the
+ * reference; it states the FP32-bulk error against an FP64 run at every
+ * checkpoint of a precision study.  Waits for b's stream, runs on a's. */
+gpe_status gpe_compare_fields(gpe_solver *a, int field_a, gpe_solver *b, int field_b, double out[3]);
+
 gpe_status gpe_synchronize(gpe_solver *s);
 gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info);
 
@@ -184,6 +193,17 @@ typedef struct gpe_vortex_state gpe_vortex_state;
 /* VortexState::load (koodit/GrossPitaevskii/VortexState.cpp:49-72). */
 gpe_status gpe_vortex_state_load(const char *path, gpe_vortex_state **out);
 void gpe_vortex_state_free(gpe_vortex_state *vs);
+/* A VortexState from memory, without a file: the fields of the reference's
+ * object (koodit/GrossPitaevskii/VortexState.hpp:16-76, in the order load
+ * reads them, VortexState.cpp:49-72) and its rsize x zsize table f(r, z),
+ * r-major (f[ir * zsize + iz]), copied.  A caller holding a reference
+ * VortexState passes its members here to call integrateInTime through
+ * gpe_integrate_in_time (GPUGrossPitaevskii_BCC.cu:218). */
+gpe_status gpe_vortex_state_from_table(uint32_t kappa, double g, uint32_t rsize, uint32_t zsize, double rplug,
+                                       double rstep, double zstep, double mu, const double *f,
+                                       gpe_vortex_state **out);
+/* Copies the rsize x zsize table (r-major) into f. */
+gpe_status gpe_vortex_state_table(const gpe_vortex_state *vs, double *f);
 /* {kappa, g, rsize, zsize, rplug, rstep, zstep, mu} */
 void gpe_vortex_state_params(const gpe_vortex_state *vs, double out[8]);
 /* VortexState::getPsi(Vector3) (VortexState.cpp:204-212). */

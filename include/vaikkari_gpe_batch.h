@@ -30,7 +30,10 @@ typedef enum gpe_init_kind {
     GPE_INIT_HOST = 0,    /* state supplied with gpe_batch_set_state */
     GPE_INIT_GAUSSIAN = 1 /* device recipe: (pi sigma^2)^(-3/4) exp(-r^2/(2 sigma^2))
                              * (1 + noise (xr + i xi)), xr, xi ~ N(0,1) from a
-                             * counter-based generator keyed by (seed, node);
+                             * counter-based generator keyed by (seed, node),
+                             * then scaled to unit discrete norm w sum |psi|^2
+                             * (FP64 reduction on the device: a box that clips
+                             * the Gaussian would otherwise start below 1);
                              * odd = even exp(i mu0 dt/2); mu0 = NaN: the
                              * state's own mu = (K + P + I)/N (FP64, on the
                              * device), which starts the leapfrog without

@@ -54,6 +54,9 @@ typedef struct gpe_completion {
     double arrival, admitted, finished; /* server clock, seconds */
     double node_steps;                  /* 12 n^3 * steps */
     double norm, energy;                /* FP64 observables of the final state */
+    int32_t status;                     /* GPE_OK, or why the request was dropped at admission
+                                           (then node_steps is 0 and norm, energy are NaN) */
+    uint32_t reserved;
 } gpe_completion;
 
 typedef struct gpe_server_stats {
@@ -62,6 +65,7 @@ typedef struct gpe_server_stats {
     double node_steps_done;  /* over all ticks */
     double device_ms;        /* device time of all chunks */
     double ticks;
+    uint64_t failed;         /* requests dropped at admission (a failed completion each) */
 } gpe_server_stats;
 
 gpe_status gpe_server_create(uint32_t precision, uint64_t hbm_budget_bytes, uint32_t chunk_steps,
@@ -69,9 +73,16 @@ gpe_status gpe_server_create(uint32_t precision, uint64_t hbm_budget_bytes, uint
 void gpe_server_destroy(gpe_server *s);
 double gpe_server_now(const gpe_server *s);
 
-/* Queues a request that arrived at `arrival` (server clock). */
+/* Queues a request that arrived at `arrival` (server clock).  GPE_EINVAL for a
+ * malformed request or one larger than the whole HBM budget, GPE_EUNSTABLE
+ * when dt*(6/b^2 + max V) >= 2 on its box. */
 gpe_status gpe_server_submit(gpe_server *s, const gpe_request *req, double arrival, uint64_t *ticket);
-/* Admit, run one chunk, retire.  *completed = solves finished in this tick. */
+/* Admit, run one chunk, retire.  *completed = solves finished in this tick,
+ * failed admissions included.  A request whose admission fails for a reason of
+ * its own (GPE_EINVAL, GPE_EUNSTABLE, or GPE_ENOMEM with nothing else resident)
+ * is dropped from the queue with a completion whose status says why; the tick
+ * goes on with the others.  A device error ends the tick with that status after
+ * the queue has been brought up to date. */
 gpe_status gpe_server_tick(gpe_server *s, uint32_t *completed);
 /* Completions since the last call (at most max). */
 gpe_status gpe_server_poll(gpe_server *s, gpe_completion *out, uint32_t max, uint32_t *n);
@@ -87,6 +98,10 @@ typedef enum gpe_serve_policy {
 gpe_status gpe_server_set_policy(gpe_server *s, uint32_t policy, uint64_t max_active_nodes);
 /* Arrival time of the oldest queued request (server clock), or -1 when the queue is empty. */
 double gpe_server_oldest_queued(const gpe_server *s);
+/* How the shared chunks are launched: GPE_BATCH_GRAPH (default, one captured
+ * hipGraph per chunk) or GPE_BATCH_STREAM (the same kernels enqueued one by
+ * one), as gpe_batch_set_launch_mode.  Results are identical either way. */
+gpe_status gpe_server_set_launch_mode(gpe_server *s, uint32_t mode);
 
 #ifdef __cplusplus
 }

@@ -67,6 +67,20 @@ int main(int argc, char **argv) {
     gpe_vortex_state_psi(back, 0.7, -0.3, 0.4, s1);
     check(s0[0] == s1[0] && s0[1] == s1[1], "psi survives save/load");
     gpe_vortex_state_free(back);
+
+    // the same state handed over in memory (no file): identical parameters and psi
+    double ftab[40 * 30];
+    check(gpe_vortex_state_table(vs, ftab) == GPE_OK, "table");
+    gpe_vortex_state *mem = nullptr;
+    check(gpe_vortex_state_from_table((uint32_t)p0[0], p0[1], (uint32_t)p0[2], (uint32_t)p0[3], p0[4], p0[5], p0[6],
+                                      p0[7], ftab, &mem) == GPE_OK,
+          "from_table");
+    double p2[8], s2[2];
+    gpe_vortex_state_params(mem, p2);
+    gpe_vortex_state_psi(mem, 0.7, -0.3, 0.4, s2);
+    check(std::memcmp(p0, p2, sizeof p0) == 0, "parameters survive the in-memory handoff");
+    check(s0[0] == s2[0] && s0[1] == s2[1], "psi survives the in-memory handoff");
+    gpe_vortex_state_free(mem);
     gpe_vortex_state_free(vs);
 
     std::printf("dims %u %u %u\n", dims[0], dims[1], dims[2]);

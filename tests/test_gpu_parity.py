@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import vaikkari_amd as va
+from vaikkari_amd._lib import EINVAL, GpeError
 from oracle_lib import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -89,11 +90,33 @@ def test_fused_every_tile(orc, precision, tile):
     after 30 steps."""
     dims = (67, 13, 37)
     shape = [v for v in va.variants() if v["kernel"] == "fused" and v["tile"] == tile][0]
-    if precision == "fp64" and shape["waves"] == 16:
-        pytest.skip("16-row FP64 fused tile exceeds the LDS budget (rejected by the library)")
     announce(f"fused tile {tile} {shape} {precision}")
     h, g, dt, ratio = 0.4, 500.0, 1e-3, 1.2
     b, origin, even, odd, V = random_case(dims, h, ratio, seed=21)
+    if precision == "fp64" and shape["waves"] == 16:
+        # 16 rows of FP64 exchange slots exceed the LDS: refused at creation and by
+        # set_variant, and a refused set_variant leaves the state and the variant as they were
+        with pytest.raises(GpeError) as ei:
+            va.Solver(dims, b, dt, g, origin, "fp64", "real", "trap", ratio, "fused", tile)
+        assert ei.value.status == EINVAL
+        s = va.Solver(dims, b, dt, g, origin, "fp64", "real", "trap", ratio, "fused", 1)
+        s.set_state(even, odd)
+        s.step(3)
+        before = s.get_state()
+        with pytest.raises(GpeError) as ei:
+            s.set_variant("fused", tile)
+        assert ei.value.status == EINVAL
+        after = s.get_state()
+        np.testing.assert_array_equal(after[0], before[0])
+        np.testing.assert_array_equal(after[1], before[1])
+        s.step(27)
+        ge, go = s.get_state()
+        lf, lf0, gdt = va.leapfrog_coefs(b, dt, g)
+        oe, oo = orc.leapfrog(even, odd, V * dt, dims, lf, lf0, gdt, 30)
+        np.testing.assert_array_equal(ge, oe)
+        np.testing.assert_array_equal(go, oo)
+        s.close()
+        return
     s = va.Solver(dims, b, dt, g, origin, precision, "real", "trap", ratio, "fused", tile)
     s.set_state(even, odd)
     s.step(30)

@@ -1,10 +1,17 @@
 """GPU tests of the continuous-batching server: every request completes, a request's result
-is bit-identical whether it is served alone or among others arriving while it runs, and
-admission respects the HBM budget."""
+is bit-identical whether it is served alone or among others arriving while it runs, equals the
+single solver and the oracle on the same initial state, and admission respects the HBM budget
+and drops a request that cannot run exactly once."""
+import math
+
 import numpy as np
 import pytest
 
-from vaikkari_amd.serve import Request, Server, poisson_workload, run_serving
+import vaikkari_amd as va
+from vaikkari_amd._lib import ENOMEM, EUNSTABLE, GpeError, OK
+from vaikkari_amd.batch import Batch, instance_config
+from vaikkari_amd.serve import Request, Server, poisson_workload, run_serving, stable_dt
+from oracle_lib import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -107,4 +114,106 @@ def test_policy_order_and_node_cap(policy):
         assert done[ta]["finished"] <= min(done[tb]["admitted"], done[tc]["admitted"])
     with pytest.raises(Exception):
         srv.set_policy("lottery")
+    srv.close()
+
+
+def _initial_state(r: Request, precision: str):
+    """The initial even/odd fields the server gives request r: the batch's device recipe on
+    the same instance parameters (Server.submit: sigma 1.5, noise 1e-3, the state's own mu)."""
+    dt = stable_dt(r.n, r.h, r.g, r.ratio, mode=r.mode)
+    B = Batch(precision)
+    i = B.add(instance_config((r.n,) * 3, r.h, r.g, dt, r.steps, r.mode, r.ratio, sigma=1.5, noise=1e-3,
+                              seed=r.seed))
+    e, o = B.get_state(i)
+    B.close()
+    return dt, e, o
+
+
+@pytest.mark.parametrize("launch", ["graph", "stream"])
+def test_served_results_match_solver_and_oracle(launch):
+    """A real-time and an imaginary-time request of the serving form, served FP64 among requests
+    of other sizes that arrive while they run: the real-time result is bit-identical to the single
+    solver and to the oracle's leapfrog (GPUGrossPitaevskii_BCC.cu:205-215,475-481) from the same
+    initial state, the ground state bit-identical to the single solver and within 1e-12 of the
+    oracle's imaginary-time steps."""
+    announce(f"served vs solver + oracle ({launch})")
+    orc = Oracle()
+    real = Request(0.0, 22, 260, 1300.0, 2.2, "real", 424242)
+    imag = Request(0.0, 19, 150, 700.0, 0.8, "imag", 515151)
+    others = poisson_workload(seed=8, rate=400.0, duration=0.04, n_min=10, n_max=30, steps_min=30, steps_max=200)
+    srv = Server("fp64", 0, chunk_steps=23, policy="fifo", launch=launch)
+    tickets = {}
+    pending = [real] + others[: len(others) // 2] + [imag] + others[len(others) // 2:]
+    for q, r in enumerate(pending):
+        tickets[q] = srv.submit(r, arrival=srv.now(), keep_result=r in (real, imag))
+        if q % 2 == 0:
+            srv.tick()  # the next requests arrive while these run
+    while srv.stats()["completed"] < len(pending):
+        srv.tick()
+    done = srv.poll()
+    assert sorted(d["ticket"] for d in done) == sorted(tickets.values())
+    assert all(d["status"] == OK for d in done)
+    served_real = srv.fetch(tickets[pending.index(real)], real.n, "real")
+    served_imag = srv.fetch(tickets[pending.index(imag)], imag.n, "imag")
+    srv.close()
+
+    for r, (ge, go) in ((real, served_real), (imag, served_imag)):
+        dims = (r.n,) * 3
+        b = r.h / math.sqrt(8)
+        origin = va.centered_origin(dims, b)
+        V = va.trap_potential(dims, origin, b, r.ratio)
+        dt, e0, o0 = _initial_state(r, "fp64")
+        assert va.field_norm(dims, b, e0) == pytest.approx(1.0, abs=1e-13)  # unit norm on the grid
+        lf, lf0, gdt = va.leapfrog_coefs(b, dt, r.g)
+        s = va.Solver(dims, b, dt, r.g, origin, "fp64", r.mode, "trap", r.ratio, "gather")
+        if r.mode == "real":
+            s.set_state(e0, o0)
+            s.step(r.steps)
+            se, so = s.get_state()
+            np.testing.assert_array_equal(ge, se)
+            np.testing.assert_array_equal(go, so)
+            oe, oo = orc.leapfrog(e0, o0, V * dt, dims, lf, lf0, gdt, r.steps)
+            np.testing.assert_array_equal(ge, oe)
+            np.testing.assert_array_equal(go, oo)
+        else:
+            s.set_state(e0)
+            s.step(r.steps)
+            se = s.get_state(odd=False)
+            np.testing.assert_array_equal(ge, se)
+            oe, _ = orc.imag_steps(e0, V * dt, dims, lf, lf0, gdt, b, r.steps)
+            assert np.linalg.norm(ge - oe) / np.linalg.norm(oe) < 1e-12
+        s.close()
+
+
+def test_unstable_and_failed_requests():
+    """An unstable request is refused at submission (GPE_EUNSTABLE).  A request that passes
+    submission but cannot be allocated while nothing else is resident is dropped once, with a
+    failed completion, and the requests around it are admitted once each and complete."""
+    announce("serve failed admission")
+    srv = Server("fp32", 0, chunk_steps=10)
+    bad = Request(0.0, 20, 30, 500.0, 1.0, "real", 9)
+    with pytest.raises(GpeError) as ei:
+        srv.submit(bad, arrival=0.0, dt=0.05)
+    assert ei.value.status == EUNSTABLE
+    a = Request(0.0, 16, 40, 500.0, 1.0, "real", 1)
+    huge = Request(0.0, 2048, 10, 500.0, 1.0, "real", 2)  # ~1.7 TB of fields: the allocation fails
+    c = Request(0.0, 18, 40, 500.0, 1.0, "imag", 3)
+    ta = srv.submit(a, arrival=0.0)
+    srv.tick()  # a is resident: huge must wait, not fail
+    th = srv.submit(huge, arrival=0.0)
+    tc = srv.submit(c, arrival=0.0)
+    seen = []
+    for _ in range(200):
+        srv.tick()
+        seen.extend(srv.poll())
+        st = srv.stats()
+        if st["active"] == 0 and st["queued"] == 0:
+            break
+    tickets = [d["ticket"] for d in seen]
+    assert sorted(tickets) == sorted([ta, th, tc])  # each exactly once
+    by = {d["ticket"]: d for d in seen}
+    assert by[th]["status"] == ENOMEM and math.isnan(by[th]["norm"]) and by[th]["node_steps"] == 0
+    assert by[ta]["status"] == OK and by[tc]["status"] == OK
+    st = srv.stats()
+    assert st["completed"] == 2 and st["failed"] == 1 and st["admitted"] == 2 and st["resident_bytes"] == 0
     srv.close()

@@ -158,3 +158,30 @@ def test_recipes():
     odd = va.phase_shift(dims, g, 2.0, 1e-3)
     np.testing.assert_allclose(odd.view(np.complex128), g.view(np.complex128) * np.exp(1j * 2.0 * 1e-3 / 2),
                                rtol=1e-15, atol=1e-18)
+
+
+def test_vortex_state_from_table_equals_file(tmp_path):
+    """gpe_vortex_state_from_table (the in-memory handoff of a VortexState) gives the same state as
+    saving those members in the reference's file format and loading it: same parameters, table,
+    psi, curvature, and the same integrateInTime setup bit for bit."""
+    rng = np.random.default_rng(3)
+    f = np.abs(rng.normal(size=(40, 55))) * 0.1
+    vs = va.VortexState.from_table(3, 420.0, 0.2, 0.07, 0.09, 6.25, f)
+    np.testing.assert_array_equal(vs.table(), f)
+    path = str(tmp_path / "t.dat")
+    assert vs.save(path)
+    back = va.VortexState()
+    assert back.load(path)
+    np.testing.assert_array_equal(vs._params(), back._params())
+    np.testing.assert_array_equal(back.table(), f)
+    for p in ((0.5, -0.3, 0.2), (1.1, 0.4, -1.9), (0.0, 0.0, 0.3)):
+        assert vs.getPsi(p) == back.getPsi(p)
+    assert vs.integrateCurvature() == back.integrateCurvature()
+    ext = (-2.5, -2.5, -4.0), (2.5, 2.5, 4.0)
+    a = va.reference_setup(vs, 0.3, *ext, 0.05, 0.7)
+    b = va.reference_setup(back, 0.3, *ext, 0.05, 0.7)
+    assert a[0] == b[0]
+    for x, y in zip(a[1:], b[1:]):
+        np.testing.assert_array_equal(x, y)
+    with pytest.raises(_lib.GpeError):
+        va.VortexState.from_table(1, 1.0, 0.0, -0.1, 0.1, 1.0, f)  # non-positive step

@@ -58,3 +58,28 @@ def test_instance_bytes_accounting():
     xp = ((16 + 32 + 1 + 15) // 16) * 16
     per_field = 34 * 34 * 12 * xp * 8
     assert per_field > 12 * 32 ** 3 * 8
+
+
+def test_latency_report_counts_unfinished_requests():
+    """Completed-only percentiles against percentiles over every submitted request, with the
+    unfinished ones at their age at the end (right-censored), a dropped request in neither,
+    and per-size-class offered against served load."""
+    from vaikkari_amd.serve import Request, latency_report
+
+    reqs = [Request(float(q), n, 100, 500.0, 1.0, "real", q) for q, n in enumerate((16, 20, 40, 150, 18))]
+    arrivals = {q + 1: (r, r.arrival) for q, r in enumerate(reqs)}
+    done = [{"ticket": 1, "finished": 2.0, "node_steps": reqs[0].node_steps(), "status": 0},
+            {"ticket": 3, "finished": 9.0, "node_steps": reqs[2].node_steps(), "status": 0},
+            {"ticket": 5, "finished": 4.0, "node_steps": 0.0, "status": _lib.ENOMEM}]
+    rep = latency_report(reqs, arrivals, done, end=20.0, duration=10.0)
+    # completed: ticket 1 took 2 s, ticket 3 took 7 s
+    assert rep["p50_s"] == pytest.approx(4.5) and rep["failed"] == 1 and rep["censored"] == 2
+    # all: 2, 7 and the unfinished tickets 2 (age 19) and 4 (age 17)
+    assert rep["p50_all_s"] == pytest.approx(np.percentile([2, 7, 19, 17], 50))
+    assert rep["p99_all_s"] > rep["p99_s"]
+    cls = {tuple(c["n"]): c for c in rep["size_classes"]}
+    assert cls[(16, 31)]["submitted"] == 2 and cls[(16, 31)]["completed"] == 1  # ticket 5 was dropped
+    assert cls[(32, 63)]["completed_site_steps_per_s"] == pytest.approx(reqs[2].node_steps() / 10.0)
+    assert cls[(128, 160)]["p50_all_s"] == pytest.approx(17.0) and cls[(128, 160)]["p50_s"] is None
+    assert cls[(16, 31)]["offered_site_steps_per_s"] == pytest.approx(
+        (reqs[0].node_steps() + reqs[1].node_steps() + reqs[4].node_steps()) / 10.0)

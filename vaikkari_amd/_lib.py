@@ -116,6 +116,7 @@ class GpeCompletion(C.Structure):
         ("ticket", C.c_uint64),
         ("arrival", C.c_double), ("admitted", C.c_double), ("finished", C.c_double),
         ("node_steps", C.c_double), ("norm", C.c_double), ("energy", C.c_double),
+        ("status", C.c_int32), ("reserved", C.c_uint32),
     ]
 
 
@@ -124,6 +125,7 @@ class GpeServerStats(C.Structure):
         ("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
         ("queued", C.c_uint64), ("active", C.c_uint64), ("resident_bytes", C.c_uint64),
         ("node_steps_done", C.c_double), ("device_ms", C.c_double), ("ticks", C.c_double),
+        ("failed", C.c_uint64),
     ]
 
 
@@ -194,6 +196,7 @@ SIGNATURES = [
     ("gpe_time_steps", _I, [_P, _U64, _DP, C.POINTER(_U64)]),
     ("gpe_observables", _I, [_P, _I, _DP]),
     ("gpe_step_observe", _I, [_P, _U64, _DP]),
+    ("gpe_compare_fields", _I, [_P, _I, _P, _I, _DP]),
     ("gpe_synchronize", _I, [_P]),
     ("gpe_get_info", _I, [_P, C.POINTER(GpeSolverInfo)]),
     ("gpe_set_variant", _I, [_P, _U32, _U32]),
@@ -228,6 +231,7 @@ SIGNATURES = [
     ("gpe_server_get_stats", _I, [_P, C.POINTER(GpeServerStats)]),
     ("gpe_server_set_policy", _I, [_P, _U32, _U64]),
     ("gpe_server_oldest_queued", _D, [_P]),
+    ("gpe_server_set_launch_mode", _I, [_P, _U32]),
     ("gpe_primal_create", _I, [C.POINTER(GpePrimalConfig), C.POINTER(_P)]),
     ("gpe_primal_destroy", None, [_P]),
     ("gpe_primal_set_state", _I, [_P, _DP, _DP]),
@@ -250,6 +254,8 @@ SIGNATURES = [
     ("gpe_bcc_stencil_derived", _I, [_I32P]),
     ("gpe_vortex_state_load", _I, [C.c_char_p, C.POINTER(_P)]),
     ("gpe_vortex_state_free", None, [_P]),
+    ("gpe_vortex_state_from_table", _I, [_U32, _D, _U32, _U32, _D, _D, _D, _D, _DP, C.POINTER(_P)]),
+    ("gpe_vortex_state_table", _I, [_P, _DP]),
     ("gpe_vortex_state_params", None, [_P, _DP]),
     ("gpe_vortex_state_psi", None, [_P, _D, _D, _D, _DP]),
     ("gpe_vortex_state_phase", None, [_P, _D, _DP]),

@@ -617,19 +617,46 @@ BInst *get_inst(gpe_batch *B, int32_t id) {
     return &B->inst[id];
 }
 
-// odd = even exp(i mu dt/2) with mu = (K + P + I)/N of the instance's even
-// field (FP64 observables), the phase of the reference's start of the
-// leapfrog (GPUGrossPitaevskii_BCC.cu:325,342-343) with the state's own mu.
-gpe_status auto_odd(gpe_batch *B, int32_t id) {
+// Scales the even field of a device-initialised instance so that its
+// discrete norm w * sum |psi|^2 (FP64, on the device) is 1: the continuum
+// amplitude (pi sigma^2)^(-3/4) of the recipe gives less than 1 on a box that
+// clips the Gaussian.
+gpe_status normalize_even(gpe_batch *B, int32_t id) {
     double obs[4];
     gpe_status st = gpe_batch_observables(B, id, obs);
     if (st != GPE_OK) return st;
     BInst &b = B->inst[id];
-    const double mu = (obs[1] + obs[2] + obs[3]) / obs[0];
-    if (!std::isfinite(mu)) {
-        set_error("gpe_batch_add: initial state has no finite chemical potential");
+    if (!(obs[0] > 0) || !std::isfinite(obs[0])) {
+        set_error("gpe_batch_add: initial state has no positive finite norm");
         return GPE_EUNSTABLE;
     }
+    const double s = 1.0 / std::sqrt(obs[0]);
+    if (B->precision == GPE_FP64)
+        hipLaunchKernelGGL((k_scale_const<double>), dim3(1024), dim3(256), 0, B->stream, (double2 *)b.buf[0],
+                           b.L.elems, s);
+    else
+        hipLaunchKernelGGL((k_scale_const<float>), dim3(1024), dim3(256), 0, B->stream, (float2 *)b.buf[0],
+                           b.L.elems, s);
+    GPE_HIP_TRY(hipGetLastError());
+    GPE_HIP_TRY(hipStreamSynchronize(B->stream));
+    return GPE_OK;
+}
+
+// odd = even exp(i mu dt/2), the phase of the reference's start of the
+// leapfrog (GPUGrossPitaevskii_BCC.cu:325,342-343).  mu = NaN: the state's own
+// mu = (K + P + I)/N of the instance's even field (FP64 observables).
+gpe_status start_odd(gpe_batch *B, int32_t id, double mu) {
+    if (std::isnan(mu)) {
+        double obs[4];
+        gpe_status st = gpe_batch_observables(B, id, obs);
+        if (st != GPE_OK) return st;
+        mu = (obs[1] + obs[2] + obs[3]) / obs[0];
+        if (!std::isfinite(mu)) {
+            set_error("gpe_batch_add: initial state has no finite chemical potential");
+            return GPE_EUNSTABLE;
+        }
+    }
+    BInst &b = B->inst[id];
     const double ph = 0.5 * mu * b.cfg.dt;
     if (B->precision == GPE_FP64)
         hipLaunchKernelGGL((k_phase_copy<double>), dim3(1024), dim3(256), 0, B->stream, (double2 *)b.buf[1],
@@ -642,15 +669,14 @@ gpe_status auto_odd(gpe_batch *B, int32_t id) {
     return GPE_OK;
 }
 
+// The even field of the Gaussian recipe; normalize_even and start_odd follow.
 template <typename T>
 gpe_status init_gaussian(gpe_batch *B, BInst &b) {
     const double s = b.cfg.sigma > 0 ? b.cfg.sigma : 1.0;
     const double amp = std::pow(3.14159265358979323846 * s * s, -0.75);
-    const double ph = std::isnan(b.cfg.mu0) ? 0.0 : 0.5 * b.cfg.mu0 * b.cfg.dt;  // NaN: set after init
-    cplx_t<T> *odd = b.cfg.mode == GPE_REAL_TIME ? (cplx_t<T> *)b.buf[1] : nullptr;
-    hipLaunchKernelGGL((k_init_gaussian<T>), dim3(1024), dim3(256), 0, B->stream, (cplx_t<T> *)b.buf[0], odd, b.L,
-                       b.trap, 1.0 / (2.0 * s * s), amp, b.cfg.noise, (unsigned long long)b.cfg.seed, std::cos(ph),
-                       std::sin(ph));
+    hipLaunchKernelGGL((k_init_gaussian<T>), dim3(1024), dim3(256), 0, B->stream, (cplx_t<T> *)b.buf[0],
+                       (cplx_t<T> *)nullptr, b.L, b.trap, 1.0 / (2.0 * s * s), amp, b.cfg.noise,
+                       (unsigned long long)b.cfg.seed, 1.0, 0.0);
     GPE_HIP_TRY(hipGetLastError());
     return GPE_OK;
 }
@@ -728,6 +754,8 @@ gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id
     const int nfields = 2;
     const size_t fbytes = (size_t)b.L.elems * B->elem();
     auto fail = [&](gpe_status st) {
+        (void)hipStreamSynchronize(B->stream);  // nothing enqueued above may still use these buffers
+        (void)hipGetLastError();  // a failed allocation must not surface at a later launch check
         for (auto *&p : b.buf)
             if (p) (void)hipFree(p);
         if (b.partials) (void)hipFree(b.partials);
@@ -742,14 +770,15 @@ gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id
         }
         if (hipMemsetAsync(b.buf[f], 0, fbytes, B->stream) != hipSuccess) return fail(GPE_EDEVICE);
     }
+    // scal's initial value: lives on this frame until the stream is synchronised below
+    const double scal_init[8] = {1.0, 1.0, 0, 0, 0, 0, 0, 0};
     if (c->mode == GPE_IMAG_TIME) {
         if (hipMalloc(&b.partials, (size_t)b.ntiles * sizeof(double)) != hipSuccess ||
             hipMalloc(&b.scal, 8 * sizeof(double)) != hipSuccess) {
             set_error("gpe_batch_add: out of device memory");
             return fail(GPE_ENOMEM);
         }
-        const double init[8] = {1.0, 1.0, 0, 0, 0, 0, 0, 0};
-        if (hipMemcpyAsync(b.scal, init, sizeof(init), hipMemcpyHostToDevice, B->stream) != hipSuccess)
+        if (hipMemcpyAsync(b.scal, scal_init, sizeof(scal_init), hipMemcpyHostToDevice, B->stream) != hipSuccess)
             return fail(GPE_EDEVICE);
     }
     b.steps_left = c->steps;
@@ -778,9 +807,11 @@ gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id
     B->resident += b.bytes;
     B->desc_dirty = true;
     *id = slot;
-    if (c->init == GPE_INIT_GAUSSIAN && c->mode == GPE_REAL_TIME && std::isnan(c->mu0)) {
-        // start the leapfrog with the state's own chemical potential
-        const gpe_status st = auto_odd(B, slot);
+    if (c->init == GPE_INIT_GAUSSIAN) {
+        // unit norm on the grid, then the odd level (real time; mu0 = NaN: the
+        // state's own chemical potential)
+        gpe_status st = normalize_even(B, slot);
+        if (st == GPE_OK && c->mode == GPE_REAL_TIME) st = start_odd(B, slot, c->mu0);
         if (st != GPE_OK) {
             (void)gpe_batch_remove(B, slot);
             *id = -1;

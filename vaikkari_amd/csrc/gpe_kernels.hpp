@@ -518,6 +518,40 @@ __global__ void __launch_bounds__(64 * TY) k_observables(const cplx_t<T> *__rest
     }
 }
 
+// Distance of two fields on the same cell grid, each in its own layout and
+// precision: per block {sum |a - b|^2, sum |b|^2, sum |a|^2} over its nodes,
+// in FP64, for k_finalize<3>.
+template <typename TA, typename TB, int TY>
+__global__ void __launch_bounds__(64 * TY) k_field_distance(const cplx_t<TA> *__restrict__ a, Layout La,
+                                                            const cplx_t<TB> *__restrict__ b, Layout Lb,
+                                                            double *__restrict__ partials) {
+    __shared__ double lds[3 * TY];
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (i < La.nx && j < La.ny) {
+        const long long ba = La.origin + (long long)k * La.sz + (long long)j * La.sy + i;
+        const long long bb = Lb.origin + (long long)k * Lb.sz + (long long)j * Lb.sy + i;
+#pragma unroll
+        for (int l = 0; l < 12; l++) {
+            const cplx_t<TA> u = a[ba + l * La.xp];
+            const cplx_t<TB> v = b[bb + l * Lb.xp];
+            const double ur = (double)u.x, ui = (double)u.y, vr = (double)v.x, vi = (double)v.y;
+            const double dr = ur - vr, di = ui - vi;
+            acc[0] += dr * dr + di * di;
+            acc[1] += vr * vr + vi * vi;
+            acc[2] += ur * ur + ui * ui;
+        }
+    }
+    block_sum<3, TY>(acc, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const long long bl = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+#pragma unroll
+        for (int q = 0; q < 3; q++) partials[3 * bl + q] = acc[q];
+    }
+}
+
 // One block of 256 threads reduces NV interleaved partial streams in a
 // fixed order (Neumaier inside each thread, then a fixed tree), multiplies by
 // w and writes out[q].  For the imaginary-time norm (NV == 1, renorm != 0)
@@ -580,6 +614,18 @@ __global__ void k_scale_field(cplx_t<T> *__restrict__ a, long long n, const doub
         v.x = s * v.x;
         v.y = s * v.y;
         a[e] = v;
+    }
+}
+
+// f *= s, elementwise over the whole padded field (the zero halo stays zero).
+template <typename T>
+__global__ void k_scale_const(cplx_t<T> *__restrict__ f, long long n, double s) {
+#pragma clang fp contract(off)
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        cplx_t<T> v = f[e];
+        v.x = (T)(s * (double)v.x);
+        v.y = (T)(s * (double)v.y);
+        f[e] = v;
     }
 }
 

@@ -936,6 +936,7 @@ gpe_status gpe_primal::init() {
     const int nfields = kPrimalTiles[tile].fused ? 4 : 2;
     for (int f = 0; f < nfields; f++) {
         if (hipMalloc(&field[f], fbytes) != hipSuccess) {
+            (void)hipGetLastError();  // a failed allocation must not surface at a later launch check
             field[f] = nullptr;
             set_error("gpe_primal_create: cannot allocate %zu bytes of device memory", fbytes);
             return GPE_ENOMEM;
@@ -1006,6 +1007,7 @@ gpe_status gpe_primal::enqueue(uint64_t nsteps, uint64_t *launches) {
                 if (f != ie && f != io) spare[n++] = f;
             e = launch_primal_fused<T, FORM>(tile, (C *)field[spare[0]], (C *)field[spare[1]], (const C *)field[ie],
                                              (const C *)field[io], L, H, trap, dt, stream);
+            if (e != hipSuccess) break;  // nothing was written: the state stays where it is
             ie = spare[0];
             io = spare[1];
             nl += 1;

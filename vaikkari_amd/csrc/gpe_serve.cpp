@@ -41,6 +41,7 @@ struct gpe_server {
     uint32_t policy = GPE_SERVE_FIFO;
     uint64_t max_active_nodes = 0;  // 0: no cap
     uint64_t active_nodes = 0;
+    uint64_t budget = 0;  // HBM budget of the batch (0: none)
     std::chrono::steady_clock::time_point t0;
     uint64_t next_ticket = 1;
     std::deque<Pending> queue;
@@ -86,6 +87,7 @@ gpe_status gpe_server_create(uint32_t precision, uint64_t budget, uint32_t chunk
         return st;
     }
     s->chunk = chunk_steps;
+    s->budget = budget;
     s->t0 = std::chrono::steady_clock::now();
     *out = s;
     return GPE_OK;
@@ -105,9 +107,28 @@ gpe_status gpe_server_submit(gpe_server *s, const gpe_request *r, double arrival
         return GPE_EINVAL;
     }
     const gpe_instance_config c = to_instance(*r);
-    if (gpe_batch_instance_bytes(s->batch, &c) == 0) {
+    const uint64_t bytes = gpe_batch_instance_bytes(s->batch, &c);
+    if (bytes == 0) {
         gpe::set_error("gpe_server_submit: invalid request");
         return GPE_EINVAL;
+    }
+    if (s->budget && bytes > s->budget) {
+        // could never be admitted: it would wait in the queue for ever
+        gpe::set_error("gpe_server_submit: request needs %llu bytes, more than the whole budget of %llu",
+                       (unsigned long long)bytes, (unsigned long long)s->budget);
+        return GPE_EINVAL;
+    }
+    {
+        // the batch's own admission test, made here so that an unstable
+        // request never enters the queue
+        const uint32_t dims[3] = {r->n, r->n, r->n};
+        const double a = std::sqrt(8.0) * c.block_scale;
+        const double origin[3] = {-0.5 * a * r->n, -0.5 * a * r->n, -0.5 * a * r->n};
+        const double vmax = gpe::trap_vmax(dims, origin, c.block_scale, r->trap_ratio);
+        if (!gpe::linear_step_stable(r->dt, c.block_scale, vmax)) {
+            gpe::set_error("gpe_server_submit: dt = %g violates dt*(6/b^2 + max V) < 2 (max V = %g)", r->dt, vmax);
+            return GPE_EUNSTABLE;
+        }
     }
     *ticket = s->next_ticket++;
     s->queue.push_back({*ticket, *r, arrival});
@@ -129,6 +150,8 @@ gpe_status gpe_server_tick(gpe_server *s, uint32_t *completed) {
             return request_work(s->queue[a].req) < request_work(s->queue[b].req);
         });
     std::vector<char> taken(s->queue.size(), 0);
+    uint32_t ndone = 0;
+    gpe_status device_error = GPE_OK;
     for (size_t q : order) {
         const Pending &p = s->queue[q];
         const uint64_t nodes = request_nodes(p.req);
@@ -137,7 +160,29 @@ gpe_status gpe_server_tick(gpe_server *s, uint32_t *completed) {
         int32_t id = -1;
         const gpe_status st = gpe_batch_add(s->batch, &c, &id);
         if (st == GPE_EBUSY) continue;
-        if (st != GPE_OK) return st;
+        // out of device memory while other solves hold it: wait for them
+        if (st == GPE_ENOMEM && !s->running.empty()) continue;
+        if (st != GPE_OK) {
+            // the request cannot run: drop it with a failed completion
+            const double now = gpe_server_now(s);
+            gpe_completion cpl{};
+            cpl.ticket = p.ticket;
+            cpl.arrival = p.arrival;
+            cpl.admitted = now;
+            cpl.finished = now;
+            cpl.node_steps = 0.0;
+            cpl.norm = cpl.energy = NAN;
+            cpl.status = st;
+            s->done.push_back(cpl);
+            s->st.failed++;
+            taken[q] = 1;
+            ndone++;
+            if (st == GPE_EDEVICE) {
+                device_error = st;
+                break;
+            }
+            continue;
+        }
         s->running.push_back({p.ticket, p.req, p.arrival, gpe_server_now(s), id});
         s->active_nodes += nodes;
         s->st.admitted++;
@@ -149,7 +194,10 @@ gpe_status gpe_server_tick(gpe_server *s, uint32_t *completed) {
             if (!taken[q]) rest.push_back(s->queue[q]);
         s->queue.swap(rest);
     }
-    uint32_t ndone = 0;
+    if (device_error != GPE_OK) {
+        if (completed) *completed = ndone;
+        return device_error;
+    }
     if (!s->running.empty()) {
         const uint64_t ns = gpe_batch_chunk_node_steps(s->batch, s->chunk);
         double ms = 0.0;
@@ -179,6 +227,8 @@ gpe_status gpe_server_tick(gpe_server *s, uint32_t *completed) {
             cpl.node_steps = 12.0 * r.req.n * r.req.n * (double)r.req.n * (double)r.req.steps;
             cpl.norm = obs[0];
             cpl.energy = obs[1] + obs[2] + 0.5 * obs[3];
+            cpl.status = GPE_OK;
+            cpl.reserved = 0;
             s->done.push_back(cpl);
             s->active_nodes -= request_nodes(r.req);
             if (r.req.keep_result) {
@@ -235,6 +285,14 @@ double gpe_server_oldest_queued(const gpe_server *s) {
     double t = s->queue.front().arrival;
     for (const Pending &p : s->queue) t = std::min(t, p.arrival);
     return t;
+}
+
+gpe_status gpe_server_set_launch_mode(gpe_server *s, uint32_t mode) {
+    if (!s) {
+        gpe::set_error("gpe_server_set_launch_mode: null server");
+        return GPE_EINVAL;
+    }
+    return gpe_batch_set_launch_mode(s->batch, mode);
 }
 
 gpe_status gpe_server_get_stats(const gpe_server *s, gpe_server_stats *out) {

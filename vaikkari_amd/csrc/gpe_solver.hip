@@ -93,6 +93,12 @@ TileShape tile_shape(int kernel, int tile) {
                                         : kGatherTiles[tile];
 }
 
+// A fused tile of 16 rows holds its FP64 exchange slots in more than the
+// 160 KiB of LDS: that (family, tile, precision) is refused up front.
+bool tile_supported(int kernel, int tile, uint32_t precision) {
+    return !(kernel == GPE_KERNEL_FUSED && precision == GPE_FP64 && tile_shape(kernel, tile).ty == 16);
+}
+
 FusedGrid make_fused_grid(const Layout &L, const TileShape &t) {
     FusedGrid G;
     G.ntx = (L.nx + 61) / 62;
@@ -285,6 +291,10 @@ gpe_status gpe_solver::init(const gpe_config &c) {
         return GPE_EINVAL;
     }
     tile = c.tile == 0 ? default_tile(kernel, c.precision) : (int)c.tile - 1;
+    if (!tile_supported(kernel, tile, c.precision)) {
+        set_error("gpe_create: fused tile %u does not fit LDS in FP64", c.tile);
+        return GPE_EINVAL;
+    }
     GPE_HIP_TRY(hipGetDevice(&device));
     GPE_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     GPE_HIP_TRY(hipEventCreate(&ev0));
@@ -313,6 +323,7 @@ gpe_status gpe_solver::alloc_field(int f) {
     if (field[f]) return GPE_OK;
     const size_t fbytes = (size_t)L.elems * elem_bytes();
     if (hipMalloc(&field[f], fbytes) != hipSuccess) {
+        (void)hipGetLastError();  // a failed allocation must not surface at a later launch check
         field[f] = nullptr;
         set_error("gpe_create: cannot allocate %zu bytes of device memory", fbytes);
         return GPE_ENOMEM;
@@ -426,6 +437,7 @@ gpe_status gpe_solver::upload_potential() {
                     else d32[e] = (float)v;
                 }
     if (!pot && hipMalloc(&pot, (size_t)L.elems * eb) != hipSuccess) {
+        (void)hipGetLastError();
         pot = nullptr;
         set_error("gpe_set_potential: cannot allocate the potential");
         return GPE_ENOMEM;
@@ -456,6 +468,7 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
             const C *ei = (const C *)field[ie], *oi = (const C *)field[io];
             if (arr) e = launch_fused<T, FORM, POT_ARRAY>(tile, eo, oo, ei, oi, p, L, H, trap, co.dt, stream);
             else e = launch_fused<T, FORM, POT_TRAP>(tile, eo, oo, ei, oi, p, L, H, trap, co.dt, stream);
+            if (e != hipSuccess) break;  // nothing was written: the state stays where it is
             std::swap(ie, sp[0]);
             std::swap(io, sp[1]);
             nl += 1;
@@ -597,6 +610,13 @@ gpe_status gpe_solver::observables_t(int which, double out[4]) {
     GPE_HIP_TRY(hipMemcpyAsync(out, scal + 4, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
     GPE_HIP_TRY(hipStreamSynchronize(stream));
     return GPE_OK;
+}
+
+template <typename TA, typename TB>
+static void launch_distance(gpe_solver *a, int fa, const gpe_solver *b, int fb) {
+    const dim3 grid((a->L.nx + 63) / 64, (a->L.ny + kTY - 1) / kTY, a->L.nz), block(64, kTY);
+    hipLaunchKernelGGL((k_field_distance<TA, TB, kTY>), grid, block, 0, a->stream,
+                       (const cplx_t<TA> *)a->field[fa], a->L, (const cplx_t<TB> *)b->field[fb], b->L, a->partials);
 }
 
 // ---------------------------------------------------------------------
@@ -791,6 +811,31 @@ gpe_status gpe_step_observe(gpe_solver *s, uint64_t nsteps, double out[4]) {
     return GPE_OK;
 }
 
+gpe_status gpe_compare_fields(gpe_solver *a, int field_a, gpe_solver *b, int field_b, double out[3]) {
+    if (!a || !b || !out || (field_a != 0 && field_a != 1) || (field_b != 0 && field_b != 1)) {
+        set_error("gpe_compare_fields: invalid argument");
+        return GPE_EINVAL;
+    }
+    if (a->cfg.nx != b->cfg.nx || a->cfg.ny != b->cfg.ny || a->cfg.nz != b->cfg.nz) {
+        set_error("gpe_compare_fields: the solvers' grids differ");
+        return GPE_EINVAL;
+    }
+    const int fa = field_a == 0 ? a->ie : a->io, fb = field_b == 0 ? b->ie : b->io;
+    GPE_HIP_TRY(hipStreamSynchronize(b->stream));  // b's field is read on a's stream
+    const bool a64 = a->cfg.precision == GPE_FP64, b64 = b->cfg.precision == GPE_FP64;
+    if (a64 && b64) launch_distance<double, double>(a, fa, b, fb);
+    else if (a64) launch_distance<double, float>(a, fa, b, fb);
+    else if (b64) launch_distance<float, double>(a, fa, b, fb);
+    else launch_distance<float, float>(a, fa, b, fb);
+    GPE_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_finalize<3>), dim3(1), dim3(256), 0, a->stream, a->partials, obs_blocks(a->L), 1.0, 0,
+                       a->scal + 4);
+    GPE_HIP_TRY(hipGetLastError());
+    GPE_HIP_TRY(hipMemcpyAsync(out, a->scal + 4, 3 * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+    GPE_HIP_TRY(hipStreamSynchronize(a->stream));
+    return GPE_OK;
+}
+
 gpe_status gpe_synchronize(gpe_solver *s) {
     if (!s) return GPE_EINVAL;
     GPE_HIP_TRY(hipStreamSynchronize(s->stream));
@@ -806,6 +851,11 @@ gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile) {
         set_error("gpe_set_variant: tile %u out of range", tile);
         return GPE_EINVAL;
     }
+    const int t0 = tile == 0 ? default_tile((int)kernel, s->cfg.precision) : (int)tile - 1;
+    if (!tile_supported((int)kernel, t0, s->cfg.precision)) {
+        set_error("gpe_set_variant: fused tile %u does not fit LDS in FP64", tile);
+        return GPE_EINVAL;
+    }
     GPE_HIP_TRY(hipStreamSynchronize(s->stream));
     if (kernel == GPE_KERNEL_FUSED) {
         for (int f = 0; f < 4; f++) {
@@ -814,7 +864,7 @@ gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile) {
         }
     }
     s->kernel = (int)kernel;
-    s->tile = tile == 0 ? default_tile((int)kernel, s->cfg.precision) : (int)tile - 1;
+    s->tile = t0;
     return GPE_OK;
 }
 

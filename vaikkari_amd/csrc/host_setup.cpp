@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <random>
 #include <algorithm>
 #include <array>
@@ -154,6 +155,39 @@ gpe_status gpe_vortex_state_load(const char *path, gpe_vortex_state **out) {
 }
 
 void gpe_vortex_state_free(gpe_vortex_state *vs) { delete vs; }
+
+gpe_status gpe_vortex_state_from_table(uint32_t kappa, double g, uint32_t rsize, uint32_t zsize, double rplug,
+                                       double rstep, double zstep, double mu, const double *f,
+                                       gpe_vortex_state **out) {
+    if (!f || !out || rsize == 0 || zsize == 0 || rsize > (1u << 20) || zsize > (1u << 20) || !(rstep > 0) ||
+        !(zstep > 0) || !std::isfinite(rplug) || !std::isfinite(g) || !std::isfinite(mu)) {
+        set_error("gpe_vortex_state_from_table: invalid argument");
+        return GPE_EINVAL;
+    }
+    auto *h = new (std::nothrow) gpe_vortex_state();
+    if (!h) return GPE_ENOMEM;
+    VortexState &v = h->vs;
+    v.kappa = kappa;
+    v.g = g;
+    v.rsize = rsize;
+    v.zsize = zsize;
+    v.rplug = rplug;
+    v.rstep = rstep;
+    v.zstep = zstep;
+    v.mu = mu;
+    v.f.assign(f, f + (size_t)rsize * zsize);
+    *out = h;
+    return GPE_OK;
+}
+
+gpe_status gpe_vortex_state_table(const gpe_vortex_state *h, double *f) {
+    if (!h || !f) {
+        set_error("gpe_vortex_state_table: null argument");
+        return GPE_EINVAL;
+    }
+    std::copy(h->vs.f.begin(), h->vs.f.end(), f);
+    return GPE_OK;
+}
 
 void gpe_vortex_state_params(const gpe_vortex_state *h, double out[8]) {
     const VortexState &v = h->vs;

@@ -73,16 +73,24 @@ def poisson_workload(seed: int, rate: float, duration: float, n_min: int = 16, n
 POLICIES = {"fifo": 0, "shortest_first": 1}
 
 
+LAUNCH_MODES = {"graph": 0, "stream": 1}
+
+
 class Server:
     def __init__(self, precision: str = "fp32", hbm_budget_bytes: int = 200 * 10 ** 9, chunk_steps: int = 50,
-                 policy: str = "fifo", max_active_nodes: int = 0):
+                 policy: str = "fifo", max_active_nodes: int = 0, launch: str = "graph"):
         self._h = C.c_void_p()
         check(lib.gpe_server_create(PRECISIONS[precision], int(hbm_budget_bytes), int(chunk_steps),
                                     C.byref(self._h)), "gpe_server_create")
         self.set_policy(policy, max_active_nodes)
+        self.set_launch_mode(launch)
 
     def set_policy(self, policy: str, max_active_nodes: int = 0) -> None:
         check(lib.gpe_server_set_policy(self._h, POLICIES[policy], int(max_active_nodes)), "gpe_server_set_policy")
+
+    def set_launch_mode(self, mode: str) -> None:
+        """'graph' (captured chunk, the default) or 'stream' (kernels enqueued one by one)."""
+        check(lib.gpe_server_set_launch_mode(self._h, LAUNCH_MODES[mode]), "gpe_server_set_launch_mode")
 
     def oldest_queued(self) -> Optional[float]:
         t = lib.gpe_server_oldest_queued(self._h)
@@ -102,13 +110,15 @@ class Server:
     def now(self) -> float:
         return lib.gpe_server_now(self._h)
 
-    def submit(self, r: Request, arrival: Optional[float] = None, keep_result: bool = False) -> int:
+    def submit(self, r: Request, arrival: Optional[float] = None, keep_result: bool = False,
+               dt: Optional[float] = None) -> int:
+        """Queues r with the time step stable_dt gives it (or `dt`); returns its ticket."""
         q = _lib.GpeRequest()
         q.n = r.n
         q.mode = MODES[r.mode]
         q.steps = r.steps
         q.h = r.h
-        q.dt = stable_dt(r.n, r.h, r.g, r.ratio, mode=r.mode)
+        q.dt = stable_dt(r.n, r.h, r.g, r.ratio, mode=r.mode) if dt is None else float(dt)
         q.g = r.g
         q.trap_ratio = r.ratio
         q.sigma, q.noise, q.mu0 = 1.5, 1e-3, float("nan")  # leapfrog started with the state's own mu
@@ -129,7 +139,8 @@ class Server:
         n = C.c_uint32()
         check(lib.gpe_server_poll(self._h, buf, max_items, C.byref(n)), "gpe_server_poll")
         return [{"ticket": c.ticket, "arrival": c.arrival, "admitted": c.admitted, "finished": c.finished,
-                 "node_steps": c.node_steps, "norm": c.norm, "energy": c.energy} for c in buf[:n.value]]
+                 "node_steps": c.node_steps, "norm": c.norm, "energy": c.energy, "status": c.status}
+                for c in buf[:n.value]]
 
     def fetch(self, ticket: int, n: int, mode: str):
         nodes = nodes_of((n, n, n))
@@ -144,18 +155,70 @@ class Server:
         return {k: getattr(s, k) for k, _ in _lib.GpeServerStats._fields_}
 
 
+SIZE_CLASSES = ((16, 32), (32, 64), (64, 128), (128, 161))
+
+
+def _pct(a, q):
+    return float(np.percentile(a, q)) if len(a) else None
+
+
+def latency_report(workload: List[Request], arrivals: dict, done: List[dict], end: float, duration: float) -> dict:
+    """Latency figures of a serving run.
+
+    arrivals: ticket -> (request, arrival on the server clock) of every submitted request;
+    done: completions; end: server clock at the end.  Time in system is finished - arrival for a
+    completed request and end - arrival (a lower bound: the request is still queued or running)
+    for the others, which are right-censored.  Percentiles over completed requests alone favour
+    the short solves that shortest-first admission finishes first; the 'all' figures count every
+    submitted request and are lower bounds on what those callers see.  Per size class (cells per
+    axis) the offered load (node-steps/s of the requests arriving in the window) sits beside the
+    served load (node-steps/s of the completed ones)."""
+    fin = {d["ticket"]: d for d in done if d.get("status", 0) == 0}
+    failed = sum(1 for d in done if d.get("status", 0) != 0)
+    dropped = {d["ticket"] for d in done if d.get("status", 0) != 0}
+    lat_done, lat_all = [], []
+    for t, (r, arr) in arrivals.items():
+        if t in fin:
+            lat_done.append(fin[t]["finished"] - arr)
+            lat_all.append(fin[t]["finished"] - arr)
+        elif t not in dropped:
+            lat_all.append(end - arr)
+    out = {
+        "p50_s": _pct(lat_done, 50), "p99_s": _pct(lat_done, 99),
+        "mean_s": float(np.mean(lat_done)) if lat_done else None,
+        "p50_all_s": _pct(lat_all, 50), "p99_all_s": _pct(lat_all, 99),
+        "censored": len(lat_all) - len(lat_done), "failed": failed,
+        "latency_note": "p50_s/p99_s: completed requests only; p50_all_s/p99_all_s: every submitted request, "
+                        "unfinished ones at their age at the end (lower bounds)",
+    }
+    classes = []
+    for lo, hi in SIZE_CLASSES:
+        ts = [t for t, (r, _) in arrivals.items() if lo <= r.n < hi and t not in dropped]
+        ld = [fin[t]["finished"] - arrivals[t][1] for t in ts if t in fin]
+        la = [fin[t]["finished"] - arrivals[t][1] if t in fin else end - arrivals[t][1] for t in ts]
+        offered = sum(r.node_steps() for r in workload if r.arrival < duration and lo <= r.n < hi) / duration
+        served = sum(fin[t]["node_steps"] for t in ts if t in fin) / duration
+        classes.append({"n": [lo, hi - 1], "submitted": len(ts), "completed": len(ld),
+                        "p50_s": _pct(ld, 50), "p99_s": _pct(ld, 99), "p50_all_s": _pct(la, 50),
+                        "p99_all_s": _pct(la, 99), "offered_site_steps_per_s": offered,
+                        "completed_site_steps_per_s": served})
+    out["size_classes"] = classes
+    return out
+
+
 def run_serving(workload: List[Request], duration: float, precision: str = "fp32",
                 hbm_budget_bytes: int = 200 * 10 ** 9, chunk_steps: int = 50, log=None, policy: str = "fifo",
-                max_active_nodes: int = 0, sample_every: float = 10.0) -> dict:
+                max_active_nodes: int = 0, sample_every: float = 10.0, launch: str = "graph") -> dict:
     """Replays the workload against the wall clock for `duration` seconds: requests are
     submitted when their arrival time has passed, the server ticks continuously.  Every
     `sample_every` seconds the queue depth, resident solves and bytes, completions and the age
     of the oldest queued request are recorded (and logged)."""
-    srv = Server(precision, hbm_budget_bytes, chunk_steps, policy, max_active_nodes)
+    srv = Server(precision, hbm_budget_bytes, chunk_steps, policy, max_active_nodes, launch)
     t_start = srv.now()
     pending = sorted(workload, key=lambda r: r.arrival)
     k = 0
     done = []
+    arrivals = {}
     samples = []
     last_sample = t_start
     while True:
@@ -163,7 +226,8 @@ def run_serving(workload: List[Request], duration: float, precision: str = "fp32
         if now >= duration:
             break
         while k < len(pending) and pending[k].arrival <= now:
-            srv.submit(pending[k], arrival=t_start + pending[k].arrival)
+            arr = t_start + pending[k].arrival
+            arrivals[srv.submit(pending[k], arrival=arr)] = (pending[k], arr)
             k += 1
         srv.tick()
         done.extend(srv.poll())
@@ -177,9 +241,10 @@ def run_serving(workload: List[Request], duration: float, precision: str = "fp32
                 log(f"serve t={now:.0f}s completed={st['completed']} active={st['active']} queued={st['queued']} "
                     f"resident={st['resident_bytes'] / 1e9:.1f} GB")
             last_sample = srv.now()
-    wall = srv.now() - t_start
+    end = srv.now()
+    wall = end - t_start
     st = srv.stats()
-    lat = np.array([d["finished"] - d["arrival"] for d in done]) if done else np.array([])
+    ok = [d for d in done if d.get("status", 0) == 0]
     offered = sum(r.node_steps() for r in workload if r.arrival < duration) / duration
     res = {
         "duration_s": wall,
@@ -188,15 +253,13 @@ def run_serving(workload: List[Request], duration: float, precision: str = "fp32
         "served_site_steps_per_s": st["node_steps_done"] / wall if wall > 0 else 0.0,
         "device_site_steps_per_s": st["node_steps_done"] / (st["device_ms"] * 1e-3) if st["device_ms"] else 0.0,
         "offered_site_steps_per_s": offered,
-        "p50_s": float(np.percentile(lat, 50)) if lat.size else None,
-        "p99_s": float(np.percentile(lat, 99)) if lat.size else None,
-        "completed_norm_range": [float(min(d["norm"] for d in done)), float(max(d["norm"] for d in done))]
-        if done else None,
+        "completed_norm_range": [float(min(d["norm"] for d in ok)), float(max(d["norm"] for d in ok))]
+        if ok else None,
         "ticks": int(st["ticks"]),
-        "policy": policy, "max_active_nodes": int(max_active_nodes),
-        "mean_s": float(lat.mean()) if lat.size else None,
-        "completed_work_site_steps": float(sum(d["node_steps"] for d in done)),
+        "policy": policy, "max_active_nodes": int(max_active_nodes), "launch": launch,
+        "completed_work_site_steps": float(sum(d["node_steps"] for d in ok)),
         "samples": samples,
     }
+    res.update(latency_report(workload, arrivals, done, end, duration))
     srv.close()
     return res

@@ -165,6 +165,15 @@ class Solver:
         N, K, P, I = (float(x) for x in out)
         return {"N": N, "K": K, "P": P, "I": I, "E": K + P + 0.5 * I, "mu": (K + P + I) / N if N else float("nan")}
 
+    def compare(self, other: "Solver", field: int = 0, other_field: int = 0) -> dict:
+        """Distance of this solver's field to `other`'s on the same grid, reduced on the device in
+        FP64 (neither field leaves the GPU): rel_l2 = ||self - other|| / ||other||."""
+        out = np.zeros(3)
+        check(lib.gpe_compare_fields(self._h, int(field), other._h, int(other_field), dptr(out)),
+              "gpe_compare_fields")
+        d2, b2, a2 = (float(x) for x in out)
+        return {"diff2": d2, "other2": b2, "self2": a2, "rel_l2": math.sqrt(d2 / b2) if b2 > 0 else float("nan")}
+
     def set_variant(self, kernel: str, tile: int) -> None:
         check(lib.gpe_set_variant(self._h, KERNELS[kernel], int(tile)), "gpe_set_variant")
 
@@ -256,6 +265,29 @@ class VortexState:
         self.close()
         self._h = h
         return True
+
+    @classmethod
+    def from_table(cls, kappa: int, g: float, rplug: float, rstep: float, zstep: float, mu: float,
+                   f: np.ndarray) -> "VortexState":
+        """A state from memory: f is the rsize x zsize table f(r, z) (r-major), the rest the
+        reference object's members (VortexState.hpp:16-76)."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        if f.ndim != 2:
+            raise ValueError("f must be an rsize x zsize array")
+        h = C.c_void_p()
+        check(lib.gpe_vortex_state_from_table(int(kappa), float(g), f.shape[0], f.shape[1], float(rplug),
+                                              float(rstep), float(zstep), float(mu), dptr(f), C.byref(h)),
+              "gpe_vortex_state_from_table")
+        vs = cls.__new__(cls)
+        vs._h = h
+        return vs
+
+    def table(self) -> np.ndarray:
+        """The rsize x zsize table f(r, z)."""
+        p = self._params()
+        f = np.empty((int(p[2]), int(p[3])))
+        check(lib.gpe_vortex_state_table(self._h, dptr(f)), "gpe_vortex_state_table")
+        return f
 
     def close(self):
         if self._h:
