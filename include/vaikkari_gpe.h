@@ -157,6 +157,27 @@ gpe_status gpe_compare_fields(gpe_solver *a, int field_a, gpe_solver *b, int fie
 gpe_status gpe_synchronize(gpe_solver *s);
 gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info);
 
+/* Device layout of one field ("class-row SoA"): node (i, j, k, class l) of an
+ * nx x ny x nz cell grid is complex element origin + k*sz + j*sy + l*xp + i of
+ * the buffer, each element (re, im) in 2 x 8 bytes (GPE_FP64) or 2 x 4 bytes
+ * (GPE_FP32_BULK); the buffer holds elems elements, and every element outside
+ * the box (the one-cell halo and the row padding) is zero.  No counterpart in
+ * the reference, whose buffers are pitched arrays of 12 double2 per cell
+ * (GPUGrossPitaevskii_BCC.cu:144-159,269-287). */
+typedef struct gpe_field_layout {
+    uint64_t elems;       /* complex elements of the whole buffer */
+    uint64_t origin;      /* element of node (0, 0, 0, 0) */
+    uint64_t xp, sy, sz;  /* class-row, cell-row and layer strides, in elements */
+    uint32_t elem_bytes;  /* 16 or 8 */
+    uint32_t reserved;
+} gpe_field_layout;
+gpe_status gpe_get_field_layout(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t precision, gpe_field_layout *out);
+/* Device address of a solver's field 0 (even) or 1 (odd), in the layout
+ * above.  It stays valid until the solver is destroyed; after gpe_step of the
+ * fused family or gpe_set_variant ask again (those may move the time levels
+ * between buffers). */
+gpe_status gpe_get_field_device(gpe_solver *s, int field, void **dev);
+
 /* Kernel variants (the autotuner's search space).  gpe_set_variant switches
  * the solver's kernel family and tile shape between steps; results do not
  * depend on the variant (every variant performs the same arithmetic per

@@ -185,3 +185,25 @@ def test_vortex_state_from_table_equals_file(tmp_path):
         np.testing.assert_array_equal(x, y)
     with pytest.raises(_lib.GpeError):
         va.VortexState.from_table(1, 1.0, 0.0, -0.1, 0.1, 1.0, f)  # non-positive step
+
+
+@pytest.mark.parametrize("precision,eb", [("fp64", 16), ("fp32", 8)])
+def test_field_layout(precision, eb):
+    """gpe_get_field_layout: rows start on 128-byte lines behind a one-line left pad that holds
+    the x halo, 12 class rows per cell row, a one-cell halo all round; pack_field puts node
+    (i, j, k, l) at origin + k sz + j sy + l xp + i and leaves everything else zero."""
+    dims = (37, 5, 4)
+    L = va.field_layout(dims, precision)
+    line = 128 // eb
+    assert L["elem_bytes"] == eb and L["xp"] % line == 0 and L["xp"] >= line + dims[0] + 1
+    assert L["sy"] == 12 * L["xp"] and L["sz"] == (dims[1] + 2) * L["sy"]
+    assert L["origin"] == L["sz"] + L["sy"] + line and L["elems"] == (dims[2] + 2) * L["sz"]
+    rng = np.random.default_rng(1)
+    psi = rng.uniform(-1, 1, 2 * va.nodes_of(dims))
+    packed = va.pack_field(dims, psi, precision)
+    assert packed.size == 2 * L["elems"] and np.count_nonzero(packed) == psi.size
+    i, j, k, l = 36, 4, 3, 11
+    n = ((k * dims[1] + j) * dims[0] + i) * 12 + l
+    e = L["origin"] + k * L["sz"] + j * L["sy"] + l * L["xp"] + i
+    assert packed[2 * e] == np.asarray(psi[2 * n], packed.dtype)
+    assert packed[2 * e + 1] == np.asarray(psi[2 * n + 1], packed.dtype)

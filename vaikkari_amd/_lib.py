@@ -129,6 +129,13 @@ class GpeServerStats(C.Structure):
     ]
 
 
+class GpeFieldLayout(C.Structure):
+    _fields_ = [
+        ("elems", C.c_uint64), ("origin", C.c_uint64), ("xp", C.c_uint64), ("sy", C.c_uint64), ("sz", C.c_uint64),
+        ("elem_bytes", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
 class GpeRelaxReport(C.Structure):
     _fields_ = [
         ("iterations", C.c_uint32),
@@ -197,6 +204,8 @@ SIGNATURES = [
     ("gpe_observables", _I, [_P, _I, _DP]),
     ("gpe_step_observe", _I, [_P, _U64, _DP]),
     ("gpe_compare_fields", _I, [_P, _I, _P, _I, _DP]),
+    ("gpe_get_field_layout", _I, [_U32, _U32, _U32, _U32, C.POINTER(GpeFieldLayout)]),
+    ("gpe_get_field_device", _I, [_P, _I, C.POINTER(_P)]),
     ("gpe_synchronize", _I, [_P]),
     ("gpe_get_info", _I, [_P, C.POINTER(GpeSolverInfo)]),
     ("gpe_set_variant", _I, [_P, _U32, _U32]),

@@ -836,6 +836,32 @@ gpe_status gpe_compare_fields(gpe_solver *a, int field_a, gpe_solver *b, int fie
     return GPE_OK;
 }
 
+gpe_status gpe_get_field_layout(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t precision, gpe_field_layout *out) {
+    if (!out || nx == 0 || ny == 0 || nz == 0 || nx > 65536 || ny > 65536 || nz > 65536 || precision > GPE_FP32_BULK) {
+        set_error("gpe_get_field_layout: invalid argument");
+        return GPE_EINVAL;
+    }
+    const size_t eb = precision == GPE_FP64 ? 16 : 8;
+    const Layout L = make_layout(nx, ny, nz, eb);
+    out->elems = (uint64_t)L.elems;
+    out->origin = (uint64_t)L.origin;
+    out->xp = (uint64_t)L.xp;
+    out->sy = (uint64_t)L.sy;
+    out->sz = (uint64_t)L.sz;
+    out->elem_bytes = (uint32_t)eb;
+    out->reserved = 0;
+    return GPE_OK;
+}
+
+gpe_status gpe_get_field_device(gpe_solver *s, int field, void **dev) {
+    if (!s || !dev || (field != 0 && field != 1)) {
+        set_error("gpe_get_field_device: invalid argument");
+        return GPE_EINVAL;
+    }
+    *dev = s->field[field == 0 ? s->ie : s->io];
+    return GPE_OK;
+}
+
 gpe_status gpe_synchronize(gpe_solver *s) {
     if (!s) return GPE_EINVAL;
     GPE_HIP_TRY(hipStreamSynchronize(s->stream));

@@ -174,6 +174,12 @@ class Solver:
         d2, b2, a2 = (float(x) for x in out)
         return {"diff2": d2, "other2": b2, "self2": a2, "rel_l2": math.sqrt(d2 / b2) if b2 > 0 else float("nan")}
 
+    def device_field(self, field: int = 0) -> int:
+        """Device address of field 0 (even) or 1 (odd) in the layout field_layout() describes."""
+        p = C.c_void_p()
+        check(lib.gpe_get_field_device(self._h, int(field), C.byref(p)), "gpe_get_field_device")
+        return int(p.value or 0)
+
     def set_variant(self, kernel: str, tile: int) -> None:
         check(lib.gpe_set_variant(self._h, KERNELS[kernel], int(tile)), "gpe_set_variant")
 
@@ -192,6 +198,29 @@ class Solver:
         if a.size != 2 * self.nodes:
             raise ValueError(f"field has {a.size} doubles, expected {2 * self.nodes}")
         return a
+
+
+def field_layout(dims: Sequence[int], precision: str = "fp64") -> dict:
+    """Device layout of one field: node (i, j, k, l) is complex element
+    origin + k*sz + j*sy + l*xp + i of a buffer of `elems` elements of `elem_bytes` bytes."""
+    L = _lib.GpeFieldLayout()
+    check(lib.gpe_get_field_layout(*(int(d) for d in dims), PRECISIONS[precision], C.byref(L)),
+          "gpe_get_field_layout")
+    return {k: int(getattr(L, k)) for k, _ in _lib.GpeFieldLayout._fields_ if k != "reserved"}
+
+
+def pack_field(dims: Sequence[int], psi: np.ndarray, precision: str = "fp64") -> np.ndarray:
+    """Host field ([k][j][i][l], re/im interleaved) in the device layout, as a flat array of
+    float64 (fp64) or float32 (fp32) re/im pairs; zero outside the box."""
+    L = field_layout(dims, precision)
+    nx, ny, nz = (int(d) for d in dims)
+    out = np.zeros(2 * L["elems"], dtype=np.float64 if precision == "fp64" else np.float32)
+    c = np.asarray(psi, dtype=np.float64).reshape(nz, ny, nx, 12, 2)
+    k, j, i, l = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), np.arange(12), indexing="ij")
+    e = L["origin"] + k * L["sz"] + j * L["sy"] + l * L["xp"] + i
+    out[2 * e] = c[..., 0]
+    out[2 * e + 1] = c[..., 1]
+    return out
 
 
 def variants():
