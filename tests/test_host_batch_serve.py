@@ -83,3 +83,21 @@ def test_latency_report_counts_unfinished_requests():
     assert cls[(128, 160)]["p50_all_s"] == pytest.approx(17.0) and cls[(128, 160)]["p50_s"] is None
     assert cls[(16, 31)]["offered_site_steps_per_s"] == pytest.approx(
         (reqs[0].node_steps() + reqs[1].node_steps() + reqs[4].node_steps()) / 10.0)
+
+
+@pytest.mark.parametrize("n", [6, 8, 12, 16, 24])
+@pytest.mark.parametrize("mode", ["real", "imag"])
+def test_stable_dt_holds_for_the_normalised_initial_state(n, mode):
+    """Batched and served Gaussians are normalised on the grid, which raises the peak density of a
+    clipped one (a 8^3 box of edge 2 keeps about 28 % of a sigma = 1.5 Gaussian): stable_dt must
+    keep dt * (6/b^2 + max V + g max|psi0|^2) < 2 with that peak, for the largest g of the forms."""
+    h, g, ratio, sigma = 0.25, 5000.0, 4.0, 1.5
+    b = h / math.sqrt(8)
+    dims = (n, n, n)
+    origin = va.centered_origin(dims, b)
+    psi = va.gaussian(dims, origin, b, sigma, 0.0, 1)  # normalised on the grid, no noise
+    assert va.field_norm(dims, b, psi) == pytest.approx(1.0, rel=1e-12)
+    peak = float(np.max(psi[0::2] ** 2 + psi[1::2] ** 2))
+    vmax = float(va.trap_potential(dims, origin, b, ratio).max())
+    dt = stable_dt(n, h, g, ratio, sigma=sigma, mode=mode)
+    assert dt * (6.0 / b ** 2 + vmax + g * peak) < 2.0

@@ -25,11 +25,13 @@ def stable_dt(n: int, h: float, g: float, ratio: float, sigma: float = 1.5, dt_n
               mode: str = "real") -> float:
     """Time step for an n^3-cell request: the nominal 1e-3*(32/n) (real time), capped by the
     reference's own rule dt <= 1/(max(V + g|psi0|^2) + 3/b^2) (GPUGrossPitaevskii_BCC.cu:298).
-    Imaginary time uses half the cap."""
+    Imaginary time uses half the cap.  max |psi0|^2 is the peak of the Gaussian initial state
+    normalised on the box, (pi sigma^2)^(-3/2) / erf(n h / (2 sigma))^3: a box that clips the
+    Gaussian raises the peak by the norm it cuts off."""
     b = h / math.sqrt(8.0)
     half = 0.5 * n * h
     vmax = 0.5 * (2 * half * half + ratio * ratio * half * half)
-    rho = (math.pi * sigma * sigma) ** -1.5
+    rho = (math.pi * sigma * sigma) ** -1.5 / math.erf(half / sigma) ** 3
     cap = 1.0 / (vmax + g * rho + 3.0 / (b * b))
     if mode == "imag":
         return 0.5 * cap
