@@ -178,6 +178,25 @@ gpe_status gpe_get_field_layout(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t 
  * between buffers). */
 gpe_status gpe_get_field_device(gpe_solver *s, int field, void **dev);
 
+/* gpe_observables for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The caller passes the field in device memory
+ * in the layout of gpe_get_field_layout (its own buffer, or a solver's from
+ * gpe_get_field_device) with cfg describing it (nx, ny, nz, precision,
+ * block_scale, dt, g, origin, potential, trap_ratio; the other members are
+ * ignored), pot_dt = V*dt in that layout with one real value of the field's
+ * precision per element when cfg->potential is GPE_POT_ARRAY (else NULL), and
+ * out = device memory for the 4 doubles {N, K, P, I} of gpe_observables, equal
+ * to what gpe_observables gives for the same field.  The call only enqueues two
+ * kernels on `stream` (a hipStream_t; NULL: the null stream) and returns: it
+ * allocates nothing and does not synchronise.  The block partials live in an
+ * 8 MiB device array of the library, one per GPU, held until the process ends:
+ * grids of more than 2^18 blocks of 64 x 4 cells per layer (about 390^3
+ * cells) are refused with GPE_EINVAL.  Calls that overlap on one GPU (other
+ * streams, other threads) share that array and race. */
+gpe_status gpe_observables_noalloc(const gpe_config *cfg, const void *field, const void *pot_dt, double *out,
+                                   void *stream);
+
 /* Kernel variants (the autotuner's search space).  gpe_set_variant switches
  * the solver's kernel family and tile shape between steps; results do not
  * depend on the variant (every variant performs the same arithmetic per

@@ -207,3 +207,27 @@ def test_field_layout(precision, eb):
     e = L["origin"] + k * L["sz"] + j * L["sy"] + l * L["xp"] + i
     assert packed[2 * e] == np.asarray(psi[2 * n], packed.dtype)
     assert packed[2 * e + 1] == np.asarray(psi[2 * n + 1], packed.dtype)
+
+
+def test_observables_noalloc_argument_checks():
+    """gpe_observables_noalloc refuses, before touching the device, a grid whose block partials
+    exceed its 8 MiB scratch (more than 2^18 blocks of 64 x 4 cells per layer), a missing output,
+    field or streamed potential, and a bad configuration."""
+    cfg = _lib.GpeConfig()
+    cfg.nx, cfg.ny, cfg.nz = 512, 512, 512
+    cfg.precision, cfg.block_scale, cfg.dt, cfg.g = _lib.FP64, 0.05, 1e-4, 10.0
+    cfg.potential, cfg.trap_ratio = _lib.POT_TRAP, 1.0
+    dummy = 0x1000  # never dereferenced: every call below is refused first
+    with pytest.raises(_lib.GpeError) as ei:
+        va.observables_noalloc(cfg, dummy, dummy)
+    assert ei.value.status == _lib.EINVAL and "scratch" in str(ei.value)
+    cfg.nx, cfg.ny, cfg.nz = 8, 8, 8
+    for args in ((0, dummy), (dummy, 0)):
+        with pytest.raises(_lib.GpeError):
+            va.observables_noalloc(cfg, *args)
+    cfg.potential = _lib.POT_ARRAY
+    with pytest.raises(_lib.GpeError):
+        va.observables_noalloc(cfg, dummy, dummy)
+    cfg.potential, cfg.block_scale = _lib.POT_TRAP, -1.0
+    with pytest.raises(_lib.GpeError):
+        va.observables_noalloc(cfg, dummy, dummy)

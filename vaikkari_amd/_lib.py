@@ -206,6 +206,7 @@ SIGNATURES = [
     ("gpe_compare_fields", _I, [_P, _I, _P, _I, _DP]),
     ("gpe_get_field_layout", _I, [_U32, _U32, _U32, _U32, C.POINTER(GpeFieldLayout)]),
     ("gpe_get_field_device", _I, [_P, _I, C.POINTER(_P)]),
+    ("gpe_observables_noalloc", _I, [C.POINTER(GpeConfig), _P, _P, _P, _P]),
     ("gpe_synchronize", _I, [_P]),
     ("gpe_get_info", _I, [_P, C.POINTER(GpeSolverInfo)]),
     ("gpe_set_variant", _I, [_P, _U32, _U32]),

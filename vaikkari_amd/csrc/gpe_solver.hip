@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -620,6 +621,64 @@ static void launch_distance(gpe_solver *a, int fa, const gpe_solver *b, int fb) 
 }
 
 // ---------------------------------------------------------------------
+// allocation-free observables (gpe_observables_noalloc)
+// ---------------------------------------------------------------------
+namespace {
+// Block partials of gpe_observables_noalloc, 4 doubles per block of the
+// observables grid (64 x 4 cells per block and layer): grids of up to this many
+// blocks, about 390^3 cells.
+constexpr long long kNoAllocBlocks = 1LL << 18;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+// 8 MiB per GPU, held until the process ends.  Every block of the
+// observables kernel writes its own 4 partials before the finalize reads
+// them, so nothing in it needs clearing between calls.  In host code the name
+// below is a host-side stand-in: kernels get the address scratch_address()
+// looked up on the current device.
+static __device__ __align__(256) double scratch[4 * kNoAllocBlocks];
+
+// Device address of `scratch` on the current device, looked up once per device.
+static gpe_status scratch_address(double **out) {
+    int dev = 0;
+    GPE_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("gpe_observables_noalloc: device %d beyond the %d supported", dev, kMaxDevices);
+        return GPE_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        GPE_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = (double *)p;
+    }
+    *out = g_scratch[dev];
+    return GPE_OK;
+}
+
+template <typename T>
+static gpe_status observables_noalloc_t(const gpe_config &c, const Layout &L, const void *field, const void *pot,
+                                        double *out, hipStream_t st, double *partials) {
+    const double b = c.block_scale;
+    const Trap trap = make_trap(c.origin, b, c.trap_ratio);
+    const dim3 grid((L.nx + 63) / 64, (L.ny + kTY - 1) / kTY, L.nz), block(64, kTY);
+    const cplx_t<T> *f = (const cplx_t<T> *)field;
+    const double lap = kHodge / (b * b);
+    if (c.potential == GPE_POT_ARRAY)
+        hipLaunchKernelGGL((k_observables<T, POT_ARRAY, kTY>), grid, block, 0, st, f, (const T *)pot, L, trap,
+                           1.0 / c.dt, lap, c.g, partials);
+    else
+        hipLaunchKernelGGL((k_observables<T, POT_TRAP, kTY>), grid, block, 0, st, f, (const T *)nullptr, L, trap,
+                           1.0 / c.dt, lap, c.g, partials);
+    GPE_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_finalize<4>), dim3(1), dim3(256), 0, st, partials, obs_blocks(L), node_measure(b), 0, out);
+    GPE_HIP_TRY(hipGetLastError());
+    return GPE_OK;
+}
+
+// ---------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------
 extern "C" {
@@ -860,6 +919,38 @@ gpe_status gpe_get_field_device(gpe_solver *s, int field, void **dev) {
     }
     *dev = s->field[field == 0 ? s->ie : s->io];
     return GPE_OK;
+}
+
+gpe_status gpe_observables_noalloc(const gpe_config *cfg, const void *field, const void *pot_dt, double *out,
+                                   void *stream) {
+    if (!cfg || !field || !out) {
+        set_error("gpe_observables_noalloc: null argument");
+        return GPE_EINVAL;
+    }
+    const gpe_config &c = *cfg;
+    if (c.nx == 0 || c.ny == 0 || c.nz == 0 || c.nx > 65536 || c.ny > 65536 || c.nz > 65536 ||
+        c.precision > GPE_FP32_BULK || c.potential > GPE_POT_TRAP || !(c.block_scale > 0) || !(c.dt > 0) ||
+        !std::isfinite(c.g) || !std::isfinite(c.trap_ratio)) {
+        set_error("gpe_observables_noalloc: invalid configuration");
+        return GPE_EINVAL;
+    }
+    if (c.potential == GPE_POT_ARRAY && !pot_dt) {
+        set_error("gpe_observables_noalloc: potential kind is ARRAY but pot_dt is NULL");
+        return GPE_EINVAL;
+    }
+    const Layout L = make_layout(c.nx, c.ny, c.nz, c.precision == GPE_FP64 ? 16 : 8);
+    if (obs_blocks(L) > kNoAllocBlocks) {
+        set_error("gpe_observables_noalloc: %u x %u x %u cells need %lld blocks of partials, more than the %lld "
+                  "the scratch holds",
+                  c.nx, c.ny, c.nz, obs_blocks(L), kNoAllocBlocks);
+        return GPE_EINVAL;
+    }
+    double *partials = nullptr;
+    const gpe_status st = scratch_address(&partials);
+    if (st != GPE_OK) return st;
+    const hipStream_t hs = (hipStream_t)stream;
+    if (c.precision == GPE_FP64) return observables_noalloc_t<double>(c, L, field, pot_dt, out, hs, partials);
+    return observables_noalloc_t<float>(c, L, field, pot_dt, out, hs, partials);
 }
 
 gpe_status gpe_synchronize(gpe_solver *s) {

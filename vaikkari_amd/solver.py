@@ -200,6 +200,14 @@ class Solver:
         return a
 
 
+def observables_noalloc(cfg, field: int, out: int, pot_dt: int = 0, stream: int = 0) -> None:
+    """gpe_observables_noalloc: enqueue the observables {N, K, P, I} of the device field at
+    address `field` (described by the GpeConfig cfg) into the 4 device doubles at `out` on
+    `stream`, allocating nothing.  Asynchronous: synchronise the stream before reading out."""
+    check(lib.gpe_observables_noalloc(C.byref(cfg), C.c_void_p(field), C.c_void_p(pot_dt or None),
+                                      C.c_void_p(out), C.c_void_p(stream or None)), "gpe_observables_noalloc")
+
+
 def field_layout(dims: Sequence[int], precision: str = "fp64") -> dict:
     """Device layout of one field: node (i, j, k, l) is complex element
     origin + k*sz + j*sy + l*xp + i of a buffer of `elems` elements of `elem_bytes` bytes."""
