@@ -17,6 +17,14 @@ are reported beside it.
 
 Inputs are resident in HBM before the timed region; timing uses HIP events
 on the solver's stream around exactly --steps steps.
+
+Beside the line of record (all untimed or timed separately): fp32_check, FP32
+against FP64 over a fixed --check-steps from the phase-2 state, psi relative L2
+(reduced on the device) with norm and energy drift at four checkpoints; dual256,
+both precisions at the precision form's 256^3 grid; the batched form (64
+instances), a 20 s sample of the serving form, the 14-point primal stepper, and
+cpu_baseline, the CPU oracle on the host's cores (warm-up, then the median of
+timed runs) at the 128^3 grid and at the 32^3 CPU-form grid.
 """
 from __future__ import annotations
 
@@ -43,7 +51,13 @@ def parse():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-warmup", type=int, default=10, help="untimed oracle steps before the timed runs")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed oracle runs (the median is reported)")
+    ap.add_argument("--cpu-steps", type=int, default=100, help="steps per timed oracle run at 128^3")
+    ap.add_argument("--check-steps", type=int, default=1000,
+                    help="length of the untimed FP32-vs-FP64 check from the benchmark state")
+    ap.add_argument("--dual256-steps", type=int, default=100,
+                    help="timed steps of the 256^3 precision-form grid, FP64 and FP32 (0 = skip)")
     ap.add_argument("--batch-instances", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--serve-seconds", type=float, default=20.0,
@@ -62,6 +76,9 @@ def parse():
     ap.add_argument("--batch-launch", default="graph", choices=["graph", "stream"],
                     help="batched workload: replay a captured hipGraph per chunk, or enqueue the kernels "
                          "directly (for per-dispatch counter runs)")
+    ap.add_argument("--serve-launch", default="graph", choices=["graph", "stream"],
+                    help="serving: replay a captured hipGraph per chunk (default), or enqueue the kernels "
+                         "directly (for kernel traces of the serving loop)")
     ap.add_argument("--serve-policy", default="shortest_first", choices=["shortest_first", "fifo"],
                     help="admission policy of --suite serving (shortest_first is capped at 32 M resident nodes)")
     ap.add_argument("--primal-cells", default="128,256",
@@ -92,47 +109,69 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(dims, b, g, dt, ratio, seconds):
-    """The CPU oracle (oracle/liboracle.so, C + OpenMP, FP64, reference AoS layout) on the same
-    grid and coefficients: rate = nodes*(n2-n1)/(t2-t1) from two runs, so padding copies cancel."""
+def host_cpu():
+    """CPU model and core counts of this host from lscpu (physical cores = sockets x cores per socket)."""
+    import subprocess
+
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    def num(k):
+        try:
+            return int(info.get(k, ""))
+        except ValueError:
+            return None
+    sockets, per, tpc = num("Socket(s)"), num("Core(s) per socket"), num("Thread(s) per core")
+    return {"model": info.get("Model name", "unknown"), "sockets": sockets, "cores_per_socket": per,
+            "threads_per_core": tpc, "physical_cores": sockets * per if sockets and per else None,
+            "logical_cpus": num("CPU(s)"), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(warmup, runs, steps128):
+    """The oracle (oracle/liboracle.so: C + OpenMP, FP64, the reference's AoS layout of 12 complex
+    values per cell, -O2 -ffp-contract=off) on this host's cores, timed as the CPU-baseline plan says:
+    `warmup` untimed steps, then the median of `runs` runs of >= 100 steps, each run timed inside the
+    oracle with the monotonic clock (the fields are padded once, outside the timing).  Two grids:
+    the benchmark's own (128^3 cells, h 0.1, g 2000, ratio 2, dt 2e-4: `value`) and the CPU form's
+    (32^3 cells, h 0.25, g 500, isotropic trap, dt 1e-3, 1000 steps per run)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle  # noqa: E402
     import vaikkari_amd as va
 
     orc = Oracle()
-    origin = va.centered_origin(dims, b)
-    even = va.gaussian(dims, origin, b, 1.5, 1e-3, 1)
-    odd = va.phase_shift(dims, even, 10.0, dt)
-    V = va.trap_potential(dims, origin, b, ratio) * dt
-    lf, lf0, gdt = va.leapfrog_coefs(b, dt, g)
-    n = va.nodes_of(dims)
-    t0 = time.perf_counter()
-    orc.leapfrog(even, odd, V, dims, lf, lf0, gdt, 1)
-    t1 = time.perf_counter()
-    orc.leapfrog(even, odd, V, dims, lf, lf0, gdt, 3)
-    t1b = time.perf_counter()
-    # a call's fixed cost (layout copies, thread start) cancels in the difference of 3 steps and 1
-    per_step = max(((t1b - t1) - (t1 - t0)) / 2, 1e-3)
-    n2 = max(2, int(seconds / per_step))
-    t2 = time.perf_counter()
-    orc.leapfrog(even, odd, V, dims, lf, lf0, gdt, n2)
-    t3 = time.perf_counter()
-    rate = n * (n2 - 1) / ((t3 - t2) - (t1 - t0))
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    model = "unknown CPU"
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
-    # the oracle streams prev, next (read + write) and V per node-update, 16 + 32 + 8 bytes
-    host_gbs = rate * 2 * 56 / 1e9
-    return {"value": rate, "unit": "site-steps/s", "cores": threads, "kind": "port",
-            "cpu": model, "host_GBs": host_gbs,
-            "sample": f"oracle FP64 leapfrog on the same {dims[0]}^3-cell grid ({n} nodes), "
-                      f"{n2} steps timed ({t3 - t2:.1f} s) minus a 1-step run"}
+    grids = []
+    for label, n, h, g, dt, ratio, nsteps in (("drop_in_128", 128, 0.1, 2000.0, 2e-4, 2.0, steps128),
+                                              ("cpu_form_32", 32, 0.25, 500.0, 1e-3, 1.0, 1000)):
+        dims = (n, n, n)
+        b = h / math.sqrt(8)
+        origin = va.centered_origin(dims, b)
+        even = va.gaussian(dims, origin, b, 1.5, 1e-3, 54363)
+        odd = va.phase_shift(dims, even, 10.0, dt)
+        V = va.trap_potential(dims, origin, b, ratio) * dt
+        lf, lf0, gdt = va.leapfrog_coefs(b, dt, g)
+        secs, threads = orc.leapfrog_timed(even, odd, V, dims, lf, lf0, gdt, warmup, runs, nsteps)
+        nodes = va.nodes_of(dims)
+        med = float(np.median(secs))
+        rate = nodes * nsteps / med
+        grids.append({"grid": label, "cells": list(dims), "nodes": nodes, "steps_per_run": nsteps,
+                      "runs_s": [round(float(x), 4) for x in secs], "median_s": med, "site_steps_per_s": rate,
+                      # the oracle streams prev, next (read + write) and V per node-update: 16 + 32 + 8 bytes
+                      "host_GBs": rate * 2 * 56 / 1e9, "threads": threads})
+        log(f"cpu oracle {label}: {rate / 1e6:.1f} M site-steps/s (median of {runs} x {nsteps} steps, "
+            f"{threads} threads)")
+    main = grids[0]
+    cpu = host_cpu()
+    return {"value": main["site_steps_per_s"], "unit": "site-steps/s", "cores": main["threads"], "kind": "port",
+            "cpu": cpu, "grids": grids,
+            "sample": f"oracle FP64 leapfrog on the benchmark's 128^3-cell grid ({main['nodes']} nodes): "
+                      f"{warmup} warm-up steps, median of {runs} runs of {main['steps_per_run']} steps "
+                      f"({main['median_s']:.2f} s each) on {main['threads']} OpenMP threads; "
+                      f"the 32^3 CPU-form grid timed the same way in 'grids'"}
 
 
 def batched_workload(n_inst, steps, seed=4, log=log, launch="graph", tile=None, group_mib=None):
@@ -141,20 +180,12 @@ def batched_workload(n_inst, steps, seed=4, log=log, launch="graph", tile=None, 
     the reference's stability rule, FP32 bulk; Gaussian + noise initial states (device recipe).
     tile / group_mib fix the real-time tile (waves, cached, slabs) and the group size instead of
     autotuning them."""
-    import vaikkari_amd as va
     from vaikkari_amd.autotune import tune_batch_groups, tune_batch_tile
-    from vaikkari_amd.batch import Batch, instance_config
-    from vaikkari_amd.serve import stable_dt
+    from vaikkari_amd.batch import Batch
 
-    rng = np.random.default_rng(seed)
-    cfgs = []
-    for q in range(n_inst):
-        n = int(rng.choice([24, 32, 48, 64, 96]))
-        g = float(np.exp(rng.uniform(np.log(100.0), np.log(5000.0))))
-        ratio = float(rng.uniform(0.5, 4.0))
-        dt = stable_dt(n, 0.25, g, ratio)
-        cfgs.append(instance_config((n, n, n), 0.25, g, dt, steps, "real", ratio, sigma=1.5, noise=1e-3,
-                                    seed=seed * 100003 + q))
+    from vaikkari_amd import workloads as wl
+
+    cfgs = wl.instance_configs(wl.batched_form(n_inst, steps, seed=seed))
 
     def make(waves, cfg_list=cfgs):
         b = Batch("fp32")
@@ -209,14 +240,15 @@ def batched_workload(n_inst, steps, seed=4, log=log, launch="graph", tile=None, 
 SERVE_CAP_NODES = 32 * 10 ** 6
 
 
-def serving_workload(seconds, policy="shortest_first", cap=SERVE_CAP_NODES, log=log):
+def serving_workload(seconds, policy="shortest_first", cap=SERVE_CAP_NODES, log=log, launch="graph"):
     """Poisson stream (rate 20/s, seed 5; n log-uniform in [16, 160], steps in [500, 20000], 30 % ground
     states), FP32 bulk, 200 GB HBM budget, 50-step chunks, replayed against the wall clock."""
-    from vaikkari_amd.serve import poisson_workload, run_serving
+    from vaikkari_amd import workloads as wl
+    from vaikkari_amd.serve import run_serving
 
-    w = poisson_workload(seed=5, rate=20.0, duration=seconds)
+    w = wl.serving_form(seconds)
     res = run_serving(w, duration=seconds, precision="fp32", hbm_budget_bytes=200 * 10 ** 9, chunk_steps=50,
-                      log=log, policy=policy, max_active_nodes=cap)
+                      log=log, policy=policy, max_active_nodes=cap, launch=launch)
     res["rate_per_s"] = 20.0
     res["hbm_budget_GB"] = 200
     res["chunk_steps"] = 50
@@ -228,7 +260,7 @@ def serving_suite(args):
     admission policy; prints one JSON line with latency percentiles, offered vs served load and the
     queue-depth time series."""
     cap = SERVE_CAP_NODES if args.serve_policy == "shortest_first" else 0
-    res = serving_workload(args.serve_seconds, policy=args.serve_policy, cap=cap)
+    res = serving_workload(args.serve_seconds, policy=args.serve_policy, cap=cap, launch=args.serve_launch)
     print(json.dumps({"suite": "serving", **res}), flush=True)
 
 
@@ -337,42 +369,46 @@ def output_timing(s, dims, reps=5):
 
 
 def precision_study(args):
-    """256^3 cells (201,326,592 nodes), h = 0.06, g = 5000, isotropic trap: Thomas-Fermi profile
-    with a seeded vortex ring, dt = 5e-5; FP32 bulk vs the FP64 path (bit-identical to the CPU
-    oracle) every `every` steps: psi relative L2, norm drift and energy drift."""
+    """The precision form (256^3 cells, 201,326,592 nodes, h = 0.06, g = 5000, isotropic trap,
+    Thomas-Fermi profile with the seed-3 vortex ring, dt = 5e-5): FP32 bulk beside the FP64 path
+    (bit-identical to the CPU oracle) every `every` steps: psi relative L2 (reduced on the device),
+    norm drift and energy drift."""
     import vaikkari_amd as va
+    from vaikkari_amd import workloads as wl
 
-    N = args.precision_cells
-    dims = (N, N, N)
-    h, g, dt = 0.06, 5000.0, 5e-5
-    b = h / math.sqrt(8)
-    origin = va.centered_origin(dims, b)
-    rng = np.random.default_rng(3)
-    psi, mu_tf = va.thomas_fermi(dims, origin, b, g, 1.0)
-    R = float(rng.uniform(2.0, 4.0))
-    ax = rng.normal(size=3)
-    ax /= np.linalg.norm(ax)
-    psi = va.vortex_ring(dims, origin, b, (0.0, 0.0, 0.0), ax, R, psi)
-    log(f"precision: {dims} cells, ring R={R:.3f} axis={ax.round(3).tolist()}, mu_TF={mu_tf:.3f}")
-    s64 = va.Solver(dims, b, dt, g, origin, "fp64", "real", "trap", 1.0)
-    s32 = va.Solver(dims, b, dt, g, origin, "fp32", "real", "trap", 1.0)
+    w = wl.precision_form(args.precision_cells)
+    log(f"precision: {w.dims} cells, ring R={w.ring_radius:.3f} axis={np.round(w.ring_axis, 3).tolist()}, "
+        f"mu_TF={w.mu:.3f}")
+    s64 = va.Solver(w.dims, w.b, w.dt, w.g, w.origin, "fp64", "real", "trap", 1.0)
+    s32 = va.Solver(w.dims, w.b, w.dt, w.g, w.origin, "fp32", "real", "trap", 1.0)
     for s in (s64, s32):
-        s.set_state(psi)
-        s.init_odd(mu_tf)
-    del psi
+        s.set_state(w.psi)
+        s.init_odd(w.mu)
+    rows, ms64, ms32 = precision_checkpoints(s64, s32, args.precision_steps, args.precision_every)
+    nodes = va.nodes_of(w.dims)
+    done = rows[-1]["step"]
+    out = {"suite": "precision", "cells": list(w.dims), "nodes": nodes, "h": w.h, "g": w.g, "dt": w.dt,
+           "steps": args.precision_steps, "ring_radius": w.ring_radius, "ring_axis": list(w.ring_axis),
+           "fp64_site_steps_per_s": nodes * done / (ms64 * 1e-3),
+           "fp32_site_steps_per_s": nodes * done / (ms32 * 1e-3),
+           "checkpoints": rows, "max_psi_rel_l2": max(r["psi_rel_l2"] for r in rows)}
+    print(json.dumps(out), flush=True)
+
+
+def precision_checkpoints(s64, s32, steps, every):
+    """Steps an FP64 and an FP32 solver side by side from the same state; every `every` steps the
+    psi relative L2 error of FP32 against FP64 (gpe_compare_fields: on the device, FP64 sums),
+    and the norm and energy drift of both.  Returns (rows, FP64 device ms, FP32 device ms)."""
     o64, o32 = s64.observables(0), s32.observables(0)
     rows = []
     done = 0
     ms64 = ms32 = 0.0
-    while done < args.precision_steps:
-        k = min(args.precision_every, args.precision_steps - done)
+    while done < steps:
+        k = min(every, steps - done)
         ms64 += s64.time_steps(k)[0]
         ms32 += s32.time_steps(k)[0]
         done += k
-        e64 = s64.get_state(odd=False)
-        e32 = s32.get_state(odd=False)
-        err = float(np.linalg.norm(e32 - e64) / np.linalg.norm(e64))
-        del e32
+        err = s32.compare(s64)["rel_l2"]
         a64, a32 = s64.observables(0), s32.observables(0)
         rows.append({"step": done, "psi_rel_l2": err,
                      "norm_drift_fp64": a64["N"] / o64["N"] - 1, "norm_drift_fp32": a32["N"] / o32["N"] - 1,
@@ -380,13 +416,38 @@ def precision_study(args):
                      "norm_fp32_minus_fp64": a32["N"] - a64["N"], "energy_fp32_minus_fp64": a32["E"] - a64["E"]})
         log(f"precision step {done}: rel-L2 {err:.3e}, N drift fp64 {rows[-1]['norm_drift_fp64']:.2e} "
             f"fp32 {rows[-1]['norm_drift_fp32']:.2e}")
+    return rows, ms64, ms32
+
+
+def dual256_leg(steps, warmup=5):
+    """The dual-node stepper at the precision form's grid (256^3 cells, 201,326,592 nodes, h = 0.06,
+    g = 5000, isotropic trap, dt = 5e-5), FP64 and FP32 bulk, from the device Gaussian (sigma 1.5,
+    1e-3 noise, seed 3, normalised on the grid; no host state): `steps` timed steps each with HIP
+    events on the solver's stream, the hot kernel's average launch time against the roofline."""
+    import vaikkari_amd as va
+
+    n = 256
+    dims, h, g, dt = (n, n, n), 0.06, 5000.0, 5e-5
+    b = h / math.sqrt(8)
     nodes = va.nodes_of(dims)
-    out = {"suite": "precision", "cells": list(dims), "nodes": nodes, "h": h, "g": g, "dt": dt,
-           "steps": args.precision_steps, "ring_radius": R, "ring_axis": ax.tolist(),
-           "fp64_site_steps_per_s": nodes * done / (ms64 * 1e-3),
-           "fp32_site_steps_per_s": nodes * done / (ms32 * 1e-3),
-           "checkpoints": rows, "max_psi_rel_l2": max(r["psi_rel_l2"] for r in rows)}
-    print(json.dumps(out), flush=True)
+    out = {"cells": list(dims), "nodes": nodes, "h": h, "g": g, "dt": dt, "steps": steps}
+    for prec in ("fp64", "fp32"):
+        with va.Solver(dims, b, dt, g, None, prec, "real", "trap", 1.0) as s:
+            s.init_gaussian(1.5, 1e-3, 3)
+            s.step(warmup)
+            s.synchronize()
+            ms, launches = s.time_steps(steps)
+            info = s.info()
+        upd_bytes = info["bytes_per_node_step"] / 2.0 * nodes
+        avg = ms * 1e-3 / launches
+        gbs = upd_bytes / avg / 1e9
+        out[prec] = {"site_steps_per_s": nodes * steps / (ms * 1e-3), "ms_per_step": ms / steps,
+                     "avg_launch_us": avg * 1e6, "achieved_GBs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS,
+                     "kernel": info["kernel"], "tile": info["tile"]}
+        log(f"256^3 {prec}: {out[prec]['site_steps_per_s'] / 1e9:.2f} G site-steps/s, {gbs:.0f} GB/s "
+            f"({gbs / HBM_PEAK_GBS * 100:.1f}% of peak)")
+    out["fp32_vs_fp64"] = out["fp32"]["site_steps_per_s"] / out["fp64"]["site_steps_per_s"]
+    return out
 
 
 def main():
@@ -421,12 +482,12 @@ def main():
               "(the stepper shards by z-slabs across GPUs by design, not built here)", file=sys.stderr)
         sys.exit(2)
     import vaikkari_amd as va
+    from vaikkari_amd import workloads as wl
 
     N = args.cells
-    dims = (N, N, N)
-    h, g, ratio, dt = 0.1, 2000.0, 2.0, 2e-4
-    b = h / math.sqrt(8)
-    origin = va.centered_origin(dims, b)
+    w = wl.drop_in_form(N)
+    dims, h, g, ratio, dt = w.dims, w.h, w.g, w.ratio, w.dt
+    b, origin = w.b, w.origin
     nodes = va.nodes_of(dims)
     log(f"grid {dims} cells, {nodes} nodes")
     copy_gbs = va.copy_bandwidth(1 << 30, 10)
@@ -436,10 +497,8 @@ def main():
     log(f"stencil-pattern stream (read + non-temporal RMW, no stencil): {rmw_gbs:.0f} GB/s; read-only: {read_gbs:.0f} GB/s")
 
     # ---- phase 1: imaginary time, FP64 -----------------------------------
-    psi0 = va.gaussian(dims, origin, b, 1.5, 1e-3, 1)
-    V = va.trap_potential(dims, origin, b, ratio)
-    dtau = 1.0 / (6.0 / b ** 2 + V.max() + g * float(np.max(psi0[0::2] ** 2 + psi0[1::2] ** 2)))
-    del V
+    psi0 = w.initial()
+    dtau = w.dtau(psi0)
     s64 = va.Solver(dims, b, dtau, g, origin, "fp64", "imag", "trap", ratio, args.kernel)
     s64.set_state(psi0)
     ms_imag, _ = s64.time_steps(args.imag_steps)
@@ -448,9 +507,7 @@ def main():
     ground = s64.get_state(odd=False)
 
     # ---- phase 2 input: vortex line at a seeded offset ----------------------
-    rng = np.random.default_rng(2)
-    x0, y0 = rng.uniform(-0.5, 0.5, 2)
-    psi = va.vortex_line(dims, origin, b, x0, y0, ground.copy())
+    psi = w.imprint(ground)
     del ground
     mu = obs["mu"]
 
@@ -481,14 +538,14 @@ def main():
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(nodes, prec, info),
                          "algorithmic_bytes_per_launch": upd_bytes},
-            "state": s.get_state(odd=False),
         }
         log(f"{prec}: {rate / 1e9:.2f} G site-steps/s, {ms / args.steps * 1e3:.1f} us/step, "
             f"{achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS * 100:.1f}% of peak), kernel {info['kernel']}")
         if prec == "fp32":
-            s.close()
-    err = float(np.linalg.norm(results["fp32"]["state"] - results["fp64"]["state"])
-                / np.linalg.norm(results["fp64"]["state"]))
+            s32 = s
+    # psi error of the timed FP32 run against the timed FP64 run (same state, same steps)
+    err = s32.compare(s64)["rel_l2"]
+    s32.close()
     output = output_timing(s64, dims)
     monitor = monitor_timing(s64)
     log(f"monitoring: a step with fused observables costs {monitor['observed_step_extra_us']:.1f} us more than a "
@@ -496,6 +553,26 @@ def main():
     log(f"output: volume map {output['volume_map_call_ms']:.2f} ms, slice {output['slice_picture_call_ms']:.2f} ms "
         f"per call vs {output['field_download_ms']:.1f} ms to download the even field")
     s64.close()
+
+    # FP32 against FP64 over a fixed length, whatever --steps is: both from the phase-2 state,
+    # compared on the device every 250 steps (untimed)
+    c64 = va.Solver(dims, b, dt, g, origin, "fp64", "real", "trap", ratio, args.kernel)
+    c32 = va.Solver(dims, b, dt, g, origin, "fp32", "real", "trap", ratio, args.kernel)
+    for s in (c64, c32):
+        s.set_state(psi)
+        s.init_odd(mu)
+    del psi
+    check_rows, _, _ = precision_checkpoints(c64, c32, args.check_steps, max(1, args.check_steps // 4))
+    c64.close()
+    c32.close()
+    check = {"steps": args.check_steps, "checkpoints": check_rows,
+             "psi_rel_l2": check_rows[-1]["psi_rel_l2"],
+             "max_abs_norm_drift_fp32": max(abs(r["norm_drift_fp32"]) for r in check_rows),
+             "max_abs_energy_drift_fp32": max(abs(r["energy_drift_fp32"]) for r in check_rows),
+             "max_abs_norm_drift_fp64": max(abs(r["norm_drift_fp64"]) for r in check_rows),
+             "max_abs_energy_drift_fp64": max(abs(r["energy_drift_fp64"]) for r in check_rows)}
+
+    dual256 = dual256_leg(args.dual256_steps) if args.dual256_steps > 0 else None
 
     batched = None
     if args.batch_instances > 0:
@@ -506,11 +583,13 @@ def main():
     if args.serve_seconds > 0:
         serving = serving_workload(args.serve_seconds)
         fifo = serving_workload(args.serve_seconds, policy="fifo", cap=0)
-        serving["fifo_uncapped"] = {k: fifo[k] for k in ("completed", "p50_s", "p99_s", "served_site_steps_per_s",
+        serving["fifo_uncapped"] = {k: fifo[k] for k in ("completed", "p50_s", "p99_s", "p50_all_s", "p99_all_s",
+                                                          "censored", "served_site_steps_per_s",
                                                           "active_at_end", "queued_at_end")}
         for name, r in (("shortest-first, capped", serving), ("fifo, uncapped", fifo)):
-            log(f"serving ({name}): {r['completed']} done, p50 {r['p50_s']} s, p99 {r['p99_s']} s, "
-                f"served {r['served_site_steps_per_s'] / 1e9:.1f} G site-steps/s")
+            log(f"serving ({name}): {r['completed']} done, p50 {r['p50_s']} s, p99 {r['p99_s']} s "
+                f"(all {r['submitted']} submitted, unfinished at their age: p50 {r['p50_all_s']} s, "
+                f"p99 {r['p99_all_s']} s), served {r['served_site_steps_per_s'] / 1e9:.1f} G site-steps/s")
 
     primal = None
     if args.primal_cells.strip():
@@ -518,8 +597,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(dims, b, g, dt, ratio, args.cpu_seconds)
-        log(f"cpu oracle: {cpu['value'] / 1e6:.1f} M site-steps/s on {cpu['cores']} threads")
+        cpu = cpu_baseline(args.cpu_warmup, args.cpu_runs, args.cpu_steps)
 
     r64, r32 = results["fp64"], results["fp32"]
     line = {
@@ -545,8 +623,12 @@ def main():
         "cpu_baseline": cpu,
         "fp32_bulk": {
             "value": r32["rate"], "ms_per_step": r32["ms_per_step"], "ratio_vs_fp64": r32["rate"] / r64["rate"],
-            "psi_rel_l2_vs_fp64": err, "roofline": r32["roofline"], "kernel": r32["kernel"],
+            "psi_rel_l2_vs_fp64": err, "psi_rel_l2_note": f"after the {args.warmup + args.steps} steps of the "
+                                                             "timed runs; fixed-length check in fp32_check",
+            "roofline": r32["roofline"], "kernel": r32["kernel"],
         },
+        "fp32_check": check,
+        "dual256": dual256,
         "phase1_imag": {"steps": args.imag_steps, "ms": ms_imag, "site_steps_per_s": nodes * args.imag_steps / (ms_imag * 1e-3),
                         "E": obs["E"], "mu": obs["mu"], "N": obs["N"]},
         "bytes_per_node_step": {"fp64": r64["bytes_per_node_step"], "fp32": r32["bytes_per_node_step"]},

@@ -123,6 +123,14 @@ gpe_status gpe_set_mode(gpe_solver *s, uint32_t mode, double dt);
  * (odd = even * getPhase(-dt/2), GPUGrossPitaevskii_BCC.cu:325,342-343). */
 gpe_status gpe_init_odd(gpe_solver *s, double mu);
 
+/* The even field set on the device, without a host array: the Gaussian
+ * (pi sigma^2)^(-3/4) exp(-r^2/(2 sigma^2)) (1 + noise (xr + i xi)) centred on
+ * 0, xr, xi ~ N(0,1) from a counter-based generator keyed by (seed, node),
+ * scaled to unit discrete norm (FP64 reduction); the same recipe as a batch
+ * instance's GPE_INIT_GAUSSIAN.  Real time: odd = even exp(i mu dt/2) as
+ * gpe_init_odd, with mu = NaN meaning the state's own (K + P + I)/N. */
+gpe_status gpe_init_gaussian(gpe_solver *s, double sigma, double noise, uint64_t seed, double mu);
+
 /* nsteps time steps.  Real time: nsteps x (odd update, even update).
  * Imaginary time: nsteps x (Euler step, FP64 renormalisation). */
 gpe_status gpe_step(gpe_solver *s, uint64_t nsteps);

@@ -31,11 +31,16 @@
  * exported functions take unpadded host arrays in [k][j][i][l] order (class
  * fastest, complex interleaved re,im) and pad internally.
  */
+#define _POSIX_C_SOURCE 199309L /* clock_gettime */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define OR_CLASSES 12
 #define OR_FACES 4
@@ -186,6 +191,40 @@ int or_leapfrog(double *even, double *odd, const double *pot_dt, int X, int Y, i
 }
 
 /* One half-update only (next updated in place), for unit tests. */
+/* CPU-baseline timing of or_leapfrog's stepping loop (test infrastructure: the
+ * reported CPU rate, not a product path): the fields are padded once, nwarm
+ * steps run untimed, then nruns runs of nsteps steps each are timed with the
+ * monotonic clock; seconds[r] receives run r's wall time and *threads the
+ * OpenMP team size used.  The fields are left stepped (not copied back). */
+int or_leapfrog_timed(const double *even, const double *odd, const double *pot_dt, int X, int Y, int Z,
+                      double lapfac, double lapfac0, double g_dt, int nwarm, int nruns, long long nsteps,
+                      double *seconds, int *threads) {
+    or_grid gr = or_mkgrid(X, Y, Z);
+    double *pe = or_pad_complex(even, &gr), *po = or_pad_complex(odd, &gr), *pp = or_pad_real(pot_dt, &gr);
+    if (!pe || !po || !pp) { free(pe); free(po); free(pp); return 1; }
+    for (int s = 0; s < nwarm; s++) {
+        or_update_padded(po, pe, pp, &gr, lapfac, lapfac0, g_dt, 0);
+        or_update_padded(pe, po, pp, &gr, lapfac, lapfac0, g_dt, 0);
+    }
+    for (int r = 0; r < nruns; r++) {
+        struct timespec t0, t1;
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+        for (long long s = 0; s < nsteps; s++) {
+            or_update_padded(po, pe, pp, &gr, lapfac, lapfac0, g_dt, 0);
+            or_update_padded(pe, po, pp, &gr, lapfac, lapfac0, g_dt, 0);
+        }
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        seconds[r] = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+    }
+#ifdef _OPENMP
+    *threads = omp_get_max_threads();
+#else
+    *threads = 1;
+#endif
+    free(pe); free(po); free(pp);
+    return 0;
+}
+
 int or_half_update(double *next, const double *prev, const double *pot_dt, int X, int Y, int Z,
                    double lapfac, double lapfac0, double g_dt, int imag) {
     or_grid gr = or_mkgrid(X, Y, Z);

@@ -28,6 +28,7 @@ def load_oracle():
     o = C.CDLL(ORACLE_SO)
     sig = {
         "or_leapfrog": (_I, [_DP, _DP, _DP, _I, _I, _I, _D, _D, _D, _LL]),
+        "or_leapfrog_timed": (_I, [_DP, _DP, _DP, _I, _I, _I, _D, _D, _D, _I, _I, _LL, _DP, C.POINTER(_I)]),
         "or_half_update": (_I, [_DP, _DP, _DP, _I, _I, _I, _D, _D, _D, _I]),
         "or_observables": (_I, [_DP, _DP, _I, _I, _I, _D, _D, _DP]),
         "or_overlap": (_D, [_DP, _DP, _LL]),
@@ -109,6 +110,17 @@ class Oracle:
         rc = self.o.or_leapfrog(_dp(e), _dp(d), _dp(p), *map(int, dims), lapfac, lapfac0, g_dt, int(nsteps))
         assert rc == 0
         return e, d
+
+    def leapfrog_timed(self, even, odd, pot_dt, dims, lapfac, lapfac0, g_dt, nwarm, nruns, nsteps):
+        """Wall seconds of nruns runs of nsteps leapfrog steps after nwarm untimed ones (fields
+        padded once, outside the timing), and the OpenMP threads used."""
+        secs = np.zeros(int(nruns))
+        th = C.c_int()
+        rc = self.o.or_leapfrog_timed(_dp(np.ascontiguousarray(even)), _dp(np.ascontiguousarray(odd)),
+                                      _dp(np.ascontiguousarray(pot_dt)), *map(int, dims), lapfac, lapfac0, g_dt,
+                                      int(nwarm), int(nruns), int(nsteps), _dp(secs), C.byref(th))
+        assert rc == 0
+        return secs, th.value
 
     def half_update(self, nxt, prev, pot_dt, dims, lapfac, lapfac0, g_dt, imag):
         n = np.array(nxt, dtype=np.float64, copy=True)

@@ -412,3 +412,33 @@ def test_imag_then_real_workflow(orc):
     np.testing.assert_array_equal(re, oe2)
     np.testing.assert_array_equal(ro, oo2)
     s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_device_gaussian_init_equals_batch_recipe(precision):
+    """gpe_init_gaussian sets the even field with the batch's device recipe, normalised on the grid:
+    bit-identical to a batch instance of the same parameters, unit norm, and odd = even turned by
+    the state's own chemical potential exactly as gpe_init_odd turns it."""
+    from vaikkari_amd.batch import Batch, instance_config
+    from vaikkari_amd.serve import stable_dt
+
+    announce(f"device gaussian init {precision}")
+    dims, h, g, ratio = (21, 17, 13), 0.3, 700.0, 1.7
+    b = h / math.sqrt(8)
+    dt = stable_dt(13, h, g, ratio)
+    s = va.Solver(dims, b, dt, g, None, precision, "real", "trap", ratio)
+    s.init_gaussian(1.5, 1e-3, 99)
+    se, so = s.get_state()
+    B = Batch(precision)
+    i = B.add(instance_config(dims, h, g, dt, 10, "real", ratio, sigma=1.5, noise=1e-3, seed=99))
+    be, _ = B.get_state(i)
+    B.close()
+    np.testing.assert_array_equal(se, be)
+    assert va.field_norm(dims, b, se) == pytest.approx(1.0, abs=1e-13 if precision == "fp64" else 1e-6)
+    mu = s.observables(0)["mu"]
+    ref = va.phase_shift(dims, se, mu, dt)
+    if precision == "fp64":
+        np.testing.assert_array_equal(so, ref)
+    else:
+        assert rel_l2(so, ref) < 1e-6
+    s.close()

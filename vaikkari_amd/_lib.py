@@ -199,6 +199,7 @@ SIGNATURES = [
     ("gpe_get_state", _I, [_P, _DP, _DP]),
     ("gpe_set_mode", _I, [_P, _U32, _D]),
     ("gpe_init_odd", _I, [_P, _D]),
+    ("gpe_init_gaussian", _I, [_P, _D, _D, _U64, _D]),
     ("gpe_step", _I, [_P, _U64]),
     ("gpe_time_steps", _I, [_P, _U64, _DP, C.POINTER(_U64)]),
     ("gpe_observables", _I, [_P, _I, _DP]),

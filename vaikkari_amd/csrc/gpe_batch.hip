@@ -221,48 +221,6 @@ __global__ void k_batch_advance(InstDesc<T> *__restrict__ R, InstDesc<T> *__rest
     d->parity = (int)((d->parity + steps) & 1);
 }
 
-// Counter-based N(0,1) pair for node n: splitmix64 of (seed, n), Box-Muller.
-__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
-    x += 0x9E3779B97F4A7C15ULL;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
-    return x ^ (x >> 31);
-}
-
-template <typename T>
-__global__ void k_init_gaussian(cplx_t<T> *__restrict__ even, cplx_t<T> *__restrict__ odd, Layout L, Trap trap,
-                                double inv2s2, double amp, double noise, unsigned long long seed, double cph,
-                                double sph) {
-    const long long cells = (long long)L.nx * L.ny * L.nz;
-    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < cells;
-         c += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(c % L.nx), j = (int)((c / L.nx) % L.ny), k = (int)(c / ((long long)L.nx * L.ny));
-        const long long base = L.origin + (long long)k * L.sz + (long long)j * L.sy + i;
-        for (int l = 0; l < 12; l++) {
-            const double x = trap.p0[0] + trap.b * ((double)(unsigned)i * trap.bw + trap.bpos[l][0]);
-            const double y = trap.p0[1] + trap.b * ((double)(unsigned)j * trap.bw + trap.bpos[l][1]);
-            const double z = trap.p0[2] + trap.b * ((double)(unsigned)k * trap.bw + trap.bpos[l][2]);
-            const double a = amp * exp(-(x * x + y * y + z * z) * inv2s2);
-            const unsigned long long h1 = splitmix64(seed ^ (unsigned long long)(c * 12 + l) * 0xD1B54A32D192ED03ULL);
-            const unsigned long long h2 = splitmix64(h1);
-            const double u1 = ((double)(h1 >> 11) + 1.0) * (1.0 / 9007199254740992.0);
-            const double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);
-            const double r = sqrt(-2.0 * log(u1));
-            const double xr = r * cos(6.2831853071795864770 * u2), xi = r * sin(6.2831853071795864770 * u2);
-            const double er = a * (1.0 + noise * xr), ei = a * (noise * xi);
-            cplx_t<T> e, o;
-            e.x = (T)er;
-            e.y = (T)ei;
-            even[base + l * L.xp] = e;
-            if (odd) {
-                o.x = (T)(cph * er - sph * ei);
-                o.y = (T)(sph * er + cph * ei);
-                odd[base + l * L.xp] = o;
-            }
-        }
-    }
-}
-
 }  // namespace gpe
 
 namespace {

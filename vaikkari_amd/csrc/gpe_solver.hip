@@ -678,6 +678,27 @@ static gpe_status observables_noalloc_t(const gpe_config &c, const Layout &L, co
     return GPE_OK;
 }
 
+template <typename T>
+static gpe_status init_gaussian_t(gpe_solver *s, double sigma, double noise, uint64_t seed) {
+    using C = cplx_t<T>;
+    const double amp = std::pow(3.14159265358979323846 * sigma * sigma, -0.75);
+    hipLaunchKernelGGL((k_init_gaussian<T>), dim3(1024), dim3(256), 0, s->stream, (C *)s->field[s->ie], (C *)nullptr,
+                       s->L, s->trap, 1.0 / (2.0 * sigma * sigma), amp, noise, (unsigned long long)seed, 1.0, 0.0);
+    GPE_HIP_TRY(hipGetLastError());
+    double obs[4];
+    gpe_status st = s->observables_t<T>(s->ie, obs);
+    if (st != GPE_OK) return st;
+    if (!(obs[0] > 0) || !std::isfinite(obs[0])) {
+        set_error("gpe_init_gaussian: the Gaussian has no positive finite norm on this grid");
+        return GPE_EUNSTABLE;
+    }
+    hipLaunchKernelGGL((k_scale_const<T>), dim3(1024), dim3(256), 0, s->stream, (C *)s->field[s->ie], s->L.elems,
+                       1.0 / std::sqrt(obs[0]));
+    GPE_HIP_TRY(hipGetLastError());
+    GPE_HIP_TRY(hipStreamSynchronize(s->stream));
+    return GPE_OK;
+}
+
 // ---------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------
@@ -803,6 +824,32 @@ gpe_status gpe_init_odd(gpe_solver *s, double mu) {
     GPE_HIP_TRY(hipStreamSynchronize(s->stream));
     s->odd_valid = true;
     return GPE_OK;
+}
+
+gpe_status gpe_init_gaussian(gpe_solver *s, double sigma, double noise, uint64_t seed, double mu) {
+    if (!s || !(sigma > 0) || !std::isfinite(sigma) || !std::isfinite(noise) || std::isinf(mu)) {
+        set_error("gpe_init_gaussian: invalid argument");
+        return GPE_EINVAL;
+    }
+    GPE_HIP_TRY(hipStreamSynchronize(s->stream));
+    const gpe_status st = s->cfg.precision == GPE_FP64 ? init_gaussian_t<double>(s, sigma, noise, seed)
+                                                       : init_gaussian_t<float>(s, sigma, noise, seed);
+    if (st != GPE_OK) return st;
+    const double one = 1.0;
+    GPE_HIP_TRY(hipMemcpy(s->scal + 1, &one, sizeof(double), hipMemcpyHostToDevice));
+    s->odd_valid = false;
+    if (s->cfg.mode != GPE_REAL_TIME) return GPE_OK;
+    if (std::isnan(mu)) {
+        double obs[4];
+        const gpe_status so = gpe_observables(s, 0, obs);
+        if (so != GPE_OK) return so;
+        mu = (obs[1] + obs[2] + obs[3]) / obs[0];
+        if (!std::isfinite(mu)) {
+            set_error("gpe_init_gaussian: the state has no finite chemical potential");
+            return GPE_EUNSTABLE;
+        }
+    }
+    return gpe_init_odd(s, mu);
 }
 
 gpe_status gpe_step(gpe_solver *s, uint64_t nsteps) {

@@ -137,6 +137,11 @@ class Solver:
     def init_odd(self, mu: float) -> None:
         check(lib.gpe_init_odd(self._h, float(mu)), "gpe_init_odd")
 
+    def init_gaussian(self, sigma: float = 1.5, noise: float = 1e-3, seed: int = 0, mu: float = float("nan")) -> None:
+        """Even field = the device Gaussian recipe normalised on the grid (no host array); in real
+        time odd = even exp(i mu dt/2), mu NaN = the state's own chemical potential."""
+        check(lib.gpe_init_gaussian(self._h, float(sigma), float(noise), int(seed), float(mu)), "gpe_init_gaussian")
+
     # -- stepping --------------------------------------------------------
     def step(self, nsteps: int) -> None:
         check(lib.gpe_step(self._h, int(nsteps)), "gpe_step")
