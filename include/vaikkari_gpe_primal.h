@@ -68,11 +68,19 @@ gpe_status gpe_primal_synchronize(gpe_primal *p);
  * read-modify-write, and how a lane gets its x-neighbours (DPP lane shift,
  * cross-lane shuffle or a second load); tiles 21-25 fuse both half-updates
  * of a real-time step into one out-of-place pass (temporal blocking, two
- * more field buffers).  Every shape gives the same bits.
+ * more field buffers); tiles 26-42 are z-marches that address the fields
+ * through buffer resources (32-bit offsets: fields below 2 GiB, else
+ * GPE_EINVAL at creation), with one (26-31, 37-42) or two (32-36) layers of
+ * prefetch; tiles 37-42 hold two cells per lane in FP32 (16-byte accesses;
+ * FP64 runs them one cell per lane).  In FP64 every shape gives the same
+ * bits; in FP32 the buffer tiles add V*dt as an x-y part and a z part, each
+ * rounded to FP32.
  * Tile 0 in the config picks by size: when both fields fit in 192 MB (the
- * Infinity Cache holds them), one layer per block with a cached
- * read-modify-write (tile 9 FP64, tile 11 FP32); otherwise the z-march of
- * 16 layers (tile 13 FP64, tile 19 FP32). */
+ * Infinity Cache holds them) a cached read-modify-write (tile 9 FP64: one
+ * layer per block; tile 41 FP32: buffer z-march, two cells per lane); below
+ * 2 GiB per field the buffer z-march of 16 layers (tile 28 FP64, tile 37
+ * FP32); larger fields the z-march on 64-bit addresses (tile 13 FP64, tile
+ * 19 FP32). */
 int gpe_primal_tiles(void);
 
 /* Host helpers, the arithmetic the kernels use. */

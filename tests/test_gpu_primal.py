@@ -20,6 +20,8 @@ from oracle_lib import PrimalOracle
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(37, 19, 11), (64, 32, 6), (96, 8, 5), (5, 3, 2)]
+# the z-march tiles with buffer-resource addressing (26.. in the tile table)
+BUF_TILES = list(range(26, va.primal_tiles() + 1))
 # degenerate grids: one cell, single rows and columns
 TINY = [(1, 1, 1), (2, 1, 3), (17, 1, 1), (1, 9, 2)]
 
@@ -58,9 +60,8 @@ def setup(dims, a, dt, g, ratio):
 
 
 @pytest.mark.parametrize("dims", SHAPES)
-@pytest.mark.parametrize("tile", range(1, 26))
+@pytest.mark.parametrize("tile", range(1, va.primal_tiles() + 1))
 def test_fp64_leapfrog_bit_exact(po, dims, tile):
-    assert va.primal_tiles() == 25
     a, dt, g, ratio = 0.35, 0.004, 1.3, 1.3
     origin, V, coef = setup(dims, a, dt, g, ratio)
     even, odd = random_fields(dims, seed=11 + tile)
@@ -91,7 +92,7 @@ def test_fp64_degenerate_grids_bit_exact(po, dims, tile):
 
 
 @pytest.mark.parametrize("dims", SHAPES[:3])
-@pytest.mark.parametrize("tile", [1, 2, 13, 21])
+@pytest.mark.parametrize("tile", [1, 2, 13, 21] + BUF_TILES[::5])
 def test_fp64_imaginary_time(po, dims, tile):
     a, dt, g, ratio = 0.35, 0.004, 1.3, 1.3
     origin, V, coef = setup(dims, a, dt, g, ratio)
@@ -123,7 +124,8 @@ def test_fp64_observables(po, dims):
                 assert abs(got[key] - want[q]) <= 1e-12 * abs(want[q]) + 1e-300, (key, got[key], want[q])
 
 
-def test_fp32_observables_and_imaginary_time(po):
+@pytest.mark.parametrize("tile", [0] + BUF_TILES[::5])
+def test_fp32_observables_and_imaginary_time(po, tile):
     """FP32 fields: the observables reduce in FP64, so on FP32-rounded input they
     match the oracle to <= 1e-12 relative; 20 imaginary-time steps stay within
     1e-5 relative L2 of the FP64 oracle and end normalised."""
@@ -131,8 +133,8 @@ def test_fp32_observables_and_imaginary_time(po):
     a, dt, g, ratio = 0.35, 0.004, 1.3, 1.3
     origin, V, coef = setup(dims, a, dt, g, ratio)
     psi = gaussian(dims, origin, a).astype(np.float32).astype(np.float64)
-    announce("primal fp32 observables + imag")
-    with va.PrimalSolver(dims, a, dt, g, origin, ratio, "fp32", "imag") as s:
+    announce(f"primal fp32 observables + imag tile={tile}")
+    with va.PrimalSolver(dims, a, dt, g, origin, ratio, "fp32", "imag", tile) as s:
         s.set_state(psi)
         got = s.observables(0)
         want = po.observables(psi, V, dims, a, g)
@@ -159,7 +161,7 @@ def test_init_odd_phase():
     np.testing.assert_allclose(odd.view(np.complex128), want, rtol=1e-15, atol=1e-18)
 
 
-@pytest.mark.parametrize("tile", [1, 2, 13, 21])
+@pytest.mark.parametrize("tile", [1, 2, 13, 21] + BUF_TILES)
 def test_fp32_leapfrog_tolerance(po, tile):
     """FP32 storage and stencil (difference form) against the FP64 oracle:
     relative L2 error <= 1e-5 after 3 leapfrog steps."""

@@ -25,10 +25,12 @@
 
 namespace gpe {
 
-// lane i <- lane i-1 (value of the x-1 neighbour); lane 0 gets 0.
-__device__ __forceinline__ int dpp_from_xm(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+// lane i <- lane i-1 (value of the x-1 neighbour); lane 0 gets 0 (bound
+// control: a lane whose source is outside the wave reads zero, so no old
+// value has to be set first).
+__device__ __forceinline__ int dpp_from_xm(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
 // lane i <- lane i+1 (value of the x+1 neighbour); lane 63 gets 0.
-__device__ __forceinline__ int dpp_from_xp(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_from_xp(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
 
 template <bool FROM_XM>
 __device__ __forceinline__ float shift_lane(float v) {

@@ -77,7 +77,11 @@ struct PCoef {
 //            sums left to right from the first element.
 //  FORM_DIFF (FP32): cD*sum_8 (d - psi) + cA*sum_6 (x - psi) + (V + g|psi|^2) psi,
 //            algebraically equal (c0 = -(8 cD + 6 cA)) and free of the
-//            cancellation between the neighbour sums and c0 psi.
+//            cancellation between the neighbour sums and c0 psi; the products
+//            are fused multiply-adds.
+__device__ __forceinline__ float ffma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double ffma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename T, int FORM>
 __device__ __forceinline__ cplx_t<T> primal_apply(const cplx_t<T> (&d)[8], const cplx_t<T> (&x)[6], cplx_t<T> p,
                                                   T pot, const PCoef<T> &H) {
@@ -116,7 +120,13 @@ __device__ __forceinline__ cplx_t<T> primal_apply(const cplx_t<T> (&d)[8], const
             ar = ar + (x[q].x - p.x);
             ai = ai + (x[q].y - p.y);
         }
-        c = pot + H.g * nsq;
+        // FP32: fused multiply-adds (one rounding fewer per product, no bit
+        // target to keep)
+        const T n2 = ffma(p.x, p.x, p.y * p.y);
+        c = ffma(H.g, n2, pot);
+        s.x = ffma(c, p.x, ffma(H.cD, dr, H.cA * ar));
+        s.y = ffma(c, p.y, ffma(H.cD, di, H.cA * ai));
+        return s;
     }
     s.x = (H.cD * dr + H.cA * ar) + c * p.x;
     s.y = (H.cD * di + H.cA * ai) + c * p.y;
@@ -491,6 +501,480 @@ __global__ void __launch_bounds__(64 * TY) k_primal_zm(cplx_t<T> *__restrict__ n
     }
 }
 
+// Z-MARCH with buffer-resource addressing: the same column walk, layer
+// order, prefetch and arithmetic as primal_column, but every access is a raw
+// buffer load or store with one per-lane 32-bit byte offset (the column) and a
+// wave-uniform one (layer and row, advanced by scalar adds), so a row access
+// costs no 64-bit vector address arithmetic.  Fields must be smaller than
+// 2 GiB (the host checks).  Lanes without a cell store to an offset beyond the
+// buffer, which the buffer's range check discards, so stores need no branch.
+// FP32 (FORM_DIFF) takes V*dt as (0.5 (x^2 + y^2) dt) of the column plus
+// (0.5 rr z^2 dt) of the layer from a per-layer table, both rounded to FP32
+// (about 1 ulp from the FP64 expression); FP64 evaluates site_v exactly as
+// primal_column, so its results keep the same bits.
+template <typename T>
+struct BufC;
+template <>
+struct BufC<float> {
+    template <int AUX>
+    static __device__ __forceinline__ cplx_t<float> ld(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        return __builtin_bit_cast(cplx_t<float>, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, AUX));
+    }
+    template <int AUX>
+    static __device__ __forceinline__ void st(cplx_t<float> x, __amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        __builtin_amdgcn_raw_buffer_store_b64(
+            __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), x), r, v, s, AUX);
+    }
+};
+template <>
+struct BufC<double> {
+    template <int AUX>
+    static __device__ __forceinline__ cplx_t<double> ld(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        return __builtin_bit_cast(cplx_t<double>, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, AUX));
+    }
+    template <int AUX>
+    static __device__ __forceinline__ void st(cplx_t<double> x, __amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        __builtin_amdgcn_raw_buffer_store_b128(
+            __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0)), x), r, v, s, AUX);
+    }
+};
+constexpr int kBufNT = 2;             // cache-policy bits of a non-temporal buffer access
+constexpr int kBufWord3 = 0x00020000;  // raw buffer, 32-bit data format
+constexpr uint32_t kBufDiscard = 0x80000000u;  // byte offset beyond every buffer this kernel addresses
+
+template <typename T, int FORM, int MODE, int TY, int CZ, bool NT, bool LY, int PD>
+__device__ __forceinline__ double primal_column_b(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                                  const Layout &L, const PCoef<T> &H, const PTrap &trap, double dt,
+                                                  const float *__restrict__ vzt, const double *__restrict__ scale,
+                                                  int bx, int by, int bz, int lsh) {
+#pragma clang fp contract(off)
+    using C = cplx_t<T>;
+    constexpr uint32_t ES = sizeof(C);
+    const int bytes = (int)((long long)L.elems * ES);
+    const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc((void *)prev, (short)0, bytes, kBufWord3);
+    const __amdgpu_buffer_rsrc_t rn = __builtin_amdgcn_make_buffer_rsrc((void *)next, (short)0, bytes, kBufWord3);
+    const int lw = 1 << lsh;
+    const int xl = threadIdx.x & (lw - 1);
+    const int i = (bx << lsh) + xl;
+    const int j = ((by * TY + threadIdx.y) << (6 - lsh)) + (threadIdx.x >> lsh);
+    const int k0 = bz * CZ;
+    const int k1 = min(k0 + CZ, L.nz);
+    const bool act = i < L.nx && j < L.ny;
+    const bool left_end = act && xl == 0, right_end = act && xl == lw - 1;
+    const uint32_t xp = (uint32_t)L.xp, sy = (uint32_t)L.sy, sz = (uint32_t)L.sz;
+    const int ie = act ? i : -1;
+    const int je = act ? j : min(j, L.ny - 1);
+    // element of row (dz, dy, s) of layer k: cb + k sz + R(dz, dy, s), every term >= 0
+    const uint32_t cb = (uint32_t)(L.origin - L.sz - L.sy + (long long)je * L.sy + ie);
+    auto R = [&](int dz, int dy, int sl) -> uint32_t {
+        return (uint32_t)(dz + 1) * sz + (uint32_t)(dy + 1) * sy + (uint32_t)sl * xp;
+    };
+    // the same row offsets as wave-uniform values: the compiler may share an
+    // R(...) with the per-lane offsets below, and a row offset it holds in a
+    // vector register would turn each load into a loop over distinct values
+    auto RS = [&](int dz, int dy, int sl) -> uint32_t { return __builtin_amdgcn_readfirstlane(R(dz, dy, sl)); };
+    const uint32_t vcol = cb * ES;                              // per-lane byte offset
+    const uint32_t vst = act ? vcol : kBufDiscard;              // stores of lanes without a cell are discarded
+    // row-end loads: per-lane element deltas (left end: x-1 values, right end:
+    // x+1 values, other lanes their own corner site)
+    const uint32_t own = R(0, 0, 0);
+    const uint32_t vt0 = (cb + (left_end ? R(0, -1, 1) - 1 : right_end ? R(1, 0, 0) + 1 : own)) * ES;
+    const uint32_t vt1 = (cb + (left_end ? R(0, 0, 1) - 1 : right_end ? R(1, 1, 0) + 1 : own)) * ES;
+    const uint32_t vt2 = (cb + (left_end ? R(0, 0, 0) - 1 : right_end ? R(0, 0, 1) + 1 : own)) * ES;
+    const uint32_t vi0 = (cb + (left_end ? R(-1, -1, 1) - 1 : right_end ? R(0, 1, 0) + 1 : own)) * ES;
+    const uint32_t vi1 = (cb + (left_end ? R(-1, 0, 1) - 1 : right_end ? R(0, 0, 0) + 1 : own)) * ES;
+    T sc = (T)1;
+    if (MODE == MODE_IMAG) sc = (T)(*scale);
+    auto ldv = [&](uint32_t v, uint32_t soff) {
+        C x = BufC<T>::template ld<0>(rp, v, soff);
+        if (MODE == MODE_IMAG) x = cscale<T>(x, sc);
+        return x;
+    };
+    auto ld = [&](uint32_t lay, int dz, int dy, int sl) { return ldv(vcol, (lay + RS(dz, dy, sl)) * ES); };
+    auto fetch = [&](uint32_t lay, ZLayer<T> &f) {  // lay = k sz of the layer
+        f.czp = ld(lay, 1, 0, 0);
+        f.v3 = ld(lay, 1, 1, 0);
+        f.zzp = ld(lay, 1, 0, 1);
+        if (!LY) {
+            f.cym = ld(lay, 0, -1, 0);
+            f.u2 = ld(lay, 0, -1, 1);
+            f.zyp = ld(lay, 0, 1, 1);
+        }
+        f.t0 = ldv(vt0, lay * ES);
+        f.t1 = ldv(vt1, lay * ES);
+        f.t2 = ldv(vt2, lay * ES);
+        if (MODE == MODE_REAL) {
+            f.n0 = BufC<T>::template ld<NT ? kBufNT : 0>(rn, vcol, (lay + RS(0, 0, 0)) * ES);
+            f.n1 = BufC<T>::template ld<NT ? kBufNT : 0>(rn, vcol, (lay + RS(0, 0, 1)) * ES);
+        }
+    };
+    uint32_t lay = (uint32_t)k0 * sz;
+    C c0 = ld(lay, 0, 0, 0), c1 = ld(lay, 0, 0, 1), czm = ld(lay, -1, 0, 0), u1 = ld(lay, -1, 0, 1),
+      u0 = ld(lay, -1, -1, 1);
+    C cyp = ld(lay, 0, 1, 0);
+    const C t0 = ldv(vi0, lay * ES);
+    const C t1 = ldv(vi1, lay * ES);
+    // PD + 1 layer buffers used in turn: the loads of layer k + PD are issued
+    // while layer k is computed (the loop is unrolled, so buf[kk % (PD + 1)]
+    // names fixed registers)
+    ZLayer<T> buf[PD + 1];
+#pragma unroll
+    for (int q = 0; q < PD; q++) fetch(lay + (uint32_t)q * sz, buf[q]);
+    C w0 = shift_lane_c<true>(u0), w1 = shift_lane_c<true>(u1);
+    C e1 = shift_lane_c<false>(cyp), xq = shift_lane_c<false>(c0);
+    if (left_end) {
+        w0 = t0;
+        w1 = t1;
+    }
+    if (right_end) {
+        e1 = t0;
+        xq = t1;
+    }
+    const double xy0 = site_xy(trap, i, j, 0.0), xy1 = site_xy(trap, i, j, 0.5);
+    const T vxy0 = (T)(0.5 * xy0 * dt), vxy1 = (T)(0.5 * xy1 * dt);
+    double part = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < CZ; kk++, lay += sz) {
+        const int k = k0 + kk;
+        if (k >= k1) break;
+        ZLayer<T> &cur = buf[kk % (PD + 1)];
+        if (LY) {
+            cur.cym = ld(lay, 0, -1, 0);
+            cur.u2 = ld(lay, 0, -1, 1);
+            cur.zyp = ld(lay, 0, 1, 1);
+        }
+        // the prefetch also runs at the chunk's last layers (a branch here would
+        // make the compiler merge the buffers with register copies): beyond the
+        // halo layer nz + 1 the buffer's range check returns zeros, never used
+        fetch(lay + (uint32_t)PD * sz, buf[(kk + PD) % (PD + 1)]);
+        __builtin_amdgcn_sched_barrier(0);
+        C w2 = shift_lane_c<true>(cur.u2), w3 = shift_lane_c<true>(c1), xm = shift_lane_c<true>(c0);
+        C e2 = shift_lane_c<false>(cur.czp), e3 = shift_lane_c<false>(cur.v3), zxp = shift_lane_c<false>(c1);
+        if (left_end) {
+            w2 = cur.t0;
+            w3 = cur.t1;
+            xm = cur.t2;
+        }
+        if (right_end) {
+            e2 = cur.t0;
+            e3 = cur.t1;
+            zxp = cur.t2;
+        }
+        T v0, v1;
+        if (FORM == FORM_REF) {
+            v0 = (T)(site_v_xy(trap, xy0, k, 0.0) * dt);
+            v1 = (T)(site_v_xy(trap, xy1, k, 0.5) * dt);
+        } else {
+            v0 = vxy0 + (T)vzt[2 * k];
+            v1 = vxy1 + (T)vzt[2 * k + 1];
+        }
+        const C dC[8] = {w0, u0, w1, u1, w2, cur.u2, w3, c1};
+        const C aC[6] = {xm, xq, cur.cym, cyp, czm, cur.czp};
+        const C s0 = primal_apply<T, FORM>(dC, aC, c0, v0, H);
+        const C dZ[8] = {c0, xq, cyp, e1, cur.czp, e2, cur.v3, e3};
+        const C aZ[6] = {w3, zxp, cur.u2, cur.zyp, u1, cur.zzp};
+        const C s1 = primal_apply<T, FORM>(dZ, aZ, c1, v1, H);
+        C o0, o1;
+        if (MODE == MODE_REAL) {
+            o0.x = cur.n0.x + s0.y;
+            o0.y = cur.n0.y + (-s0.x);
+            o1.x = cur.n1.x + s1.y;
+            o1.y = cur.n1.y + (-s1.x);
+            BufC<T>::template st<NT ? kBufNT : 0>(o0, rn, vst, (lay + RS(0, 0, 0)) * ES);
+            BufC<T>::template st<NT ? kBufNT : 0>(o1, rn, vst, (lay + RS(0, 0, 1)) * ES);
+        } else {
+            o0.x = c0.x - s0.x;
+            o0.y = c0.y - s0.y;
+            o1.x = c1.x - s1.x;
+            o1.y = c1.y - s1.y;
+            BufC<T>::template st<kBufNT>(o0, rn, vst, (lay + RS(0, 0, 0)) * ES);
+            BufC<T>::template st<kBufNT>(o1, rn, vst, (lay + RS(0, 0, 1)) * ES);
+            if (act)
+                part += ((double)o0.x * (double)o0.x + (double)o0.y * (double)o0.y) +
+                        ((double)o1.x * (double)o1.x + (double)o1.y * (double)o1.y);
+        }
+        czm = c0;
+        c0 = cur.czp;
+        u1 = c1;
+        c1 = cur.zzp;
+        u0 = cur.u2;
+        w0 = w2;
+        w1 = w3;
+        cyp = cur.v3;
+        e1 = e3;
+        xq = e2;
+    }
+    return part;
+}
+
+template <typename T, int FORM, int MODE, int TY, int CZ, bool NT, bool LY, int PD>
+__global__ void __launch_bounds__(64 * TY)
+    k_primal_zb(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev, Layout L, PCoef<T> H, PTrap trap,
+                double dt, const float *__restrict__ vzt, const double *__restrict__ scale,
+                double *__restrict__ partials, Grid3 G) {
+    __shared__ double lds[TY];
+    int bx, by, bz;
+    block_coords(G, bx, by, bz);
+    double part[1];
+    part[0] = primal_column_b<T, FORM, MODE, TY, CZ, NT, LY, PD>(next, prev, L, H, trap, dt, vzt, scale, bx, by, bz,
+                                                                 G.lsh);
+    if (MODE == MODE_IMAG) {
+        block_sum<1, TY>(part, lds);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partials[bx + G.gx * (by + G.gy * bz)] = part[0];
+    }
+}
+
+// Z-MARCH, FP32, two cells per lane: as primal_column_b (buffer addressing,
+// one layer of prefetch, the same neighbour order and arithmetic per site),
+// but lane l of a wave holds the cells 2l and 2l + 1 of a 128-cell row, so
+// every row access is one 16-byte load or store per lane: half the
+// vector-memory instructions, and half the texture-address work, of one cell
+// per lane (the FP32 z-march is bound by that unit: TA busy about 90 % of
+// the kernel time at 256^3).  x-neighbours inside the lane are the other
+// cell, across lanes one DPP wave shift; the row-end lanes load the cell
+// beyond their end.  A lane whose second cell is the zero halo column x = nx
+// stores zero there; a lane whose first cell is beyond the grid discards
+// its stores.  Wave rows always span 128 cells (lsh = 6).
+template <typename T>
+struct CP {
+    cplx_t<T> a, b;  // cells 2l, 2l + 1
+};
+__device__ __forceinline__ CP<float> ld_pair(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+    return __builtin_bit_cast(CP<float>, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
+}
+template <int AUX>
+__device__ __forceinline__ CP<float> ld_pair_aux(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+    return __builtin_bit_cast(CP<float>, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, AUX));
+}
+template <int AUX>
+__device__ __forceinline__ void st_pair(CP<float> x, __amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+    __builtin_amdgcn_raw_buffer_store_b128(
+        __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0)), x), r, v, s, AUX);
+}
+// x-neighbours of both cells: from the left (x - 1) and from the right (x + 1)
+__device__ __forceinline__ CP<float> pair_from_left(const CP<float> &v) {
+    CP<float> o;
+    o.a = shift_lane_c<true>(v.b);
+    o.b = v.a;
+    return o;
+}
+__device__ __forceinline__ CP<float> pair_from_right(const CP<float> &v) {
+    CP<float> o;
+    o.a = v.b;
+    o.b = shift_lane_c<false>(v.a);
+    return o;
+}
+
+struct ZLayer2 {
+    CP<float> czp, v3, zzp, cym, u2, zyp;  // rows of prev
+    cplx_t<float> t0, t1, t2;              // row-end x-neighbours (one cell)
+    CP<float> n0, n1;                      // next, for the read-modify-write
+};
+
+template <int MODE, int TY, int CZ, bool NT, bool LY>
+__device__ __forceinline__ double primal_column_p(cplx_t<float> *__restrict__ next,
+                                                  const cplx_t<float> *__restrict__ prev, const Layout &L,
+                                                  const PCoef<float> &H, const PTrap &trap, double dt,
+                                                  const float *__restrict__ vzt, const double *__restrict__ scale,
+                                                  int bx, int by, int bz) {
+#pragma clang fp contract(off)
+    using C = cplx_t<float>;
+    using P = CP<float>;
+    constexpr uint32_t ES = sizeof(C);
+    const int bytes = (int)((long long)L.elems * ES);
+    const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc((void *)prev, (short)0, bytes, kBufWord3);
+    const __amdgpu_buffer_rsrc_t rn = __builtin_amdgcn_make_buffer_rsrc((void *)next, (short)0, bytes, kBufWord3);
+    const int lane = threadIdx.x;
+    const int i = bx * 128 + 2 * lane;  // first cell of the lane
+    const int j = by * TY + threadIdx.y;
+    const int k0 = bz * CZ;
+    const int k1 = min(k0 + CZ, L.nz);
+    const bool row = j < L.ny;
+    const bool actA = row && i < L.nx, actB = row && i + 1 < L.nx;
+    const bool left_end = row && lane == 0, right_end = row && lane == 63;
+    const uint32_t xp = (uint32_t)L.xp, sy = (uint32_t)L.sy, sz = (uint32_t)L.sz;
+    const int je = row ? j : L.ny - 1;
+    // lanes past the row end read whatever follows it (inside the buffer, or
+    // zeros past its end): those values reach only cells beyond the grid
+    const uint32_t cb = (uint32_t)(L.origin - L.sz - L.sy + (long long)je * L.sy + i);
+    auto R = [&](int dz, int dy, int sl) -> uint32_t {
+        return (uint32_t)(dz + 1) * sz + (uint32_t)(dy + 1) * sy + (uint32_t)sl * xp;
+    };
+    auto RS = [&](int dz, int dy, int sl) -> uint32_t { return __builtin_amdgcn_readfirstlane(R(dz, dy, sl)); };
+    const uint32_t vcol = cb * ES;
+    const uint32_t vst = actA ? vcol : kBufDiscard;
+    // row-end loads (one cell): left end x - 1 of its first cell, right end
+    // x + 1 of its second; other lanes their own first cell
+    const uint32_t own = R(0, 0, 0);
+    const uint32_t vt0 = (cb + (left_end ? R(0, -1, 1) - 1 : right_end ? R(1, 0, 0) + 2 : own)) * ES;
+    const uint32_t vt1 = (cb + (left_end ? R(0, 0, 1) - 1 : right_end ? R(1, 1, 0) + 2 : own)) * ES;
+    const uint32_t vt2 = (cb + (left_end ? R(0, 0, 0) - 1 : right_end ? R(0, 0, 1) + 2 : own)) * ES;
+    const uint32_t vi0 = (cb + (left_end ? R(-1, -1, 1) - 1 : right_end ? R(0, 1, 0) + 2 : own)) * ES;
+    const uint32_t vi1 = (cb + (left_end ? R(-1, 0, 1) - 1 : right_end ? R(0, 0, 0) + 2 : own)) * ES;
+    float sc = 1.0f;
+    if (MODE == MODE_IMAG) sc = (float)(*scale);
+    auto scl = [&](P v) {
+        if (MODE == MODE_IMAG) {
+            v.a = cscale<float>(v.a, sc);
+            v.b = cscale<float>(v.b, sc);
+        }
+        return v;
+    };
+    auto ld = [&](uint32_t lay, int dz, int dy, int sl) { return scl(ld_pair(rp, vcol, (lay + RS(dz, dy, sl)) * ES)); };
+    auto ld1 = [&](uint32_t v, uint32_t lay) {
+        C x = BufC<float>::template ld<0>(rp, v, lay * ES);
+        if (MODE == MODE_IMAG) x = cscale<float>(x, sc);
+        return x;
+    };
+    auto fetch = [&](uint32_t lay, ZLayer2 &f) {
+        f.czp = ld(lay, 1, 0, 0);
+        f.v3 = ld(lay, 1, 1, 0);
+        f.zzp = ld(lay, 1, 0, 1);
+        if (!LY) {
+            f.cym = ld(lay, 0, -1, 0);
+            f.u2 = ld(lay, 0, -1, 1);
+            f.zyp = ld(lay, 0, 1, 1);
+        }
+        f.t0 = ld1(vt0, lay);
+        f.t1 = ld1(vt1, lay);
+        f.t2 = ld1(vt2, lay);
+        if (MODE == MODE_REAL) {
+            f.n0 = ld_pair_aux<NT ? kBufNT : 0>(rn, vcol, (lay + RS(0, 0, 0)) * ES);
+            f.n1 = ld_pair_aux<NT ? kBufNT : 0>(rn, vcol, (lay + RS(0, 0, 1)) * ES);
+        }
+    };
+    uint32_t lay = (uint32_t)k0 * sz;
+    P c0 = ld(lay, 0, 0, 0), c1 = ld(lay, 0, 0, 1), czm = ld(lay, -1, 0, 0), u1 = ld(lay, -1, 0, 1),
+      u0 = ld(lay, -1, -1, 1);
+    P cyp = ld(lay, 0, 1, 0);
+    const C t0 = ld1(vi0, lay);
+    const C t1 = ld1(vi1, lay);
+    ZLayer2 buf[2];
+    fetch(lay, buf[0]);
+    P w0 = pair_from_left(u0), w1 = pair_from_left(u1);
+    P e1 = pair_from_right(cyp), xq = pair_from_right(c0);
+    if (left_end) {
+        w0.a = t0;
+        w1.a = t1;
+    }
+    if (right_end) {
+        e1.b = t0;
+        xq.b = t1;
+    }
+    const double yy = trap.p0[1] + trap.a * (double)j, yy1 = trap.p0[1] + trap.a * ((double)j + 0.5);
+    const double xa = trap.p0[0] + trap.a * (double)i, xb = trap.p0[0] + trap.a * (double)(i + 1);
+    const double xa1 = trap.p0[0] + trap.a * ((double)i + 0.5), xb1 = trap.p0[0] + trap.a * ((double)(i + 1) + 0.5);
+    const float vA0 = (float)(0.5 * (xa * xa + yy * yy) * dt), vB0 = (float)(0.5 * (xb * xb + yy * yy) * dt);
+    const float vA1 = (float)(0.5 * (xa1 * xa1 + yy1 * yy1) * dt), vB1 = (float)(0.5 * (xb1 * xb1 + yy1 * yy1) * dt);
+    double part = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < CZ; kk++, lay += sz) {
+        const int k = k0 + kk;
+        if (k >= k1) break;
+        ZLayer2 &cur = buf[kk & 1];
+        if (LY) {
+            cur.cym = ld(lay, 0, -1, 0);
+            cur.u2 = ld(lay, 0, -1, 1);
+            cur.zyp = ld(lay, 0, 1, 1);
+        }
+        fetch(lay + sz, buf[(kk + 1) & 1]);
+        __builtin_amdgcn_sched_barrier(0);
+        P w2 = pair_from_left(cur.u2), w3 = pair_from_left(c1), xm = pair_from_left(c0);
+        P e2 = pair_from_right(cur.czp), e3 = pair_from_right(cur.v3), zxp = pair_from_right(c1);
+        if (left_end) {
+            w2.a = cur.t0;
+            w3.a = cur.t1;
+            xm.a = cur.t2;
+        }
+        if (right_end) {
+            e2.b = cur.t0;
+            e3.b = cur.t1;
+            zxp.b = cur.t2;
+        }
+        const float vz0 = vzt[2 * k], vz1 = vzt[2 * k + 1];
+        P o0, o1;
+        {
+            const C dC[8] = {w0.a, u0.a, w1.a, u1.a, w2.a, cur.u2.a, w3.a, c1.a};
+            const C aC[6] = {xm.a, xq.a, cur.cym.a, cyp.a, czm.a, cur.czp.a};
+            const C s0 = primal_apply<float, FORM_DIFF>(dC, aC, c0.a, vA0 + vz0, H);
+            const C dZ[8] = {c0.a, xq.a, cyp.a, e1.a, cur.czp.a, e2.a, cur.v3.a, e3.a};
+            const C aZ[6] = {w3.a, zxp.a, cur.u2.a, cur.zyp.a, u1.a, cur.zzp.a};
+            const C s1 = primal_apply<float, FORM_DIFF>(dZ, aZ, c1.a, vA1 + vz1, H);
+            if (MODE == MODE_REAL) {
+                o0.a.x = cur.n0.a.x + s0.y;
+                o0.a.y = cur.n0.a.y + (-s0.x);
+                o1.a.x = cur.n1.a.x + s1.y;
+                o1.a.y = cur.n1.a.y + (-s1.x);
+            } else {
+                o0.a.x = c0.a.x - s0.x;
+                o0.a.y = c0.a.y - s0.y;
+                o1.a.x = c1.a.x - s1.x;
+                o1.a.y = c1.a.y - s1.y;
+            }
+        }
+        {
+            const C dC[8] = {w0.b, u0.b, w1.b, u1.b, w2.b, cur.u2.b, w3.b, c1.b};
+            const C aC[6] = {xm.b, xq.b, cur.cym.b, cyp.b, czm.b, cur.czp.b};
+            const C s0 = primal_apply<float, FORM_DIFF>(dC, aC, c0.b, vB0 + vz0, H);
+            const C dZ[8] = {c0.b, xq.b, cyp.b, e1.b, cur.czp.b, e2.b, cur.v3.b, e3.b};
+            const C aZ[6] = {w3.b, zxp.b, cur.u2.b, cur.zyp.b, u1.b, cur.zzp.b};
+            const C s1 = primal_apply<float, FORM_DIFF>(dZ, aZ, c1.b, vB1 + vz1, H);
+            if (MODE == MODE_REAL) {
+                o0.b.x = cur.n0.b.x + s0.y;
+                o0.b.y = cur.n0.b.y + (-s0.x);
+                o1.b.x = cur.n1.b.x + s1.y;
+                o1.b.y = cur.n1.b.y + (-s1.x);
+            } else {
+                o0.b.x = c0.b.x - s0.x;
+                o0.b.y = c0.b.y - s0.y;
+                o1.b.x = c1.b.x - s1.x;
+                o1.b.y = c1.b.y - s1.y;
+            }
+        }
+        if (!actB) {  // the second cell is the halo column x = nx (or beyond): it stays zero
+            o0.b.x = o0.b.y = 0.0f;
+            o1.b.x = o1.b.y = 0.0f;
+        }
+        st_pair<NT || MODE == MODE_IMAG ? kBufNT : 0>(o0, rn, vst, (lay + RS(0, 0, 0)) * ES);
+        st_pair<NT || MODE == MODE_IMAG ? kBufNT : 0>(o1, rn, vst, (lay + RS(0, 0, 1)) * ES);
+        if (MODE == MODE_IMAG) {
+            if (actA)
+                part += ((double)o0.a.x * (double)o0.a.x + (double)o0.a.y * (double)o0.a.y) +
+                        ((double)o1.a.x * (double)o1.a.x + (double)o1.a.y * (double)o1.a.y);
+            if (actB)
+                part += ((double)o0.b.x * (double)o0.b.x + (double)o0.b.y * (double)o0.b.y) +
+                        ((double)o1.b.x * (double)o1.b.x + (double)o1.b.y * (double)o1.b.y);
+        }
+        czm = c0;
+        c0 = cur.czp;
+        u1 = c1;
+        c1 = cur.zzp;
+        u0 = cur.u2;
+        w0 = w2;
+        w1 = w3;
+        cyp = cur.v3;
+        e1 = e3;
+        xq = e2;
+    }
+    return part;
+}
+
+template <int MODE, int TY, int CZ, bool NT, bool LY>
+__global__ void __launch_bounds__(64 * TY)
+    k_primal_zp(cplx_t<float> *__restrict__ next, const cplx_t<float> *__restrict__ prev, Layout L, PCoef<float> H,
+                PTrap trap, double dt, const float *__restrict__ vzt, const double *__restrict__ scale,
+                double *__restrict__ partials, Grid3 G) {
+    __shared__ double lds[TY];
+    int bx, by, bz;
+    block_coords(G, bx, by, bz);
+    double part[1];
+    part[0] = primal_column_p<MODE, TY, CZ, NT, LY>(next, prev, L, H, trap, dt, vzt, scale, bx, by, bz);
+    if (MODE == MODE_IMAG) {
+        block_sum<1, TY>(part, lds);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partials[bx + G.gx * (by + G.gy * bz)] = part[0];
+    }
+}
+
 // FUSED (real time): one pass per leapfrog step, out of place,
 //   O' = O - i dt H(E) E,   E' = E - i dt H(O') O',
 // reading E and O once and writing E' and O' once (64 B per site-step in
@@ -740,6 +1224,9 @@ struct PTile {
     int ty, remap, nt, xs, cz;
     int ly;     // z-march: y-neighbour rows loaded in the layer instead of prefetched
     int fused;  // real time: both half-updates in one z-march pass (k_primal_fused)
+    int buf;    // z-march with buffer-resource addressing (k_primal_zb; fields below 2 GiB):
+                // the layers of prefetch (1 or 2)
+    int pair;   // FP32: two cells per lane (k_primal_zp); FP64 runs k_primal_zb of the same shape
 };
 constexpr PTile kPrimalTiles[] = {
     {4, 2, 1, XS_DPP, 0},  {4, 2, 1, XS_LOAD, 0}, {2, 2, 1, XS_DPP, 0},  {8, 2, 1, XS_DPP, 0},
@@ -748,12 +1235,20 @@ constexpr PTile kPrimalTiles[] = {
     {4, 2, 1, XS_DPP, 16}, {4, 2, 1, XS_DPP, 4},  {2, 2, 1, XS_DPP, 8},  {8, 2, 1, XS_DPP, 8},
     {4, 0, 1, XS_DPP, 8},  {4, 2, 0, XS_DPP, 8},  {4, 2, 1, XS_DPP, 16, 1}, {4, 2, 1, XS_DPP, 8, 1},
     {8, 2, 0, XS_DPP, 16, 0, 1}, {8, 2, 0, XS_DPP, 8, 0, 1}, {6, 2, 0, XS_DPP, 16, 0, 1},
-    {8, 0, 0, XS_DPP, 16, 0, 1}, {4, 2, 0, XS_DPP, 16, 0, 1}};
+    {8, 0, 0, XS_DPP, 16, 0, 1}, {4, 2, 0, XS_DPP, 16, 0, 1},
+    {4, 2, 1, XS_DPP, 16, 1, 0, 1}, {4, 2, 1, XS_DPP, 8, 1, 0, 1}, {4, 2, 1, XS_DPP, 16, 0, 0, 1},
+    {2, 2, 1, XS_DPP, 16, 1, 0, 1}, {8, 2, 1, XS_DPP, 16, 1, 0, 1}, {4, 2, 0, XS_DPP, 16, 1, 0, 1},
+    {4, 2, 1, XS_DPP, 16, 1, 0, 2}, {4, 2, 1, XS_DPP, 16, 0, 0, 2}, {2, 2, 1, XS_DPP, 16, 1, 0, 2},
+    {4, 2, 0, XS_DPP, 16, 1, 0, 2}, {8, 2, 1, XS_DPP, 16, 1, 0, 2},
+    {4, 2, 1, XS_DPP, 16, 0, 0, 1, 1}, {4, 2, 1, XS_DPP, 16, 1, 0, 1, 1}, {2, 2, 1, XS_DPP, 16, 0, 0, 1, 1},
+    {8, 2, 1, XS_DPP, 16, 0, 0, 1, 1}, {4, 2, 0, XS_DPP, 16, 0, 0, 1, 1}, {4, 2, 1, XS_DPP, 8, 0, 0, 1, 1}};
 constexpr int kNumPrimalTiles = sizeof(kPrimalTiles) / sizeof(kPrimalTiles[0]);
-constexpr int kDefaultPrimalTile = 12;  // {4, 2, 1, XS_DPP, 16}
+constexpr int kDefaultPrimalTile = 12;  // {4, 2, 1, XS_DPP, 16}: fields of 2 GiB or more, FP64
+constexpr int kMarchTileFP32 = 18;      // {4, 2, 1, XS_DPP, 16, 1}: fields of 2 GiB or more, FP32
 constexpr int kCachedTileFP64 = 8;      // {4, 2, 0, XS_DPP, 0}
-constexpr int kCachedTileFP32 = 10;     // {4, 2, 0, XS_LOAD, 0}
-constexpr int kMarchTileFP32 = 18;      // {4, 2, 1, XS_DPP, 16, 1}
+constexpr int kCachedTileFP32 = 40;     // {4, 2, 0, XS_DPP, 16, 0, 0, 1, 1}: two cells per lane
+constexpr int kBufMarchTileFP64 = 27;   // {4, 2, 1, XS_DPP, 16, 0, 0, 1}
+constexpr int kBufMarchTileFP32 = 36;   // {4, 2, 1, XS_DPP, 16, 0, 0, 1, 1}: two cells per lane
 
 Layout primal_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes) {
     Layout L{};
@@ -772,6 +1267,16 @@ Layout primal_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes) {
 
 Grid3 primal_grid(const PTile &t, const Layout &L) {
     Grid3 G{};
+    if (t.pair && L.padl == 16) {
+        // FP32 (16 elements per 128-byte line) two cells per lane: 128-cell rows
+        G.lsh = 6;
+        G.gx = (L.nx + 127) / 128;
+        G.gy = (L.ny + t.ty - 1) / t.ty;
+        G.gz = (L.nz + t.cz - 1) / t.cz;
+        G.remap = t.remap;
+        if (G.remap == 2 && G.gy % 8 != 0) G.remap = 0;
+        return G;
+    }
     if (t.fused) {
         // 64-lane rows, interior 62 x (ty - 2) columns per block
         G.lsh = 6;
@@ -804,11 +1309,11 @@ void launch_t(cplx_t<T> *next, const cplx_t<T> *prev, const Layout &L, const PCo
 template <typename T, int FORM, int MODE, int IDX = 0>
 hipError_t launch_primal(int tile, cplx_t<T> *next, const cplx_t<T> *prev, const Layout &L, const PCoef<T> &H,
                          const PTrap &trap, double dt, const double *scale, double *partials, hipStream_t st,
-                         bool reverse) {
+                         bool reverse, const float *vzt) {
     if constexpr (IDX < kNumPrimalTiles) {
         if (tile != IDX)
             return launch_primal<T, FORM, MODE, IDX + 1>(tile, next, prev, L, H, trap, dt, scale, partials, st,
-                                                         reverse);
+                                                         reverse, vzt);
         constexpr PTile t = kPrimalTiles[IDX];
         if constexpr (t.fused) {
             return hipErrorInvalidValue;  // launched by launch_primal_fused
@@ -816,7 +1321,13 @@ hipError_t launch_primal(int tile, cplx_t<T> *next, const cplx_t<T> *prev, const
         Grid3 G = primal_grid(t, L);
         G.rev = reverse ? 1 : 0;
         constexpr bool nt = MODE == MODE_REAL && t.nt;
-        if constexpr (t.cz > 0)
+        if constexpr (t.pair && sizeof(T) == 4)
+            hipLaunchKernelGGL((k_primal_zp<MODE, t.ty, t.cz, nt, t.ly != 0>), dim3(G.gx * G.gy * G.gz),
+                               dim3(64, t.ty), 0, st, next, prev, L, H, trap, dt, vzt, scale, partials, G);
+        else if constexpr (t.buf)
+            hipLaunchKernelGGL((k_primal_zb<T, FORM, MODE, t.ty, t.cz, nt, t.ly != 0, t.buf>), dim3(G.gx * G.gy * G.gz),
+                               dim3(64, t.ty), 0, st, next, prev, L, H, trap, dt, vzt, scale, partials, G);
+        else if constexpr (t.cz > 0)
             hipLaunchKernelGGL((k_primal_zm<T, FORM, MODE, t.ty, t.cz, nt, t.ly != 0>), dim3(G.gx * G.gy * G.gz), dim3(64, t.ty),
                                0, st, next, prev, L, H, trap, dt, scale, partials, G);
         else
@@ -886,6 +1397,9 @@ struct gpe_primal {
     double *scal = nullptr;      // [0] last N, [1] read scale, [4..7] observables
     double *partials = nullptr;  // per-block partial sums
     long long npartials = 0;
+    // FP32: (0.5 rr z^2 dt) per layer and sublattice, rounded to FP32, for the
+    // buffer-addressed z-march (recomputed when dt changes)
+    float *vzt = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t steps_done = 0;
@@ -897,6 +1411,7 @@ struct gpe_primal {
             if (p) (void)hipFree(p);
         if (scal) (void)hipFree(scal);
         if (partials) (void)hipFree(partials);
+        if (vzt) (void)hipFree(vzt);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -908,6 +1423,7 @@ struct gpe_primal {
         w = 0.5 * cfg.a * cfg.a * cfg.a;
     }
     gpe_status init();
+    gpe_status upload_vz();
     template <typename T> gpe_status upload(int f, const double *h);
     template <typename T> gpe_status download(int f, double *h);
     template <typename T, int FORM> gpe_status enqueue(uint64_t nsteps, uint64_t *launches);
@@ -920,14 +1436,25 @@ gpe_status gpe_primal::init() {
     trap.a = cfg.a;
     trap.rr = cfg.trap_ratio * cfg.trap_ratio;
     // default by size, the fastest shapes measured on MI355X: when both fields
-    // fit the 256 MB Infinity Cache (128^3 cells) one layer per block with a
-    // cached read-modify-write (FP64 DPP exchange, FP32 loads); larger grids
-    // (256^3) the z-march of 16 layers with one layer of prefetch (FP32: with
-    // the y-neighbour rows loaded in the layer, which leaves room for 5 waves)
-    const bool cached = 2.0 * (double)L.elems * (double)elem_bytes() < 192e6;
+    // fit the 256 MB Infinity Cache (128^3 cells) a cached read-modify-write,
+    // FP64 one layer per block with DPP exchange, FP32 the buffer-addressed
+    // z-march with two cells per lane; larger grids (256^3) the
+    // buffer-addressed z-march of 16 layers with one layer of prefetch (FP32:
+    // two cells per lane); fields of 2 GiB or more (beyond 32-bit offsets) the
+    // z-march on 64-bit addresses
+    const double fbytes_total = (double)L.elems * (double)elem_bytes();
+    const bool cached = 2.0 * fbytes_total < 192e6;
+    const bool small = fbytes_total < 2147483648.0;
     const bool fp64 = cfg.precision == GPE_FP64;
-    imag_tile = cached ? (fp64 ? kCachedTileFP64 : kCachedTileFP32) : (fp64 ? kDefaultPrimalTile : kMarchTileFP32);
+    if (cached) imag_tile = fp64 ? kCachedTileFP64 : kCachedTileFP32;
+    else if (small) imag_tile = fp64 ? kBufMarchTileFP64 : kBufMarchTileFP32;
+    else imag_tile = fp64 ? kDefaultPrimalTile : kMarchTileFP32;
     tile = cfg.tile != 0 ? (int)cfg.tile - 1 : imag_tile;
+    if (kPrimalTiles[tile].buf && (double)L.elems * (double)elem_bytes() >= 2147483648.0) {
+        set_error("gpe_primal_create: tile %d addresses fields by 32-bit offsets; this grid's fields exceed 2 GiB",
+                  tile + 1);
+        return GPE_EINVAL;
+    }
     set_coefs();
     GPE_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     GPE_HIP_TRY(hipEventCreate(&ev0));
@@ -951,6 +1478,24 @@ gpe_status gpe_primal::init() {
     const double init_scal[8] = {1.0, 1.0, 0, 0, 0, 0, 0, 0};
     GPE_HIP_TRY(hipMemcpyAsync(scal, init_scal, sizeof(init_scal), hipMemcpyHostToDevice, stream));
     GPE_HIP_TRY(hipStreamSynchronize(stream));
+    if (!fp64) {
+        GPE_HIP_TRY(hipMalloc(&vzt, (size_t)L.nz * 2 * sizeof(float)));
+        return upload_vz();
+    }
+    return GPE_OK;
+}
+
+// The z part of V*dt per layer k and sublattice s (z = p0 + a (k + s/2)), in
+// site_v's operation order, rounded to FP32.
+gpe_status gpe_primal::upload_vz() {
+    if (!vzt) return GPE_OK;
+    std::vector<float> h((size_t)L.nz * 2);
+    for (int k = 0; k < L.nz; k++)
+        for (int sl = 0; sl < 2; sl++) {
+            const double z = trap.p0[2] + trap.a * ((double)k + (sl ? 0.5 : 0.0));
+            h[2 * k + sl] = (float)(0.5 * (trap.rr * z * z) * cfg.dt);
+        }
+    GPE_HIP_TRY(hipMemcpy(vzt, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     return GPE_OK;
 }
 
@@ -1017,9 +1562,9 @@ gpe_status gpe_primal::enqueue(uint64_t nsteps, uint64_t *launches) {
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
             // the even update walks the tiles backwards, starting on the layers
             // the odd update touched last
-            e = launch_primal<T, FORM, MODE_REAL>(tile, od, ev, L, H, trap, dt, nullptr, nullptr, stream, false);
+            e = launch_primal<T, FORM, MODE_REAL>(tile, od, ev, L, H, trap, dt, nullptr, nullptr, stream, false, vzt);
             if (e == hipSuccess)
-                e = launch_primal<T, FORM, MODE_REAL>(tile, ev, od, L, H, trap, dt, nullptr, nullptr, stream, true);
+                e = launch_primal<T, FORM, MODE_REAL>(tile, ev, od, L, H, trap, dt, nullptr, nullptr, stream, true, vzt);
             nl += 2;
         }
     } else {
@@ -1031,7 +1576,7 @@ gpe_status gpe_primal::enqueue(uint64_t nsteps, uint64_t *launches) {
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
             C *in = (C *)field[ie], *out = (C *)field[io];
             const bool rev = (steps_done + s) & 1;
-            e = launch_primal<T, FORM, MODE_IMAG>(itile, out, in, L, H, trap, dt, scal + 1, partials, stream, rev);
+            e = launch_primal<T, FORM, MODE_IMAG>(itile, out, in, L, H, trap, dt, scal + 1, partials, stream, rev, vzt);
             if (e != hipSuccess) break;
             hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(256), 0, stream, partials, nb, w, 1, scal);
             e = hipGetLastError();
@@ -1206,7 +1751,7 @@ gpe_status gpe_primal_set_mode(gpe_primal *p, uint32_t mode, double dt) {
     p->cfg.mode = mode;
     p->cfg.dt = dt;
     p->set_coefs();
-    return GPE_OK;
+    return p->upload_vz();
 }
 
 gpe_status gpe_primal_init_odd(gpe_primal *p, double mu) {
