@@ -214,7 +214,8 @@ gpe_status gpe_observables_noalloc(const gpe_config *cfg, const void *field, con
  * flags bit 0 = XCD-contiguous block order, bit 1 = streaming
  * (non-temporal) access to the updated field, bit 2 = each XCD owns a
  * y-band of every z layer (grids whose block rows divide by 8; others run
- * in plain order).
+ * in plain order), bit 3 = fused tile of one independent wavefront per cell
+ * row (fields below 2 GiB only).
  * GPE_KERNEL_FUSED needs two more field buffers (allocated on selection);
  * imaginary-time steps of a fused solver run the z-march kernel. */
 gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile);

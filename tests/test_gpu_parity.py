@@ -131,6 +131,35 @@ def test_fused_every_tile(orc, precision, tile):
     s.close()
 
 
+def _row_tiles():
+    return [v["tile"] for v in va.variants() if v["kernel"] == "fused" and v["row"]]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("tile", _row_tiles())
+def test_fused_row_tiles_array_potential(orc, precision, tile):
+    """The one-wave-per-row fused tiles with a streamed potential, on a grid of three x-strips
+    (the last one 1 column wide), rows that do not fill the last block and a last z-chunk of
+    one layer: FP64 bit-identical to the oracle, FP32 within 2e-6 relative L2 after 12 steps."""
+    dims = (125, 11, 33)
+    announce(f"fused row tile {tile} array {precision} {dims}")
+    h, g, dt, ratio = 0.4, 400.0, 1e-3, 0.8
+    b, origin, even, odd, V = random_case(dims, h, ratio, seed=41)
+    s = va.Solver(dims, b, dt, g, origin, precision, "real", "array", ratio, "fused", tile)
+    s.set_potential(V)
+    s.set_state(even, odd)
+    s.step(12)
+    ge, go = s.get_state()
+    lf, lf0, gdt = va.leapfrog_coefs(b, dt, g)
+    oe, oo = orc.leapfrog(even, odd, V * dt, dims, lf, lf0, gdt, 12)
+    if precision == "fp64":
+        np.testing.assert_array_equal(ge, oe)
+        np.testing.assert_array_equal(go, oo)
+    else:
+        assert rel_l2(ge, oe) < 2e-6 and rel_l2(go, oo) < 2e-6
+    s.close()
+
+
 @pytest.mark.parametrize("variant", [(v["kernel"], v["tile"]) for v in va.variants()
                                      if not (v["kernel"] == "fused" and v["waves"] == 16)])
 def test_every_variant_fp64_bit_exact(orc, variant):

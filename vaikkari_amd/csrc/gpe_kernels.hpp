@@ -41,10 +41,20 @@ struct Layout {
 struct Trap {
     double p0[3];
     double b;
-    double bw;          // sqrt(8)
-    double rr;          // ratio*ratio
-    double bpos[12][3]; // q_l/sqrt(8)
+    double bw;    // sqrt(8)
+    double rr;    // ratio*ratio
+    double q[4];  // (2n + 1)/sqrt(8): the four coordinates a node takes inside its cell
 };
+
+// Node position of class l inside its cell, per axis, in units of b/sqrt(8)
+// (1, 3, 5 or 7; GPUGrossPitaevskii_BCC.cu:76-88), as the index n of
+// Trap::q.  Kernels index it with compile-time classes, so the trap's
+// offsets are the 4 values q[n] rather than 36.
+__host__ __device__ constexpr int class_q(int l, int a) {
+    constexpr int pos[12][3] = {{1, 3, 7}, {7, 1, 3}, {7, 3, 1}, {7, 3, 5}, {7, 5, 3}, {1, 7, 3},
+                                {3, 7, 1}, {3, 7, 5}, {5, 7, 3}, {3, 1, 7}, {3, 5, 7}, {5, 3, 7}};
+    return pos[l][a] >> 1;
+}
 
 // Scalars of one solve.  *_dt are multiplied by the time step like the
 // reference (GPUGrossPitaevskii_BCC.cu:304-307).
@@ -61,9 +71,9 @@ enum { MODE_REAL = 0, MODE_IMAG = 1 };
 
 __device__ __forceinline__ double trap_v(const Trap &t, int i, int j, int k, int l) {
 #pragma clang fp contract(off)
-    const double x = t.p0[0] + t.b * ((double)(unsigned)i * t.bw + t.bpos[l][0]);
-    const double y = t.p0[1] + t.b * ((double)(unsigned)j * t.bw + t.bpos[l][1]);
-    const double z = t.p0[2] + t.b * ((double)(unsigned)k * t.bw + t.bpos[l][2]);
+    const double x = t.p0[0] + t.b * ((double)(unsigned)i * t.bw + t.q[class_q(l, 0)]);
+    const double y = t.p0[1] + t.b * ((double)(unsigned)j * t.bw + t.q[class_q(l, 1)]);
+    const double z = t.p0[2] + t.b * ((double)(unsigned)k * t.bw + t.q[class_q(l, 2)]);
     return 0.5 * (x * x + y * y + t.rr * z * z);
 }
 
@@ -638,9 +648,9 @@ __global__ void k_init_gaussian(cplx_t<T> *__restrict__ even, cplx_t<T> *__restr
         const int i = (int)(c % L.nx), j = (int)((c / L.nx) % L.ny), k = (int)(c / ((long long)L.nx * L.ny));
         const long long base = L.origin + (long long)k * L.sz + (long long)j * L.sy + i;
         for (int l = 0; l < 12; l++) {
-            const double x = trap.p0[0] + trap.b * ((double)(unsigned)i * trap.bw + trap.bpos[l][0]);
-            const double y = trap.p0[1] + trap.b * ((double)(unsigned)j * trap.bw + trap.bpos[l][1]);
-            const double z = trap.p0[2] + trap.b * ((double)(unsigned)k * trap.bw + trap.bpos[l][2]);
+            const double x = trap.p0[0] + trap.b * ((double)(unsigned)i * trap.bw + trap.q[class_q(l, 0)]);
+            const double y = trap.p0[1] + trap.b * ((double)(unsigned)j * trap.bw + trap.q[class_q(l, 1)]);
+            const double z = trap.p0[2] + trap.b * ((double)(unsigned)k * trap.bw + trap.q[class_q(l, 2)]);
             const double a = amp * exp(-(x * x + y * y + z * z) * inv2s2);
             const unsigned long long h1 = splitmix64(seed ^ (unsigned long long)(c * 12 + l) * 0xD1B54A32D192ED03ULL);
             const unsigned long long h2 = splitmix64(h1);

@@ -534,8 +534,7 @@ struct BufC<double> {
     }
     template <int AUX>
     static __device__ __forceinline__ void st(cplx_t<double> x, __amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
-        __builtin_amdgcn_raw_buffer_store_b128(
-            __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0)), x), r, v, s, AUX);
+        buffer_store_128<AUX>(x, r, v, s);
     }
 };
 constexpr int kBufNT = 2;             // cache-policy bits of a non-temporal buffer access
@@ -748,8 +747,7 @@ __device__ __forceinline__ CP<float> ld_pair_aux(__amdgpu_buffer_rsrc_t r, uint3
 }
 template <int AUX>
 __device__ __forceinline__ void st_pair(CP<float> x, __amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
-    __builtin_amdgcn_raw_buffer_store_b128(
-        __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0)), x), r, v, s, AUX);
+    buffer_store_128<AUX>(x, r, v, s);
 }
 // x-neighbours of both cells: from the left (x - 1) and from the right (x + 1)
 __device__ __forceinline__ CP<float> pair_from_left(const CP<float> &v) {

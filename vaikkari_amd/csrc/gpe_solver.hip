@@ -13,6 +13,7 @@
 
 #include "gpe_internal.hpp"
 #include "gpe_fused.hpp"
+#include "gpe_fused_row.hpp"
 #include "gpe_hip_check.hpp"
 #include "gpe_kernels.hpp"
 #include "gpe_solver.hpp"
@@ -43,8 +44,7 @@ Trap make_trap(const double origin[3], double b, double ratio) {
     t.bw = std::sqrt(8.0);
     t.rr = ratio * ratio;
     const double sq18 = 1.0 / std::sqrt(8.0);
-    for (int l = 0; l < 12; l++)
-        for (int a = 0; a < 3; a++) t.bpos[l][a] = sq18 * kClassPos[l][a];
+    for (int n = 0; n < 4; n++) t.q[n] = sq18 * (2 * n + 1);
     return t;
 }
 
@@ -63,6 +63,7 @@ constexpr int kTY = 4;  // waves per block of the observables kernel
 // 1 contiguous eighths, 2 y-bands).
 struct TileShape {
     int ty, cz, remap, nt;  // nt: streaming (non-temporal) read-modify-write of the updated field
+    int row;                // fused family: one independent wave per cell row (k_fused_row)
 };
 constexpr TileShape kGatherTiles[] = {{4, 1, 0, 1}, {4, 1, 0, 0}, {4, 1, 1, 0}, {1, 1, 1, 0}, {2, 1, 1, 0},
                                       {8, 1, 1, 0}, {4, 1, 1, 1}, {2, 1, 0, 1}, {8, 1, 0, 1}, {1, 1, 0, 1},
@@ -71,8 +72,11 @@ constexpr TileShape kZmarchTiles[] = {{4, 8, 1, 0}, {4, 8, 0, 1}, {1, 8, 1, 0}, 
                                       {4, 4, 1, 1}, {4, 16, 1, 1}, {2, 16, 1, 0}, {1, 16, 1, 1}, {2, 4, 1, 1},
                                       {4, 4, 2, 1}, {8, 8, 2, 1}};
 // fused: ty = waves per block (footprint rows, interior ty-2), cz = layers per block
+// fused, row design (row = 1): ty = waves (rows) per block, cz = layers per z-chunk
 constexpr TileShape kFusedTiles[] = {{8, 16, 1, 0}, {4, 16, 1, 0}, {4, 32, 1, 0}, {8, 32, 1, 0}, {8, 8, 1, 0},
-                                     {16, 16, 1, 0}, {4, 8, 1, 0}, {8, 16, 0, 0}, {6, 16, 1, 0}, {6, 32, 1, 0}};
+                                     {16, 16, 1, 0}, {4, 8, 1, 0}, {8, 16, 0, 0}, {6, 16, 1, 0}, {6, 32, 1, 0},
+                                     {4, 16, 1, 0, 1}, {4, 32, 1, 0, 1}, {2, 32, 1, 0, 1}, {8, 32, 1, 0, 1},
+                                     {4, 64, 1, 0, 1}, {1, 32, 1, 0, 1}, {4, 32, 0, 0, 1}};
 constexpr int kNumGatherTiles = sizeof(kGatherTiles) / sizeof(kGatherTiles[0]);
 constexpr int kNumZmarchTiles = sizeof(kZmarchTiles) / sizeof(kZmarchTiles[0]);
 constexpr int kNumFusedTiles = sizeof(kFusedTiles) / sizeof(kFusedTiles[0]);
@@ -95,9 +99,14 @@ TileShape tile_shape(int kernel, int tile) {
 }
 
 // A fused tile of 16 rows holds its FP64 exchange slots in more than the
-// 160 KiB of LDS: that (family, tile, precision) is refused up front.
-bool tile_supported(int kernel, int tile, uint32_t precision) {
-    return !(kernel == GPE_KERNEL_FUSED && precision == GPE_FP64 && tile_shape(kernel, tile).ty == 16);
+// 160 KiB of LDS, and the one-wave-per-row fused tiles address a field with
+// 32-bit byte offsets: those (family, tile, precision, size) are refused up
+// front.
+bool tile_supported(int kernel, int tile, uint32_t precision, const Layout &L) {
+    if (kernel != GPE_KERNEL_FUSED) return true;
+    const TileShape t = tile_shape(kernel, tile);
+    if (t.row) return (unsigned long long)L.elems * (precision == GPE_FP64 ? 16u : 8u) < (1ull << 31);
+    return !(precision == GPE_FP64 && t.ty == 16);
 }
 
 FusedGrid make_fused_grid(const Layout &L, const TileShape &t) {
@@ -120,11 +129,40 @@ void launch_fused_t(cplx_t<T> *eo, cplx_t<T> *oo, const cplx_t<T> *ei, const cpl
                        pot, L, H, trap, dt, G);
 }
 
+template <typename T, int FORM, int POT, int TY, int CZ>
+void launch_fused_row_t(cplx_t<T> *eo, cplx_t<T> *oo, const cplx_t<T> *ei, const cplx_t<T> *oi, const T *pot,
+                        const Layout &L, const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, int remap,
+                        hipStream_t st) {
+    RowGrid G;
+    G.ntx = (L.nx + 61) / 62;
+    G.nty = (L.ny + TY - 1) / TY;
+    G.ntz = (L.nz + CZ - 1) / CZ;
+    G.cz = CZ;
+    G.remap = remap;
+    hipLaunchKernelGGL((k_fused_row<T, FORM, POT, TY, CZ>), dim3(G.ntx * G.nty * G.ntz), dim3(64, TY), 0, st, eo, oo,
+                       ei, oi, pot, L, H, trap, dt, G);
+}
+
 template <typename T, int FORM, int POT>
 hipError_t launch_fused(int tile, cplx_t<T> *eo, cplx_t<T> *oo, const cplx_t<T> *ei, const cplx_t<T> *oi,
                         const T *pot, const Layout &L, const Hamiltonian<T, FORM> &H, const Trap &trap, double dt,
                         hipStream_t st) {
     const TileShape t = tile_shape(GPE_KERNEL_FUSED, tile);
+    if (t.row) {
+#define GPE_ROW(TYV, CZV) \
+    launch_fused_row_t<T, FORM, POT, TYV, CZV>(eo, oo, ei, oi, pot, L, H, trap, dt, t.remap, st)
+        switch (t.ty * 1000 + t.cz) {
+            case 1032: GPE_ROW(1, 32); break;
+            case 2032: GPE_ROW(2, 32); break;
+            case 4016: GPE_ROW(4, 16); break;
+            case 4032: GPE_ROW(4, 32); break;
+            case 4064: GPE_ROW(4, 64); break;
+            case 8032: GPE_ROW(8, 32); break;
+            default: return hipErrorInvalidValue;
+        }
+#undef GPE_ROW
+        return hipGetLastError();
+    }
     const FusedGrid G = make_fused_grid(L, t);
     switch (t.ty) {
         case 4: launch_fused_t<T, FORM, POT, 4>(eo, oo, ei, oi, pot, L, H, trap, dt, G, st); break;
@@ -292,8 +330,9 @@ gpe_status gpe_solver::init(const gpe_config &c) {
         return GPE_EINVAL;
     }
     tile = c.tile == 0 ? default_tile(kernel, c.precision) : (int)c.tile - 1;
-    if (!tile_supported(kernel, tile, c.precision)) {
-        set_error("gpe_create: fused tile %u does not fit LDS in FP64", c.tile);
+    if (!tile_supported(kernel, tile, c.precision, L)) {
+        set_error("gpe_create: fused tile %u does not fit LDS in FP64, or is a row tile on a field of 2 GiB or more",
+                  c.tile);
         return GPE_EINVAL;
     }
     GPE_HIP_TRY(hipGetDevice(&device));
@@ -1016,8 +1055,9 @@ gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile) {
         return GPE_EINVAL;
     }
     const int t0 = tile == 0 ? default_tile((int)kernel, s->cfg.precision) : (int)tile - 1;
-    if (!tile_supported((int)kernel, t0, s->cfg.precision)) {
-        set_error("gpe_set_variant: fused tile %u does not fit LDS in FP64", tile);
+    if (!tile_supported((int)kernel, t0, s->cfg.precision, s->L)) {
+        set_error("gpe_set_variant: fused tile %u does not fit LDS in FP64, or is a row tile on a field of 2 GiB "
+                  "or more", tile);
         return GPE_EINVAL;
     }
     GPE_HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1043,7 +1083,7 @@ void gpe_variant_shape(uint32_t kernel, uint32_t tile, int32_t out[3]) {
         const TileShape t = tile_shape((int)kernel, (int)tile - 1);
         out[0] = t.ty;
         out[1] = t.cz;
-        out[2] = (t.remap == 1 ? 1 : 0) | (t.nt << 1) | (t.remap == 2 ? 4 : 0);
+        out[2] = (t.remap == 1 ? 1 : 0) | (t.nt << 1) | (t.remap == 2 ? 4 : 0) | (t.row << 3);
     }
 }
 
