@@ -26,6 +26,11 @@ def test_tune_persist_and_parity(tmp_path):
     assert not r1["cached"] and r1["candidates"] >= 10
     r2 = autotune.tune(s, steps=4, path=path)
     assert r2["cached"] and (r2["kernel"], r2["tile"]) == (r1["kernel"], r1["tile"])
+    assert r1["kernel"] in autotune.DEFAULT_FAMILIES
+    # the fused family is tuned on request only, and cached apart from the default tuning
+    rf = autotune.tune(s, steps=2, path=path, families=("fused",))
+    assert not rf["cached"] and rf["kernel"] == "fused"
+    assert autotune.tune(s, steps=2, path=path)["cached"]
     assert s.info()["kernel"] == r1["kernel"]
     s.set_state(even, odd)
     s.step(10)

@@ -5,10 +5,12 @@
 // out of place, reading E and O once and writing E' and O' once (64 B per
 // node-step in FP64 instead of the 96 B of two passes).
 //
-// Layout of the work.  A wave owns one cell row j and a strip of 64 x-columns
-// xs - 1 .. xs + 62 (xs = 62 bx): lanes 1..62 are its interior, lanes 0 and
-// 63 a one-column ring whose O' the interior reads as x-neighbours (DPP
-// wave shifts).  It marches the layers of a z-chunk: at step t it computes
+// Layout of the work.  A wave owns one cell row j and a strip of w <= 62
+// x-columns xs .. xs + w - 1 (xs = w bx; w is a launch parameter): lanes
+// 1..w are its interior, lanes 0 and w + 1 a one-column ring
+// whose O' the interior reads as x-neighbours (DPP wave shifts), lanes
+// beyond w + 1 idle.  It marches the layers of a z-chunk (the host sizes the
+// chunks so that one round of waves fills the chip): at step t it computes
 // O'(t) of its row and E'(t - 1) of its interior.  E'(t - 1) needs O' of the
 // rows j - 1 and j + 1 as well, but only some classes of them: the classes
 // the row below exports upwards are {5, 6, 7, 8} and the row above exports
@@ -31,12 +33,15 @@
 // operations on the same inputs, so they carry the same bits as those rows'
 // own results.
 #pragma once
+#include <type_traits>
+
 #include "gpe_fused.hpp"
 
 namespace gpe {
 
 struct RowGrid {
-    int ntx, nty, ntz;  // strips of 62 columns, blocks of TY rows, z-chunks
+    int ntx, nty, ntz;  // x-strips, blocks of TY rows, z-chunks
+    int w;              // interior columns per strip (at most 62)
     int cz;             // layers per chunk
     int remap;          // XCD-contiguous block order
 };
@@ -73,8 +78,13 @@ __device__ __forceinline__ C keep_if(bool keep, C v) {
     return v;
 }
 
+// FP64 needs about all 256 registers a wave can have at 2 waves per SIMD:
+// the compiler is held to that occupancy rather than trading it for more
+// loads in flight.
+// CZ > 0: layers per chunk fixed at compile time (a constant trip count the
+// compiler schedules better); CZ = 0: taken from the grid.
 template <typename T, int FORM, int POT, int TY, int CZ>
-__global__ void __launch_bounds__(64 * TY)
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 3)))
     k_fused_row(cplx_t<T> *__restrict__ eout, cplx_t<T> *__restrict__ oout, const cplx_t<T> *__restrict__ ein,
                 const cplx_t<T> *__restrict__ oin, const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
                 Trap trap, double dt, RowGrid G) {
@@ -84,11 +94,13 @@ __global__ void __launch_bounds__(64 * TY)
     const int bx = w % G.ntx, by = (w / G.ntx) % G.nty, bz = w / (G.ntx * G.nty);
     const int j = by * TY + (int)threadIdx.y;
     if (j >= L.ny) return;  // the whole wave: rows beyond the box
-    const int k0 = bz * CZ, k1 = min(k0 + CZ, L.nz);
+    const int cz = CZ > 0 ? CZ : G.cz;
+    const int k0 = bz * cz, k1 = min(k0 + cz, L.nz);
     const int lane = threadIdx.x;
-    const int i = bx * 62 - 1 + lane;
-    const bool dom = i >= 0 && i < L.nx;
-    const bool inner = dom && lane >= 1 && lane <= 62;
+    const int i = bx * G.w - 1 + lane;
+    // lanes past the right ring column own nothing and act as zero halo
+    const bool dom = i >= 0 && i < L.nx && lane <= G.w + 1;
+    const bool inner = dom && lane >= 1 && lane <= G.w;
     const bool ym_ok = j >= 1, yp_ok = j + 1 < L.ny;  // wave-uniform: rows j -+ 1 inside the box
     // lanes outside the box read the zero halo column x = -1 of their row
     const long long col = L.origin + (long long)j * L.sy + (dom ? i : -1);
@@ -135,8 +147,14 @@ __global__ void __launch_bounds__(64 * TY)
     };
 
     // ---- march state ---------------------------------------------------
-    // E(t-1): own classes 0, 9, 10, 11; class 9 of row j+1, class 10 of row j-1
-    C Edn0 = zero, Edn9 = zero, Edn10 = zero, Edn11 = zero, Eyp_dn9 = zero, Eym_dn10 = zero;
+    // E(t-1): own row (all classes in FP32, where E'(t-1) takes its base from
+    // here; FP64 keeps only classes 0, 9, 10, 11 and reloads the base from the
+    // caches, for registers); class 9 of row j+1, class 10 of row j-1
+    constexpr bool kCarryE = sizeof(T) == 4;
+    C Edn[12];
+#pragma unroll
+    for (int l = 0; l < 12; l++) Edn[l] = zero;
+    C Eyp_dn9 = zero, Eym_dn10 = zero;
     // O'(t-1): own row, the exported classes of rows j-1 {5, 7, 8} and j+1 {1, 9}
     C Op[12];
 #pragma unroll
@@ -146,23 +164,35 @@ __global__ void __launch_bounds__(64 * TY)
     C Opp0 = zero, Opp10 = zero, Opp11 = zero, Opp9yp = zero;
     if (k0 - 2 >= -1) {
         const uint32_t vp = vlay(k0 - 2);
-        Edn0 = ld(rE, vp, 0, 0);
-        Edn9 = ld(rE, vp, 9, 0);
-        Edn10 = ld(rE, vp, 10, 0);
-        Edn11 = ld(rE, vp, 11, 0);
-        if (yp_ok) Eyp_dn9 = ld(rE, vp, 9, 1);
-        if (ym_ok) Eym_dn10 = ld(rE, vp, 10, -1);
+        Edn[0] = ld(rE, vp, 0, 0);
+        Edn[9] = ld(rE, vp, 9, 0);
+        Edn[10] = ld(rE, vp, 10, 0);
+        Edn[11] = ld(rE, vp, 11, 0);
+        Eyp_dn9 = ld(rE, vp, 9, 1);
+        Eym_dn10 = ld(rE, vp, 10, -1);
     }
+    // rows j -+ 1 outside the box are the zero halo: their loads read zeros,
+    // and the values computed for them are forced to zero
+    const bool ym_dom = dom && ym_ok, yp_dom = dom && yp_ok;
 
-    for (int t = k0 - 1; t <= k1; t++) {
-        const bool tin = t >= 0 && t < L.nz;  // O'(t) is computed inside the box only
+    // One march step.  STEADY steps (layers k0 + 1 .. k1 - 1) have every
+    // condition true at compile time, so that their body is one basic block
+    // the scheduler can order freely; the two first steps and the last one
+    // test them.
+    auto step = [&](auto steady, int t) {
+        constexpr bool S = decltype(steady)::value;
+        const bool tin = S || (t >= 0 && t < L.nz);  // O'(t) is computed inside the box only
         const int te = t - 1;
-        const bool ework = te >= k0 && te < k1;  // E'(t-1) of this chunk
+        const bool ework = S || (te >= k0 && te < k1);  // E'(t-1) of this chunk
+        const bool wr = S || (t >= k0 && t < k1);       // O'(t) of this chunk
         const uint32_t vt = vlay(t), vte = vt - szb, vtu = vt + szb;
         // stores of lanes that own no cell go beyond the buffer and are discarded
         const uint32_t ste = inner ? vte : 0x80000000u, sto = inner ? vt : 0x80000000u;
         const RowPot<T, POT> pe{rP, vte, srow[1], trap, dt, i, j, te};
-        auto ebase = [&](int l) { return ld(rE, vte, l, 0); };
+        auto ebase = [&](int l) {
+            if constexpr (kCarryE) return Edn[l];
+            else return ld(rE, vte, l, 0);
+        };
         auto esink = [&](int l, C v) { st(wE, ste, l, v); };
 
         // ---- E'(t-1), classes 1..8: no O'(t) needed ------------------------
@@ -204,7 +234,6 @@ __global__ void __launch_bounds__(64 * TY)
             Oc[l] = zero;
         }
         OffColumn<C> oe;
-        const bool wr = t >= k0 && t < k1;  // wave-uniform: O'(t) of this chunk
         const RowPot<T, POT> po{rP, vt, srow[1], trap, dt, i, j, t};
         auto obase = [&](int l) { return ld(rO, vt, l, 0); };
         auto osink = [&](int l, C v) {
@@ -225,7 +254,7 @@ __global__ void __launch_bounds__(64 * TY)
             oe.xp0 = shift_lane_c<false>(Ec[0]);
             oe.xp5 = shift_lane_c<false>(Ec[5]);
             oe.xm_up2 = shift_lane_c<true>(nEup2);
-            oe.xp_dn0 = shift_lane_c<false>(Edn0);
+            oe.xp_dn0 = shift_lane_c<false>(Edn[0]);
             oe.ym7 = Eym7;
             oe.ym8 = Eym8;
             oe.yp1 = Eyp1;
@@ -234,9 +263,9 @@ __global__ void __launch_bounds__(64 * TY)
             oe.yp_dn9 = Eyp_dn9;
             oe.xm_yp1 = shift_lane_c<true>(Eyp1);
             oe.xp_ym5 = shift_lane_c<false>(Eym5);
-            update_class<2, T, FORM>(Ec, Edn10, Edn11, nEup2, nEup6, oe, po, H, obase, osink);
-            update_class<6, T, FORM>(Ec, Edn10, Edn11, nEup2, nEup6, oe, po, H, obase, osink);
-            if (ym_ok) {
+            update_class<2, T, FORM>(Ec, Edn[10], Edn[11], nEup2, nEup6, oe, po, H, obase, osink);
+            update_class<6, T, FORM>(Ec, Edn[10], Edn[11], nEup2, nEup6, oe, po, H, obase, osink);
+            {
                 // row j-1: its classes 4..8 and 10 at layer t, 10 at t-1; its
                 // y+1 neighbours are this row's values (each class reads its
                 // own value as well as its four neighbours)
@@ -257,11 +286,11 @@ __global__ void __launch_bounds__(64 * TY)
                 orr.xm_yp1 = shift_lane_c<true>(Ec[1]);
                 orr.yp1 = Ec[1];
                 orr.yp9 = Ec[9];
-                orr.yp_dn9 = Edn9;
+                orr.yp_dn9 = Edn[9];
                 const RowPot<T, POT> pr{rP, vt, srow[0], trap, dt, i, j - 1, t};
                 auto base = [&](int l) { return ld(rO, vt, l, -1); };
                 auto sink = [&](int l, C v) {
-                    v = keep_if(dom, v);
+                    v = keep_if(ym_dom, v);
                     if (l == 5) Ocym5 = v;
                     else if (l == 6) Ocym6 = v;
                     else if (l == 7) Ocym7 = v;
@@ -272,7 +301,7 @@ __global__ void __launch_bounds__(64 * TY)
                 update_class<7, T, FORM>(Er, Eym_dn10, zero, zero, zero, orr, pr, H, base, sink);
                 update_class<8, T, FORM>(Er, Eym_dn10, zero, zero, zero, orr, pr, H, base, sink);
             }
-            if (yp_ok) {
+            {
                 // row j+1: its classes 0..3, 9 and 11 at layer t; its y-1
                 // neighbours are this row's values
                 C Er[12];
@@ -294,7 +323,7 @@ __global__ void __launch_bounds__(64 * TY)
                 const RowPot<T, POT> pr{rP, vt, srow[2], trap, dt, i, j + 1, t};
                 auto base = [&](int l) { return ld(rO, vt, l, 1); };
                 auto sink = [&](int l, C v) {
-                    v = keep_if(dom, v);
+                    v = keep_if(yp_dom, v);
                     if (l == 1) Ocyp1 = v;
                     else Ocyp9 = v;
                 };
@@ -320,16 +349,16 @@ __global__ void __launch_bounds__(64 * TY)
 
         // ---- O'(t): the other own classes -----------------------------------
         if (tin) {
-            update_class<0, T, FORM>(Ec, Edn10, Edn11, nEup2, nEup6, oe, po, H, obase, osink);
-            update_class<1, T, FORM>(Ec, Edn10, Edn11, nEup2, nEup6, oe, po, H, obase, osink);
-            update_class<3, T, FORM>(Ec, Edn10, Edn11, nEup2, nEup6, oe, po, H, obase, osink);
-            update_class<4, T, FORM>(Ec, Edn10, Edn11, nEup2, nEup6, oe, po, H, obase, osink);
-            update_class<5, T, FORM>(Ec, Edn10, Edn11, nEup2, nEup6, oe, po, H, obase, osink);
-            update_class<7, T, FORM>(Ec, Edn10, Edn11, nEup2, nEup6, oe, po, H, obase, osink);
-            update_class<8, T, FORM>(Ec, Edn10, Edn11, nEup2, nEup6, oe, po, H, obase, osink);
-            update_class<9, T, FORM>(Ec, Edn10, Edn11, nEup2, nEup6, oe, po, H, obase, osink);
-            update_class<10, T, FORM>(Ec, Edn10, Edn11, nEup2, nEup6, oe, po, H, obase, osink);
-            update_class<11, T, FORM>(Ec, Edn10, Edn11, nEup2, nEup6, oe, po, H, obase, osink);
+            update_class<0, T, FORM>(Ec, Edn[10], Edn[11], nEup2, nEup6, oe, po, H, obase, osink);
+            update_class<1, T, FORM>(Ec, Edn[10], Edn[11], nEup2, nEup6, oe, po, H, obase, osink);
+            update_class<3, T, FORM>(Ec, Edn[10], Edn[11], nEup2, nEup6, oe, po, H, obase, osink);
+            update_class<4, T, FORM>(Ec, Edn[10], Edn[11], nEup2, nEup6, oe, po, H, obase, osink);
+            update_class<5, T, FORM>(Ec, Edn[10], Edn[11], nEup2, nEup6, oe, po, H, obase, osink);
+            update_class<7, T, FORM>(Ec, Edn[10], Edn[11], nEup2, nEup6, oe, po, H, obase, osink);
+            update_class<8, T, FORM>(Ec, Edn[10], Edn[11], nEup2, nEup6, oe, po, H, obase, osink);
+            update_class<9, T, FORM>(Ec, Edn[10], Edn[11], nEup2, nEup6, oe, po, H, obase, osink);
+            update_class<10, T, FORM>(Ec, Edn[10], Edn[11], nEup2, nEup6, oe, po, H, obase, osink);
+            update_class<11, T, FORM>(Ec, Edn[10], Edn[11], nEup2, nEup6, oe, po, H, obase, osink);
         }
 
         // ---- advance --------------------------------------------------------
@@ -340,13 +369,19 @@ __global__ void __launch_bounds__(64 * TY)
         Opym8 = Ocym8;
         Opyp1 = Ocyp1;
         Opyp9 = Ocyp9;
-        Edn0 = Ec[0];
-        Edn9 = Ec[9];
-        Edn10 = Ec[10];
-        Edn11 = Ec[11];
+#pragma unroll
+        for (int l = 0; l < 12; l++)
+            if (kCarryE || l == 0 || l == 9 || l == 10 || l == 11) Edn[l] = Ec[l];
         Eyp_dn9 = Eyp9;
         Eym_dn10 = Eym10;
-    }
+    };
+
+    using Edge = std::integral_constant<bool, false>;
+    using Steady = std::integral_constant<bool, true>;
+    int t = k0 - 1;
+    for (; t <= k0; t++) step(Edge{}, t);
+    for (; t < k1; t++) step(Steady{}, t);
+    step(Edge{}, k1);
 }
 
 }  // namespace gpe

@@ -75,8 +75,8 @@ constexpr TileShape kZmarchTiles[] = {{4, 8, 1, 0}, {4, 8, 0, 1}, {1, 8, 1, 0}, 
 // fused, row design (row = 1): ty = waves (rows) per block, cz = layers per z-chunk
 constexpr TileShape kFusedTiles[] = {{8, 16, 1, 0}, {4, 16, 1, 0}, {4, 32, 1, 0}, {8, 32, 1, 0}, {8, 8, 1, 0},
                                      {16, 16, 1, 0}, {4, 8, 1, 0}, {8, 16, 0, 0}, {6, 16, 1, 0}, {6, 32, 1, 0},
-                                     {4, 16, 1, 0, 1}, {4, 32, 1, 0, 1}, {2, 32, 1, 0, 1}, {8, 32, 1, 0, 1},
-                                     {4, 64, 1, 0, 1}, {1, 32, 1, 0, 1}, {4, 32, 0, 0, 1}};
+                                     {2, 32, 1, 0, 1}, {1, 32, 1, 0, 1}, {4, 32, 1, 0, 1}, {2, 16, 1, 0, 1},
+                                     {1, 16, 1, 0, 1}, {2, 0, 1, 0, 1}, {1, 0, 1, 0, 1}};
 constexpr int kNumGatherTiles = sizeof(kGatherTiles) / sizeof(kGatherTiles[0]);
 constexpr int kNumZmarchTiles = sizeof(kZmarchTiles) / sizeof(kZmarchTiles[0]);
 constexpr int kNumFusedTiles = sizeof(kFusedTiles) / sizeof(kFusedTiles[0]);
@@ -87,7 +87,9 @@ int num_tiles(int kernel) {
 
 // Tile index used when the caller asks for tile 0.
 int default_tile(int kernel, uint32_t precision) {
-    if (kernel == GPE_KERNEL_FUSED) return precision == GPE_FP64 ? 1 : 0;
+    // the one-wave-per-row tiles measured fastest: {2 waves, 32 layers} in
+    // FP64, {2 waves, 16 layers} in FP32
+    if (kernel == GPE_KERNEL_FUSED) return precision == GPE_FP64 ? 10 : 13;
     if (kernel == GPE_KERNEL_GATHER) return 10;  // 4 waves, streaming RMW, y-bands per XCD
     return 0;
 }
@@ -129,18 +131,43 @@ void launch_fused_t(cplx_t<T> *eo, cplx_t<T> *oo, const cplx_t<T> *ei, const cpl
                        pot, L, H, trap, dt, G);
 }
 
+// Grid of the row-fused kernel: x-strips of 62 interior columns (the last
+// one partial); z-chunks of cz layers, or (cz = 0) as many layers as make one
+// round of waves fill every wave slot of the chip, since each wave marches
+// its chunk from start to end and a second, partial round would leave most
+// slots idle.
 template <typename T, int FORM, int POT, int TY, int CZ>
-void launch_fused_row_t(cplx_t<T> *eo, cplx_t<T> *oo, const cplx_t<T> *ei, const cplx_t<T> *oi, const T *pot,
-                        const Layout &L, const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, int remap,
-                        hipStream_t st) {
+hipError_t launch_fused_row_t(cplx_t<T> *eo, cplx_t<T> *oo, const cplx_t<T> *ei, const cplx_t<T> *oi, const T *pot,
+                              const Layout &L, const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, int cz,
+                              int remap, hipStream_t st) {
+    const auto kern = k_fused_row<T, FORM, POT, TY, CZ>;
     RowGrid G;
     G.ntx = (L.nx + 61) / 62;
+    G.w = 62;  // full strips measured faster than strips of nx / ntx columns
     G.nty = (L.ny + TY - 1) / TY;
-    G.ntz = (L.nz + CZ - 1) / CZ;
-    G.cz = CZ;
+    if (cz <= 0) {
+        static int blocks_per_cu = -1;  // per instantiation: a property of the compiled kernel
+        if (blocks_per_cu < 0) {
+            int nb = 0;
+            const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 64 * TY, 0);
+            if (e != hipSuccess) return e;
+            blocks_per_cu = nb > 0 ? nb : 1;
+        }
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        const long long slots = (long long)blocks_per_cu * cus;  // blocks resident at once
+        const long long cols = (long long)G.ntx * G.nty;
+        const int ntz = (int)std::max(1LL, std::min<long long>(L.nz, slots / cols));
+        cz = (L.nz + ntz - 1) / ntz;
+    }
+    G.cz = cz;
+    G.ntz = (L.nz + cz - 1) / cz;
     G.remap = remap;
-    hipLaunchKernelGGL((k_fused_row<T, FORM, POT, TY, CZ>), dim3(G.ntx * G.nty * G.ntz), dim3(64, TY), 0, st, eo, oo,
-                       ei, oi, pot, L, H, trap, dt, G);
+    hipLaunchKernelGGL(kern, dim3(G.ntx * G.nty * G.ntz), dim3(64, TY), 0, st, eo, oo, ei, oi, pot, L, H, trap, dt,
+                       G);
+    return hipGetLastError();
 }
 
 template <typename T, int FORM, int POT>
@@ -150,18 +177,18 @@ hipError_t launch_fused(int tile, cplx_t<T> *eo, cplx_t<T> *oo, const cplx_t<T> 
     const TileShape t = tile_shape(GPE_KERNEL_FUSED, tile);
     if (t.row) {
 #define GPE_ROW(TYV, CZV) \
-    launch_fused_row_t<T, FORM, POT, TYV, CZV>(eo, oo, ei, oi, pot, L, H, trap, dt, t.remap, st)
+    return launch_fused_row_t<T, FORM, POT, TYV, CZV>(eo, oo, ei, oi, pot, L, H, trap, dt, CZV, t.remap, st)
         switch (t.ty * 1000 + t.cz) {
-            case 1032: GPE_ROW(1, 32); break;
-            case 2032: GPE_ROW(2, 32); break;
-            case 4016: GPE_ROW(4, 16); break;
-            case 4032: GPE_ROW(4, 32); break;
-            case 4064: GPE_ROW(4, 64); break;
-            case 8032: GPE_ROW(8, 32); break;
+            case 1000: GPE_ROW(1, 0);
+            case 2000: GPE_ROW(2, 0);
+            case 1016: GPE_ROW(1, 16);
+            case 2016: GPE_ROW(2, 16);
+            case 1032: GPE_ROW(1, 32);
+            case 2032: GPE_ROW(2, 32);
+            case 4032: GPE_ROW(4, 32);
             default: return hipErrorInvalidValue;
         }
 #undef GPE_ROW
-        return hipGetLastError();
     }
     const FusedGrid G = make_fused_grid(L, t);
     switch (t.ty) {
