@@ -59,7 +59,7 @@ struct RowPot {
     double dt;
     int i, j, k;
     __device__ __forceinline__ T operator[](int l) const {
-        if (POT == POT_TRAP) return (T)(trap_v(trap, i, j, k, l) * dt);
+        if (POT == POT_TRAP) return (T)(trap_v4(trap, i, j, k, l) * dt);
         const uint32_t s = __builtin_amdgcn_readfirstlane(srow[l] >> 1);
         if constexpr (sizeof(T) == 8)
             return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, vlayer >> 1, s, 0));

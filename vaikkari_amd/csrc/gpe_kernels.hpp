@@ -41,15 +41,15 @@ struct Layout {
 struct Trap {
     double p0[3];
     double b;
-    double bw;    // sqrt(8)
-    double rr;    // ratio*ratio
-    double q[4];  // (2n + 1)/sqrt(8): the four coordinates a node takes inside its cell
+    double bw;           // sqrt(8)
+    double rr;           // ratio*ratio
+    double bpos[12][3];  // q_l/sqrt(8)
+    double q[4];         // (2n + 1)/sqrt(8): the four values bpos takes
 };
 
 // Node position of class l inside its cell, per axis, in units of b/sqrt(8)
 // (1, 3, 5 or 7; GPUGrossPitaevskii_BCC.cu:76-88), as the index n of
-// Trap::q.  Kernels index it with compile-time classes, so the trap's
-// offsets are the 4 values q[n] rather than 36.
+// Trap::q (bpos[l][a] == q[class_q(l, a)]).
 __host__ __device__ constexpr int class_q(int l, int a) {
     constexpr int pos[12][3] = {{1, 3, 7}, {7, 1, 3}, {7, 3, 1}, {7, 3, 5}, {7, 5, 3}, {1, 7, 3},
                                 {3, 7, 1}, {3, 7, 5}, {5, 7, 3}, {3, 1, 7}, {3, 5, 7}, {5, 3, 7}};
@@ -71,10 +71,31 @@ enum { MODE_REAL = 0, MODE_IMAG = 1 };
 
 __device__ __forceinline__ double trap_v(const Trap &t, int i, int j, int k, int l) {
 #pragma clang fp contract(off)
+    const double x = t.p0[0] + t.b * ((double)(unsigned)i * t.bw + t.bpos[l][0]);
+    const double y = t.p0[1] + t.b * ((double)(unsigned)j * t.bw + t.bpos[l][1]);
+    const double z = t.p0[2] + t.b * ((double)(unsigned)k * t.bw + t.bpos[l][2]);
+    return 0.5 * (x * x + y * y + t.rr * z * z);
+}
+
+// The same value from the 4 distinct offsets.  Kernels with many live
+// values use this form: the 36 offsets of bpos take 72 scalar registers.
+// The one-thread-per-cell kernels keep bpos, where the compiler's sharing of
+// the 4 offsets across classes costs vector registers and occupancy.
+__device__ __forceinline__ double trap_v4(const Trap &t, int i, int j, int k, int l) {
+#pragma clang fp contract(off)
     const double x = t.p0[0] + t.b * ((double)(unsigned)i * t.bw + t.q[class_q(l, 0)]);
     const double y = t.p0[1] + t.b * ((double)(unsigned)j * t.bw + t.q[class_q(l, 1)]);
     const double z = t.p0[2] + t.b * ((double)(unsigned)k * t.bw + t.q[class_q(l, 2)]);
     return 0.5 * (x * x + y * y + t.rr * z * z);
+}
+
+// V of a node in a one-thread-per-cell kernel computing in T (the same bits
+// either way): FP64 takes the per-class offsets, which leave it at 5 waves
+// per SIMD; FP32 the 4-offset form, 1.4 % faster there at 128^3 cells.
+template <typename T>
+__device__ __forceinline__ double trap_vt(const Trap &t, int i, int j, int k, int l) {
+    if constexpr (sizeof(T) == 4) return trap_v4(t, i, j, k, l);
+    else return trap_v(t, i, j, k, l);
 }
 
 // s = dt*H(psi)psi from the four neighbour values, |psi|^2 and V*dt.
@@ -259,7 +280,7 @@ __device__ __forceinline__ double gather_tile(cplx_t<T> *__restrict__ next, cons
                 }
             }
             T pv;
-            if (POT == POT_TRAP) pv = (T)(trap_v(trap, i, j, k, l) * dt);
+            if (POT == POT_TRAP) pv = (T)(trap_vt<T>(trap, i, j, k, l) * dt);
             else pv = pot[base + l * L.xp];
             if (OBS) {
                 const double pr = (double)own[l].x, pi = (double)own[l].y;
@@ -414,7 +435,7 @@ __device__ __forceinline__ double zmarch_tile(cplx_t<T> *__restrict__ next, cons
                     }
                 }
                 T pv;
-                if (POT == POT_TRAP) pv = (T)(trap_v(trap, i, j, k, l) * dt);
+                if (POT == POT_TRAP) pv = (T)(trap_vt<T>(trap, i, j, k, l) * dt);
                 else pv = pot[base + l * L.xp];
                 const C sres = H.apply(n, cur[l], pv);
                 const long long at = base + l * L.xp;
@@ -648,9 +669,9 @@ __global__ void k_init_gaussian(cplx_t<T> *__restrict__ even, cplx_t<T> *__restr
         const int i = (int)(c % L.nx), j = (int)((c / L.nx) % L.ny), k = (int)(c / ((long long)L.nx * L.ny));
         const long long base = L.origin + (long long)k * L.sz + (long long)j * L.sy + i;
         for (int l = 0; l < 12; l++) {
-            const double x = trap.p0[0] + trap.b * ((double)(unsigned)i * trap.bw + trap.q[class_q(l, 0)]);
-            const double y = trap.p0[1] + trap.b * ((double)(unsigned)j * trap.bw + trap.q[class_q(l, 1)]);
-            const double z = trap.p0[2] + trap.b * ((double)(unsigned)k * trap.bw + trap.q[class_q(l, 2)]);
+            const double x = trap.p0[0] + trap.b * ((double)(unsigned)i * trap.bw + trap.bpos[l][0]);
+            const double y = trap.p0[1] + trap.b * ((double)(unsigned)j * trap.bw + trap.bpos[l][1]);
+            const double z = trap.p0[2] + trap.b * ((double)(unsigned)k * trap.bw + trap.bpos[l][2]);
             const double a = amp * exp(-(x * x + y * y + z * z) * inv2s2);
             const unsigned long long h1 = splitmix64(seed ^ (unsigned long long)(c * 12 + l) * 0xD1B54A32D192ED03ULL);
             const unsigned long long h2 = splitmix64(h1);

@@ -44,6 +44,8 @@ Trap make_trap(const double origin[3], double b, double ratio) {
     t.bw = std::sqrt(8.0);
     t.rr = ratio * ratio;
     const double sq18 = 1.0 / std::sqrt(8.0);
+    for (int l = 0; l < 12; l++)
+        for (int a = 0; a < 3; a++) t.bpos[l][a] = sq18 * kClassPos[l][a];
     for (int n = 0; n < 4; n++) t.q[n] = sq18 * (2 * n + 1);
     return t;
 }
