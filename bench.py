@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--batch-tile", default="",
                     help="batched suite: fixed real-time tile instead of the autotuner's, waves followed by "
                          "c (cached read-modify-write) and/or s (slab order), e.g. 2 or 1cs")
+    ap.add_argument("--batch-sizes", default="",
+                    help="batched suite: cube edges to draw the instances from (comma list; default "
+                         "24,32,48,64,96)")
     ap.add_argument("--batch-group-mib", type=int, default=None,
                     help="batched suite: fixed group size of the real-time schedule in MiB (0 = one group)")
     ap.add_argument("--batch-launch", default="graph", choices=["graph", "stream"],
@@ -84,6 +87,8 @@ def parse():
     ap.add_argument("--primal-cells", default="128,256",
                     help="cube sizes of the 14-point primal stepper sweep (comma list; empty = skip)")
     ap.add_argument("--primal-steps", type=int, default=200)
+    ap.add_argument("--primal-tiles", default="",
+                    help="primal suite: tile numbers to sweep (comma list; default every tile)")
     ap.add_argument("--precision-cells", type=int, default=256)
     ap.add_argument("--precision-steps", type=int, default=10000)
     ap.add_argument("--precision-every", type=int, default=1000)
@@ -174,7 +179,7 @@ def cpu_baseline(warmup, runs, steps128):
                       f"the 32^3 CPU-form grid timed the same way in 'grids'"}
 
 
-def batched_workload(n_inst, steps, seed=4, log=log, launch="graph", tile=None, group_mib=None):
+def batched_workload(n_inst, steps, seed=4, log=log, launch="graph", tile=None, group_mib=None, sizes=None):
     """64 independent condensates: n^3 cells with n uniform in {24,32,48,64,96}, cube edge 0.25,
     g log-uniform in [100, 5000], trap anisotropy uniform in [0.5, 4], dt = 1e-3*32/n capped by
     the reference's stability rule, FP32 bulk; Gaussian + noise initial states (device recipe).
@@ -185,7 +190,7 @@ def batched_workload(n_inst, steps, seed=4, log=log, launch="graph", tile=None, 
 
     from vaikkari_amd import workloads as wl
 
-    cfgs = wl.instance_configs(wl.batched_form(n_inst, steps, seed=seed))
+    cfgs = wl.instance_configs(wl.batched_form(n_inst, steps, seed=seed, sizes=sizes or wl.BATCH_SIZES))
 
     def make(waves, cfg_list=cfgs):
         b = Batch("fp32")
@@ -264,12 +269,13 @@ def serving_suite(args):
     print(json.dumps({"suite": "serving", **res}), flush=True)
 
 
-def primal_workload(sizes, steps, warmup=5, log=log):
+def primal_workload(sizes, steps, warmup=5, log=log, tiles_only=None):
     """The 14-point primal BCC stepper (2 sites per cubic cell): FP64 and FP32
     leapfrog steps of a Gaussian condensate (sigma 2, kick e^{0.3ix}) in the trap
     0.5 r^2 on a box of edge 16, g = 100, dt = 0.4 / (7.5/a^2 + max V), over every
     tile shape.  One launch updates every site once: FP64 reads prev (16 B) and
-    reads + writes next (32 B), 48 B per site; FP32 24 B."""
+    reads + writes next (32 B), 48 B per site; FP32 24 B.  tiles_only limits the sweep to
+    those tile numbers (the default shape is always timed)."""
     import vaikkari_amd as va
 
     out = []
@@ -290,7 +296,7 @@ def primal_workload(sizes, steps, warmup=5, log=log):
         mu = None
         for prec, bps in (("fp64", 48), ("fp32", 24)):
             tiles = {}
-            for tile in range(1, va.primal_tiles() + 1):
+            for tile in (tiles_only or range(1, va.primal_tiles() + 1)):
                 with va.PrimalSolver(dims, a, dt, g, origin, 1.0, prec, "real", tile) as s:
                     s.set_state(psi)
                     if mu is None:
@@ -461,7 +467,8 @@ def main():
             import vaikkari_amd as va
 
             copy_gbs = va.copy_bandwidth(1 << 30, 3)
-            res = primal_workload([int(x) for x in args.primal_cells.split(",")], args.primal_steps)
+            only = [int(x) for x in args.primal_tiles.split(",")] if args.primal_tiles else None
+            res = primal_workload([int(x) for x in args.primal_cells.split(",")], args.primal_steps, tiles_only=only)
             print(json.dumps({"suite": "primal", "hbm_copy_GBs": copy_gbs, "primal14": res}), flush=True)
         elif args.suite == "batched":
             import vaikkari_amd as va
@@ -471,8 +478,9 @@ def main():
             if args.batch_tile:
                 w, flags = int(args.batch_tile.rstrip("cs")), args.batch_tile.lstrip("0123456789")
                 tile = (w, "c" in flags, "s" in flags)
+            sizes = [int(x) for x in args.batch_sizes.split(",")] if args.batch_sizes else None
             res = batched_workload(args.batch_instances, args.batch_steps, launch=args.batch_launch, tile=tile,
-                                   group_mib=args.batch_group_mib)
+                                   group_mib=args.batch_group_mib, sizes=sizes)
             print(json.dumps({"suite": "batched", "hbm_copy_GBs": copy_gbs, "batched": res}), flush=True)
         else:
             serving_suite(args)

@@ -189,15 +189,21 @@ def test_vortex_state_from_table_equals_file(tmp_path):
 
 @pytest.mark.parametrize("precision,eb", [("fp64", 16), ("fp32", 8)])
 def test_field_layout(precision, eb):
-    """gpe_get_field_layout: rows start on 128-byte lines behind a one-line left pad that holds
-    the x halo, 12 class rows per cell row, a one-cell halo all round; pack_field puts node
-    (i, j, k, l) at origin + k sz + j sy + l xp + i and leaves everything else zero."""
+    """gpe_get_field_layout: rows start on 128-byte lines and hold nx + 1 elements or more (the
+    x halo at nx, and x = -1 as the last padding element of the row before), 12 class rows per
+    cell row, a one-cell halo all round, one line in front of the first row; pack_field puts
+    node (i, j, k, l) at origin + k sz + j sy + l xp + i and leaves everything else zero."""
     dims = (37, 5, 4)
     L = va.field_layout(dims, precision)
     line = 128 // eb
-    assert L["elem_bytes"] == eb and L["xp"] % line == 0 and L["xp"] >= line + dims[0] + 1
+    assert L["elem_bytes"] == eb and L["xp"] % line == 0 and L["xp"] >= dims[0] + 1
+    assert L["xp"] - line < dims[0] + 1  # no line of a row is padding alone
     assert L["sy"] == 12 * L["xp"] and L["sz"] == (dims[1] + 2) * L["sy"]
-    assert L["origin"] == L["sz"] + L["sy"] + line and L["elems"] == (dims[2] + 2) * L["sz"]
+    assert L["origin"] == L["sz"] + L["sy"] + line and L["elems"] == (dims[2] + 2) * L["sz"] + line
+    # x = -1 of the first halo row and x = nx of the last one lie inside the buffer
+    assert L["origin"] - L["sz"] - L["sy"] - 1 >= 0
+    top = L["origin"] + dims[2] * L["sz"] + dims[1] * L["sy"] + 11 * L["xp"] + dims[0]
+    assert top < L["elems"]
     rng = np.random.default_rng(1)
     psi = rng.uniform(-1, 1, 2 * va.nodes_of(dims))
     packed = va.pack_field(dims, psi, precision)

@@ -10,8 +10,11 @@
 // shifted by dx + dy*sy + dz*sz: still coalesced, no gathers.  One zero halo
 // cell surrounds the box on every side (homogeneous Dirichlet boundary, as
 // the never-written halo of the reference's pitched buffers,
-// GPUGrossPitaevskii_BCC.cu:269-287); the x halo sits inside a 128-byte left
-// pad so every interior row starts cache-line aligned.
+// GPUGrossPitaevskii_BCC.cu:269-287).  Rows start on 128-byte lines and
+// xp >= nx + 1, so the x halo at x = nx is padding of its own row and the one
+// at x = -1 is the last padding element of the row before it (one line in
+// front of the first row holds that of the first): no line of a row holds
+// only halo on the left, and the line that ends a row serves both halos.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -27,7 +30,7 @@ template <typename T> using cplx_t = typename cplx<T>::type;
 
 struct Layout {
     int nx, ny, nz;
-    int padl;            // elements of left pad (x halo at padl-1)
+    int line;            // elements per 128-byte line
     long long xp;        // class-row pitch in elements
     long long sy;        // 12*xp
     long long sz;        // (ny+2)*sy
@@ -182,6 +185,7 @@ struct Grid3 {
     int gx, gy, gz, remap;
     int lsh;  // log2 of the lanes one x-row of cells takes (6: a wave per row; 5, 4: 2 or 4 rows per wave)
     int rev;  // 1: tiles in descending order (the pass starts where the previous one ended)
+    int cz;   // layers per z-chunk of z-march kernels that take it at launch (0: compile-time chunks)
 };
 
 // Lanes per x-row for an nx-cell grid: the width in {64, 32, 16} with the

@@ -545,7 +545,7 @@ template <typename T, int FORM, int MODE, int TY, int CZ, bool NT, bool LY, int 
 __device__ __forceinline__ double primal_column_b(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                                   const Layout &L, const PCoef<T> &H, const PTrap &trap, double dt,
                                                   const float *__restrict__ vzt, const double *__restrict__ scale,
-                                                  int bx, int by, int bz, int lsh) {
+                                                  int bx, int by, int bz, int lsh, int czr) {
 #pragma clang fp contract(off)
     using C = cplx_t<T>;
     constexpr uint32_t ES = sizeof(C);
@@ -556,8 +556,10 @@ __device__ __forceinline__ double primal_column_b(cplx_t<T> *__restrict__ next, 
     const int xl = threadIdx.x & (lw - 1);
     const int i = (bx << lsh) + xl;
     const int j = ((by * TY + threadIdx.y) << (6 - lsh)) + (threadIdx.x >> lsh);
-    const int k0 = bz * CZ;
-    const int k1 = min(k0 + CZ, L.nz);
+    // CZ > 0: layers per chunk at compile time; CZ = 0: czr, from the launch
+    const int cz = CZ > 0 ? CZ : czr;
+    const int k0 = bz * cz;
+    const int k1 = min(k0 + cz, L.nz);
     const bool act = i < L.nx && j < L.ny;
     const bool left_end = act && xl == 0, right_end = act && xl == lw - 1;
     const uint32_t xp = (uint32_t)L.xp, sy = (uint32_t)L.sy, sz = (uint32_t)L.sz;
@@ -632,11 +634,8 @@ __device__ __forceinline__ double primal_column_b(cplx_t<T> *__restrict__ next, 
     const double xy0 = site_xy(trap, i, j, 0.0), xy1 = site_xy(trap, i, j, 0.5);
     const T vxy0 = (T)(0.5 * xy0 * dt), vxy1 = (T)(0.5 * xy1 * dt);
     double part = 0.0;
-#pragma unroll
-    for (int kk = 0; kk < CZ; kk++, lay += sz) {
-        const int k = k0 + kk;
-        if (k >= k1) break;
-        ZLayer<T> &cur = buf[kk % (PD + 1)];
+    // layer k (lay = k sz) from the buffer cur, prefetching layer k + PD into nxt
+    auto step = [&](int k, uint32_t lay, ZLayer<T> &cur, ZLayer<T> &nxt) {
         if (LY) {
             cur.cym = ld(lay, 0, -1, 0);
             cur.u2 = ld(lay, 0, -1, 1);
@@ -645,7 +644,7 @@ __device__ __forceinline__ double primal_column_b(cplx_t<T> *__restrict__ next, 
         // the prefetch also runs at the chunk's last layers (a branch here would
         // make the compiler merge the buffers with register copies): beyond the
         // halo layer nz + 1 the buffer's range check returns zeros, never used
-        fetch(lay + (uint32_t)PD * sz, buf[(kk + PD) % (PD + 1)]);
+        fetch(lay + (uint32_t)PD * sz, nxt);
         __builtin_amdgcn_sched_barrier(0);
         C w2 = shift_lane_c<true>(cur.u2), w3 = shift_lane_c<true>(c1), xm = shift_lane_c<true>(c0);
         C e2 = shift_lane_c<false>(cur.czp), e3 = shift_lane_c<false>(cur.v3), zxp = shift_lane_c<false>(c1);
@@ -702,6 +701,24 @@ __device__ __forceinline__ double primal_column_b(cplx_t<T> *__restrict__ next, 
         cyp = cur.v3;
         e1 = e3;
         xq = e2;
+    };
+    if constexpr (CZ > 0) {
+#pragma unroll
+        for (int kk = 0; kk < CZ; kk++, lay += sz) {
+            if (k0 + kk >= k1) break;
+            step(k0 + kk, lay, buf[kk % (PD + 1)], buf[(kk + PD) % (PD + 1)]);
+        }
+    } else {
+        // a run-time chunk: groups of PD + 1 layers, so that every buffer
+        // keeps fixed registers, then the rest
+        int k = k0;
+        for (; k + PD < k1; k += PD + 1, lay += (uint32_t)(PD + 1) * sz) {
+#pragma unroll
+            for (int q = 0; q <= PD; q++) step(k + q, lay + (uint32_t)q * sz, buf[q], buf[(q + PD) % (PD + 1)]);
+        }
+#pragma unroll
+        for (int q = 0; q < PD; q++)
+            if (k + q < k1) step(k + q, lay + (uint32_t)q * sz, buf[q], buf[(q + PD) % (PD + 1)]);
     }
     return part;
 }
@@ -716,7 +733,7 @@ __global__ void __launch_bounds__(64 * TY)
     block_coords(G, bx, by, bz);
     double part[1];
     part[0] = primal_column_b<T, FORM, MODE, TY, CZ, NT, LY, PD>(next, prev, L, H, trap, dt, vzt, scale, bx, by, bz,
-                                                                 G.lsh);
+                                                                 G.lsh, G.cz);
     if (MODE == MODE_IMAG) {
         block_sum<1, TY>(part, lds);
         if (threadIdx.x == 0 && threadIdx.y == 0) partials[bx + G.gx * (by + G.gy * bz)] = part[0];
@@ -774,7 +791,7 @@ __device__ __forceinline__ double primal_column_p(cplx_t<float> *__restrict__ ne
                                                   const cplx_t<float> *__restrict__ prev, const Layout &L,
                                                   const PCoef<float> &H, const PTrap &trap, double dt,
                                                   const float *__restrict__ vzt, const double *__restrict__ scale,
-                                                  int bx, int by, int bz) {
+                                                  int bx, int by, int bz, int czr) {
 #pragma clang fp contract(off)
     using C = cplx_t<float>;
     using P = CP<float>;
@@ -785,8 +802,9 @@ __device__ __forceinline__ double primal_column_p(cplx_t<float> *__restrict__ ne
     const int lane = threadIdx.x;
     const int i = bx * 128 + 2 * lane;  // first cell of the lane
     const int j = by * TY + threadIdx.y;
-    const int k0 = bz * CZ;
-    const int k1 = min(k0 + CZ, L.nz);
+    const int cz = CZ > 0 ? CZ : czr;  // CZ = 0: layers per chunk from the launch
+    const int k0 = bz * cz;
+    const int k1 = min(k0 + cz, L.nz);
     const bool row = j < L.ny;
     const bool actA = row && i < L.nx, actB = row && i + 1 < L.nx;
     const bool left_end = row && lane == 0, right_end = row && lane == 63;
@@ -865,17 +883,14 @@ __device__ __forceinline__ double primal_column_p(cplx_t<float> *__restrict__ ne
     const float vA0 = (float)(0.5 * (xa * xa + yy * yy) * dt), vB0 = (float)(0.5 * (xb * xb + yy * yy) * dt);
     const float vA1 = (float)(0.5 * (xa1 * xa1 + yy1 * yy1) * dt), vB1 = (float)(0.5 * (xb1 * xb1 + yy1 * yy1) * dt);
     double part = 0.0;
-#pragma unroll
-    for (int kk = 0; kk < CZ; kk++, lay += sz) {
-        const int k = k0 + kk;
-        if (k >= k1) break;
-        ZLayer2 &cur = buf[kk & 1];
+    // layer k (lay = k sz) from the buffer cur, prefetching layer k + 1 into nxt
+    auto step = [&](int k, uint32_t lay, ZLayer2 &cur, ZLayer2 &nxt) {
         if (LY) {
             cur.cym = ld(lay, 0, -1, 0);
             cur.u2 = ld(lay, 0, -1, 1);
             cur.zyp = ld(lay, 0, 1, 1);
         }
-        fetch(lay + sz, buf[(kk + 1) & 1]);
+        fetch(lay + sz, nxt);
         __builtin_amdgcn_sched_barrier(0);
         P w2 = pair_from_left(cur.u2), w3 = pair_from_left(c1), xm = pair_from_left(c0);
         P e2 = pair_from_right(cur.czp), e3 = pair_from_right(cur.v3), zxp = pair_from_right(c1);
@@ -953,12 +968,29 @@ __device__ __forceinline__ double primal_column_p(cplx_t<float> *__restrict__ ne
         cyp = cur.v3;
         e1 = e3;
         xq = e2;
+    };
+    if constexpr (CZ > 0) {
+#pragma unroll
+        for (int kk = 0; kk < CZ; kk++, lay += sz) {
+            if (k0 + kk >= k1) break;
+            step(k0 + kk, lay, buf[kk & 1], buf[(kk + 1) & 1]);
+        }
+    } else {
+        // a run-time chunk: pairs of layers (fixed registers per buffer), then the rest
+        int k = k0;
+        for (; k + 1 < k1; k += 2, lay += 2 * sz) {
+            step(k, lay, buf[0], buf[1]);
+            step(k + 1, lay + sz, buf[1], buf[0]);
+        }
+        if (k < k1) step(k, lay, buf[0], buf[1]);
     }
     return part;
 }
 
-template <int MODE, int TY, int CZ, bool NT, bool LY>
-__global__ void __launch_bounds__(64 * TY)
+// W > 0 holds the compiler to W waves per SIMD (at most 512 / W registers
+// per lane) instead of its own trade of occupancy for registers.
+template <int MODE, int TY, int CZ, bool NT, bool LY, int W>
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1)))
     k_primal_zp(cplx_t<float> *__restrict__ next, const cplx_t<float> *__restrict__ prev, Layout L, PCoef<float> H,
                 PTrap trap, double dt, const float *__restrict__ vzt, const double *__restrict__ scale,
                 double *__restrict__ partials, Grid3 G) {
@@ -966,7 +998,7 @@ __global__ void __launch_bounds__(64 * TY)
     int bx, by, bz;
     block_coords(G, bx, by, bz);
     double part[1];
-    part[0] = primal_column_p<MODE, TY, CZ, NT, LY>(next, prev, L, H, trap, dt, vzt, scale, bx, by, bz);
+    part[0] = primal_column_p<MODE, TY, CZ, NT, LY>(next, prev, L, H, trap, dt, vzt, scale, bx, by, bz, G.cz);
     if (MODE == MODE_IMAG) {
         block_sum<1, TY>(part, lds);
         if (threadIdx.x == 0 && threadIdx.y == 0) partials[bx + G.gx * (by + G.gy * bz)] = part[0];
@@ -1217,7 +1249,8 @@ __global__ void __launch_bounds__(64 * kObsTY) k_primal_observables(const cplx_t
 // Tile shapes: waves per block, block order over the XCDs (block_coords),
 // non-temporal read-modify-write, x-neighbour exchange (XS_*), and layers per
 // block (0: one layer per block, primal_cell; > 0: a z-march, primal_column,
-// which always exchanges by DPP).
+// which always exchanges by DPP; -1: a buffer z-march whose chunks are sized
+// at launch, runtime_chunks).
 struct PTile {
     int ty, remap, nt, xs, cz;
     int ly;     // z-march: y-neighbour rows loaded in the layer instead of prefetched
@@ -1225,6 +1258,7 @@ struct PTile {
     int buf;    // z-march with buffer-resource addressing (k_primal_zb; fields below 2 GiB):
                 // the layers of prefetch (1 or 2)
     int pair;   // FP32: two cells per lane (k_primal_zp); FP64 runs k_primal_zb of the same shape
+    int waves;  // FP32 two cells per lane: waves per SIMD the registers must allow (0: the compiler's choice)
 };
 constexpr PTile kPrimalTiles[] = {
     {4, 2, 1, XS_DPP, 0},  {4, 2, 1, XS_LOAD, 0}, {2, 2, 1, XS_DPP, 0},  {8, 2, 1, XS_DPP, 0},
@@ -1239,7 +1273,11 @@ constexpr PTile kPrimalTiles[] = {
     {4, 2, 1, XS_DPP, 16, 1, 0, 2}, {4, 2, 1, XS_DPP, 16, 0, 0, 2}, {2, 2, 1, XS_DPP, 16, 1, 0, 2},
     {4, 2, 0, XS_DPP, 16, 1, 0, 2}, {8, 2, 1, XS_DPP, 16, 1, 0, 2},
     {4, 2, 1, XS_DPP, 16, 0, 0, 1, 1}, {4, 2, 1, XS_DPP, 16, 1, 0, 1, 1}, {2, 2, 1, XS_DPP, 16, 0, 0, 1, 1},
-    {8, 2, 1, XS_DPP, 16, 0, 0, 1, 1}, {4, 2, 0, XS_DPP, 16, 0, 0, 1, 1}, {4, 2, 1, XS_DPP, 8, 0, 0, 1, 1}};
+    {8, 2, 1, XS_DPP, 16, 0, 0, 1, 1}, {4, 2, 0, XS_DPP, 16, 0, 0, 1, 1}, {4, 2, 1, XS_DPP, 8, 0, 0, 1, 1},
+    {4, 2, 1, XS_DPP, 16, 1, 0, 1, 1, 4}, {4, 2, 1, XS_DPP, 16, 0, 0, 1, 1, 4}, {4, 2, 0, XS_DPP, 8, 0, 0, 1, 1},
+    {4, 2, 0, XS_DPP, 4, 0, 0, 1, 1}, {2, 2, 0, XS_DPP, 8, 0, 0, 1, 1}, {4, 2, 0, XS_DPP, 8, 1, 0, 1, 1, 4},
+    {4, 2, 1, XS_DPP, -1, 0, 0, 1, 1}, {4, 2, 1, XS_DPP, -1, 1, 0, 1, 1}, {2, 2, 1, XS_DPP, -1, 0, 0, 1, 1},
+    {8, 2, 1, XS_DPP, -1, 0, 0, 1, 1}, {4, 2, 0, XS_DPP, -1, 0, 0, 1, 1}, {4, 2, 1, XS_DPP, -1, 0, 0, 2}};
 constexpr int kNumPrimalTiles = sizeof(kPrimalTiles) / sizeof(kPrimalTiles[0]);
 constexpr int kDefaultPrimalTile = 12;  // {4, 2, 1, XS_DPP, 16}: fields of 2 GiB or more, FP64
 constexpr int kMarchTileFP32 = 18;      // {4, 2, 1, XS_DPP, 16, 1}: fields of 2 GiB or more, FP32
@@ -1254,23 +1292,25 @@ Layout primal_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes) {
     L.ny = (int)ny;
     L.nz = (int)nz;
     const long long line = (long long)(128 / elem_bytes);  // elements per 128-byte line
-    L.padl = (int)line;
-    L.xp = ((L.padl + (long long)nx + 1 + line - 1) / line) * line;
+    L.line = (int)line;
+    // as the dual layout: x = nx and, in the row before, x = -1 share the
+    // row's last padding line; one line in front of the first row
+    L.xp = (((long long)nx + 1 + line - 1) / line) * line;
     L.sy = 2LL * L.xp;
     L.sz = (long long)(ny + 2) * L.sy;
-    L.origin = L.sz + L.sy + L.padl;
-    L.elems = (long long)(nz + 2) * L.sz;
+    L.origin = L.sz + L.sy + line;
+    L.elems = (long long)(nz + 2) * L.sz + line;
     return L;
 }
 
 Grid3 primal_grid(const PTile &t, const Layout &L) {
     Grid3 G{};
-    if (t.pair && L.padl == 16) {
+    if (t.pair && L.line == 16) {
         // FP32 (16 elements per 128-byte line) two cells per lane: 128-cell rows
         G.lsh = 6;
         G.gx = (L.nx + 127) / 128;
         G.gy = (L.ny + t.ty - 1) / t.ty;
-        G.gz = (L.nz + t.cz - 1) / t.cz;
+        G.gz = t.cz > 0 ? (L.nz + t.cz - 1) / t.cz : L.nz;  // run-time chunks: at most one layer each
         G.remap = t.remap;
         if (G.remap == 2 && G.gy % 8 != 0) G.remap = 0;
         return G;
@@ -1289,10 +1329,35 @@ Grid3 primal_grid(const PTile &t, const Layout &L) {
     const int lw = 1 << G.lsh, rows = t.ty * (64 >> G.lsh);
     G.gx = (L.nx + lw - 1) / lw;
     G.gy = (L.ny + rows - 1) / rows;
-    G.gz = t.cz > 0 ? (L.nz + t.cz - 1) / t.cz : L.nz;
+    G.gz = t.cz > 0 ? (L.nz + t.cz - 1) / t.cz : L.nz;  // one layer per block, or run-time chunks (at most)
     G.remap = t.remap;
     if (G.remap == 2 && G.gy % 8 != 0) G.remap = 0;  // y-bands need gy divisible by the 8 XCDs
     return G;
+}
+
+// Run-time z-chunks (tile cz < 0): as many chunks per column as make one
+// round of blocks fill every block slot of the chip (the kernel's own
+// occupancy), so that no second, partial round of waves runs, and chunks as
+// long as that allows (fewer chunk boundaries, whose halo layers are read
+// twice).  Sets G.gz and G.cz.
+template <typename K>
+hipError_t runtime_chunks(K kern, int ty, const Layout &L, Grid3 &G) {
+    static int blocks_per_cu = -1;  // per kernel instantiation: a property of the compiled kernel
+    if (blocks_per_cu < 0) {
+        int nb = 0;
+        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 64 * ty, 0);
+        if (e != hipSuccess) return e;
+        blocks_per_cu = nb > 0 ? nb : 1;
+    }
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    const long long slots = (long long)blocks_per_cu * cus, cols = (long long)G.gx * G.gy;
+    const int ntz = (int)std::max(1LL, std::min<long long>(L.nz, slots / cols));
+    G.cz = (L.nz + ntz - 1) / ntz;
+    G.gz = (L.nz + G.cz - 1) / G.cz;
+    return hipSuccess;
 }
 
 template <typename T, int FORM, int MODE, int TY, bool NT, int XSM>
@@ -1304,14 +1369,16 @@ void launch_t(cplx_t<T> *next, const cplx_t<T> *prev, const Layout &L, const PCo
 
 // Instantiates one kernel per tile shape (imaginary time always stores
 // streaming, so its shapes differ only in waves, order and exchange).
+// *nblocks (if given) receives the number of blocks launched, whose partial
+// sums an imaginary-time pass leaves in partials[0 .. nblocks).
 template <typename T, int FORM, int MODE, int IDX = 0>
 hipError_t launch_primal(int tile, cplx_t<T> *next, const cplx_t<T> *prev, const Layout &L, const PCoef<T> &H,
                          const PTrap &trap, double dt, const double *scale, double *partials, hipStream_t st,
-                         bool reverse, const float *vzt) {
+                         bool reverse, const float *vzt, long long *nblocks = nullptr) {
     if constexpr (IDX < kNumPrimalTiles) {
         if (tile != IDX)
             return launch_primal<T, FORM, MODE, IDX + 1>(tile, next, prev, L, H, trap, dt, scale, partials, st,
-                                                         reverse, vzt);
+                                                         reverse, vzt, nblocks);
         constexpr PTile t = kPrimalTiles[IDX];
         if constexpr (t.fused) {
             return hipErrorInvalidValue;  // launched by launch_primal_fused
@@ -1319,17 +1386,30 @@ hipError_t launch_primal(int tile, cplx_t<T> *next, const cplx_t<T> *prev, const
         Grid3 G = primal_grid(t, L);
         G.rev = reverse ? 1 : 0;
         constexpr bool nt = MODE == MODE_REAL && t.nt;
-        if constexpr (t.pair && sizeof(T) == 4)
-            hipLaunchKernelGGL((k_primal_zp<MODE, t.ty, t.cz, nt, t.ly != 0>), dim3(G.gx * G.gy * G.gz),
-                               dim3(64, t.ty), 0, st, next, prev, L, H, trap, dt, vzt, scale, partials, G);
-        else if constexpr (t.buf)
-            hipLaunchKernelGGL((k_primal_zb<T, FORM, MODE, t.ty, t.cz, nt, t.ly != 0, t.buf>), dim3(G.gx * G.gy * G.gz),
-                               dim3(64, t.ty), 0, st, next, prev, L, H, trap, dt, vzt, scale, partials, G);
-        else if constexpr (t.cz > 0)
+        constexpr int czt = t.cz < 0 ? 0 : t.cz;  // 0: chunks sized at launch (buffer z-marches only)
+        static_assert(t.cz >= 0 || t.buf, "run-time z-chunks need a buffer-addressed z-march");
+        if constexpr (t.pair && sizeof(T) == 4) {
+            const auto kern = k_primal_zp<MODE, t.ty, czt, nt, t.ly != 0, t.waves>;
+            if constexpr (t.cz < 0) {
+                const hipError_t e = runtime_chunks(kern, t.ty, L, G);
+                if (e != hipSuccess) return e;
+            }
+            hipLaunchKernelGGL(kern, dim3(G.gx * G.gy * G.gz), dim3(64, t.ty), 0, st, next, prev, L, H, trap, dt, vzt,
+                               scale, partials, G);
+        } else if constexpr (t.buf) {
+            const auto kern = k_primal_zb<T, FORM, MODE, t.ty, czt, nt, t.ly != 0, t.buf>;
+            if constexpr (t.cz < 0) {
+                const hipError_t e = runtime_chunks(kern, t.ty, L, G);
+                if (e != hipSuccess) return e;
+            }
+            hipLaunchKernelGGL(kern, dim3(G.gx * G.gy * G.gz), dim3(64, t.ty), 0, st, next, prev, L, H, trap, dt, vzt,
+                               scale, partials, G);
+        } else if constexpr (t.cz > 0)
             hipLaunchKernelGGL((k_primal_zm<T, FORM, MODE, t.ty, t.cz, nt, t.ly != 0>), dim3(G.gx * G.gy * G.gz), dim3(64, t.ty),
                                0, st, next, prev, L, H, trap, dt, scale, partials, G);
         else
             launch_t<T, FORM, MODE, t.ty, nt, t.xs>(next, prev, L, H, trap, dt, scale, partials, G, st);
+        if (nblocks) *nblocks = (long long)G.gx * G.gy * G.gz;
         return hipGetLastError();
     } else {
         return hipErrorInvalidValue;
@@ -1570,11 +1650,12 @@ gpe_status gpe_primal::enqueue(uint64_t nsteps, uint64_t *launches) {
         // the next read through scal[1] = 1/sqrt(N).  (A fused tile is a
         // leapfrog step; imaginary time runs the size's default shape.)
         const int itile = kPrimalTiles[tile].fused ? imag_tile : tile;
-        const long long nb = primal_blocks(itile, L);
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
             C *in = (C *)field[ie], *out = (C *)field[io];
             const bool rev = (steps_done + s) & 1;
-            e = launch_primal<T, FORM, MODE_IMAG>(itile, out, in, L, H, trap, dt, scal + 1, partials, stream, rev, vzt);
+            long long nb = 0;  // blocks launched, each leaving one partial sum
+            e = launch_primal<T, FORM, MODE_IMAG>(itile, out, in, L, H, trap, dt, scal + 1, partials, stream, rev, vzt,
+                                                  &nb);
             if (e != hipSuccess) break;
             hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(256), 0, stream, partials, nb, w, 1, scal);
             e = hipGetLastError();

@@ -28,12 +28,14 @@ Layout make_layout(uint32_t nx, uint32_t ny, uint32_t nz, size_t elem_bytes) {
     L.ny = (int)ny;
     L.nz = (int)nz;
     const long long line = (long long)(128 / elem_bytes);  // elements per 128-byte line
-    L.padl = (int)line;
-    L.xp = ((L.padl + (long long)nx + 1 + line - 1) / line) * line;
+    L.line = (int)line;
+    // x = nx and, in the row before, x = -1 share the row's last padding line
+    L.xp = (((long long)nx + 1 + line - 1) / line) * line;
     L.sy = 12LL * L.xp;
     L.sz = (long long)(ny + 2) * L.sy;
-    L.origin = L.sz + L.sy + L.padl;
-    L.elems = (long long)(nz + 2) * L.sz;
+    // one line in front of the halo row (j, k) = (-1, -1) holds its x = -1
+    L.origin = L.sz + L.sy + line;
+    L.elems = (long long)(nz + 2) * L.sz + line;
     return L;
 }
 

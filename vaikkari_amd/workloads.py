@@ -109,15 +109,16 @@ def precision_form(n: int = 256, seed: int = 3) -> PrecisionForm:
 BATCH_SIZES = (24, 32, 48, 64, 96)
 
 
-def batched_form(n_inst: int = 64, steps: int = 2000, seed: int = 4, h: float = 0.25):
+def batched_form(n_inst: int = 64, steps: int = 2000, seed: int = 4, h: float = 0.25, sizes=BATCH_SIZES):
     """Instance configurations of the batched form: list of dicts (dims, h, g, dt, steps, mode,
-    trap_ratio, seed) for vaikkari_amd.batch.instance_config."""
+    trap_ratio, seed) for vaikkari_amd.batch.instance_config.  `sizes` replaces the set the cube
+    edges are drawn from (one size: a batch of equal grids)."""
     from .serve import stable_dt
 
     rng = np.random.default_rng(seed)
     out = []
     for q in range(n_inst):
-        n = int(rng.choice(BATCH_SIZES))
+        n = int(rng.choice(tuple(sizes)))
         g = float(np.exp(rng.uniform(np.log(100.0), np.log(5000.0))))
         ratio = float(rng.uniform(0.5, 4.0))
         out.append({"dims": (n, n, n), "h": h, "g": g, "dt": stable_dt(n, h, g, ratio), "steps": int(steps),
