@@ -70,7 +70,8 @@ def parse():
                          "workload alone (with the same probe)")
     ap.add_argument("--batch-tile", default="",
                     help="batched suite: fixed real-time tile instead of the autotuner's, waves followed by "
-                         "c (cached read-modify-write) and/or s (slab order), e.g. 2 or 1cs")
+                         "c (cached read-modify-write) and/or s (slab order per instance) or g (slab order "
+                         "per group), e.g. 2, 1cs or 2g")
     ap.add_argument("--batch-sizes", default="",
                     help="batched suite: cube edges to draw the instances from (comma list; default "
                          "24,32,48,64,96)")
@@ -82,6 +83,9 @@ def parse():
     ap.add_argument("--serve-launch", default="graph", choices=["graph", "stream"],
                     help="serving: replay a captured hipGraph per chunk (default), or enqueue the kernels "
                          "directly (for kernel traces of the serving loop)")
+    ap.add_argument("--serve-tile", default="",
+                    help="serving suite: real-time tile of the shared launches (as --batch-tile; default the "
+                         "batch's own)")
     ap.add_argument("--serve-policy", default="shortest_first", choices=["shortest_first", "fifo"],
                     help="admission policy of --suite serving (shortest_first is capped at 32 M resident nodes)")
     ap.add_argument("--primal-cells", default="128,256",
@@ -205,7 +209,8 @@ def batched_workload(n_inst, steps, seed=4, log=log, launch="graph", tile=None, 
     else:
         tuned = {"waves": tile[0], "cached": tile[1], "slabs": tile[2], "tile": tuple(tile), "ms": "fixed"}
     log(f"batched: tile {tuned['waves']} waves, {'cached' if tuned['cached'] else 'streamed'} "
-        f"read-modify-write, {'slab' if tuned['slabs'] else 'y-band'} order ({tuned['ms']})")
+        f"read-modify-write, {'group-slab' if tuned['slabs'] == 2 else 'slab' if tuned['slabs'] else 'y-band'} "
+        f"order ({tuned['ms']})")
     if group_mib is None:
         T = make(tuned["tile"])  # the whole workload, only to tune the group size of the schedule
         grouping = tune_batch_groups(T, chunk_steps=10, chunks=2)
@@ -245,7 +250,7 @@ def batched_workload(n_inst, steps, seed=4, log=log, launch="graph", tile=None, 
 SERVE_CAP_NODES = 32 * 10 ** 6
 
 
-def serving_workload(seconds, policy="shortest_first", cap=SERVE_CAP_NODES, log=log, launch="graph"):
+def serving_workload(seconds, policy="shortest_first", cap=SERVE_CAP_NODES, log=log, launch="graph", tile=None):
     """Poisson stream (rate 20/s, seed 5; n log-uniform in [16, 160], steps in [500, 20000], 30 % ground
     states), FP32 bulk, 200 GB HBM budget, 50-step chunks, replayed against the wall clock."""
     from vaikkari_amd import workloads as wl
@@ -253,7 +258,7 @@ def serving_workload(seconds, policy="shortest_first", cap=SERVE_CAP_NODES, log=
 
     w = wl.serving_form(seconds)
     res = run_serving(w, duration=seconds, precision="fp32", hbm_budget_bytes=200 * 10 ** 9, chunk_steps=50,
-                      log=log, policy=policy, max_active_nodes=cap, launch=launch)
+                      log=log, policy=policy, max_active_nodes=cap, launch=launch, tile=tile)
     res["rate_per_s"] = 20.0
     res["hbm_budget_GB"] = 200
     res["chunk_steps"] = 50
@@ -265,7 +270,12 @@ def serving_suite(args):
     admission policy; prints one JSON line with latency percentiles, offered vs served load and the
     queue-depth time series."""
     cap = SERVE_CAP_NODES if args.serve_policy == "shortest_first" else 0
-    res = serving_workload(args.serve_seconds, policy=args.serve_policy, cap=cap, launch=args.serve_launch)
+    tile = None
+    if args.serve_tile:
+        w, flags = int(args.serve_tile.rstrip("csg")), args.serve_tile.lstrip("0123456789")
+        tile = (w, "c" in flags, 2 if "g" in flags else "s" in flags)
+    res = serving_workload(args.serve_seconds, policy=args.serve_policy, cap=cap, launch=args.serve_launch,
+                           tile=tile)
     print(json.dumps({"suite": "serving", **res}), flush=True)
 
 
@@ -476,8 +486,8 @@ def main():
             copy_gbs = va.copy_bandwidth(1 << 30, 3)
             tile = None
             if args.batch_tile:
-                w, flags = int(args.batch_tile.rstrip("cs")), args.batch_tile.lstrip("0123456789")
-                tile = (w, "c" in flags, "s" in flags)
+                w, flags = int(args.batch_tile.rstrip("csg")), args.batch_tile.lstrip("0123456789")
+                tile = (w, "c" in flags, 2 if "g" in flags else "s" in flags)
             sizes = [int(x) for x in args.batch_sizes.split(",")] if args.batch_sizes else None
             res = batched_workload(args.batch_instances, args.batch_steps, launch=args.batch_launch, tile=tile,
                                    group_mib=args.batch_group_mib, sizes=sizes)

@@ -96,12 +96,16 @@ uint64_t gpe_batch_chunk_node_steps(const gpe_batch *b, uint32_t chunk_steps);
  * | GPE_BATCH_TILE_CACHED for a cached read-modify-write of the updated field
  * (without it the field streams past the caches, non-temporal) and
  * | GPE_BATCH_TILE_SLABS to give each XCD a contiguous eighth (a slab of
- * layers) of every instance's tiles (without it y-bands of every layer when
- * an instance's block rows divide by 8, else plain order); only while the
- * batch is empty.  A new batch starts with 2 | GPE_BATCH_TILE_SLABS.  Results
- * do not depend on it.  The autotuner picks it. */
+ * layers) of every instance's tiles, or | GPE_BATCH_TILE_GROUP_SLABS a
+ * contiguous eighth of a whole group's tiles (whole instances, each in
+ * layer order; small instances then keep their neighbour layers in one L2)
+ * (without either y-bands of every layer when an instance's block rows
+ * divide by 8, else plain order); only while the batch is empty.  A new
+ * batch starts with 2 | GPE_BATCH_TILE_SLABS.  Results do not depend on it.
+ * The autotuner picks it. */
 #define GPE_BATCH_TILE_CACHED 0x100u
 #define GPE_BATCH_TILE_SLABS 0x200u
+#define GPE_BATCH_TILE_GROUP_SLABS 0x400u
 gpe_status gpe_batch_set_tile(gpe_batch *b, uint32_t tile);
 /* How gpe_batch_run issues a chunk: GPE_BATCH_GRAPH replays a captured
  * hipGraph of the chunk (the default), GPE_BATCH_STREAM enqueues the chunk's

@@ -66,21 +66,28 @@ gpe_status gpe_primal_synchronize(gpe_primal *p);
 /* Number of tile shapes: one layer per block or a z-march of 4 to 16
  * layers, waves per block (2, 4, 8), block order over the XCDs, streaming
  * read-modify-write, and how a lane gets its x-neighbours (DPP lane shift,
- * cross-lane shuffle or a second load); tiles 21-25 fuse both half-updates
- * of a real-time step into one out-of-place pass (temporal blocking, two
- * more field buffers); tiles 26-42 are z-marches that address the fields
- * through buffer resources (32-bit offsets: fields below 2 GiB, else
- * GPE_EINVAL at creation), with one (26-31, 37-42) or two (32-36) layers of
- * prefetch; tiles 37-42 hold two cells per lane in FP32 (16-byte accesses;
- * FP64 runs them one cell per lane).  In FP64 every shape gives the same
- * bits; in FP32 the buffer tiles add V*dt as an x-y part and a z part, each
- * rounded to FP32.
+ * cross-lane shuffle or a second load).
+ *   1-20   one layer per block, or z-marches on 64-bit addresses;
+ *   21-25  both half-updates of a real-time step in one out-of-place pass
+ *          (temporal blocking, two more field buffers);
+ *   26-65  z-marches that address the fields through buffer resources
+ *          (32-bit offsets: fields below 2 GiB, else GPE_EINVAL at creation):
+ *          26-36 one cell per lane with one (26-31) or two (32-36) layers of
+ *          prefetch; 37-48 and 55-65 two cells per lane in FP32 (16-byte
+ *          accesses; FP64 runs them one cell per lane), 43, 44, 48, 56 and 57
+ *          with the registers held to 4 waves per SIMD, 55-57 loading the
+ *          row-end values in the layer, 58-65 varying waves, z-chunk and
+ *          block order; 49-54 z-chunks sized at launch so that one round of
+ *          blocks fills the chip (54 one cell per lane, two layers of
+ *          prefetch).
+ * In FP64 every shape gives the same bits; in FP32 the buffer tiles add V*dt
+ * as an x-y part and a z part, each rounded to FP32.
  * Tile 0 in the config picks by size: when both fields fit in 192 MB (the
- * Infinity Cache holds them) a cached read-modify-write (tile 9 FP64: one
- * layer per block; tile 41 FP32: buffer z-march, two cells per lane); below
- * 2 GiB per field the buffer z-march of 16 layers (tile 28 FP64, tile 37
- * FP32); larger fields the z-march on 64-bit addresses (tile 13 FP64, tile
- * 19 FP32). */
+ * Infinity Cache holds them) a cached read-modify-write in the buffer
+ * z-march with a contiguous slab of layers per XCD (tile 60 FP64: 16 layers
+ * per chunk; tile 59 FP32: 8 layers, two cells per lane); below 2 GiB per
+ * field the buffer z-march of 16 layers (tile 28 FP64, tile 37 FP32); larger
+ * fields the z-march on 64-bit addresses (tile 13 FP64, tile 19 FP32). */
 int gpe_primal_tiles(void);
 
 /* Host helpers, the arithmetic the kernels use. */

@@ -243,6 +243,7 @@ SIGNATURES = [
     ("gpe_server_set_policy", _I, [_P, _U32, _U64]),
     ("gpe_server_oldest_queued", _D, [_P]),
     ("gpe_server_set_launch_mode", _I, [_P, _U32]),
+    ("gpe_server_set_tile", _I, [_P, _U32]),
     ("gpe_primal_create", _I, [C.POINTER(GpePrimalConfig), C.POINTER(_P)]),
     ("gpe_primal_destroy", None, [_P]),
     ("gpe_primal_set_state", _I, [_P, _DP, _DP]),

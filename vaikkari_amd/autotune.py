@@ -100,7 +100,9 @@ def tune(s: Solver, steps: int = 20, families=DEFAULT_FAMILIES, use_cache: bool 
     return dict(out, cached=False)
 
 
-BATCH_TILES = tuple((w, c, sl) for sl in (False, True) for c in (False, True) for w in (1, 2, 4, 8))
+# tile orders: False y-bands (or plain), True a slab of every instance per XCD, 2 a slab of the
+# whole group per XCD
+BATCH_TILES = tuple((w, c, sl) for sl in (False, True, 2) for c in (False, True) for w in (1, 2, 4, 8))
 
 
 def _tile_triple(t):
@@ -122,7 +124,7 @@ def tune_batch_tile(make_batch, chunk_steps: int = 10, chunks: int = 2, candidat
     best = min(timings, key=timings.get)
     waves, cached, slabs = _tile_triple(best)
     return {"waves": waves, "cached": cached, "slabs": slabs, "tile": best,
-            "ms": {"%d%s%s" % (w, "c" if c else "", "s" if sl else ""): v
+            "ms": {"%d%s%s" % (w, "c" if c else "", "g" if sl == 2 else "s" if sl else ""): v
                    for (w, c, sl), v in ((_tile_triple(k), v) for k, v in timings.items())}}
 
 

@@ -18,6 +18,7 @@ from .solver import MODES, PRECISIONS, nodes_of
 
 TILE_CACHED = 0x100  # GPE_BATCH_TILE_CACHED: cached read-modify-write of the real-time pass
 TILE_SLABS = 0x200  # GPE_BATCH_TILE_SLABS: each XCD takes a contiguous eighth of an instance's tiles
+TILE_GROUP_SLABS = 0x400  # GPE_BATCH_TILE_GROUP_SLABS: ... of a whole group's tiles (whole instances)
 
 
 def instance_config(dims: Sequence[int], h: float, g: float, dt: float, steps: int, mode: str = "real",
@@ -43,6 +44,15 @@ def instance_config(dims: Sequence[int], h: float, g: float, dt: float, steps: i
     return c
 
 
+def tile_code(waves, cached: bool = False, slabs=False) -> int:
+    """gpe_batch_set_tile's argument for (waves, cached, slabs); slabs False/0, True/1 or 2 (see
+    Batch.set_tile).  A (waves, cached) or (waves, cached, slabs) tuple is accepted as waves."""
+    if isinstance(waves, tuple):
+        waves, cached, slabs = (tuple(waves) + (False, False))[:3]
+    order = int(slabs)
+    return int(waves) | (TILE_CACHED if cached else 0) | {0: 0, 1: TILE_SLABS, 2: TILE_GROUP_SLABS}[order]
+
+
 class Batch:
     def __init__(self, precision: str = "fp32", hbm_budget_bytes: int = 0):
         self._h = C.c_void_p()
@@ -61,15 +71,13 @@ class Batch:
         except Exception:
             pass
 
-    def set_tile(self, waves, cached: bool = False, slabs: bool = False) -> None:
+    def set_tile(self, waves, cached: bool = False, slabs=False) -> None:
         """Real-time tile: waves per block (1, 2, 4, 8), whether the updated field's
-        read-modify-write is cached (default: streamed) and whether each XCD takes a contiguous
-        slab of an instance's tiles (default: y-bands or plain order).  A (waves, cached) or
-        (waves, cached, slabs) tuple is accepted."""
-        if isinstance(waves, tuple):
-            waves, cached, slabs = (tuple(waves) + (False, False))[:3]
-        tile = int(waves) | (TILE_CACHED if cached else 0) | (TILE_SLABS if slabs else 0)
-        check(lib.gpe_batch_set_tile(self._h, tile), "gpe_batch_set_tile")
+        read-modify-write is cached (default: streamed) and how the tiles go to the XCDs: slabs
+        True (or 1) gives each XCD a contiguous slab of every instance's tiles, 2 a contiguous
+        eighth of a whole group's tiles (whole instances in layer order), False y-bands or plain
+        order.  A (waves, cached) or (waves, cached, slabs) tuple is accepted."""
+        check(lib.gpe_batch_set_tile(self._h, tile_code(waves, cached, slabs)), "gpe_batch_set_tile")
 
     def set_group_bytes(self, nbytes: int) -> None:
         """Real-time instances advance a chunk group by group, groups of at most nbytes of fields

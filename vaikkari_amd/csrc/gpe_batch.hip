@@ -118,7 +118,15 @@ constexpr int kFinalizeThreads = 256;
 // GPUGrossPitaevskii_BCC.cu:478,480).
 // The launch covers the work items [item0, item0 + nitems) of one group of
 // instances (item0 and nitems are multiples of 8).
-template <typename T, int TY, bool NT>
+// GS (group slabs): block b runs on XCD b mod 8 (work-groups go to the XCDs
+// round-robin), and XCD x takes the x-th contiguous eighth of the group's
+// work items in order -- whole instances, layer after layer -- so the
+// neighbour rows and layers a tile reads were fetched into that XCD's L2 by
+// the tiles just before it; only the instances cut by an eighth's ends are
+// shared between L2s.  The instances' own tile order is then the plain one.
+// Half 1 walks each eighth backwards.  Otherwise half 1 walks all the work
+// items backwards.
+template <typename T, int TY, bool NT, bool GS>
 __global__ void __launch_bounds__(64 * TY)
     k_batch_real(const InstDesc<T> *__restrict__ D, const int *__restrict__ items, int item0, int nitems, int s,
                  int chunk, int half) {
@@ -128,7 +136,13 @@ __global__ void __launch_bounds__(64 * TY)
         // (those fields are still in the caches)
         const int local = base + (int)blockIdx.x;
         if (local >= nitems) continue;
-        const int w = item0 + (half ? nitems - 1 - local : local);
+        int w;
+        if (GS) {
+            const int per = nitems >> 3, slot = local >> 3;
+            w = item0 + (local & 7) * per + (half ? per - 1 - slot : slot);
+        } else {
+            w = item0 + (half ? nitems - 1 - local : local);
+        }
         const int q = __builtin_amdgcn_readfirstlane(items[w]);
         const InstDesc<T> &d = D[q];
         const long long steps = d.steps_left < chunk ? d.steps_left : chunk;
@@ -251,7 +265,8 @@ struct gpe_batch {
     uint64_t budget = 0;
     int real_ty = kRealTY;  // waves per block of the real-time pass (fixed while instances exist)
     bool real_nt = true;    // streaming (non-temporal) read-modify-write of the updated field
-    bool real_slabs = true;   // each XCD takes a contiguous eighth of every instance's tiles
+    int real_order = 1;       // tile order over the XCDs: 0 y-bands (or plain), 1 a contiguous eighth
+                              // of every instance's tiles, 2 of the whole group's work items
     // Real-time instances advance a whole chunk group by group: contiguous
     // runs of instances whose fields total at most group_bytes (0: one group)
     uint64_t group_bytes = kDefaultGroupBytes;
@@ -314,7 +329,7 @@ bool valid_instance(const gpe_instance_config *c) {
            std::isfinite(c->trap_ratio) && c->init <= GPE_INIT_GAUSSIAN;
 }
 
-size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, bool real_slabs, Layout *Lout,
+size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, int real_order, Layout *Lout,
                       Grid3 *gg, Grid3 *zg, int *zcz, int *ntiles) {
     const Layout L = make_layout(c->nx, c->ny, c->nz, elem);
     size_t bytes = 2 * (size_t)L.elems * elem;  // even and odd (real time) or ping-pong pair (imaginary time)
@@ -325,7 +340,9 @@ size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, bo
         G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
         G.gy = (L.ny + real_ty * (64 >> G.lsh) - 1) / (real_ty * (64 >> G.lsh));
         G.gz = L.nz;
-        G.remap = real_slabs ? 1 : G.gy % 8 == 0 ? 2 : 0;
+        // group slabs: the kernel spreads the group over the XCDs, each
+        // instance's tiles in plain (layer-major) order
+        G.remap = real_order == 2 ? 0 : real_order == 1 ? 1 : G.gy % 8 == 0 ? 2 : 0;
         tiles = G.gx * G.gy * G.gz;
         if (gg) *gg = G;
     } else {
@@ -459,18 +476,27 @@ void enqueue_chunk(gpe_batch *B, uint32_t chunk, hipStream_t st) {
         const int grid = std::max(1, std::min(grp.second, kMaxBatchGrid));
         for (uint32_t s = 0; s < chunk; s++) {
             for (int half = 0; half < 2; half++) {
-#define GPE_BATCH_REAL(TY, NT)                                                                          \
-    hipLaunchKernelGGL((k_batch_real<T, TY, NT>), dim3(grid), dim3(64, TY), 0, st, R, B->d_items, grp.first, \
-                       grp.second, (int)s, (int)chunk, half)
-                switch (B->real_ty * 2 + (B->real_nt ? 1 : 0)) {
-                    case 2: GPE_BATCH_REAL(1, false); break;
-                    case 3: GPE_BATCH_REAL(1, true); break;
-                    case 4: GPE_BATCH_REAL(2, false); break;
-                    case 5: GPE_BATCH_REAL(2, true); break;
-                    case 8: GPE_BATCH_REAL(4, false); break;
-                    case 16: GPE_BATCH_REAL(8, false); break;
-                    case 17: GPE_BATCH_REAL(8, true); break;
-                    default: GPE_BATCH_REAL(4, true);
+#define GPE_BATCH_REAL(TY, NT, GS)                                                                      \
+    hipLaunchKernelGGL((k_batch_real<T, TY, NT, GS>), dim3(grid), dim3(64, TY), 0, st, R, B->d_items,     \
+                       grp.first, grp.second, (int)s, (int)chunk, half)
+                const bool gs = B->real_order == 2;
+                switch (B->real_ty * 4 + (B->real_nt ? 2 : 0) + (gs ? 1 : 0)) {
+                    case 4: GPE_BATCH_REAL(1, false, false); break;
+                    case 5: GPE_BATCH_REAL(1, false, true); break;
+                    case 6: GPE_BATCH_REAL(1, true, false); break;
+                    case 7: GPE_BATCH_REAL(1, true, true); break;
+                    case 8: GPE_BATCH_REAL(2, false, false); break;
+                    case 9: GPE_BATCH_REAL(2, false, true); break;
+                    case 10: GPE_BATCH_REAL(2, true, false); break;
+                    case 11: GPE_BATCH_REAL(2, true, true); break;
+                    case 16: GPE_BATCH_REAL(4, false, false); break;
+                    case 17: GPE_BATCH_REAL(4, false, true); break;
+                    case 19: GPE_BATCH_REAL(4, true, true); break;
+                    case 32: GPE_BATCH_REAL(8, false, false); break;
+                    case 33: GPE_BATCH_REAL(8, false, true); break;
+                    case 34: GPE_BATCH_REAL(8, true, false); break;
+                    case 35: GPE_BATCH_REAL(8, true, true); break;
+                    default: GPE_BATCH_REAL(4, true, false);
                 }
 #undef GPE_BATCH_REAL
             }
@@ -668,7 +694,7 @@ void gpe_batch_destroy(gpe_batch *b) { delete b; }
 
 uint64_t gpe_batch_instance_bytes(const gpe_batch *B, const gpe_instance_config *c) {
     if (!B || !valid_instance(c)) return 0;
-    return instance_bytes(c, B->elem(), B->real_ty, B->real_slabs, nullptr, nullptr, nullptr, nullptr, nullptr);
+    return instance_bytes(c, B->elem(), B->real_ty, B->real_order, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id) {
@@ -678,7 +704,7 @@ gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id
     }
     BInst b;
     b.cfg = *c;
-    b.bytes = instance_bytes(c, B->elem(), B->real_ty, B->real_slabs, &b.L, &b.gg, &b.zg, &b.zcz, &b.ntiles);
+    b.bytes = instance_bytes(c, B->elem(), B->real_ty, B->real_order, &b.L, &b.gg, &b.zg, &b.zcz, &b.ntiles);
     if (B->budget && B->resident + b.bytes > B->budget) {
         set_error("gpe_batch_add: %zu bytes would exceed the budget (%llu of %llu resident)", b.bytes,
                   (unsigned long long)B->resident, (unsigned long long)B->budget);
@@ -987,9 +1013,10 @@ uint64_t gpe_batch_group_bytes(const gpe_batch *B) { return B ? B->group_bytes :
 gpe_status gpe_batch_set_tile(gpe_batch *B, uint32_t tile) {
     const uint32_t waves = tile & 0xff;
     if (!B || (waves != 1 && waves != 2 && waves != 4 && waves != 8) ||
-        (tile & ~(0xffu | GPE_BATCH_TILE_CACHED | GPE_BATCH_TILE_SLABS))) {
+        (tile & ~(0xffu | GPE_BATCH_TILE_CACHED | GPE_BATCH_TILE_SLABS | GPE_BATCH_TILE_GROUP_SLABS)) ||
+        ((tile & GPE_BATCH_TILE_SLABS) && (tile & GPE_BATCH_TILE_GROUP_SLABS))) {
         set_error("gpe_batch_set_tile: waves must be 1, 2, 4 or 8 (optionally | GPE_BATCH_TILE_CACHED, "
-                  "| GPE_BATCH_TILE_SLABS)");
+                  "| GPE_BATCH_TILE_SLABS or | GPE_BATCH_TILE_GROUP_SLABS)");
         return GPE_EINVAL;
     }
     for (auto &b : B->inst)
@@ -999,7 +1026,7 @@ gpe_status gpe_batch_set_tile(gpe_batch *B, uint32_t tile) {
         }
     B->real_ty = (int)waves;
     B->real_nt = !(tile & GPE_BATCH_TILE_CACHED);
-    B->real_slabs = (tile & GPE_BATCH_TILE_SLABS) != 0;
+    B->real_order = (tile & GPE_BATCH_TILE_GROUP_SLABS) ? 2 : (tile & GPE_BATCH_TILE_SLABS) ? 1 : 0;
     B->graph_dirty = true;
     return GPE_OK;
 }

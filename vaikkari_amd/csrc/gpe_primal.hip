@@ -576,14 +576,18 @@ __device__ __forceinline__ double primal_column_b(cplx_t<T> *__restrict__ next, 
     auto RS = [&](int dz, int dy, int sl) -> uint32_t { return __builtin_amdgcn_readfirstlane(R(dz, dy, sl)); };
     const uint32_t vcol = cb * ES;                              // per-lane byte offset
     const uint32_t vst = act ? vcol : kBufDiscard;              // stores of lanes without a cell are discarded
-    // row-end loads: per-lane element deltas (left end: x-1 values, right end:
-    // x+1 values, other lanes their own corner site)
-    const uint32_t own = R(0, 0, 0);
-    const uint32_t vt0 = (cb + (left_end ? R(0, -1, 1) - 1 : right_end ? R(1, 0, 0) + 1 : own)) * ES;
-    const uint32_t vt1 = (cb + (left_end ? R(0, 0, 1) - 1 : right_end ? R(1, 1, 0) + 1 : own)) * ES;
-    const uint32_t vt2 = (cb + (left_end ? R(0, 0, 0) - 1 : right_end ? R(0, 0, 1) + 1 : own)) * ES;
-    const uint32_t vi0 = (cb + (left_end ? R(-1, -1, 1) - 1 : right_end ? R(0, 1, 0) + 1 : own)) * ES;
-    const uint32_t vi1 = (cb + (left_end ? R(-1, 0, 1) - 1 : right_end ? R(0, 0, 0) + 1 : own)) * ES;
+    // row-end loads: per-lane byte offsets (left end: x-1 values, right end:
+    // x+1 values); the other lanes take an offset beyond the buffer, whose
+    // range check returns zero without a cache access, so these loads cost
+    // the memory pipeline two lanes' lines instead of a whole row's
+    auto vend = [&](uint32_t l, uint32_t r) -> uint32_t {
+        return left_end ? (cb + l - 1) * ES : right_end ? (cb + r + 1) * ES : kBufDiscard;
+    };
+    const uint32_t vt0 = vend(R(0, -1, 1), R(1, 0, 0));
+    const uint32_t vt1 = vend(R(0, 0, 1), R(1, 1, 0));
+    const uint32_t vt2 = vend(R(0, 0, 0), R(0, 0, 1));
+    const uint32_t vi0 = vend(R(-1, -1, 1), R(0, 1, 0));
+    const uint32_t vi1 = vend(R(-1, 0, 1), R(0, 0, 0));
     T sc = (T)1;
     if (MODE == MODE_IMAG) sc = (T)(*scale);
     auto ldv = [&](uint32_t v, uint32_t soff) {
@@ -786,7 +790,10 @@ struct ZLayer2 {
     CP<float> n0, n1;                      // next, for the read-modify-write
 };
 
-template <int MODE, int TY, int CZ, bool NT, bool LY>
+// LY: 0 every load of a layer is prefetched a layer ahead; 1 the y-neighbour
+// rows are loaded in the layer itself (fewer registers); 2 the row-end loads
+// as well.
+template <int MODE, int TY, int CZ, bool NT, int LY>
 __device__ __forceinline__ double primal_column_p(cplx_t<float> *__restrict__ next,
                                                   const cplx_t<float> *__restrict__ prev, const Layout &L,
                                                   const PCoef<float> &H, const PTrap &trap, double dt,
@@ -820,13 +827,16 @@ __device__ __forceinline__ double primal_column_p(cplx_t<float> *__restrict__ ne
     const uint32_t vcol = cb * ES;
     const uint32_t vst = actA ? vcol : kBufDiscard;
     // row-end loads (one cell): left end x - 1 of its first cell, right end
-    // x + 1 of its second; other lanes their own first cell
-    const uint32_t own = R(0, 0, 0);
-    const uint32_t vt0 = (cb + (left_end ? R(0, -1, 1) - 1 : right_end ? R(1, 0, 0) + 2 : own)) * ES;
-    const uint32_t vt1 = (cb + (left_end ? R(0, 0, 1) - 1 : right_end ? R(1, 1, 0) + 2 : own)) * ES;
-    const uint32_t vt2 = (cb + (left_end ? R(0, 0, 0) - 1 : right_end ? R(0, 0, 1) + 2 : own)) * ES;
-    const uint32_t vi0 = (cb + (left_end ? R(-1, -1, 1) - 1 : right_end ? R(0, 1, 0) + 2 : own)) * ES;
-    const uint32_t vi1 = (cb + (left_end ? R(-1, 0, 1) - 1 : right_end ? R(0, 0, 0) + 2 : own)) * ES;
+    // x + 1 of its second; the other lanes an offset beyond the buffer (zero,
+    // no cache access)
+    auto vend = [&](uint32_t l, uint32_t r) -> uint32_t {
+        return left_end ? (cb + l - 1) * ES : right_end ? (cb + r + 2) * ES : kBufDiscard;
+    };
+    const uint32_t vt0 = vend(R(0, -1, 1), R(1, 0, 0));
+    const uint32_t vt1 = vend(R(0, 0, 1), R(1, 1, 0));
+    const uint32_t vt2 = vend(R(0, 0, 0), R(0, 0, 1));
+    const uint32_t vi0 = vend(R(-1, -1, 1), R(0, 1, 0));
+    const uint32_t vi1 = vend(R(-1, 0, 1), R(0, 0, 0));
     float sc = 1.0f;
     if (MODE == MODE_IMAG) sc = (float)(*scale);
     auto scl = [&](P v) {
@@ -851,9 +861,11 @@ __device__ __forceinline__ double primal_column_p(cplx_t<float> *__restrict__ ne
             f.u2 = ld(lay, 0, -1, 1);
             f.zyp = ld(lay, 0, 1, 1);
         }
-        f.t0 = ld1(vt0, lay);
-        f.t1 = ld1(vt1, lay);
-        f.t2 = ld1(vt2, lay);
+        if (LY < 2) {
+            f.t0 = ld1(vt0, lay);
+            f.t1 = ld1(vt1, lay);
+            f.t2 = ld1(vt2, lay);
+        }
         if (MODE == MODE_REAL) {
             f.n0 = ld_pair_aux<NT ? kBufNT : 0>(rn, vcol, (lay + RS(0, 0, 0)) * ES);
             f.n1 = ld_pair_aux<NT ? kBufNT : 0>(rn, vcol, (lay + RS(0, 0, 1)) * ES);
@@ -889,6 +901,11 @@ __device__ __forceinline__ double primal_column_p(cplx_t<float> *__restrict__ ne
             cur.cym = ld(lay, 0, -1, 0);
             cur.u2 = ld(lay, 0, -1, 1);
             cur.zyp = ld(lay, 0, 1, 1);
+        }
+        if (LY == 2) {
+            cur.t0 = ld1(vt0, lay);
+            cur.t1 = ld1(vt1, lay);
+            cur.t2 = ld1(vt2, lay);
         }
         fetch(lay + sz, nxt);
         __builtin_amdgcn_sched_barrier(0);
@@ -989,7 +1006,7 @@ __device__ __forceinline__ double primal_column_p(cplx_t<float> *__restrict__ ne
 
 // W > 0 holds the compiler to W waves per SIMD (at most 512 / W registers
 // per lane) instead of its own trade of occupancy for registers.
-template <int MODE, int TY, int CZ, bool NT, bool LY, int W>
+template <int MODE, int TY, int CZ, bool NT, int LY, int W>
 __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1)))
     k_primal_zp(cplx_t<float> *__restrict__ next, const cplx_t<float> *__restrict__ prev, Layout L, PCoef<float> H,
                 PTrap trap, double dt, const float *__restrict__ vzt, const double *__restrict__ scale,
@@ -1253,7 +1270,8 @@ __global__ void __launch_bounds__(64 * kObsTY) k_primal_observables(const cplx_t
 // at launch, runtime_chunks).
 struct PTile {
     int ty, remap, nt, xs, cz;
-    int ly;     // z-march: y-neighbour rows loaded in the layer instead of prefetched
+    int ly;     // z-march: 1 y-neighbour rows loaded in the layer instead of prefetched, 2 (FP32 two
+                // cells per lane) the row-end loads as well
     int fused;  // real time: both half-updates in one z-march pass (k_primal_fused)
     int buf;    // z-march with buffer-resource addressing (k_primal_zb; fields below 2 GiB):
                 // the layers of prefetch (1 or 2)
@@ -1277,12 +1295,18 @@ constexpr PTile kPrimalTiles[] = {
     {4, 2, 1, XS_DPP, 16, 1, 0, 1, 1, 4}, {4, 2, 1, XS_DPP, 16, 0, 0, 1, 1, 4}, {4, 2, 0, XS_DPP, 8, 0, 0, 1, 1},
     {4, 2, 0, XS_DPP, 4, 0, 0, 1, 1}, {2, 2, 0, XS_DPP, 8, 0, 0, 1, 1}, {4, 2, 0, XS_DPP, 8, 1, 0, 1, 1, 4},
     {4, 2, 1, XS_DPP, -1, 0, 0, 1, 1}, {4, 2, 1, XS_DPP, -1, 1, 0, 1, 1}, {2, 2, 1, XS_DPP, -1, 0, 0, 1, 1},
-    {8, 2, 1, XS_DPP, -1, 0, 0, 1, 1}, {4, 2, 0, XS_DPP, -1, 0, 0, 1, 1}, {4, 2, 1, XS_DPP, -1, 0, 0, 2}};
+    {8, 2, 1, XS_DPP, -1, 0, 0, 1, 1}, {4, 2, 0, XS_DPP, -1, 0, 0, 1, 1}, {4, 2, 1, XS_DPP, -1, 0, 0, 2},
+    {4, 2, 0, XS_DPP, 8, 2, 0, 1, 1}, {4, 2, 0, XS_DPP, 8, 2, 0, 1, 1, 4}, {4, 2, 1, XS_DPP, 16, 2, 0, 1, 1, 4},
+    {8, 2, 0, XS_DPP, 8, 0, 0, 1, 1}, {4, 1, 0, XS_DPP, 8, 0, 0, 1, 1}, {4, 1, 0, XS_DPP, 16, 0, 0, 1, 1},
+    {2, 1, 0, XS_DPP, 8, 0, 0, 1, 1}, {4, 1, 0, XS_DPP, 4, 0, 0, 1, 1}, {8, 1, 0, XS_DPP, 8, 0, 0, 1, 1},
+    {2, 1, 0, XS_DPP, 16, 0, 0, 1, 1}, {4, 1, 1, XS_DPP, 8, 0, 0, 1, 1}};
 constexpr int kNumPrimalTiles = sizeof(kPrimalTiles) / sizeof(kPrimalTiles[0]);
 constexpr int kDefaultPrimalTile = 12;  // {4, 2, 1, XS_DPP, 16}: fields of 2 GiB or more, FP64
 constexpr int kMarchTileFP32 = 18;      // {4, 2, 1, XS_DPP, 16, 1}: fields of 2 GiB or more, FP32
-constexpr int kCachedTileFP64 = 8;      // {4, 2, 0, XS_DPP, 0}
-constexpr int kCachedTileFP32 = 40;     // {4, 2, 0, XS_DPP, 16, 0, 0, 1, 1}: two cells per lane
+// fields the Infinity Cache holds: cached read-modify-write, each XCD a
+// contiguous slab of layers (its L2 keeps the slab's halo rows)
+constexpr int kCachedTileFP64 = 59;     // {4, 1, 0, XS_DPP, 16, 0, 0, 1, 1}: buffer z-march, one cell per lane
+constexpr int kCachedTileFP32 = 58;     // {4, 1, 0, XS_DPP, 8, 0, 0, 1, 1}: two cells per lane
 constexpr int kBufMarchTileFP64 = 27;   // {4, 2, 1, XS_DPP, 16, 0, 0, 1}
 constexpr int kBufMarchTileFP32 = 36;   // {4, 2, 1, XS_DPP, 16, 0, 0, 1, 1}: two cells per lane
 
@@ -1389,7 +1413,7 @@ hipError_t launch_primal(int tile, cplx_t<T> *next, const cplx_t<T> *prev, const
         constexpr int czt = t.cz < 0 ? 0 : t.cz;  // 0: chunks sized at launch (buffer z-marches only)
         static_assert(t.cz >= 0 || t.buf, "run-time z-chunks need a buffer-addressed z-march");
         if constexpr (t.pair && sizeof(T) == 4) {
-            const auto kern = k_primal_zp<MODE, t.ty, czt, nt, t.ly != 0, t.waves>;
+            const auto kern = k_primal_zp<MODE, t.ty, czt, nt, t.ly, t.waves>;
             if constexpr (t.cz < 0) {
                 const hipError_t e = runtime_chunks(kern, t.ty, L, G);
                 if (e != hipSuccess) return e;
@@ -1514,9 +1538,9 @@ gpe_status gpe_primal::init() {
     trap.a = cfg.a;
     trap.rr = cfg.trap_ratio * cfg.trap_ratio;
     // default by size, the fastest shapes measured on MI355X: when both fields
-    // fit the 256 MB Infinity Cache (128^3 cells) a cached read-modify-write,
-    // FP64 one layer per block with DPP exchange, FP32 the buffer-addressed
-    // z-march with two cells per lane; larger grids (256^3) the
+    // fit the 256 MB Infinity Cache (128^3 cells) a cached read-modify-write
+    // in the buffer-addressed z-march, each XCD a contiguous slab (FP64 16
+    // layers per chunk, FP32 8 layers and two cells per lane); larger grids (256^3) the
     // buffer-addressed z-march of 16 layers with one layer of prefetch (FP32:
     // two cells per lane); fields of 2 GiB or more (beyond 32-bit offsets) the
     // z-march on 64-bit addresses

@@ -295,6 +295,14 @@ gpe_status gpe_server_set_launch_mode(gpe_server *s, uint32_t mode) {
     return gpe_batch_set_launch_mode(s->batch, mode);
 }
 
+gpe_status gpe_server_set_tile(gpe_server *s, uint32_t tile) {
+    if (!s) {
+        gpe::set_error("gpe_server_set_tile: null server");
+        return GPE_EINVAL;
+    }
+    return gpe_batch_set_tile(s->batch, tile);
+}
+
 gpe_status gpe_server_get_stats(const gpe_server *s, gpe_server_stats *out) {
     if (!s || !out) return GPE_EINVAL;
     *out = s->st;

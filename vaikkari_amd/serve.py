@@ -80,12 +80,20 @@ LAUNCH_MODES = {"graph": 0, "stream": 1}
 
 class Server:
     def __init__(self, precision: str = "fp32", hbm_budget_bytes: int = 200 * 10 ** 9, chunk_steps: int = 50,
-                 policy: str = "fifo", max_active_nodes: int = 0, launch: str = "graph"):
+                 policy: str = "fifo", max_active_nodes: int = 0, launch: str = "graph", tile=None):
         self._h = C.c_void_p()
         check(lib.gpe_server_create(PRECISIONS[precision], int(hbm_budget_bytes), int(chunk_steps),
                                     C.byref(self._h)), "gpe_server_create")
         self.set_policy(policy, max_active_nodes)
         self.set_launch_mode(launch)
+        if tile is not None:
+            self.set_tile(tile)
+
+    def set_tile(self, waves, cached: bool = False, slabs=False) -> None:
+        """Real-time tile of the shared launches, as Batch.set_tile (before the first admission)."""
+        from .batch import tile_code
+
+        check(lib.gpe_server_set_tile(self._h, tile_code(waves, cached, slabs)), "gpe_server_set_tile")
 
     def set_policy(self, policy: str, max_active_nodes: int = 0) -> None:
         check(lib.gpe_server_set_policy(self._h, POLICIES[policy], int(max_active_nodes)), "gpe_server_set_policy")
@@ -210,12 +218,12 @@ def latency_report(workload: List[Request], arrivals: dict, done: List[dict], en
 
 def run_serving(workload: List[Request], duration: float, precision: str = "fp32",
                 hbm_budget_bytes: int = 200 * 10 ** 9, chunk_steps: int = 50, log=None, policy: str = "fifo",
-                max_active_nodes: int = 0, sample_every: float = 10.0, launch: str = "graph") -> dict:
+                max_active_nodes: int = 0, sample_every: float = 10.0, launch: str = "graph", tile=None) -> dict:
     """Replays the workload against the wall clock for `duration` seconds: requests are
     submitted when their arrival time has passed, the server ticks continuously.  Every
     `sample_every` seconds the queue depth, resident solves and bytes, completions and the age
     of the oldest queued request are recorded (and logged)."""
-    srv = Server(precision, hbm_budget_bytes, chunk_steps, policy, max_active_nodes, launch)
+    srv = Server(precision, hbm_budget_bytes, chunk_steps, policy, max_active_nodes, launch, tile)
     t_start = srv.now()
     pending = sorted(workload, key=lambda r: r.arrival)
     k = 0
