@@ -93,11 +93,15 @@ __device__ __forceinline__ double trap_v4(const Trap &t, int i, int j, int k, in
 }
 
 // V of a node in a one-thread-per-cell kernel computing in T (the same bits
-// either way): FP64 takes the per-class offsets, which leave it at 5 waves
-// per SIMD; FP32 the 4-offset form, 1.4 % faster there at 128^3 cells.
-template <typename T>
+// either way): the FP64 real-time update takes the per-class offsets, which
+// leave it at 5 waves per SIMD; FP32 the 4-offset form, 1.4 % faster there at
+// 128^3 cells; the FP64 imaginary-time update the 4-offset form too, which
+// holds it at 3 waves per SIMD (133 registers): at 4 waves its blocks keep
+// more layers in flight than an XCD's L2 holds, and it read 1.6 x as many
+// bytes (704 against 442 MB per step at 128^3 cells) and ran 10 % slower.
+template <typename T, int MODE = MODE_REAL>
 __device__ __forceinline__ double trap_vt(const Trap &t, int i, int j, int k, int l) {
-    if constexpr (sizeof(T) == 4) return trap_v4(t, i, j, k, l);
+    if constexpr (sizeof(T) == 4 || MODE == MODE_IMAG) return trap_v4(t, i, j, k, l);
     else return trap_v(t, i, j, k, l);
 }
 
@@ -284,7 +288,7 @@ __device__ __forceinline__ double gather_tile(cplx_t<T> *__restrict__ next, cons
                 }
             }
             T pv;
-            if (POT == POT_TRAP) pv = (T)(trap_vt<T>(trap, i, j, k, l) * dt);
+            if (POT == POT_TRAP) pv = (T)(trap_vt<T, MODE>(trap, i, j, k, l) * dt);
             else pv = pot[base + l * L.xp];
             if (OBS) {
                 const double pr = (double)own[l].x, pi = (double)own[l].y;
@@ -439,7 +443,7 @@ __device__ __forceinline__ double zmarch_tile(cplx_t<T> *__restrict__ next, cons
                     }
                 }
                 T pv;
-                if (POT == POT_TRAP) pv = (T)(trap_vt<T>(trap, i, j, k, l) * dt);
+                if (POT == POT_TRAP) pv = (T)(trap_vt<T, MODE>(trap, i, j, k, l) * dt);
                 else pv = pot[base + l * L.xp];
                 const C sres = H.apply(n, cur[l], pv);
                 const long long at = base + l * L.xp;
