@@ -49,7 +49,9 @@ typedef enum gpe_status {
     GPE_EUNSTABLE = 4, /* non-finite values or a time step beyond the stability bound */
     GPE_EBUSY = 5,     /* resource budget exhausted (batch / serving admission) */
     GPE_ENOTSUP = 6,   /* combination not supported */
-    GPE_EIO = 7        /* file could not be read */
+    GPE_EIO = 7,       /* file could not be read or written, or is not what it should be */
+    GPE_ECANCELED = 8, /* serving: the request was cancelled (gpe_server_cancel) */
+    GPE_ETIMEDOUT = 9  /* serving: the request outlived the server's timeout */
 } gpe_status;
 
 typedef enum gpe_precision {
@@ -161,6 +163,23 @@ gpe_status gpe_step_observe(gpe_solver *s, uint64_t nsteps, double out[4]);
  * reference; it states the FP32-bulk error against an FP64 run at every
  * checkpoint of a precision study.  Waits for b's stream, runs on a's. */
 gpe_status gpe_compare_fields(gpe_solver *a, int field_a, gpe_solver *b, int field_b, double out[3]);
+
+/* Checkpoint and resume of a run.  No counterpart in the reference, which
+ * saves only the VortexState that starts a run (VortexState.cpp:74-96) and
+ * cannot resume one.  gpe_checkpoint_save writes the configuration in force
+ * (mode and dt as last set, kernel family and tile in use), the steps taken,
+ * both time levels (the even one alone in imaginary time) and a streamed
+ * potential to one file with a checksum; FP32 fields are stored exactly.
+ * gpe_checkpoint_load creates a solver (on the current device) holding that
+ * state: gpe_step on it continues the saved run bit for bit.  GPE_EIO for a
+ * file that cannot be written or read, or whose magic, sizes or checksum are
+ * wrong (checked before any device work). */
+gpe_status gpe_checkpoint_save(gpe_solver *s, const char *path);
+gpe_status gpe_checkpoint_load(const char *path, gpe_solver **out);
+/* The configuration in force: as created, with the mode and dt of the last
+ * gpe_set_mode, the potential kind of the last gpe_set_potential, and the
+ * kernel family and tile (1-based) in use. */
+gpe_status gpe_get_config(const gpe_solver *s, gpe_config *out);
 
 gpe_status gpe_synchronize(gpe_solver *s);
 gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info);

@@ -54,8 +54,9 @@ typedef struct gpe_completion {
     double arrival, admitted, finished; /* server clock, seconds */
     double node_steps;                  /* 12 n^3 * steps */
     double norm, energy;                /* FP64 observables of the final state */
-    int32_t status;                     /* GPE_OK, or why the request was dropped at admission
-                                           (then node_steps is 0 and norm, energy are NaN) */
+    int32_t status;                     /* GPE_OK, or why the request was dropped: at admission,
+                                           cancelled (GPE_ECANCELED) or timed out (GPE_ETIMEDOUT);
+                                           then node_steps is 0 and norm, energy are NaN */
     uint32_t reserved;
 } gpe_completion;
 
@@ -66,6 +67,8 @@ typedef struct gpe_server_stats {
     double device_ms;        /* device time of all chunks */
     double ticks;
     uint64_t failed;         /* requests dropped at admission (a failed completion each) */
+    uint64_t cancelled;      /* requests cancelled by gpe_server_cancel */
+    uint64_t timed_out;      /* requests dropped by the server's timeout */
 } gpe_server_stats;
 
 gpe_status gpe_server_create(uint32_t precision, uint64_t hbm_budget_bytes, uint32_t chunk_steps,
@@ -88,6 +91,18 @@ gpe_status gpe_server_tick(gpe_server *s, uint32_t *completed);
 gpe_status gpe_server_poll(gpe_server *s, gpe_completion *out, uint32_t max, uint32_t *n);
 /* Final state of a finished keep_result solve (odd may be NULL); frees it. */
 gpe_status gpe_server_fetch(gpe_server *s, uint64_t ticket, double *even, double *odd);
+/* Withdraws a queued or resident request: it leaves the queue, or its solve
+ * leaves the shared launches and frees its memory at once, and a completion
+ * with status GPE_ECANCELED is posted.  GPE_EINVAL for a ticket that is
+ * neither queued nor running (finished, dropped, or unknown).  No counterpart
+ * in the reference. */
+gpe_status gpe_server_cancel(gpe_server *s, uint64_t ticket);
+/* Time limit in seconds on the server clock from a request's arrival to its
+ * result (0: none, the default).  At the start of every tick each queued or
+ * resident request older than that is dropped with a GPE_ETIMEDOUT
+ * completion; a resident one frees its memory.  No counterpart in the
+ * reference. */
+gpe_status gpe_server_set_timeout(gpe_server *s, double seconds);
 gpe_status gpe_server_get_stats(const gpe_server *s, gpe_server_stats *st);
 
 typedef enum gpe_serve_policy {

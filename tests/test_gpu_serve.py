@@ -217,3 +217,86 @@ def test_unstable_and_failed_requests():
     st = srv.stats()
     assert st["completed"] == 2 and st["failed"] == 1 and st["admitted"] == 2 and st["resident_bytes"] == 0
     srv.close()
+
+
+def _req(n, steps, mode="real", seed=1, g=300.0):
+    return Request(arrival=0.0, n=n, steps=steps, g=g, ratio=1.0, mode=mode, h=0.25, seed=seed)
+
+
+def test_cancel_queued_and_running_requests():
+    """gpe_server_cancel: a queued request leaves the queue and a resident one leaves the shared
+    launches (its memory freed at once), each with one GPE_ECANCELED completion; the request that
+    was not cancelled gives the same bits as when served alone; unknown or finished tickets are
+    refused."""
+    announce("serve cancel queued + running")
+    from vaikkari_amd._lib import ECANCELED, EINVAL
+
+    a, b, c = _req(20, 400, seed=1), _req(16, 60, seed=2), _req(12, 40, seed=3)
+    srv = Server("fp32", 0, chunk_steps=8, policy="fifo", max_active_nodes=12 * 20 ** 3)
+    ta = srv.submit(a, arrival=0.0)
+    tb = srv.submit(b, arrival=0.0)
+    tc = srv.submit(c, arrival=0.0, keep_result=True)
+    srv.tick()  # a is admitted; the cap keeps b and c queued
+    st = srv.stats()
+    assert st["active"] == 1 and st["queued"] == 2 and st["resident_bytes"] > 0
+    srv.cancel(tb)
+    srv.cancel(ta)
+    st = srv.stats()
+    assert st["active"] == 0 and st["queued"] == 1 and st["resident_bytes"] == 0 and st["cancelled"] == 2
+    done = {d["ticket"]: d for d in srv.poll()}
+    assert set(done) == {ta, tb}
+    assert all(d["status"] == ECANCELED and d["node_steps"] == 0 and math.isnan(d["norm"]) for d in done.values())
+    while srv.stats()["completed"] < 1:
+        srv.tick()
+    rest = srv.poll()
+    assert [d["ticket"] for d in rest] == [tc] and rest[0]["status"] == OK
+    with pytest.raises(GpeError) as ei:
+        srv.cancel(tc)  # finished
+    assert ei.value.status == EINVAL
+    with pytest.raises(GpeError) as ei:
+        srv.cancel(10 ** 9)  # never issued
+    assert ei.value.status == EINVAL
+    mix = srv.fetch(tc, c.n, c.mode)
+    srv.close()
+    alone_srv = Server("fp32", 0, chunk_steps=8)
+    t = alone_srv.submit(c, arrival=0.0, keep_result=True)
+    while alone_srv.stats()["completed"] < 1:
+        alone_srv.tick()
+    alone = alone_srv.fetch(t, c.n, c.mode)
+    alone_srv.close()
+    np.testing.assert_array_equal(alone[0], mix[0])
+    np.testing.assert_array_equal(alone[1], mix[1])
+
+
+def test_timeout_drops_queued_and_resident_requests():
+    """gpe_server_set_timeout: at the next tick every queued or resident request older than the
+    limit is dropped with one GPE_ETIMEDOUT completion (a resident one frees its memory); a
+    negative or non-finite limit is refused."""
+    announce("serve timeout")
+    import time
+
+    from vaikkari_amd._lib import EINVAL, ETIMEDOUT
+
+    srv = Server("fp32", 0, chunk_steps=8)
+    for bad in (-1.0, float("nan"), float("inf")):
+        with pytest.raises(GpeError) as ei:
+            srv.set_timeout(bad)
+        assert ei.value.status == EINVAL
+    now = srv.now()
+    t_run = srv.submit(_req(16, 10 ** 6, seed=4), arrival=now)  # would run for a long time
+    srv.tick()
+    assert srv.stats()["active"] == 1
+    t_old = srv.submit(_req(12, 40, seed=5), arrival=now - 100.0)  # arrived long ago
+    srv.set_timeout(50.0)
+    srv.tick()  # drops the old request before admission; the young one keeps running
+    d = srv.poll()
+    assert [x["ticket"] for x in d] == [t_old] and d[0]["status"] == ETIMEDOUT
+    assert srv.stats()["active"] == 1
+    srv.set_timeout(1e-3)
+    time.sleep(0.01)
+    srv.tick()
+    d = srv.poll()
+    assert [x["ticket"] for x in d] == [t_run] and d[0]["status"] == ETIMEDOUT and math.isnan(d[0]["energy"])
+    st = srv.stats()
+    assert st["active"] == 0 and st["resident_bytes"] == 0 and st["timed_out"] == 2 and st["completed"] == 0
+    srv.close()

@@ -237,3 +237,37 @@ def test_observables_noalloc_argument_checks():
     cfg.potential, cfg.block_scale = _lib.POT_TRAP, -1.0
     with pytest.raises(_lib.GpeError):
         va.observables_noalloc(cfg, dummy, dummy)
+
+
+def _checkpoint_bytes(dims=(1, 1, 1), precision=0, mode=0, has_odd=True):
+    """A checkpoint file in the layout gpe_checkpoint_save writes (header, fields, FNV-1a 64)."""
+    import struct
+
+    nodes = 12 * dims[0] * dims[1] * dims[2]
+    body = b"VGPECKP1" + struct.pack("<I", 1)
+    body += struct.pack("<8I", *dims, precision, 1, mode, 1, 1)
+    body += struct.pack("<7d", 0.1, 1e-4, 10.0, -0.1, -0.1, -0.1, 1.0)
+    body += struct.pack("<QII", 7, 1 if has_odd else 0, 0)
+    rng = np.random.default_rng(5)
+    body += rng.uniform(-1, 1, 2 * nodes).tobytes()
+    if has_odd:
+        body += rng.uniform(-1, 1, 2 * nodes).tobytes()
+    h = 1469598103934665603
+    for b in body:
+        h = ((h ^ b) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return body + struct.pack("<Q", h)
+
+
+def test_checkpoint_load_rejects_bad_files_before_device_work(tmp_path):
+    """gpe_checkpoint_load refuses a missing file, a wrong magic, a short file and a file whose
+    checksum does not match with GPE_EIO, before any device work (so on a host without a GPU)."""
+    good = _checkpoint_bytes()
+    cases = {"missing": None, "magic": b"XXXXXXXX" + good[8:], "short": good[:-9],
+             "checksum": good[:200] + bytes([good[200] ^ 1]) + good[201:]}
+    for name, data in cases.items():
+        path = tmp_path / f"{name}.ckpt"
+        if data is not None:
+            path.write_bytes(data)
+        with pytest.raises(_lib.GpeError) as ei:
+            va.Solver.load_checkpoint(str(path))
+        assert ei.value.status == _lib.EIO, name

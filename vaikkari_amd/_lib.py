@@ -17,7 +17,7 @@ GPE_CLASSES = 12
 GPE_FACES = 4
 
 # gpe_status
-OK, EINVAL, ENOMEM, EDEVICE, EUNSTABLE, EBUSY, ENOTSUP, EIO = range(8)
+OK, EINVAL, ENOMEM, EDEVICE, EUNSTABLE, EBUSY, ENOTSUP, EIO, ECANCELED, ETIMEDOUT = range(10)
 # gpe_precision
 FP64, FP32_BULK = 0, 1
 # gpe_mode
@@ -125,7 +125,7 @@ class GpeServerStats(C.Structure):
         ("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
         ("queued", C.c_uint64), ("active", C.c_uint64), ("resident_bytes", C.c_uint64),
         ("node_steps_done", C.c_double), ("device_ms", C.c_double), ("ticks", C.c_double),
-        ("failed", C.c_uint64),
+        ("failed", C.c_uint64), ("cancelled", C.c_uint64), ("timed_out", C.c_uint64),
     ]
 
 
@@ -244,6 +244,11 @@ SIGNATURES = [
     ("gpe_server_oldest_queued", _D, [_P]),
     ("gpe_server_set_launch_mode", _I, [_P, _U32]),
     ("gpe_server_set_tile", _I, [_P, _U32]),
+    ("gpe_server_cancel", _I, [_P, _U64]),
+    ("gpe_checkpoint_save", _I, [_P, C.c_char_p]),
+    ("gpe_checkpoint_load", _I, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("gpe_get_config", _I, [_P, _P]),
+    ("gpe_server_set_timeout", _I, [_P, _D]),
     ("gpe_primal_create", _I, [C.POINTER(GpePrimalConfig), C.POINTER(_P)]),
     ("gpe_primal_destroy", None, [_P]),
     ("gpe_primal_set_state", _I, [_P, _DP, _DP]),

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 ARCH = os.environ.get("GPE_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["gpe_solver.hip", "gpe_batch.hip", "gpe_output.hip", "gpe_primal.hip", "diag.hip", "host_setup.cpp", "integrate.cpp",
-           "gpe_serve.cpp", "vortex_state.cpp"]
+           "gpe_serve.cpp", "vortex_state.cpp", "checkpoint.cpp"]
 
 FLAGS = [
     "-O3",

@@ -42,6 +42,7 @@ struct gpe_server {
     uint64_t max_active_nodes = 0;  // 0: no cap
     uint64_t active_nodes = 0;
     uint64_t budget = 0;  // HBM budget of the batch (0: none)
+    double timeout = 0.0;  // seconds from arrival to result (0: none)
     std::chrono::steady_clock::time_point t0;
     uint64_t next_ticket = 1;
     std::deque<Pending> queue;
@@ -139,8 +140,70 @@ gpe_status gpe_server_submit(gpe_server *s, const gpe_request *r, double arrival
 static uint64_t request_nodes(const gpe_request &r) { return 12ull * r.n * r.n * r.n; }
 static double request_work(const gpe_request &r) { return (double)request_nodes(r) * (double)r.steps; }
 
+// A completion without a result (dropped request): node_steps 0, NaN observables.
+static void post_dropped(gpe_server *s, uint64_t ticket, double arrival, double admitted, gpe_status why) {
+    gpe_completion cpl{};
+    cpl.ticket = ticket;
+    cpl.arrival = arrival;
+    cpl.admitted = admitted;
+    cpl.finished = gpe_server_now(s);
+    cpl.node_steps = 0.0;
+    cpl.norm = cpl.energy = NAN;
+    cpl.status = why;
+    s->done.push_back(cpl);
+}
+
+// Removes running[q]'s solve from the batch and the running list (the order
+// of running does not matter).
+static gpe_status drop_running(gpe_server *s, size_t q) {
+    Running r = s->running[q];
+    const gpe_status e = gpe_batch_remove(s->batch, r.id);
+    if (e != GPE_OK) return e;
+    s->active_nodes -= request_nodes(r.req);
+    s->running[q] = s->running.back();
+    s->running.pop_back();
+    return GPE_OK;
+}
+
+// Drops every queued or resident request older than the timeout; returns the
+// number dropped in *n.
+static gpe_status expire(gpe_server *s, uint32_t *n) {
+    *n = 0;
+    if (!(s->timeout > 0.0)) return GPE_OK;
+    const double now = gpe_server_now(s);
+    std::deque<Pending> rest;
+    for (const Pending &p : s->queue) {
+        if (now - p.arrival > s->timeout) {
+            post_dropped(s, p.ticket, p.arrival, now, GPE_ETIMEDOUT);
+            s->st.timed_out++;
+            (*n)++;
+        } else {
+            rest.push_back(p);
+        }
+    }
+    s->queue.swap(rest);
+    for (size_t q = 0; q < s->running.size();) {
+        const Running r = s->running[q];
+        if (now - r.arrival > s->timeout) {
+            const gpe_status e = drop_running(s, q);
+            if (e != GPE_OK) return e;
+            post_dropped(s, r.ticket, r.arrival, r.admitted, GPE_ETIMEDOUT);
+            s->st.timed_out++;
+            (*n)++;
+        } else {
+            q++;
+        }
+    }
+    return GPE_OK;
+}
+
 gpe_status gpe_server_tick(gpe_server *s, uint32_t *completed) {
     if (!s) return GPE_EINVAL;
+    uint32_t nexpired = 0;
+    {
+        const gpe_status e = expire(s, &nexpired);
+        if (e != GPE_OK) return e;
+    }
     // Admission: candidates in policy order, first fit within the HBM budget
     // and the node cap.
     std::vector<size_t> order(s->queue.size());
@@ -150,7 +213,7 @@ gpe_status gpe_server_tick(gpe_server *s, uint32_t *completed) {
             return request_work(s->queue[a].req) < request_work(s->queue[b].req);
         });
     std::vector<char> taken(s->queue.size(), 0);
-    uint32_t ndone = 0;
+    uint32_t ndone = nexpired;
     gpe_status device_error = GPE_OK;
     for (size_t q : order) {
         const Pending &p = s->queue[q];
@@ -268,6 +331,41 @@ gpe_status gpe_server_fetch(gpe_server *s, uint64_t ticket, double *even, double
     st = gpe_batch_remove(s->batch, it->second.id);
     s->kept.erase(it);
     return st;
+}
+
+gpe_status gpe_server_cancel(gpe_server *s, uint64_t ticket) {
+    if (!s) {
+        gpe::set_error("gpe_server_cancel: null server");
+        return GPE_EINVAL;
+    }
+    for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
+        if (it->ticket == ticket) {
+            const Pending p = *it;
+            s->queue.erase(it);
+            post_dropped(s, p.ticket, p.arrival, gpe_server_now(s), GPE_ECANCELED);
+            s->st.cancelled++;
+            return GPE_OK;
+        }
+    for (size_t q = 0; q < s->running.size(); q++)
+        if (s->running[q].ticket == ticket) {
+            const Running r = s->running[q];
+            const gpe_status e = drop_running(s, q);
+            if (e != GPE_OK) return e;
+            post_dropped(s, r.ticket, r.arrival, r.admitted, GPE_ECANCELED);
+            s->st.cancelled++;
+            return GPE_OK;
+        }
+    gpe::set_error("gpe_server_cancel: ticket %llu is neither queued nor running", (unsigned long long)ticket);
+    return GPE_EINVAL;
+}
+
+gpe_status gpe_server_set_timeout(gpe_server *s, double seconds) {
+    if (!s || !(seconds >= 0.0) || !std::isfinite(seconds)) {
+        gpe::set_error("gpe_server_set_timeout: the timeout must be a finite number of seconds >= 0");
+        return GPE_EINVAL;
+    }
+    s->timeout = seconds;
+    return GPE_OK;
 }
 
 gpe_status gpe_server_set_policy(gpe_server *s, uint32_t policy, uint64_t max_active_nodes) {

@@ -72,6 +72,8 @@ const char *gpe_status_string(gpe_status st) {
         case GPE_EBUSY: return "budget exhausted";
         case GPE_ENOTSUP: return "not supported";
         case GPE_EIO: return "i/o error";
+        case GPE_ECANCELED: return "cancelled";
+        case GPE_ETIMEDOUT: return "timed out";
     }
     return "unknown status";
 }

@@ -102,6 +102,14 @@ class Server:
         """'graph' (captured chunk, the default) or 'stream' (kernels enqueued one by one)."""
         check(lib.gpe_server_set_launch_mode(self._h, LAUNCH_MODES[mode]), "gpe_server_set_launch_mode")
 
+    def cancel(self, ticket: int) -> None:
+        """Withdraws a queued or resident request (a GPE_ECANCELED completion follows)."""
+        check(lib.gpe_server_cancel(self._h, int(ticket)), "gpe_server_cancel")
+
+    def set_timeout(self, seconds: float) -> None:
+        """Drops requests older than `seconds` since arrival at the next tick (0: no limit)."""
+        check(lib.gpe_server_set_timeout(self._h, float(seconds)), "gpe_server_set_timeout")
+
     def oldest_queued(self) -> Optional[float]:
         t = lib.gpe_server_oldest_queued(self._h)
         return None if t < 0 else t

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from typing import Optional, Sequence
 
 import numpy as np
@@ -74,6 +75,37 @@ class Solver:
         self._h = C.c_void_p()
         check(lib.gpe_create(C.byref(cfg), C.byref(self._h)), "gpe_create")
         self.nodes = nodes_of(self.dims)
+
+    # -- checkpoint and resume --------------------------------------------
+    def save_checkpoint(self, path: str) -> None:
+        """Writes the run's state (configuration in force, steps taken, both time levels, a
+        streamed potential) to `path` (gpe_checkpoint_save)."""
+        check(lib.gpe_checkpoint_save(self._h, os.fsencode(path)), "gpe_checkpoint_save")
+
+    @classmethod
+    def load_checkpoint(cls, path: str) -> "Solver":
+        """A solver resuming the run saved at `path` (gpe_checkpoint_load): stepping it continues
+        the saved run bit for bit."""
+        h = C.c_void_p()
+        check(lib.gpe_checkpoint_load(os.fsencode(path), C.byref(h)), "gpe_checkpoint_load")
+        s = cls.__new__(cls)
+        s._h = h
+        cfg = _lib.GpeConfig()
+        check(lib.gpe_get_config(h, C.byref(cfg)), "gpe_get_config")
+        s.cfg = cfg
+        s.dims = (int(cfg.nx), int(cfg.ny), int(cfg.nz))
+        s.block_scale = float(cfg.block_scale)
+        s.origin = tuple(float(o) for o in cfg.origin)
+        s.nodes = nodes_of(s.dims)
+        return s
+
+    def config(self) -> dict:
+        """The configuration in force (gpe_get_config)."""
+        cfg = _lib.GpeConfig()
+        check(lib.gpe_get_config(self._h, C.byref(cfg)), "gpe_get_config")
+        out = {k: getattr(cfg, k) for k, _ in _lib.GpeConfig._fields_}
+        out["origin"] = tuple(cfg.origin)
+        return out
 
     # -- lifecycle -------------------------------------------------------
     def close(self) -> None:
