@@ -167,7 +167,8 @@ gpe_status gpe_compare_fields(gpe_solver *a, int field_a, gpe_solver *b, int fie
 /* Checkpoint and resume of a run.  No counterpart in the reference, which
  * saves only the VortexState that starts a run (VortexState.cpp:74-96) and
  * cannot resume one.  gpe_checkpoint_save writes the configuration in force
- * (mode and dt as last set, kernel family and tile in use), the steps taken,
+ * (mode and dt as last set, kernel family, and the tile chosen or that the
+ * family's defaults are in use), the steps taken,
  * both time levels (the even one alone in imaginary time) and a streamed
  * potential to one file with a checksum; FP32 fields are stored exactly.
  * gpe_checkpoint_load creates a solver (on the current device) holding that

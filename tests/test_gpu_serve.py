@@ -165,7 +165,8 @@ def test_served_results_match_solver_and_oracle(launch):
         dt, e0, o0 = _initial_state(r, "fp64")
         assert va.field_norm(dims, b, e0) == pytest.approx(1.0, abs=1e-13)  # unit norm on the grid
         lf, lf0, gdt = va.leapfrog_coefs(b, dt, r.g)
-        s = va.Solver(dims, b, dt, r.g, origin, "fp64", r.mode, "trap", r.ratio, "gather")
+        # tile 11 (4 waves, y-bands): the tile of the server's ground-state launches
+        s = va.Solver(dims, b, dt, r.g, origin, "fp64", r.mode, "trap", r.ratio, "gather", 11)
         if r.mode == "real":
             s.set_state(e0, o0)
             s.step(r.steps)

@@ -11,7 +11,7 @@
 //
 // File layout (little-endian):
 //   "VGPECKP1", u32 version (1),
-//   u32 nx, ny, nz, precision, potential, mode, kernel, tile,
+//   u32 nx, ny, nz, precision, potential, mode, kernel, tile (0: the defaults),
 //   f64 block_scale, dt, g, origin[3], trap_ratio,
 //   u64 steps_done, u32 has_odd, u32 has_potential,
 //   f64 even[2N], f64 odd[2N] (if has_odd), f64 V[N] (if has_potential),
@@ -92,8 +92,10 @@ gpe_status gpe_checkpoint_save(gpe_solver *s, const char *path) {
     Writer w{f};
     w.put(kMagic, sizeof kMagic);
     w.val(kVersion);
-    const uint32_t u[8] = {s->cfg.nx,   s->cfg.ny,           s->cfg.nz,         s->cfg.precision,
-                           s->cfg.potential, s->cfg.mode, (uint32_t)s->kernel, (uint32_t)s->tile + 1};
+    // tile 0 when the solver runs its family's defaults (real and imaginary time may differ)
+    const uint32_t tile = s->tile_default ? 0u : (uint32_t)s->tile + 1;
+    const uint32_t u[8] = {s->cfg.nx,        s->cfg.ny,   s->cfg.nz,           s->cfg.precision,
+                           s->cfg.potential, s->cfg.mode, (uint32_t)s->kernel, tile};
     w.put(u, sizeof u);
     const double d[7] = {s->cfg.block_scale, s->cfg.dt,        s->cfg.g,          s->cfg.origin[0],
                          s->cfg.origin[1],   s->cfg.origin[2], s->cfg.trap_ratio};

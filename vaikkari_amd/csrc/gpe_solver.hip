@@ -98,6 +98,15 @@ int default_tile(int kernel, uint32_t precision) {
     return 0;
 }
 
+// Default tile of the imaginary-time Euler step.  FP64 gather: 8 waves per
+// block, y-bands ({8, 1, 2, 1}), 3-4 % faster than the real-time default
+// there (167 against 162 G site-steps/s at 128^3 cells); otherwise the
+// real-time default.
+int default_imag_tile(int kernel, uint32_t precision) {
+    if (kernel == GPE_KERNEL_GATHER && precision == GPE_FP64) return 12;
+    return default_tile(kernel, precision);
+}
+
 TileShape tile_shape(int kernel, int tile) {
     return kernel == GPE_KERNEL_ZMARCH ? kZmarchTiles[tile]
            : kernel == GPE_KERNEL_FUSED ? kFusedTiles[tile]
@@ -361,6 +370,7 @@ gpe_status gpe_solver::init(const gpe_config &c) {
         return GPE_EINVAL;
     }
     tile = c.tile == 0 ? default_tile(kernel, c.precision) : (int)c.tile - 1;
+    tile_default = c.tile == 0;
     if (!tile_supported(kernel, tile, c.precision, L)) {
         set_error("gpe_create: fused tile %u does not fit LDS in FP64, or is a row tile on a field of 2 GiB or more",
                   c.tile);
@@ -565,7 +575,9 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
         // next read through scal[1] = 1/sqrt(N).  (The fused pass is a
         // leapfrog step; imaginary time runs the single-update z-march.)
         const int kernel = this->kernel == GPE_KERNEL_FUSED ? GPE_KERNEL_ZMARCH : this->kernel;
-        const int tile = this->kernel == GPE_KERNEL_FUSED ? 0 : this->tile;
+        const int tile = this->kernel == GPE_KERNEL_FUSED ? 0
+                         : tile_default                  ? default_imag_tile(kernel, cfg.precision)
+                                                         : this->tile;
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
             C *in = (C *)field[ie], *out = (C *)field[io];
             const bool rev = (steps_done + s) & 1;  // alternate directions: start where the last step ended
@@ -1100,6 +1112,7 @@ gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile) {
     }
     s->kernel = (int)kernel;
     s->tile = t0;
+    s->tile_default = tile == 0;
     return GPE_OK;
 }
 

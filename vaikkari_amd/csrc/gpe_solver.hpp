@@ -23,6 +23,7 @@ struct gpe_solver {
     gpe::Coefs co{};
     int kernel = GPE_KERNEL_GATHER;
     int tile = 0;
+    bool tile_default = true;  // the family's default tile (0 at creation or in gpe_set_variant)
     int device = 0;
     // field[ie] holds the even time level, field[io] the odd one (or the
     // imaginary-time scratch buffer).  The fused kernel writes the step's
