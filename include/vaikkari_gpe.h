@@ -89,8 +89,13 @@ typedef struct gpe_config {
     uint32_t mode;         /* gpe_mode */
     double trap_ratio;     /* RATIO of the in-kernel trap (GPUGrossPitaevskii_BCC.cu:13) */
     uint32_t kernel;       /* gpe_kernel */
-    uint32_t tile;         /* tile shape 1..gpe_variant_tiles(kernel), 0 = the family's defaults
-                              (FP64 gather: tile 11 in real time, 13 in imaginary time) */
+    uint32_t tile;         /* tile shape 1..gpe_variant_tiles(kernel), 0 = the family's defaults;
+                              gather: by grid, precision and mode (real time with both fields in
+                              256 MiB: tile 4, or 14 on grids of at most 1024 one-wave blocks;
+                              else the y-band tile with the most waves whose block rows divide
+                              by 8: 13, 11, 12, 14 for 8 (FP64 imaginary time only), 4, 2, 1
+                              waves; none: tile 7).  128^3 cells: tile 11, FP64 imaginary
+                              time 13 */
 } gpe_config;
 
 typedef struct gpe_solver gpe_solver;

@@ -1,0 +1,56 @@
+"""GPU test of the gather family's default tile (gpe_config.tile = 0): the tile follows the grid,
+the precision and the cache fit, and stepping on it gives the bits of every other tile."""
+import numpy as np
+import pytest
+
+import vaikkari_amd as va
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+# (dims, precision) -> real-time default tile (1-based)
+CASES = [
+    ((32, 32, 32), "fp64", 14),   # both fields cached, 512 one-wave blocks in y-bands
+    ((24, 24, 24), "fp64", 4),    # cached; one-wave block rows (12) do not divide by 8
+    ((48, 48, 48), "fp32", 4),    # cached
+    ((80, 80, 80), "fp64", 4),    # 197 MB of fields: cached
+    ((96, 96, 96), "fp32", 4),    # 170 MB: cached
+    ((96, 96, 96), "fp64", 12),   # 340 MB: 4-wave rows 12, 2-wave rows 24
+    ((112, 112, 112), "fp32", 7),  # 270 MB; rows 7, 14, 28: no y-bands
+    ((128, 128, 128), "fp64", 11),
+    ((128, 128, 128), "fp32", 11),
+    ((160, 160, 160), "fp64", 12),
+]
+
+
+@pytest.mark.parametrize("dims,precision,want", CASES)
+def test_default_gather_tile_by_grid(dims, precision, want):
+    announce(f"default tile {dims} {precision}")
+    h = 0.25
+    b = h / np.sqrt(8.0)
+    s = va.Solver(dims, b, 1e-3, 500.0, va.centered_origin(dims, b), precision, "real", "trap", 1.0)
+    info = s.info()
+    s.close()
+    assert info["kernel"] == "gather" and info["tile"] == want
+
+
+def test_default_tile_steps_equal_other_tiles():
+    """At 32^3 the default (tile 14) steps the same bits as tile 11, the 128^3 default."""
+    announce("default tile 32^3 bits")
+    dims, h = (32, 32, 32), 0.25
+    b = h / np.sqrt(8.0)
+    origin = va.centered_origin(dims, b)
+    psi = va.gaussian(dims, origin, b, 1.5, 1e-3, 5)
+    out = []
+    for tile in (0, 11):
+        s = va.Solver(dims, b, 1e-3, 500.0, origin, "fp64", "real", "trap", 1.0, "gather", tile)
+        s.set_state(psi)
+        s.init_odd(10.0)
+        s.step(40)
+        out.append(s.get_state())
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])

@@ -208,7 +208,7 @@ def test_random_shapes_output_bit_exact(case, precision):
 # one layer / 8 / 4 layers per tile), hence the same norm summation order
 # the single solver's tile whose arithmetic and block partials a batched ground state of
 # zlayers layers per tile repeats: gather {4 waves, y-bands} (tile 11; the single solver's own
-# FP64 imaginary-time default is 8 waves), z-march 8 or 4 layers
+# default depends on the grid), z-march 8 or 4 layers
 IMAG_TILES = {0: ("gather", 11), 8: ("zmarch", 1), 4: ("zmarch", 6)}
 
 

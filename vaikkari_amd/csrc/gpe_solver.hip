@@ -89,22 +89,44 @@ int num_tiles(int kernel) {
     return kernel == GPE_KERNEL_ZMARCH ? kNumZmarchTiles : kernel == GPE_KERNEL_FUSED ? kNumFusedTiles : kNumGatherTiles;
 }
 
-// Tile index used when the caller asks for tile 0.
-int default_tile(int kernel, uint32_t precision) {
-    // the one-wave-per-row tiles measured fastest: {2 waves, 32 layers} in
-    // FP64, {2 waves, 16 layers} in FP32
-    if (kernel == GPE_KERNEL_FUSED) return precision == GPE_FP64 ? 10 : 13;
-    if (kernel == GPE_KERNEL_GATHER) return 10;  // 4 waves, streaming RMW, y-bands per XCD
-    return 0;
+// Block rows per layer of a gather tile of ty waves (as grid_of computes them).
+int gather_rows(int ty, const Layout &L) {
+    const int rows = ty * (64 >> row_lanes_log2(L.nx));
+    return (L.ny + rows - 1) / rows;
 }
 
-// Default tile of the imaginary-time Euler step.  FP64 gather: 8 waves per
-// block, y-bands ({8, 1, 2, 1}), 3-4 % faster than the real-time default
-// there (167 against 162 G site-steps/s at 128^3 cells); otherwise the
-// real-time default.
-int default_imag_tile(int kernel, uint32_t precision) {
-    if (kernel == GPE_KERNEL_GATHER && precision == GPE_FP64) return 12;
-    return default_tile(kernel, precision);
+// Tile index used when the caller asks for tile 0, by family, precision,
+// mode and grid.
+// Fused family: the one-wave-per-row tiles measured fastest, {2 waves,
+// 32 layers} in FP64 and {2 waves, 16 layers} in FP32.  Z-march: tile 1.
+// Gather family, a rule fitted to every gather tile timed on MI355X over 24^3
+// to 160^3 cells, 87x87x302 and 100x60x140 (both precisions, both modes; the
+// rule's choice runs at 99 % of the fastest tile on geometric mean, 92 % at
+// worst):
+//  - real time with both time levels in 256 MiB (the Infinity Cache keeps
+//    them between the two passes): {1 wave, contiguous eighths, cached
+//    read-modify-write} (tile 4), or {1 wave, y-bands, streaming} (tile 14)
+//    when its block rows divide into y-bands and it launches at most 1024
+//    blocks (32^3: 1.5x tile 4);
+//  - otherwise y-bands per XCD with streaming read-modify-write, the most
+//    waves per block (up to 8 in FP64 imaginary time, 4 else: tiles 13, 11,
+//    12, 14 for 8, 4, 2, 1) whose block rows divide by the 8 XCDs; when none
+//    does, {4 waves, contiguous eighths, streaming} (tile 7).
+// At 128^3 cells this is tile 11 (tile 13 in FP64 imaginary time).
+int default_tile(int kernel, uint32_t precision, int mode, const Layout &L) {
+    if (kernel == GPE_KERNEL_FUSED) return precision == GPE_FP64 ? 10 : 13;
+    if (kernel != GPE_KERNEL_GATHER) return 0;
+    const unsigned long long both = 2ull * (unsigned long long)L.elems * (precision == GPE_FP64 ? 16u : 8u);
+    const long long gx = (L.nx + (1 << row_lanes_log2(L.nx)) - 1) >> row_lanes_log2(L.nx);
+    if (mode == MODE_REAL && both <= (256ull << 20)) {
+        const int gy1 = gather_rows(1, L);
+        return gy1 % 8 == 0 && gx * gy1 * L.nz <= 1024 ? 13 : 3;
+    }
+    const int wmax = precision == GPE_FP64 && mode == MODE_IMAG ? 8 : 4;
+    static constexpr int kWaves[4] = {8, 4, 2, 1}, kBandTile[4] = {12, 10, 11, 13};
+    for (int i = 0; i < 4; i++)
+        if (kWaves[i] <= wmax && gather_rows(kWaves[i], L) % 8 == 0) return kBandTile[i];
+    return 6;
 }
 
 TileShape tile_shape(int kernel, int tile) {
@@ -369,7 +391,7 @@ gpe_status gpe_solver::init(const gpe_config &c) {
         set_error("gpe_create: tile %u out of range for this kernel family", c.tile);
         return GPE_EINVAL;
     }
-    tile = c.tile == 0 ? default_tile(kernel, c.precision) : (int)c.tile - 1;
+    tile = c.tile == 0 ? default_tile(kernel, c.precision, MODE_REAL, L) : (int)c.tile - 1;
     tile_default = c.tile == 0;
     if (!tile_supported(kernel, tile, c.precision, L)) {
         set_error("gpe_create: fused tile %u does not fit LDS in FP64, or is a row tile on a field of 2 GiB or more",
@@ -576,7 +598,7 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
         // leapfrog step; imaginary time runs the single-update z-march.)
         const int kernel = this->kernel == GPE_KERNEL_FUSED ? GPE_KERNEL_ZMARCH : this->kernel;
         const int tile = this->kernel == GPE_KERNEL_FUSED ? 0
-                         : tile_default                  ? default_imag_tile(kernel, cfg.precision)
+                         : tile_default                  ? default_tile(kernel, cfg.precision, MODE_IMAG, L)
                                                          : this->tile;
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
             C *in = (C *)field[ie], *out = (C *)field[io];
@@ -990,7 +1012,8 @@ gpe_status gpe_step_observe(gpe_solver *s, uint64_t nsteps, double out[4]) {
     }
     // the last step as two gather passes (any family gives the same bits),
     // the odd one accumulating the observables of the even field it reads
-    const int tile = s->kernel == GPE_KERNEL_GATHER ? s->tile : default_tile(GPE_KERNEL_GATHER, s->cfg.precision);
+    const int tile =
+        s->kernel == GPE_KERNEL_GATHER ? s->tile : default_tile(GPE_KERNEL_GATHER, s->cfg.precision, MODE_REAL, s->L);
     const gpe_status st = s->cfg.precision == GPE_FP64 ? s->observe_step<double, FORM_REF>(tile)
                                                        : s->observe_step<float, FORM_DIFF>(tile);
     if (st != GPE_OK) return st;
@@ -1097,7 +1120,7 @@ gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile) {
         set_error("gpe_set_variant: tile %u out of range", tile);
         return GPE_EINVAL;
     }
-    const int t0 = tile == 0 ? default_tile((int)kernel, s->cfg.precision) : (int)tile - 1;
+    const int t0 = tile == 0 ? default_tile((int)kernel, s->cfg.precision, MODE_REAL, s->L) : (int)tile - 1;
     if (!tile_supported((int)kernel, t0, s->cfg.precision, s->L)) {
         set_error("gpe_set_variant: fused tile %u does not fit LDS in FP64, or is a row tile on a field of 2 GiB "
                   "or more", tile);
