@@ -597,12 +597,13 @@ def main():
         batched = batched_workload(args.batch_instances, args.batch_steps)
         log(f"batched: {batched['site_steps_per_s'] / 1e9:.1f} G site-steps/s over {batched['instances']} instances "
             f"({batched['frac_of_peak'] * 100:.1f}% of peak), norm drift {batched['max_rel_norm_drift']:.1e}")
-        # the smallest size of the batched form alone: 256 instances of 24^3 cells (1 GiB of fields), the
-        # tile fixed to 1 wave per block with each XCD a contiguous eighth of a group (the best measured)
-        small = batched_workload(256, 1000, tile=(1, False, 2), group_mib=512, sizes=(24,))
+        # the smallest size of the batched form alone: 256 instances of 24^3 cells (1 GiB of fields),
+        # tile and group size autotuned as for the mixed form
+        small = batched_workload(256, 1000, sizes=(24,))
         batched["uniform_24"] = {k: small[k] for k in ("instances", "steps", "nodes_total", "site_steps_per_s",
                                                         "algorithmic_GBs", "frac_of_peak", "tile_waves",
-                                                        "tile_slabs", "group_MiB", "max_rel_norm_drift")}
+                                                        "tile_cached_rmw", "tile_slabs", "group_MiB",
+                                                        "max_rel_norm_drift")}
         log(f"batched, 256 x 24^3: {small['site_steps_per_s'] / 1e9:.1f} G site-steps/s "
             f"({small['frac_of_peak'] * 100:.1f}% of peak)")
     serving = None
