@@ -21,7 +21,8 @@ on the solver's stream around exactly --steps steps.
 Beside the line of record (all untimed or timed separately): fp32_check, FP32
 against FP64 over a fixed --check-steps from the phase-2 state, psi relative L2
 (reduced on the device) with norm and energy drift at four checkpoints; dual256,
-both precisions at the precision form's 256^3 grid; the batched form (64
+both precisions at the precision form's 256^3 grid; cpu_form_gpu, both precisions
+on the CPU form's 32^3 grid (1000 steps); the batched form (64
 instances), a 20 s sample of the serving form, the 14-point primal stepper, and
 cpu_baseline, the CPU oracle on the host's cores (warm-up, then the median of
 timed runs) at the 128^3 grid and at the 32^3 CPU-form grid.
@@ -466,6 +467,34 @@ def dual256_leg(steps, warmup=5):
     return out
 
 
+def cpu_form_leg(steps=1000, warmup=20):
+    """The CPU form's workload on the GPU (32^3 cells, 393,216 nodes, h 0.25, g 500, isotropic
+    trap, dt 1e-3; unit-norm Gaussian sigma 1.5 with 1e-3 seeded noise, odd level at mu 10): `steps`
+    real-time steps, FP64 and FP32 bulk, default tiles, timed with HIP events -- the GPU figure
+    beside cpu_baseline's 32^3 grid."""
+    import vaikkari_amd as va
+
+    dims, h, g, dt = (32, 32, 32), 0.25, 500.0, 1e-3
+    b = h / math.sqrt(8)
+    origin = va.centered_origin(dims, b)
+    nodes = va.nodes_of(dims)
+    psi = va.gaussian(dims, origin, b, 1.5, 1e-3, 54363)
+    out = {"cells": list(dims), "nodes": nodes, "h": h, "g": g, "dt": dt, "steps": steps}
+    for prec in ("fp64", "fp32"):
+        with va.Solver(dims, b, dt, g, origin, prec, "real", "trap", 1.0) as s:
+            s.set_state(psi)
+            s.init_odd(10.0)
+            s.step(warmup)
+            s.synchronize()
+            ms, launches = s.time_steps(steps)
+            info = s.info()
+        out[prec] = {"site_steps_per_s": nodes * steps / (ms * 1e-3), "ms": ms,
+                     "avg_launch_us": ms * 1e3 / launches, "kernel": info["kernel"], "tile": info["tile"]}
+        log(f"32^3 {prec}: {out[prec]['site_steps_per_s'] / 1e9:.2f} G site-steps/s, 1000 steps in {ms:.1f} ms "
+            f"(tile {info['tile']})")
+    return out
+
+
 def main():
     args = parse()
     if args.suite in ("precision", "serving", "primal", "batched"):
@@ -591,6 +620,7 @@ def main():
              "max_abs_energy_drift_fp64": max(abs(r["energy_drift_fp64"]) for r in check_rows)}
 
     dual256 = dual256_leg(args.dual256_steps) if args.dual256_steps > 0 else None
+    cpu_form_gpu = cpu_form_leg()
 
     batched = None
     if args.batch_instances > 0:
@@ -656,6 +686,7 @@ def main():
         },
         "fp32_check": check,
         "dual256": dual256,
+        "cpu_form_gpu": cpu_form_gpu,
         "phase1_imag": {"steps": args.imag_steps, "ms": ms_imag, "site_steps_per_s": nodes * args.imag_steps / (ms_imag * 1e-3),
                         "E": obs["E"], "mu": obs["mu"], "N": obs["N"]},
         "bytes_per_node_step": {"fp64": r64["bytes_per_node_step"], "fp32": r32["bytes_per_node_step"]},
