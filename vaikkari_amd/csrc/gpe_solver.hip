@@ -100,9 +100,9 @@ int gather_rows(int ty, const Layout &L) {
 // Fused family: the one-wave-per-row tiles measured fastest, {2 waves,
 // 32 layers} in FP64 and {2 waves, 16 layers} in FP32.  Z-march: tile 1.
 // Gather family, a rule fitted to every gather tile timed on MI355X over 24^3
-// to 160^3 cells, 87x87x302 and 100x60x140 (both precisions, both modes; the
-// rule's choice runs at 99 % of the fastest tile on geometric mean, 92 % at
-// worst):
+// to 256^3 cells, 87x87x302, 100x60x140 and 300x300x200 (both precisions,
+// both modes; the rule's choice runs at 99 % of the fastest tile on geometric
+// mean, 92 % at worst):
 //  - real time with both time levels in 256 MiB (the Infinity Cache keeps
 //    them between the two passes): {1 wave, contiguous eighths, cached
 //    read-modify-write} (tile 4), or {1 wave, y-bands, streaming} (tile 14)
