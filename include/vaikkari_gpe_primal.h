@@ -84,8 +84,8 @@ gpe_status gpe_primal_synchronize(gpe_primal *p);
  * as an x-y part and a z part, each rounded to FP32.
  * Tile 0 in the config picks by size: when both fields fit in 192 MB (the
  * Infinity Cache holds them) a cached read-modify-write in the buffer
- * z-march with a contiguous slab of layers per XCD (tile 60 FP64: 16 layers
- * per chunk; tile 59 FP32: 8 layers, two cells per lane); below 2 GiB per
+ * z-march with a contiguous slab of layers per XCD, 16 layers per chunk
+ * (tile 60; FP32 two cells per lane); below 2 GiB per
  * field the buffer z-march of 16 layers (tile 28 FP64, tile 37 FP32); larger
  * fields the z-march on 64-bit addresses (tile 13 FP64, tile 19 FP32). */
 int gpe_primal_tiles(void);

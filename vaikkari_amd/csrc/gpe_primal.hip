@@ -1306,7 +1306,7 @@ constexpr int kMarchTileFP32 = 18;      // {4, 2, 1, XS_DPP, 16, 1}: fields of 2
 // fields the Infinity Cache holds: cached read-modify-write, each XCD a
 // contiguous slab of layers (its L2 keeps the slab's halo rows)
 constexpr int kCachedTileFP64 = 59;     // {4, 1, 0, XS_DPP, 16, 0, 0, 1, 1}: buffer z-march, one cell per lane
-constexpr int kCachedTileFP32 = 58;     // {4, 1, 0, XS_DPP, 8, 0, 0, 1, 1}: two cells per lane
+constexpr int kCachedTileFP32 = 59;     // the same shape, two cells per lane
 constexpr int kBufMarchTileFP64 = 27;   // {4, 2, 1, XS_DPP, 16, 0, 0, 1}
 constexpr int kBufMarchTileFP32 = 36;   // {4, 2, 1, XS_DPP, 16, 0, 0, 1, 1}: two cells per lane
 
@@ -1539,8 +1539,8 @@ gpe_status gpe_primal::init() {
     trap.rr = cfg.trap_ratio * cfg.trap_ratio;
     // default by size, the fastest shapes measured on MI355X: when both fields
     // fit the 256 MB Infinity Cache (128^3 cells) a cached read-modify-write
-    // in the buffer-addressed z-march, each XCD a contiguous slab (FP64 16
-    // layers per chunk, FP32 8 layers and two cells per lane); larger grids (256^3) the
+    // in the buffer-addressed z-march, each XCD a contiguous slab, 16 layers
+    // per chunk (FP32: two cells per lane); larger grids (256^3) the
     // buffer-addressed z-march of 16 layers with one layer of prefetch (FP32:
     // two cells per lane); fields of 2 GiB or more (beyond 32-bit offsets) the
     // z-march on 64-bit addresses
