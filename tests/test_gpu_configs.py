@@ -154,7 +154,7 @@ def test_precision_form_fp64_against_oracle(orc, precision_state):
 def test_precision_form_fp32_against_fp64_10000_steps(precision_state):
     """FP32 bulk beside the FP64 path (bit-identical to the oracle, test above) for the form's
     10,000 steps at dt = 5e-5, compared on the device every 1000 steps: psi relative L2 <= 1e-5 at
-    every checkpoint (north_star's bound), and the FP32 norm and energy within 1e-7 relative of the
+    every checkpoint (the stated FP32-bulk error bound), and the FP32 norm and energy within 1e-7 relative of the
     FP64 ones.  Every checkpoint is printed."""
     w = precision_state
     announce("precision form 256^3: fp32 vs fp64 x 10000")

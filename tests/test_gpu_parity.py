@@ -410,8 +410,8 @@ def test_integrate_in_time_drop_in(orc, tmp_path):
 
 
 def test_imag_then_real_workflow(orc):
-    """Ground state by imaginary time, vortex imprint, then real time: the config-2 workflow
-    at 16^3, every stage against the oracle."""
+    """Ground state by imaginary time, vortex imprint, then real time: the drop-in benchmark's
+    workflow at 16^3, every stage against the oracle."""
     dims, h, g, ratio = (16, 16, 16), 0.3, 800.0, 2.0
     announce("imag->real workflow")
     b, origin, even, _, V = case(dims, h, g, 1e-3, ratio, seed=3, sigma=1.0)
