@@ -66,12 +66,14 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson serving sample (0 = skip)")
-    ap.add_argument("--suite", default="main", choices=["main", "precision", "serving", "primal", "batched"],
+    ap.add_argument("--suite", default="main",
+                    choices=["main", "precision", "serving", "primal", "batched", "cpu_form"],
                     help="main: the JSON line of record; precision: 256^3 FP32-vs-FP64 error study; "
                          "serving: the Poisson serving stream alone (use --serve-seconds 600); "
                          "primal: the 14-point primal stepper's tile sweep alone (with the 1 GiB copy "
                          "probe that calibrates rocprof traffic counters); batched: the 64-instance batched "
-                         "workload alone (with the same probe)")
+                         "workload alone (with the same probe); cpu_form: the CPU form's 32^3 grid on the "
+                         "GPU alone")
     ap.add_argument("--batch-tile", default="",
                     help="batched suite: fixed real-time tile instead of the autotuner's, waves followed by "
                          "c (cached read-modify-write) and/or s (slab order per instance) or g (slab order "
@@ -507,10 +509,12 @@ def cpu_form_leg(steps=1000, warmup=20):
 
 def main():
     args = parse()
-    if args.suite in ("precision", "serving", "primal", "batched"):
+    if args.suite in ("precision", "serving", "primal", "batched", "cpu_form"):
         if args.gpus != 1:
             sys.exit(2)
-        if args.suite == "precision":
+        if args.suite == "cpu_form":
+            print(json.dumps({"suite": "cpu_form", "cpu_form_gpu": cpu_form_leg()}), flush=True)
+        elif args.suite == "precision":
             precision_study(args)
         elif args.suite == "primal":
             import vaikkari_amd as va
