@@ -241,7 +241,8 @@ gpe_status gpe_observables_noalloc(const gpe_config *cfg, const void *field, con
  * (non-temporal) access to the updated field, bit 2 = each XCD owns a
  * y-band of every z layer (grids whose block rows divide by 8; others run
  * in plain order), bit 3 = fused tile of one independent wavefront per cell
- * row (fields below 2 GiB only).
+ * row (fields below 2 GiB only), bit 4 = gather tile of one thread per node
+ * (waves per block = classes per block).
  * GPE_KERNEL_FUSED needs two more field buffers (allocated on selection);
  * imaginary-time steps of a fused solver run the z-march kernel. */
 gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile);

@@ -376,6 +376,124 @@ __global__ void __launch_bounds__(64 * TY)
 }
 
 // ---------------------------------------------------------------------
+// NODE: one thread per node (i, j, k, l), the reference's own granularity
+// (GPUGrossPitaevskii_BCC.cu:176-216), for grids too small to keep the chip
+// busy with one thread per cell (32^3 cells make 512 one-wave gather blocks).
+// A wave holds one class l of 64 >> lsh cell rows; l is wave-uniform, so
+// each of the 12 code paths has its faces and trap offsets at compile time.
+// A block of TY waves holds TY consecutive classes of the same cell rows
+// (TY = 12: whole cells, whose 12 class rows then share the CU's L1).  Every
+// node reads its own value and its 4 neighbours; the arithmetic per node is
+// the gather pass's, so the bits are the same.  Logical waves: 12 x gx x gy x
+// nz with gx = ceil(nx / (1 << lsh)), gy = ceil(ny / (64 >> lsh)), classes
+// fastest; blocks ceil(that / TY).
+// ---------------------------------------------------------------------
+template <typename T, int FORM, int POT, int MODE, int CL, bool NT>
+__device__ __forceinline__ double node_update(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                              const T *__restrict__ pot, const Layout &L,
+                                              const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, T s,
+                                              int i, int j, int k) {
+#pragma clang fp contract(off)
+    using C = cplx_t<T>;
+    const long long base = L.origin + (long long)k * L.sz + (long long)j * L.sy + i;
+    const long long at = base + CL * L.xp;
+    C own = prev[at];
+    if (MODE == MODE_IMAG) own = cscale<T>(own, s);
+    C n[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const Face e = bcc_face(CL, f);
+        if (face_in_cell(e)) n[f] = prev[base + e.m * L.xp];
+        else n[f] = prev[base + e.m * L.xp + e.dx + e.dy * L.sy + e.dz * L.sz];
+        if (MODE == MODE_IMAG) n[f] = cscale<T>(n[f], s);
+    }
+    T pv;
+    if (POT == POT_TRAP) pv = (T)(trap_vt<T, MODE>(trap, i, j, k, CL) * dt);
+    else pv = pot[at];
+    const C sres = H.apply(n, own, pv);
+    if (MODE == MODE_REAL) {
+        C nx;
+        if (NT) {
+            nx.x = __builtin_nontemporal_load(&next[at].x);
+            nx.y = __builtin_nontemporal_load(&next[at].y);
+        } else {
+            nx = next[at];
+        }
+        nx.x = nx.x + sres.y;
+        nx.y = nx.y + (-sres.x);
+        if (NT) {
+            __builtin_nontemporal_store(nx.x, &next[at].x);
+            __builtin_nontemporal_store(nx.y, &next[at].y);
+        } else {
+            next[at] = nx;
+        }
+        return 0.0;
+    } else {
+        C o;
+        o.x = own.x - sres.x;
+        o.y = own.y - sres.y;
+        __builtin_nontemporal_store(o.x, &next[at].x);
+        __builtin_nontemporal_store(o.y, &next[at].y);
+        return (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+    }
+}
+
+// Imaginary time: block partials of sum|out|^2 in logical block order.
+// G.remap 1: XCD x takes the x-th contiguous eighth of the blocks (as
+// block_coords); G.rev walks the blocks backwards.
+template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
+__global__ void __launch_bounds__(64 * TY) k_node(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                                  const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
+                                                  Trap trap, double dt, const double *__restrict__ scale,
+                                                  double *__restrict__ partials, Grid3 G) {
+    __shared__ double lds[TY];
+    const int nb = (int)gridDim.x, lin = (int)blockIdx.x;
+    int b = lin;
+    if (G.remap == 1) {
+        const int q = nb >> 3, r = nb & 7, xcd = lin & 7, slot = lin >> 3;
+        b = (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+    }
+    if (G.rev) b = nb - 1 - b;
+    const int wv = b * TY + (int)threadIdx.y;  // logical wave
+    const int cl = __builtin_amdgcn_readfirstlane(wv % 12);
+    int t = wv / 12;
+    const int bx = t % G.gx;
+    t /= G.gx;
+    const int by = t % G.gy, k = t / G.gy;
+    const int i = (bx << G.lsh) + ((int)threadIdx.x & ((1 << G.lsh) - 1));
+    const int j = (by << (6 - G.lsh)) + ((int)threadIdx.x >> G.lsh);
+    double part = 0.0;
+    if (k < L.nz && i < L.nx && j < L.ny) {
+        T s = (T)1;
+        if (MODE == MODE_IMAG) s = (T)(*scale);
+#define GPE_NODE_CASE(C_)                                                                                 \
+    case C_:                                                                                              \
+        part = node_update<T, FORM, POT, MODE, C_, NT>(next, prev, pot, L, H, trap, dt, s, i, j, k);      \
+        break;
+        switch (cl) {
+            GPE_NODE_CASE(0)
+            GPE_NODE_CASE(1)
+            GPE_NODE_CASE(2)
+            GPE_NODE_CASE(3)
+            GPE_NODE_CASE(4)
+            GPE_NODE_CASE(5)
+            GPE_NODE_CASE(6)
+            GPE_NODE_CASE(7)
+            GPE_NODE_CASE(8)
+            GPE_NODE_CASE(9)
+            GPE_NODE_CASE(10)
+            default: GPE_NODE_CASE(11)
+        }
+#undef GPE_NODE_CASE
+    }
+    if (MODE == MODE_IMAG) {
+        double p[1] = {part};
+        block_sum<1, TY>(p, lds);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partials[b] = p[0];
+    }
+}
+
+// ---------------------------------------------------------------------
 // ZMARCH: one thread per (i,j) column walking CZ layers in z.  The thread
 // keeps its own cell's 12 values, classes {10,11} of the layer below and
 // classes {2,6} of the layer above in registers -- exactly the values the

@@ -68,10 +68,14 @@ constexpr int kTY = 4;  // waves per block of the observables kernel
 struct TileShape {
     int ty, cz, remap, nt;  // nt: streaming (non-temporal) read-modify-write of the updated field
     int row;                // fused family: one independent wave per cell row (k_fused_row)
+    int node;               // gather family: one thread per node (k_node), ty waves = ty classes
 };
+// gather tiles 15-18: one thread per node, 12 or 4 classes per block
 constexpr TileShape kGatherTiles[] = {{4, 1, 0, 1}, {4, 1, 0, 0}, {4, 1, 1, 0}, {1, 1, 1, 0}, {2, 1, 1, 0},
                                       {8, 1, 1, 0}, {4, 1, 1, 1}, {2, 1, 0, 1}, {8, 1, 0, 1}, {1, 1, 0, 1},
-                                      {4, 1, 2, 1}, {2, 1, 2, 1}, {8, 1, 2, 1}, {1, 1, 2, 1}};
+                                      {4, 1, 2, 1}, {2, 1, 2, 1}, {8, 1, 2, 1}, {1, 1, 2, 1},
+                                      {12, 1, 0, 1, 0, 1}, {12, 1, 0, 0, 0, 1}, {4, 1, 0, 1, 0, 1},
+                                      {4, 1, 1, 0, 0, 1}};
 constexpr TileShape kZmarchTiles[] = {{4, 8, 1, 0}, {4, 8, 0, 1}, {1, 8, 1, 0}, {2, 8, 1, 1}, {8, 8, 1, 1},
                                       {4, 4, 1, 1}, {4, 16, 1, 1}, {2, 16, 1, 0}, {1, 16, 1, 1}, {2, 4, 1, 1},
                                       {4, 4, 2, 1}, {8, 8, 2, 1}};
@@ -244,6 +248,13 @@ Grid3 grid_of(int kernel, int tile, const Layout &L) {
     const TileShape t = tile_shape(kernel, tile);
     Grid3 G{};
     G.lsh = row_lanes_log2(L.nx);
+    if (t.node) {  // k_node: gx x gy wave rows per layer, 12 classes each
+        G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
+        G.gy = (L.ny + (64 >> G.lsh) - 1) / (64 >> G.lsh);
+        G.gz = L.nz;
+        G.remap = t.remap == 1 ? 1 : 0;
+        return G;
+    }
     const int lw = 1 << G.lsh, rows = t.ty * (64 >> G.lsh);
     G.gx = (L.nx + lw - 1) / lw;
     G.gy = (L.ny + rows - 1) / rows;
@@ -259,6 +270,21 @@ void launch_gather(cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const L
                    const Grid3 &G, hipStream_t st) {
     hipLaunchKernelGGL((k_gather<T, FORM, POT, MODE, TY, NT>), dim3(G.gx * G.gy * G.gz), dim3(64, TY), 0, st, next,
                        prev, pot, L, H, trap, dt, scale, partials, G);
+}
+
+// Blocks of one launch of (kernel, tile) on grid G.
+long long grid_blocks(const TileShape &t, const Grid3 &G) {
+    const long long cells = (long long)G.gx * G.gy * G.gz;
+    return t.node ? (12 * cells + t.ty - 1) / t.ty : cells;
+}
+
+template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
+void launch_node(cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const Layout &L,
+                 const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, const double *scale, double *partials,
+                 const Grid3 &G, hipStream_t st) {
+    const long long nb = grid_blocks(TileShape{TY, 1, 0, 0, 0, 1}, G);
+    hipLaunchKernelGGL((k_node<T, FORM, POT, MODE, TY, NT>), dim3((unsigned)nb), dim3(64, TY), 0, st, next, prev, pot,
+                       L, H, trap, dt, scale, partials, G);
 }
 
 template <typename T, int FORM, int POT, int MODE, int TY, int CZ, bool NT = false>
@@ -298,6 +324,14 @@ hipError_t launch_update(int kernel, int tile, cplx_t<T> *next, const cplx_t<T> 
             case 8080: launch_zmarch<T, FORM, POT, MODE, 8, 8>(GPE_ARGS); break;
             default: return hipErrorInvalidValue;
         }
+    } else if (t.node) {
+        switch (t.ty * 10 + (MODE == MODE_REAL ? t.nt : 0)) {
+            case 40: launch_node<T, FORM, POT, MODE, 4>(GPE_ARGS); break;
+            case 41: launch_node<T, FORM, POT, MODE, 4, true>(GPE_ARGS); break;
+            case 120: launch_node<T, FORM, POT, MODE, 12>(GPE_ARGS); break;
+            case 121: launch_node<T, FORM, POT, MODE, 12, true>(GPE_ARGS); break;
+            default: return hipErrorInvalidValue;
+        }
     } else {
         switch (t.ty * 10 + (MODE == MODE_REAL ? t.nt : 0)) {
             case 10: launch_gather<T, FORM, POT, MODE, 1>(GPE_ARGS); break;
@@ -316,8 +350,7 @@ hipError_t launch_update(int kernel, int tile, cplx_t<T> *next, const cplx_t<T> 
 }
 
 long long update_blocks(int kernel, int tile, const Layout &L) {
-    const Grid3 G = grid_of(kernel, tile, L);
-    return (long long)G.gx * G.gy * G.gz;
+    return grid_blocks(tile_shape(kernel, tile), grid_of(kernel, tile, L));
 }
 
 long long max_update_blocks(const Layout &L) {
@@ -1012,8 +1045,10 @@ gpe_status gpe_step_observe(gpe_solver *s, uint64_t nsteps, double out[4]) {
     }
     // the last step as two gather passes (any family gives the same bits),
     // the odd one accumulating the observables of the even field it reads
-    const int tile =
+    int tile =
         s->kernel == GPE_KERNEL_GATHER ? s->tile : default_tile(GPE_KERNEL_GATHER, s->cfg.precision, MODE_REAL, s->L);
+    // the observables are summed per cell: a node-parallel tile hands the pass to tile 4
+    if (tile_shape(GPE_KERNEL_GATHER, tile).node) tile = 3;
     const gpe_status st = s->cfg.precision == GPE_FP64 ? s->observe_step<double, FORM_REF>(tile)
                                                        : s->observe_step<float, FORM_DIFF>(tile);
     if (st != GPE_OK) return st;
@@ -1150,7 +1185,7 @@ void gpe_variant_shape(uint32_t kernel, uint32_t tile, int32_t out[3]) {
         const TileShape t = tile_shape((int)kernel, (int)tile - 1);
         out[0] = t.ty;
         out[1] = t.cz;
-        out[2] = (t.remap == 1 ? 1 : 0) | (t.nt << 1) | (t.remap == 2 ? 4 : 0) | (t.row << 3);
+        out[2] = (t.remap == 1 ? 1 : 0) | (t.nt << 1) | (t.remap == 2 ? 4 : 0) | (t.row << 3) | (t.node << 4);
     }
 }
 
