@@ -91,7 +91,8 @@ typedef struct gpe_config {
     uint32_t kernel;       /* gpe_kernel */
     uint32_t tile;         /* tile shape 1..gpe_variant_tiles(kernel), 0 = the family's defaults;
                               gather: by grid, precision and mode (real time with both fields in
-                              256 MiB: tile 4, or 14 on grids of at most 1024 one-wave blocks;
+                              256 MiB: tile 18 FP64, 4 FP32, or 14 on grids of at most 1024
+                              one-wave blocks;
                               else the y-band tile with the most waves whose block rows divide
                               by 8: 13, 11, 12, 14 for 8 (FP64 imaginary time only), 4, 2, 1
                               waves; none: tile 7).  128^3 cells: tile 11, FP64 imaginary

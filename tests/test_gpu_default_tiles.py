@@ -15,9 +15,10 @@ def announce(msg):
 # (dims, precision) -> real-time default tile (1-based)
 CASES = [
     ((32, 32, 32), "fp64", 14),   # both fields cached, 512 one-wave blocks in y-bands
-    ((24, 24, 24), "fp64", 4),    # cached; one-wave block rows (12) do not divide by 8
+    ((24, 24, 24), "fp64", 18),   # cached: one thread per node in FP64 (one-wave rows 12, no y-bands)
+    ((24, 24, 24), "fp32", 4),    # cached, one thread per cell in FP32
     ((48, 48, 48), "fp32", 4),    # cached
-    ((80, 80, 80), "fp64", 4),    # 197 MB of fields: cached
+    ((80, 80, 80), "fp64", 18),   # 197 MB of fields: cached
     ((96, 96, 96), "fp32", 4),    # 170 MB: cached
     ((96, 96, 96), "fp64", 12),   # 340 MB: 4-wave rows 12, 2-wave rows 24
     ((112, 112, 112), "fp32", 7),  # 270 MB; rows 7, 14, 28: no y-bands
@@ -38,10 +39,12 @@ def test_default_gather_tile_by_grid(dims, precision, want):
     assert info["kernel"] == "gather" and info["tile"] == want
 
 
-def test_default_tile_steps_equal_other_tiles():
-    """At 32^3 the default (tile 14) steps the same bits as tile 11, the 128^3 default."""
-    announce("default tile 32^3 bits")
-    dims, h = (32, 32, 32), 0.25
+@pytest.mark.parametrize("n", [32, 40])
+def test_default_tile_steps_equal_other_tiles(n):
+    """The small-grid defaults (tile 14 at 32^3, the node tile 18 at 40^3) step the same bits as
+    tile 11, the 128^3 default."""
+    announce(f"default tile {n}^3 bits")
+    dims, h = (n, n, n), 0.25
     b = h / np.sqrt(8.0)
     origin = va.centered_origin(dims, b)
     psi = va.gaussian(dims, origin, b, 1.5, 1e-3, 5)

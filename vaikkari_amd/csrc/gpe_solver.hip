@@ -108,10 +108,12 @@ int gather_rows(int ty, const Layout &L) {
 // both modes; the rule's choice runs at 99 % of the fastest tile on geometric
 // mean, 92 % at worst):
 //  - real time with both time levels in 256 MiB (the Infinity Cache keeps
-//    them between the two passes): {1 wave, contiguous eighths, cached
-//    read-modify-write} (tile 4), or {1 wave, y-bands, streaming} (tile 14)
-//    when its block rows divide into y-bands and it launches at most 1024
-//    blocks (32^3: 1.5x tile 4);
+//    them between the two passes): a cached read-modify-write in contiguous
+//    eighths, in FP64 one thread per node with 4 classes per block (tile 18:
+//    +4 to +15 % over one thread per cell at 56^3-80^3), in FP32 one thread
+//    per cell, 1 wave per block (tile 4); {1 wave, y-bands, streaming}
+//    (tile 14) instead when its block rows divide into y-bands and it
+//    launches at most 1024 blocks (32^3: 1.5x tile 4);
 //  - otherwise y-bands per XCD with streaming read-modify-write, the most
 //    waves per block (up to 8 in FP64 imaginary time, 4 else: tiles 13, 11,
 //    12, 14 for 8, 4, 2, 1) whose block rows divide by the 8 XCDs; when none
@@ -124,7 +126,8 @@ int default_tile(int kernel, uint32_t precision, int mode, const Layout &L) {
     const long long gx = (L.nx + (1 << row_lanes_log2(L.nx)) - 1) >> row_lanes_log2(L.nx);
     if (mode == MODE_REAL && both <= (256ull << 20)) {
         const int gy1 = gather_rows(1, L);
-        return gy1 % 8 == 0 && gx * gy1 * L.nz <= 1024 ? 13 : 3;
+        if (gy1 % 8 == 0 && gx * gy1 * L.nz <= 1024) return 13;
+        return precision == GPE_FP64 ? 17 : 3;
     }
     const int wmax = precision == GPE_FP64 && mode == MODE_IMAG ? 8 : 4;
     static constexpr int kWaves[4] = {8, 4, 2, 1}, kBandTile[4] = {12, 10, 11, 13};
