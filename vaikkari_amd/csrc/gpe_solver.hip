@@ -103,10 +103,11 @@ int gather_rows(int ty, const Layout &L) {
 // mode and grid.
 // Fused family: the one-wave-per-row tiles measured fastest, {2 waves,
 // 32 layers} in FP64 and {2 waves, 16 layers} in FP32.  Z-march: tile 1.
-// Gather family, a rule fitted to every gather tile timed on MI355X over 24^3
-// to 256^3 cells, 87x87x302, 100x60x140 and 300x300x200 (both precisions,
-// both modes; the rule's choice runs at 99 % of the fastest tile on geometric
-// mean, 92 % at worst):
+// Gather family, a rule fitted to the gather tiles timed on MI355X (tiles
+// 1-14 over 24^3 to 256^3 cells, 87x87x302, 100x60x140 and 300x300x200,
+// tiles 15-18 over 16^3 to 128^3 and 87x87x302; both precisions, both modes;
+// against tiles 1-14 the rule's choice ran at 99 % of the fastest on
+// geometric mean, 92 % at worst):
 //  - real time with both time levels in 256 MiB (the Infinity Cache keeps
 //    them between the two passes): a cached read-modify-write in contiguous
 //    eighths, in FP64 one thread per node with 4 classes per block (tile 18:
