@@ -92,7 +92,8 @@ typedef struct gpe_config {
     uint32_t tile;         /* tile shape 1..gpe_variant_tiles(kernel), 0 = the family's defaults;
                               gather: by grid, precision and mode (real time with both fields in
                               256 MiB: tile 18 FP64, 4 FP32, or 14 on grids of at most 1024
-                              one-wave blocks;
+                              one-wave blocks; imaginary time with both fields in 256 MiB and
+                              at least 1024 four-wave blocks: tile 6 FP64, 3 FP32;
                               else the y-band tile with the most waves whose block rows divide
                               by 8: 13, 11, 12, 14 for 8 (FP64 imaginary time only), 4, 2, 1
                               waves; none: tile 7).  128^3 cells: tile 11, FP64 imaginary
@@ -105,7 +106,7 @@ typedef struct gpe_solver_info {
     uint64_t nodes;          /* 12*nx*ny*nz */
     uint64_t device_bytes;   /* resident device memory of this solver */
     uint64_t steps_done;     /* steps taken since creation */
-    uint32_t kernel;         /* gpe_kernel in use */
+    uint32_t kernel;         /* gpe_kernel that gpe_step runs in the current mode */
     uint32_t tile;           /* its tile shape (1-based) */
     uint32_t launches_per_step;
     double bytes_per_node_step; /* algorithmic HBM bytes per node-step of the hot kernel */
@@ -245,7 +246,8 @@ gpe_status gpe_observables_noalloc(const gpe_config *cfg, const void *field, con
  * row (fields below 2 GiB only), bit 4 = gather tile of one thread per node
  * (waves per block = classes per block).
  * GPE_KERNEL_FUSED needs two more field buffers (allocated on selection);
- * imaginary-time steps of a fused solver run the z-march kernel. */
+ * imaginary-time steps of a fused solver run the gather family's default
+ * tile. */
 gpe_status gpe_set_variant(gpe_solver *s, uint32_t kernel, uint32_t tile);
 int gpe_variant_tiles(uint32_t kernel);
 void gpe_variant_shape(uint32_t kernel, uint32_t tile, int32_t out[3]);

@@ -39,6 +39,32 @@ def test_default_gather_tile_by_grid(dims, precision, want):
     assert info["kernel"] == "gather" and info["tile"] == want
 
 
+# (dims, precision) -> imaginary-time default tile (1-based)
+IMAG_CASES = [
+    ((32, 32, 32), "fp64", 12),    # 256 four-wave blocks: y-bands of 2 waves
+    ((64, 64, 64), "fp64", 6),     # cached, 1024 four-wave blocks: cached stores, 8 waves
+    ((64, 64, 64), "fp32", 3),     # the same, 4 waves
+    ((96, 96, 96), "fp32", 3),     # 170 MB: cached
+    ((96, 96, 96), "fp64", 12),    # 340 MB: y-bands, 8- and 4-wave rows (6, 12) do not divide by 8
+    ((128, 128, 128), "fp64", 13),
+    ((128, 128, 128), "fp32", 11),
+]
+
+
+@pytest.mark.parametrize("dims,precision,want", IMAG_CASES)
+@pytest.mark.parametrize("kernel", ["gather", "fused"])
+def test_default_imaginary_time_tile_by_grid(dims, precision, want, kernel):
+    """gpe_get_info reports what the current mode runs: in imaginary time the gather family's
+    imaginary-time default, for the fused family too."""
+    announce(f"default imag tile {dims} {precision} {kernel}")
+    h = 0.25
+    b = h / np.sqrt(8.0)
+    s = va.Solver(dims, b, 1e-3, 500.0, va.centered_origin(dims, b), precision, "imag", "trap", 1.0, kernel)
+    info = s.info()
+    s.close()
+    assert info["kernel"] == "gather" and info["tile"] == want
+
+
 @pytest.mark.parametrize("n", [32, 40])
 def test_default_tile_steps_equal_other_tiles(n):
     """The small-grid defaults (tile 14 at 32^3, the node tile 18 at 40^3) step the same bits as

@@ -180,11 +180,12 @@ __global__ void __launch_bounds__(64 * kImagTY)
         const double *scale = global_ptr<double>(d.scal) + 1;
         // one layer per tile (the default) or a z-march of zcz layers
         if (d.zcz == 1)
-            part[0] = gather_tile<T, FORM, POT_TRAP, MODE_IMAG, kImagTY>(out, in, nullptr, d.L, d.H, d.trap, d.dt,
-                                                                         scale, bx, by, bz, d.zg.lsh);
+            part[0] = gather_tile<T, FORM, POT_TRAP, MODE_IMAG, kImagTY, true>(out, in, nullptr, d.L, d.H, d.trap,
+                                                                               d.dt, scale, bx, by, bz, d.zg.lsh);
         else
-            part[0] = zmarch_tile<T, FORM, POT_TRAP, MODE_IMAG, kImagTY>(out, in, nullptr, d.L, d.H, d.trap, d.dt,
-                                                                         scale, bx, by, bz, d.zcz, d.zg.lsh);
+            part[0] = zmarch_tile<T, FORM, POT_TRAP, MODE_IMAG, kImagTY, true>(out, in, nullptr, d.L, d.H, d.trap,
+                                                                               d.dt, scale, bx, by, bz, d.zcz,
+                                                                               d.zg.lsh);
         block_sum<1, kImagTY>(part, lds);
         // partials in logical tile order, as the single solver stores them
         if (threadIdx.x == 0 && threadIdx.y == 0)

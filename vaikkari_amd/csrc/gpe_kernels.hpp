@@ -327,9 +327,14 @@ __device__ __forceinline__ double gather_tile(cplx_t<T> *__restrict__ next, cons
                 C o;
                 o.x = own[l].x - sres.x;
                 o.y = own[l].y - sres.y;
-                // written once, read only by the next step: streaming store
-                __builtin_nontemporal_store(o.x, &next[at].x);
-                __builtin_nontemporal_store(o.y, &next[at].y);
+                // written once, read only by the next step: streaming store,
+                // or a cached one where the fields fit the cache (NT false)
+                if (NT) {
+                    __builtin_nontemporal_store(o.x, &next[at].x);
+                    __builtin_nontemporal_store(o.y, &next[at].y);
+                } else {
+                    next[at] = o;
+                }
                 part[0] += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
             }
         }
@@ -432,8 +437,12 @@ __device__ __forceinline__ double node_update(cplx_t<T> *__restrict__ next, cons
         C o;
         o.x = own.x - sres.x;
         o.y = own.y - sres.y;
-        __builtin_nontemporal_store(o.x, &next[at].x);
-        __builtin_nontemporal_store(o.y, &next[at].y);
+        if (NT) {
+            __builtin_nontemporal_store(o.x, &next[at].x);
+            __builtin_nontemporal_store(o.y, &next[at].y);
+        } else {
+            next[at] = o;
+        }
         return (double)o.x * (double)o.x + (double)o.y * (double)o.y;
     }
 }
@@ -586,8 +595,13 @@ __device__ __forceinline__ double zmarch_tile(cplx_t<T> *__restrict__ next, cons
                     o.x = cur[l].x - sres.x;
                     o.y = cur[l].y - sres.y;
                     // written once, read only by the next step: streaming store
-                    __builtin_nontemporal_store(o.x, &next[at].x);
-                    __builtin_nontemporal_store(o.y, &next[at].y);
+                    // (NT), or a cached one
+                    if (NT) {
+                        __builtin_nontemporal_store(o.x, &next[at].x);
+                        __builtin_nontemporal_store(o.y, &next[at].y);
+                    } else {
+                        next[at] = o;
+                    }
                     part[0] += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
                 }
             }

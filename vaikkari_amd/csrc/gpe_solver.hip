@@ -115,6 +115,11 @@ int gather_rows(int ty, const Layout &L) {
 //    per cell, 1 wave per block (tile 4); {1 wave, y-bands, streaming}
 //    (tile 14) instead when its block rows divide into y-bands and it
 //    launches at most 1024 blocks (32^3: 1.5x tile 4);
+//  - imaginary time with both time levels in 256 MiB and at least 1024
+//    blocks of 4 waves: cached stores in contiguous eighths, 8 waves per
+//    block in FP64 (tile 6), 4 in FP32 (tile 3): the Euler step's output is
+//    the next step's input, and a streaming store sends it to HBM (64^3-96^3:
+//    +18 to +27 %);
 //  - otherwise y-bands per XCD with streaming read-modify-write, the most
 //    waves per block (up to 8 in FP64 imaginary time, 4 else: tiles 13, 11,
 //    12, 14 for 8, 4, 2, 1) whose block rows divide by the 8 XCDs; when none
@@ -130,6 +135,8 @@ int default_tile(int kernel, uint32_t precision, int mode, const Layout &L) {
         if (gy1 % 8 == 0 && gx * gy1 * L.nz <= 1024) return 13;
         return precision == GPE_FP64 ? 17 : 3;
     }
+    if (mode == MODE_IMAG && both <= (256ull << 20) && gx * gather_rows(4, L) * L.nz >= 1024)
+        return precision == GPE_FP64 ? 5 : 2;
     const int wmax = precision == GPE_FP64 && mode == MODE_IMAG ? 8 : 4;
     static constexpr int kWaves[4] = {8, 4, 2, 1}, kBandTile[4] = {12, 10, 11, 13};
     for (int i = 0; i < 4; i++)
@@ -308,8 +315,7 @@ hipError_t launch_update(int kernel, int tile, cplx_t<T> *next, const cplx_t<T> 
     const TileShape t = tile_shape(kernel, tile);
 #define GPE_ARGS next, prev, pot, L, H, trap, dt, scale, partials, G, st
     if (kernel == GPE_KERNEL_ZMARCH) {
-        const bool nt = MODE == MODE_REAL && t.nt;
-        switch (t.ty * 1000 + t.cz * 10 + (nt ? 1 : 0)) {
+        switch (t.ty * 1000 + t.cz * 10 + (t.nt ? 1 : 0)) {
             case 1080: launch_zmarch<T, FORM, POT, MODE, 1, 8>(GPE_ARGS); break;
             case 1161: launch_zmarch<T, FORM, POT, MODE, 1, 16, true>(GPE_ARGS); break;
             case 1160: launch_zmarch<T, FORM, POT, MODE, 1, 16>(GPE_ARGS); break;
@@ -329,7 +335,7 @@ hipError_t launch_update(int kernel, int tile, cplx_t<T> *next, const cplx_t<T> 
             default: return hipErrorInvalidValue;
         }
     } else if (t.node) {
-        switch (t.ty * 10 + (MODE == MODE_REAL ? t.nt : 0)) {
+        switch (t.ty * 10 + t.nt) {
             case 40: launch_node<T, FORM, POT, MODE, 4>(GPE_ARGS); break;
             case 41: launch_node<T, FORM, POT, MODE, 4, true>(GPE_ARGS); break;
             case 120: launch_node<T, FORM, POT, MODE, 12>(GPE_ARGS); break;
@@ -337,7 +343,7 @@ hipError_t launch_update(int kernel, int tile, cplx_t<T> *next, const cplx_t<T> 
             default: return hipErrorInvalidValue;
         }
     } else {
-        switch (t.ty * 10 + (MODE == MODE_REAL ? t.nt : 0)) {
+        switch (t.ty * 10 + t.nt) {
             case 10: launch_gather<T, FORM, POT, MODE, 1>(GPE_ARGS); break;
             case 20: launch_gather<T, FORM, POT, MODE, 2>(GPE_ARGS); break;
             case 40: launch_gather<T, FORM, POT, MODE, 4>(GPE_ARGS); break;
@@ -588,6 +594,15 @@ gpe_status gpe_solver::upload_potential() {
     return GPE_OK;
 }
 
+// The family and tile of the imaginary-time Euler step: the configured ones,
+// the family's imaginary-time default for tile 0, and the gather family's
+// default for the fused family (whose pass is a leapfrog step).
+void gpe_solver::imag_variant(int &k, int &t) const {
+    const bool fused = kernel == GPE_KERNEL_FUSED;
+    k = fused ? GPE_KERNEL_GATHER : kernel;
+    t = fused || tile_default ? default_tile(k, cfg.precision, MODE_IMAG, L) : tile;
+}
+
 template <typename T, int FORM>
 gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
     using C = cplx_t<T>;
@@ -632,11 +647,9 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
     } else {
         // Euler step into the other buffer; renormalisation folded into the
         // next read through scal[1] = 1/sqrt(N).  (The fused pass is a
-        // leapfrog step; imaginary time runs the single-update z-march.)
-        const int kernel = this->kernel == GPE_KERNEL_FUSED ? GPE_KERNEL_ZMARCH : this->kernel;
-        const int tile = this->kernel == GPE_KERNEL_FUSED ? 0
-                         : tile_default                  ? default_tile(kernel, cfg.precision, MODE_IMAG, L)
-                                                         : this->tile;
+        // leapfrog step; imaginary time runs the gather family's default.)
+        int kernel, tile;
+        imag_variant(kernel, tile);
         for (uint64_t s = 0; s < nsteps && e == hipSuccess; s++) {
             C *in = (C *)field[ie], *out = (C *)field[io];
             const bool rev = (steps_done + s) & 1;  // alternate directions: start where the last step ended
@@ -1201,8 +1214,10 @@ gpe_status gpe_get_info(const gpe_solver *s, gpe_solver_info *info) {
     info->nodes = s->nodes();
     info->device_bytes = s->device_bytes();
     info->steps_done = s->steps_done;
-    info->kernel = (uint32_t)s->kernel;
-    info->tile = (uint32_t)s->tile + 1;
+    int k = s->kernel, t = s->tile;
+    if (s->cfg.mode == GPE_IMAG_TIME) s->imag_variant(k, t);
+    info->kernel = (uint32_t)k;
+    info->tile = (uint32_t)t + 1;
     info->launches_per_step = (s->cfg.mode == GPE_REAL_TIME && s->kernel != GPE_KERNEL_FUSED) ? 2 : 1;
     const double S = (double)s->elem_bytes();
     const double P = s->cfg.potential == GPE_POT_ARRAY ? S / 2 : 0.0;
