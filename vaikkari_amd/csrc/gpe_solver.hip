@@ -66,7 +66,8 @@ constexpr int kTY = 4;  // waves per block of the observables kernel
 // block, remap = block order over the XCDs (see block_coords: 0 plain,
 // 1 contiguous eighths, 2 y-bands).
 struct TileShape {
-    int ty, cz, remap, nt;  // nt: streaming (non-temporal) read-modify-write of the updated field
+    int ty, cz, remap, nt;  // nt: streaming (non-temporal) access to the updated field (real time: its
+                            // read-modify-write; imaginary time: its store)
     int row;                // fused family: one independent wave per cell row (k_fused_row)
     int node;               // gather family: one thread per node (k_node), ty waves = ty classes
 };
