@@ -17,9 +17,10 @@ are reported beside it.
 
 Inputs are resident in HBM before the timed region; timing uses HIP events
 on the solver's stream around exactly --steps steps.  Before them run the
---warmup steps and, when those took under 50 ms of device time, more untimed
-steps up to 50 ms (warmup_extra_steps): the GPU idles during the host-side
-imprint and upload, and its first milliseconds back run below the steady rate.
+--warmup steps and, when those took under 50 ms of device time on the FP64
+solver, more untimed steps up to 50 ms, as many for FP32 (warmup_extra_steps):
+the GPU idles during the host-side imprint and upload, and its first
+milliseconds back run below the steady rate.
 
 Beside the line of record (all untimed or timed separately): fp32_check, FP32
 against FP64 over a fixed --check-steps from the phase-2 state, psi relative L2
@@ -573,6 +574,7 @@ def main():
     mu = obs["mu"]
 
     results = {}
+    extra = 0
     for prec in ("fp64", "fp32"):
         if prec == "fp64":
             s = s64
@@ -581,14 +583,16 @@ def main():
             s = va.Solver(dims, b, dt, g, origin, "fp32", "real", "trap", ratio, args.kernel)
         s.set_state(psi)
         s.init_odd(mu)
-        extra = 0
         if args.warmup:
             # the GPU idled during the host-side imprint and upload; after the W warm-up steps,
-            # more untimed steps until 50 ms of device time have run, so that the timed steps see
-            # the steady rate however short W is (reported as warmup_extra_steps)
+            # more untimed steps until 50 ms of device time have run on the FP64 solver, so that
+            # the timed steps see the steady rate however short W is (reported as
+            # warmup_extra_steps).  The FP32 run takes as many, so that both runs end at the same
+            # step and the psi comparison below compares the same time.
             ms_w, _ = s.time_steps(args.warmup)
-            if 0 < ms_w < 50.0:
-                extra = int(math.ceil((50.0 - ms_w) / (ms_w / args.warmup)))
+            if prec == "fp64":
+                extra = int(math.ceil((50.0 - ms_w) / (ms_w / args.warmup))) if 0 < ms_w < 50.0 else 0
+            if extra:
                 s.step(extra)
         s.synchronize()
         t0 = time.perf_counter()
@@ -702,8 +706,9 @@ def main():
         "cpu_baseline": cpu,
         "fp32_bulk": {
             "value": r32["rate"], "ms_per_step": r32["ms_per_step"], "ratio_vs_fp64": r32["rate"] / r64["rate"],
-            "psi_rel_l2_vs_fp64": err, "psi_rel_l2_note": f"after the {args.warmup + args.steps} steps of the "
-                                                             "timed runs; fixed-length check in fp32_check",
+            "psi_rel_l2_vs_fp64": err,
+            "psi_rel_l2_note": f"after the {args.warmup + extra + args.steps} steps (warm-up included) of the "
+                               "timed runs; fixed-length check in fp32_check",
             "roofline": r32["roofline"], "kernel": r32["kernel"],
         },
         "fp32_check": check,
