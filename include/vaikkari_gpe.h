@@ -91,9 +91,9 @@ typedef struct gpe_config {
     uint32_t kernel;       /* gpe_kernel */
     uint32_t tile;         /* tile shape 1..gpe_variant_tiles(kernel), 0 = the family's defaults;
                               gather: by grid, precision and mode (real time with both fields in
-                              256 MiB: tile 18 FP64, 4 FP32, or 14 on grids of at most 1024
-                              one-wave blocks; imaginary time with both fields in 256 MiB and
-                              at least 1024 four-wave blocks: tile 6 FP64, 3 FP32;
+                              256 MiB: tile 18 FP64, 19 FP32, or 14 on grids of at most 1024
+                              one-wave blocks; imaginary time with both fields in 256 MiB:
+                              tile 20 FP64 from 256 four-wave blocks, 3 FP32 from 1024;
                               else the y-band tile with the most waves whose block rows divide
                               by 8: 13, 11, 12, 14 for 8 (FP64 imaginary time only), 4, 2, 1
                               waves; none: tile 7).  128^3 cells: tile 11, FP64 imaginary
@@ -243,8 +243,9 @@ gpe_status gpe_observables_noalloc(const gpe_config *cfg, const void *field, con
  * (non-temporal) access to the updated field, bit 2 = each XCD owns a
  * y-band of every z layer (grids whose block rows divide by 8; others run
  * in plain order), bit 3 = fused tile of one independent wavefront per cell
- * row (fields below 2 GiB only), bit 4 = gather tile of one thread per node
- * (waves per block = classes per block).
+ * row (fields below 2 GiB only), bits 4-5 = gather tile of one thread per
+ * node (1; waves per block = classes per block) or per cell and group of 4
+ * classes (2).
  * GPE_KERNEL_FUSED needs two more field buffers (allocated on selection);
  * imaginary-time steps of a fused solver run the gather family's default
  * tile. */

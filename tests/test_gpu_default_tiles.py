@@ -16,10 +16,10 @@ def announce(msg):
 CASES = [
     ((32, 32, 32), "fp64", 14),   # both fields cached, 512 one-wave blocks in y-bands
     ((24, 24, 24), "fp64", 18),   # cached: one thread per node in FP64 (one-wave rows 12, no y-bands)
-    ((24, 24, 24), "fp32", 4),    # cached, one thread per cell in FP32
-    ((48, 48, 48), "fp32", 4),    # cached
+    ((24, 24, 24), "fp32", 19),   # cached: one thread per group of 4 classes in FP32
+    ((48, 48, 48), "fp32", 19),   # cached
     ((80, 80, 80), "fp64", 18),   # 197 MB of fields: cached
-    ((96, 96, 96), "fp32", 4),    # 170 MB: cached
+    ((96, 96, 96), "fp32", 19),   # 170 MB: cached
     ((96, 96, 96), "fp64", 12),   # 340 MB: 4-wave rows 12, 2-wave rows 24
     ((112, 112, 112), "fp32", 7),  # 270 MB; rows 7, 14, 28: no y-bands
     ((128, 128, 128), "fp64", 11),
@@ -41,9 +41,11 @@ def test_default_gather_tile_by_grid(dims, precision, want):
 
 # (dims, precision) -> imaginary-time default tile (1-based)
 IMAG_CASES = [
-    ((32, 32, 32), "fp64", 12),    # 256 four-wave blocks: y-bands of 2 waves
-    ((64, 64, 64), "fp64", 6),     # cached, 1024 four-wave blocks: cached stores, 8 waves
-    ((64, 64, 64), "fp32", 3),     # the same, 4 waves
+    ((32, 32, 32), "fp64", 12),    # 128 four-wave blocks: y-bands of 2 waves
+    ((48, 48, 48), "fp64", 20),    # cached, 432 four-wave blocks: cached stores, groups of 4 classes
+    ((48, 48, 48), "fp32", 7),     # FP32 needs 1024 four-wave blocks for cached stores; no y-bands
+    ((64, 64, 64), "fp64", 20),    # cached, 1024 four-wave blocks
+    ((64, 64, 64), "fp32", 3),     # cached stores, 4 waves
     ((96, 96, 96), "fp32", 3),     # 170 MB: cached
     ((96, 96, 96), "fp64", 12),    # 340 MB: y-bands, 8- and 4-wave rows (6, 12) do not divide by 8
     ((128, 128, 128), "fp64", 13),
@@ -65,18 +67,18 @@ def test_default_imaginary_time_tile_by_grid(dims, precision, want, kernel):
     assert info["kernel"] == "gather" and info["tile"] == want
 
 
-@pytest.mark.parametrize("n", [32, 40])
-def test_default_tile_steps_equal_other_tiles(n):
-    """The small-grid defaults (tile 14 at 32^3, the node tile 18 at 40^3) step the same bits as
-    tile 11, the 128^3 default."""
-    announce(f"default tile {n}^3 bits")
+@pytest.mark.parametrize("n,precision", [(32, "fp64"), (40, "fp64"), (40, "fp32")])
+def test_default_tile_steps_equal_other_tiles(n, precision):
+    """The small-grid defaults (tile 14 at 32^3, the node tile 18 at 40^3 FP64, the class-group
+    tile 19 at 40^3 FP32) step the same bits as tile 11, the 128^3 default."""
+    announce(f"default tile {n}^3 {precision} bits")
     dims, h = (n, n, n), 0.25
     b = h / np.sqrt(8.0)
     origin = va.centered_origin(dims, b)
     psi = va.gaussian(dims, origin, b, 1.5, 1e-3, 5)
     out = []
     for tile in (0, 11):
-        s = va.Solver(dims, b, 1e-3, 500.0, origin, "fp64", "real", "trap", 1.0, "gather", tile)
+        s = va.Solver(dims, b, 1e-3, 500.0, origin, precision, "real", "trap", 1.0, "gather", tile)
         s.set_state(psi)
         s.init_odd(10.0)
         s.step(40)

@@ -503,6 +503,131 @@ __global__ void __launch_bounds__(64 * TY) k_node(cplx_t<T> *__restrict__ next, 
 }
 
 // ---------------------------------------------------------------------
+// QUAD: one thread per cell and group of 4 classes, between the gather pass
+// (12 classes per thread) and the node pass (1): three times the waves of the
+// gather pass for about the same loads.  The groups {0,9,10,11}, {1,2,3,4},
+// {5,6,7,8} keep most in-cell neighbours inside the thread's own registers;
+// the others are loads (L1 hits).  A wave holds one group (wave-uniform) of
+// 64 >> lsh cell rows; a block of TY waves holds TY consecutive (group, rows)
+// pairs, groups fastest.  Per node the arithmetic is the gather pass's.
+// Logical waves 3 x gx x gy x nz, blocks ceil(that / TY).
+// ---------------------------------------------------------------------
+__host__ __device__ constexpr int quad_class(int grp, int q) {
+    return grp == 0 ? (q == 0 ? 0 : 8 + q) : 4 * grp - 3 + q;
+}
+__host__ __device__ constexpr int quad_slot(int grp, int m) {
+    for (int q = 0; q < 4; q++)
+        if (quad_class(grp, q) == m) return q;
+    return -1;
+}
+
+template <typename T, int FORM, int POT, int MODE, int GRP, bool NT>
+__device__ __forceinline__ double quad_update(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                              const T *__restrict__ pot, const Layout &L,
+                                              const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, T s,
+                                              int i, int j, int k) {
+#pragma clang fp contract(off)
+    using C = cplx_t<T>;
+    const long long base = L.origin + (long long)k * L.sz + (long long)j * L.sy + i;
+    C own[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        own[q] = prev[base + quad_class(GRP, q) * L.xp];
+        if (MODE == MODE_IMAG) own[q] = cscale<T>(own[q], s);
+    }
+    double part = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int l = quad_class(GRP, q);
+        C n[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            const Face e = bcc_face(l, f);
+            if (face_in_cell(e) && quad_slot(GRP, e.m) >= 0) {
+                n[f] = own[quad_slot(GRP, e.m)];
+            } else {
+                n[f] = face_in_cell(e) ? prev[base + e.m * L.xp]
+                                       : prev[base + e.m * L.xp + e.dx + e.dy * L.sy + e.dz * L.sz];
+                if (MODE == MODE_IMAG) n[f] = cscale<T>(n[f], s);
+            }
+        }
+        const long long at = base + l * L.xp;
+        T pv;
+        if (POT == POT_TRAP) pv = (T)(trap_vt<T, MODE>(trap, i, j, k, l) * dt);
+        else pv = pot[at];
+        const C sres = H.apply(n, own[q], pv);
+        if (MODE == MODE_REAL) {
+            C nx;
+            if (NT) {
+                nx.x = __builtin_nontemporal_load(&next[at].x);
+                nx.y = __builtin_nontemporal_load(&next[at].y);
+            } else {
+                nx = next[at];
+            }
+            nx.x = nx.x + sres.y;
+            nx.y = nx.y + (-sres.x);
+            if (NT) {
+                __builtin_nontemporal_store(nx.x, &next[at].x);
+                __builtin_nontemporal_store(nx.y, &next[at].y);
+            } else {
+                next[at] = nx;
+            }
+        } else {
+            C o;
+            o.x = own[q].x - sres.x;
+            o.y = own[q].y - sres.y;
+            if (NT) {
+                __builtin_nontemporal_store(o.x, &next[at].x);
+                __builtin_nontemporal_store(o.y, &next[at].y);
+            } else {
+                next[at] = o;
+            }
+            part += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+        }
+    }
+    return part;
+}
+
+// Imaginary time: block partials of sum|out|^2 in logical block order.
+// G.remap 1: XCD x takes the x-th contiguous eighth of the blocks; G.rev
+// walks the blocks backwards.
+template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
+__global__ void __launch_bounds__(64 * TY) k_quad(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
+                                                  const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
+                                                  Trap trap, double dt, const double *__restrict__ scale,
+                                                  double *__restrict__ partials, Grid3 G) {
+    __shared__ double lds[TY];
+    const int nb = (int)gridDim.x, lin = (int)blockIdx.x;
+    int b = lin;
+    if (G.remap == 1) {
+        const int q = nb >> 3, r = nb & 7, xcd = lin & 7, slot = lin >> 3;
+        b = (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+    }
+    if (G.rev) b = nb - 1 - b;
+    const int wv = b * TY + (int)threadIdx.y;  // logical wave
+    const int grp = __builtin_amdgcn_readfirstlane(wv % 3);
+    int t = wv / 3;
+    const int bx = t % G.gx;
+    t /= G.gx;
+    const int by = t % G.gy, k = t / G.gy;
+    const int i = (bx << G.lsh) + ((int)threadIdx.x & ((1 << G.lsh) - 1));
+    const int j = (by << (6 - G.lsh)) + ((int)threadIdx.x >> G.lsh);
+    double part = 0.0;
+    if (k < L.nz && i < L.nx && j < L.ny) {
+        T s = (T)1;
+        if (MODE == MODE_IMAG) s = (T)(*scale);
+        if (grp == 0) part = quad_update<T, FORM, POT, MODE, 0, NT>(next, prev, pot, L, H, trap, dt, s, i, j, k);
+        else if (grp == 1) part = quad_update<T, FORM, POT, MODE, 1, NT>(next, prev, pot, L, H, trap, dt, s, i, j, k);
+        else part = quad_update<T, FORM, POT, MODE, 2, NT>(next, prev, pot, L, H, trap, dt, s, i, j, k);
+    }
+    if (MODE == MODE_IMAG) {
+        double p[1] = {part};
+        block_sum<1, TY>(p, lds);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partials[b] = p[0];
+    }
+}
+
+// ---------------------------------------------------------------------
 // ZMARCH: one thread per (i,j) column walking CZ layers in z.  The thread
 // keeps its own cell's 12 values, classes {10,11} of the layer below and
 // classes {2,6} of the layer above in registers -- exactly the values the

@@ -69,14 +69,17 @@ struct TileShape {
     int ty, cz, remap, nt;  // nt: streaming (non-temporal) access to the updated field (real time: its
                             // read-modify-write; imaginary time: its store)
     int row;                // fused family: one independent wave per cell row (k_fused_row)
-    int node;               // gather family: one thread per node (k_node), ty waves = ty classes
+    int node;               // gather family: 1 one thread per node (k_node, ty waves = ty classes),
+                            // 2 one thread per cell and group of 4 classes (k_quad, ty (group, rows) waves)
 };
-// gather tiles 15-18: one thread per node, 12 or 4 classes per block
+// gather tiles 15-18: one thread per node, 12 or 4 classes per block; 19-22:
+// one thread per group of 4 classes
 constexpr TileShape kGatherTiles[] = {{4, 1, 0, 1}, {4, 1, 0, 0}, {4, 1, 1, 0}, {1, 1, 1, 0}, {2, 1, 1, 0},
                                       {8, 1, 1, 0}, {4, 1, 1, 1}, {2, 1, 0, 1}, {8, 1, 0, 1}, {1, 1, 0, 1},
                                       {4, 1, 2, 1}, {2, 1, 2, 1}, {8, 1, 2, 1}, {1, 1, 2, 1},
                                       {12, 1, 0, 1, 0, 1}, {12, 1, 0, 0, 0, 1}, {4, 1, 0, 1, 0, 1},
-                                      {4, 1, 1, 0, 0, 1}};
+                                      {4, 1, 1, 0, 0, 1}, {3, 1, 1, 0, 0, 2}, {6, 1, 1, 0, 0, 2},
+                                      {12, 1, 1, 0, 0, 2}, {6, 1, 0, 1, 0, 2}};
 constexpr TileShape kZmarchTiles[] = {{4, 8, 1, 0}, {4, 8, 0, 1}, {1, 8, 1, 0}, {2, 8, 1, 1}, {8, 8, 1, 1},
                                       {4, 4, 1, 1}, {4, 16, 1, 1}, {2, 16, 1, 0}, {1, 16, 1, 1}, {2, 4, 1, 1},
                                       {4, 4, 2, 1}, {8, 8, 2, 1}};
@@ -113,14 +116,15 @@ int gather_rows(int ty, const Layout &L) {
 //    them between the two passes): a cached read-modify-write in contiguous
 //    eighths, in FP64 one thread per node with 4 classes per block (tile 18:
 //    +4 to +15 % over one thread per cell at 56^3-80^3), in FP32 one thread
-//    per cell, 1 wave per block (tile 4); {1 wave, y-bands, streaming}
-//    (tile 14) instead when its block rows divide into y-bands and it
-//    launches at most 1024 blocks (32^3: 1.5x tile 4);
-//  - imaginary time with both time levels in 256 MiB and at least 1024
-//    blocks of 4 waves: cached stores in contiguous eighths, 8 waves per
-//    block in FP64 (tile 6), 4 in FP32 (tile 3): the Euler step's output is
-//    the next step's input, and a streaming store sends it to HBM (64^3-96^3:
-//    +18 to +27 %);
+//    per cell and group of 4 classes, 3 waves per block (tile 19: +2 to +14 %
+//    at 24^3-96^3); {1 wave, y-bands, streaming} (tile 14) instead when its
+//    block rows divide into y-bands and it launches at most 1024 blocks
+//    (32^3: 1.5x tile 4);
+//  - imaginary time with both time levels in 256 MiB: cached stores in
+//    contiguous eighths, since the Euler step's output is the next step's
+//    input and a streaming store sends it to HBM; FP64 from 256 blocks of 4
+//    waves, one thread per group of 4 classes, 6 waves per block (tile 20);
+//    FP32 from 1024, 4 waves per block (tile 3) (48^3-96^3: +5 to +27 %);
 //  - otherwise y-bands per XCD with streaming read-modify-write, the most
 //    waves per block (up to 8 in FP64 imaginary time, 4 else: tiles 13, 11,
 //    12, 14 for 8, 4, 2, 1) whose block rows divide by the 8 XCDs; when none
@@ -134,10 +138,13 @@ int default_tile(int kernel, uint32_t precision, int mode, const Layout &L) {
     if (mode == MODE_REAL && both <= (256ull << 20)) {
         const int gy1 = gather_rows(1, L);
         if (gy1 % 8 == 0 && gx * gy1 * L.nz <= 1024) return 13;
-        return precision == GPE_FP64 ? 17 : 3;
+        return precision == GPE_FP64 ? 17 : 18;
     }
-    if (mode == MODE_IMAG && both <= (256ull << 20) && gx * gather_rows(4, L) * L.nz >= 1024)
-        return precision == GPE_FP64 ? 5 : 2;
+    if (mode == MODE_IMAG && both <= (256ull << 20)) {
+        const long long b4 = gx * gather_rows(4, L) * L.nz;
+        if (precision == GPE_FP64 && b4 >= 256) return 19;
+        if (precision != GPE_FP64 && b4 >= 1024) return 2;
+    }
     const int wmax = precision == GPE_FP64 && mode == MODE_IMAG ? 8 : 4;
     static constexpr int kWaves[4] = {8, 4, 2, 1}, kBandTile[4] = {12, 10, 11, 13};
     for (int i = 0; i < 4; i++)
@@ -260,7 +267,7 @@ Grid3 grid_of(int kernel, int tile, const Layout &L) {
     const TileShape t = tile_shape(kernel, tile);
     Grid3 G{};
     G.lsh = row_lanes_log2(L.nx);
-    if (t.node) {  // k_node: gx x gy wave rows per layer, 12 classes each
+    if (t.node) {  // k_node, k_quad: gx x gy wave rows per layer, 12 classes or 3 groups each
         G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
         G.gy = (L.ny + (64 >> G.lsh) - 1) / (64 >> G.lsh);
         G.gz = L.nz;
@@ -287,7 +294,8 @@ void launch_gather(cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const L
 // Blocks of one launch of (kernel, tile) on grid G.
 long long grid_blocks(const TileShape &t, const Grid3 &G) {
     const long long cells = (long long)G.gx * G.gy * G.gz;
-    return t.node ? (12 * cells + t.ty - 1) / t.ty : cells;
+    const long long per = t.node == 1 ? 12 : t.node == 2 ? 3 : 0;  // waves per wave row of cells
+    return t.node ? (per * cells + t.ty - 1) / t.ty : cells;
 }
 
 template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
@@ -296,6 +304,15 @@ void launch_node(cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const Lay
                  const Grid3 &G, hipStream_t st) {
     const long long nb = grid_blocks(TileShape{TY, 1, 0, 0, 0, 1}, G);
     hipLaunchKernelGGL((k_node<T, FORM, POT, MODE, TY, NT>), dim3((unsigned)nb), dim3(64, TY), 0, st, next, prev, pot,
+                       L, H, trap, dt, scale, partials, G);
+}
+
+template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
+void launch_quad(cplx_t<T> *next, const cplx_t<T> *prev, const T *pot, const Layout &L,
+                 const Hamiltonian<T, FORM> &H, const Trap &trap, double dt, const double *scale, double *partials,
+                 const Grid3 &G, hipStream_t st) {
+    const long long nb = grid_blocks(TileShape{TY, 1, 0, 0, 0, 2}, G);
+    hipLaunchKernelGGL((k_quad<T, FORM, POT, MODE, TY, NT>), dim3((unsigned)nb), dim3(64, TY), 0, st, next, prev, pot,
                        L, H, trap, dt, scale, partials, G);
 }
 
@@ -333,6 +350,14 @@ hipError_t launch_update(int kernel, int tile, cplx_t<T> *next, const cplx_t<T> 
             case 4160: launch_zmarch<T, FORM, POT, MODE, 4, 16>(GPE_ARGS); break;
             case 8081: launch_zmarch<T, FORM, POT, MODE, 8, 8, true>(GPE_ARGS); break;
             case 8080: launch_zmarch<T, FORM, POT, MODE, 8, 8>(GPE_ARGS); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else if (t.node == 2) {
+        switch (t.ty * 10 + t.nt) {
+            case 30: launch_quad<T, FORM, POT, MODE, 3>(GPE_ARGS); break;
+            case 60: launch_quad<T, FORM, POT, MODE, 6>(GPE_ARGS); break;
+            case 61: launch_quad<T, FORM, POT, MODE, 6, true>(GPE_ARGS); break;
+            case 120: launch_quad<T, FORM, POT, MODE, 12>(GPE_ARGS); break;
             default: return hipErrorInvalidValue;
         }
     } else if (t.node) {
@@ -1203,7 +1228,7 @@ void gpe_variant_shape(uint32_t kernel, uint32_t tile, int32_t out[3]) {
         const TileShape t = tile_shape((int)kernel, (int)tile - 1);
         out[0] = t.ty;
         out[1] = t.cz;
-        out[2] = (t.remap == 1 ? 1 : 0) | (t.nt << 1) | (t.remap == 2 ? 4 : 0) | (t.row << 3) | (t.node << 4);
+        out[2] = (t.remap == 1 ? 1 : 0) | (t.nt << 1) | (t.remap == 2 ? 4 : 0) | (t.row << 3) | (t.node << 4);  // bits 4-5
     }
 }
 

@@ -278,7 +278,7 @@ def variants():
             lib.gpe_variant_shape(k, t, sh)
             out.append({"kernel": name, "tile": t, "waves": sh[0], "cz": sh[1],
                         "remap": 2 if sh[2] & 4 else sh[2] & 1, "nt": (sh[2] >> 1) & 1,
-                        "row": (sh[2] >> 3) & 1, "node": (sh[2] >> 4) & 1})
+                        "row": (sh[2] >> 3) & 1, "node": (sh[2] >> 4) & 3})
     return out
 
 
