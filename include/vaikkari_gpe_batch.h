@@ -101,11 +101,16 @@ uint64_t gpe_batch_chunk_node_steps(const gpe_batch *b, uint32_t chunk_steps);
  * layer order; small instances then keep their neighbour layers in one L2)
  * (without either y-bands of every layer when an instance's block rows
  * divide by 8, else plain order); only while the batch is empty.  A new
- * batch starts with 2 | GPE_BATCH_TILE_SLABS.  Results do not depend on it.
- * The autotuner picks it. */
+ * batch starts with 3 | GPE_BATCH_TILE_QUAD | GPE_BATCH_TILE_CACHED |
+ * GPE_BATCH_TILE_GROUP_SLABS.  Results do not depend on it.  The autotuner
+ * picks it. */
 #define GPE_BATCH_TILE_CACHED 0x100u
 #define GPE_BATCH_TILE_SLABS 0x200u
 #define GPE_BATCH_TILE_GROUP_SLABS 0x400u
+/* | GPE_BATCH_TILE_QUAD: one thread per cell and group of 4 classes (three
+ * times the waves of one thread per cell, for about the same loads), waves
+ * per block 3, 6 or 12; the single solver's gather tiles 19-22.  Same bits. */
+#define GPE_BATCH_TILE_QUAD 0x800u
 gpe_status gpe_batch_set_tile(gpe_batch *b, uint32_t tile);
 /* How gpe_batch_run issues a chunk: GPE_BATCH_GRAPH replays a captured
  * hipGraph of the chunk (the default), GPE_BATCH_STREAM enqueues the chunk's
@@ -118,7 +123,7 @@ gpe_status gpe_batch_set_launch_mode(gpe_batch *b, uint32_t mode);
  * the whole chunk before the next group starts, so that much of a group's
  * data is still in the GPU's last-level cache (256 MB Infinity Cache) when
  * the next half-update reads it.  0 puts every instance in one group (each
- * half-update sweeps the whole batch).  Default 640 MiB; may change at any
+ * half-update sweeps the whole batch).  Default 256 MiB; may change at any
  * time (takes effect at the next gpe_batch_run).  Results do not depend on
  * it: instances are independent. */
 gpe_status gpe_batch_set_group_bytes(gpe_batch *b, uint64_t bytes);

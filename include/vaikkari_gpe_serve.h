@@ -118,7 +118,8 @@ double gpe_server_oldest_queued(const gpe_server *s);
  * one), as gpe_batch_set_launch_mode.  Results are identical either way. */
 gpe_status gpe_server_set_launch_mode(gpe_server *s, uint32_t mode);
 /* Real-time tile of the shared launches, as gpe_batch_set_tile (waves |
- * GPE_BATCH_TILE_CACHED | GPE_BATCH_TILE_SLABS or GPE_BATCH_TILE_GROUP_SLABS);
+ * GPE_BATCH_TILE_QUAD | GPE_BATCH_TILE_CACHED | GPE_BATCH_TILE_SLABS or
+ * GPE_BATCH_TILE_GROUP_SLABS; the batch's default unless set);
  * only before the first request is admitted (GPE_EBUSY after).  Results do
  * not depend on it.  No counterpart in the reference. */
 gpe_status gpe_server_set_tile(gpe_server *s, uint32_t tile);

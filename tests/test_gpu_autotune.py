@@ -46,7 +46,8 @@ def test_tune_persist_and_parity(tmp_path):
 def test_batch_tile_choice_does_not_change_results():
     cfg = instance_config((33, 21, 27), 0.3, 700.0, 6e-4, 25, "real", 1.6, seed=4, mu0=3.0)
     outs = []
-    for w in (1, 2, 4, 8, (1, True), (2, True), (8, True), (1, False, True), (4, False, True), (8, True, True)):
+    for w in (1, 2, 4, 8, (1, True), (2, True), (8, True), (1, False, True), (4, False, True), (8, True, True),
+              3, (6, True, True), (12, False, 2), (3, True, 2)):
         B = Batch("fp32")
         B.set_tile(w)
         i = B.add(cfg)
@@ -57,20 +58,20 @@ def test_batch_tile_choice_does_not_change_results():
         np.testing.assert_array_equal(e, outs[0][0])
         np.testing.assert_array_equal(o, outs[0][1])
     res = autotune.tune_batch_tile(lambda w: _mk(w, cfg), chunk_steps=5, chunks=1)
-    assert res["waves"] in (1, 2, 4, 8) and res["cached"] in (False, True) and res["slabs"] in (False, True, 2)
+    assert res["waves"] in (1, 2, 4, 8, 3, 6, 12) and res["cached"] in (False, True) and res["slabs"] in (False, True, 2)
 
 
 def test_batch_slab_order_mixed_instances():
     """Slab order per instance and per group over instances whose tile counts are and are not
-    multiples of 8, one of them a y-band instance (16 block rows) under the default order: states
-    equal the default order's."""
+    multiples of 8, one of them a y-band instance (16 block rows) under the default order, with one
+    thread per cell or per group of 4 classes: states equal the default order's."""
     cfgs = [instance_config((16, 128, 6), 0.3, 300.0, 6e-4, 12, "real", 1.0, seed=1, mu0=2.0),
             instance_config((33, 21, 27), 0.3, 700.0, 6e-4, 9, "real", 1.6, seed=2, mu0=3.0),
             instance_config((5, 7, 3), 0.3, 100.0, 6e-4, 12, "real", 2.0, seed=3, mu0=1.0)]
     runs = []
-    for slabs in (False, True, 2):
+    for waves, slabs in ((4, False), (4, True), (4, 2), (6, False), (6, True), (3, 2)):
         B = Batch("fp32")
-        B.set_tile(4, False, slabs)
+        B.set_tile(waves, False, slabs)
         ids = [B.add(c) for c in cfgs]
         B.run(5, 3)
         runs.append([B.get_state(i) for i in ids])

@@ -152,10 +152,11 @@ def test_random_shapes_imag_and_fp32_every_tile(orc, case):
         assert _rel_l2(ge, oe) < 2e-6 and _rel_l2(go, oo) < 2e-6, (dims, kernel, tile)
 
 
-@pytest.mark.parametrize("seed", [7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("seed", [7, 8, 9, 10, 11, 12, 13, 14, 15, 16])
 def test_random_batch_mix_bit_exact(orc, seed):
     rng = np.random.default_rng(seed)
-    tile = (int(rng.choice([1, 2, 4, 8])), bool(rng.integers(2)), [False, True, 2][int(rng.integers(3))])
+    # waves 3, 6 and 12: one thread per cell and group of 4 classes
+    tile = (int(rng.choice([1, 2, 4, 8, 3, 6, 12])), bool(rng.integers(2)), [False, True, 2][int(rng.integers(3))])
     group_bytes = int(rng.choice([0, 1, 1 << 20, 640 << 20]))
     chunk = int(rng.integers(3, 12))
     announce(f"sweep batch seed={seed} tile={tile} group_bytes={group_bytes} chunk={chunk}")

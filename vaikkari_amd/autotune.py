@@ -102,7 +102,8 @@ def tune(s: Solver, steps: int = 20, families=DEFAULT_FAMILIES, use_cache: bool 
 
 # tile orders: False y-bands (or plain), True a slab of every instance per XCD, 2 a slab of the
 # whole group per XCD
-BATCH_TILES = tuple((w, c, sl) for sl in (False, True, 2) for c in (False, True) for w in (1, 2, 4, 8))
+BATCH_TILES = tuple((w, c, sl) for sl in (False, True, 2) for c in (False, True) for w in (1, 2, 4, 8)) + \
+    tuple((w, c, sl) for sl in (True, 2) for c in (False, True) for w in (3, 6, 12))  # class-group tiles
 
 
 def _tile_triple(t):

@@ -19,6 +19,7 @@ from .solver import MODES, PRECISIONS, nodes_of
 TILE_CACHED = 0x100  # GPE_BATCH_TILE_CACHED: cached read-modify-write of the real-time pass
 TILE_SLABS = 0x200  # GPE_BATCH_TILE_SLABS: each XCD takes a contiguous eighth of an instance's tiles
 TILE_GROUP_SLABS = 0x400  # GPE_BATCH_TILE_GROUP_SLABS: ... of a whole group's tiles (whole instances)
+TILE_QUAD = 0x800  # GPE_BATCH_TILE_QUAD: one thread per cell and group of 4 classes (3, 6 or 12 waves)
 
 
 def instance_config(dims: Sequence[int], h: float, g: float, dt: float, steps: int, mode: str = "real",
@@ -46,11 +47,13 @@ def instance_config(dims: Sequence[int], h: float, g: float, dt: float, steps: i
 
 def tile_code(waves, cached: bool = False, slabs=False) -> int:
     """gpe_batch_set_tile's argument for (waves, cached, slabs); slabs False/0, True/1 or 2 (see
-    Batch.set_tile).  A (waves, cached) or (waves, cached, slabs) tuple is accepted as waves."""
+    Batch.set_tile).  A (waves, cached) or (waves, cached, slabs) tuple is accepted as waves.
+    3, 6 or 12 waves select one thread per cell and group of 4 classes (GPE_BATCH_TILE_QUAD)."""
     if isinstance(waves, tuple):
         waves, cached, slabs = (tuple(waves) + (False, False))[:3]
     order = int(slabs)
-    return int(waves) | (TILE_CACHED if cached else 0) | {0: 0, 1: TILE_SLABS, 2: TILE_GROUP_SLABS}[order]
+    quad = TILE_QUAD if int(waves) in (3, 6, 12) else 0
+    return int(waves) | quad | (TILE_CACHED if cached else 0) | {0: 0, 1: TILE_SLABS, 2: TILE_GROUP_SLABS}[order]
 
 
 class Batch:
@@ -72,7 +75,8 @@ class Batch:
             pass
 
     def set_tile(self, waves, cached: bool = False, slabs=False) -> None:
-        """Real-time tile: waves per block (1, 2, 4, 8), whether the updated field's
+        """Real-time tile: waves per block (1, 2, 4, 8; or 3, 6, 12 for one thread per cell and
+        group of 4 classes), whether the updated field's
         read-modify-write is cached (default: streamed) and how the tiles go to the XCDs: slabs
         True (or 1) gives each XCD a contiguous slab of every instance's tiles, 2 a contiguous
         eighth of a whole group's tiles (whole instances in layer order), False y-bands or plain

@@ -101,15 +101,18 @@ constexpr int round8(int n) { return (n + 7) & ~7; }
 // Field bytes per group of real-time instances.  A group's leapfrog reads and
 // writes its fields twice per step; the second half-update walks the tiles
 // backwards, so what the 256 MB Infinity Cache holds of the first is read
-// again from there, a large share for a group a few times the cache's size.
-constexpr uint64_t kDefaultGroupBytes = 640ull << 20;
+// again from there.  256 MiB keeps a group in that cache for the default
+// cached tile (a streamed tile measured best at 512-640 MiB).
+constexpr uint64_t kDefaultGroupBytes = 256ull << 20;
 // Most kernel launches one captured chunk may hold (longer chunks are split).
 constexpr uint64_t kMaxGraphLaunches = 16384;
 constexpr int kImagTY = 4;
-// Default tile of the real-time pass: 2 waves per block, streamed
-// read-modify-write, slab order (the best measured on mixes of small and
-// mid-size instances, such as a serving stream's resident set).
-constexpr int kRealTY = 2;
+// Default tile of the real-time pass: one thread per cell and group of 4
+// classes, 3 waves per block, cached read-modify-write, each XCD a contiguous
+// eighth of the group (with 256 MiB groups the best measured on uniform
+// batches of 16^3-48^3 cells, the 64-instance mixed form (+10 to +14 % over
+// one thread per cell) and a serving stream's resident set (+3 %)).
+constexpr int kRealTY = 3;
 constexpr int kFinalizeThreads = 256;
 
 
@@ -153,6 +156,54 @@ __global__ void __launch_bounds__(64 * TY)
         const cplx_t<T> *prev = global_ptr<cplx_t<T>>(d.buf[half == 0 ? 0 : 1]);
         gather_tile<T, FORM, POT_TRAP, MODE_REAL, TY, NT>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, nullptr, bx,
                                                          by, bz, d.gg.lsh);
+    }
+}
+
+// The real-time half-update with one thread per cell and group of 4 classes
+// (quad_update, as the single solver's class-group tiles): an instance's work
+// item is a block of TY (group, cell rows) waves, groups fastest; d.gg holds
+// the wave grid (gy = wave rows per layer) and d.ntiles the blocks.  Work
+// items go to the XCDs as in k_batch_real; with d.gg.remap 1 XCD x takes the
+// x-th contiguous eighth of an instance's blocks.
+template <typename T, int TY, bool NT, bool GS>
+__global__ void __launch_bounds__(64 * TY)
+    k_batch_quad(const InstDesc<T> *__restrict__ D, const int *__restrict__ items, int item0, int nitems, int s,
+                 int chunk, int half) {
+    constexpr int FORM = sizeof(T) == 8 ? FORM_REF : FORM_DIFF;
+    for (int base = 0; base < nitems; base += gridDim.x) {
+        const int local = base + (int)blockIdx.x;
+        if (local >= nitems) continue;
+        int w;
+        if (GS) {
+            const int per = nitems >> 3, slot = local >> 3;
+            w = item0 + (local & 7) * per + (half ? per - 1 - slot : slot);
+        } else {
+            w = item0 + (half ? nitems - 1 - local : local);
+        }
+        const int q = __builtin_amdgcn_readfirstlane(items[w]);
+        const InstDesc<T> &d = D[q];
+        const long long steps = d.steps_left < chunk ? d.steps_left : chunk;
+        if (s >= steps) continue;
+        const int lb = w - d.tile0, nb = d.ntiles;
+        const int b = d.gg.remap == 1 ? (lb & 7) * ((nb + 7) >> 3) + (lb >> 3) : lb;
+        if (b >= nb) continue;  // padding to the next multiple of 8
+        const int wv = b * TY + (int)threadIdx.y;
+        const int grp = __builtin_amdgcn_readfirstlane(wv % 3);
+        int t = wv / 3;
+        const int bx = t % d.gg.gx;
+        t /= d.gg.gx;
+        const int by = t % d.gg.gy, k = t / d.gg.gy;
+        const int i = (bx << d.gg.lsh) + ((int)threadIdx.x & ((1 << d.gg.lsh) - 1));
+        const int j = (by << (6 - d.gg.lsh)) + ((int)threadIdx.x >> d.gg.lsh);
+        if (k >= d.L.nz || i >= d.L.nx || j >= d.L.ny) continue;
+        cplx_t<T> *next = global_ptr<cplx_t<T>>(d.buf[half == 0 ? 1 : 0]);
+        const cplx_t<T> *prev = global_ptr<cplx_t<T>>(d.buf[half == 0 ? 0 : 1]);
+        if (grp == 0)
+            quad_update<T, FORM, POT_TRAP, MODE_REAL, 0, NT>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, (T)1, i, j, k);
+        else if (grp == 1)
+            quad_update<T, FORM, POT_TRAP, MODE_REAL, 1, NT>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, (T)1, i, j, k);
+        else
+            quad_update<T, FORM, POT_TRAP, MODE_REAL, 2, NT>(next, prev, nullptr, d.L, d.H, d.trap, d.dt, (T)1, i, j, k);
     }
 }
 
@@ -265,8 +316,9 @@ struct gpe_batch {
     uint32_t precision = GPE_FP32_BULK;
     uint64_t budget = 0;
     int real_ty = kRealTY;  // waves per block of the real-time pass (fixed while instances exist)
-    bool real_nt = true;    // streaming (non-temporal) read-modify-write of the updated field
-    int real_order = 1;       // tile order over the XCDs: 0 y-bands (or plain), 1 a contiguous eighth
+    bool real_nt = false;   // streaming (non-temporal) read-modify-write of the updated field
+    bool real_quad = true;  // one thread per cell and group of 4 classes (real_ty 3, 6 or 12 waves)
+    int real_order = 2;       // tile order over the XCDs: 0 y-bands (or plain), 1 a contiguous eighth
                               // of every instance's tiles, 2 of the whole group's work items
     // Real-time instances advance a whole chunk group by group: contiguous
     // runs of instances whose fields total at most group_bytes (0: one group)
@@ -330,12 +382,22 @@ bool valid_instance(const gpe_instance_config *c) {
            std::isfinite(c->trap_ratio) && c->init <= GPE_INIT_GAUSSIAN;
 }
 
-size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, int real_order, Layout *Lout,
-                      Grid3 *gg, Grid3 *zg, int *zcz, int *ntiles) {
+size_t instance_bytes(const gpe_instance_config *c, size_t elem, int real_ty, int real_order, bool real_quad,
+                      Layout *Lout, Grid3 *gg, Grid3 *zg, int *zcz, int *ntiles) {
     const Layout L = make_layout(c->nx, c->ny, c->nz, elem);
     size_t bytes = 2 * (size_t)L.elems * elem;  // even and odd (real time) or ping-pong pair (imaginary time)
     int tiles = 0;
-    if (c->mode == GPE_REAL_TIME) {
+    if (c->mode == GPE_REAL_TIME && real_quad) {
+        // k_batch_quad: the wave grid, 3 group waves per wave row of cells
+        Grid3 G{};
+        G.lsh = row_lanes_log2(L.nx);
+        G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
+        G.gy = (L.ny + (64 >> G.lsh) - 1) / (64 >> G.lsh);
+        G.gz = L.nz;
+        G.remap = real_order == 1 ? 1 : 0;
+        tiles = (int)((3LL * G.gx * G.gy * G.gz + real_ty - 1) / real_ty);
+        if (gg) *gg = G;
+    } else if (c->mode == GPE_REAL_TIME) {
         Grid3 G{};
         G.lsh = row_lanes_log2(L.nx);
         G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
@@ -481,6 +543,27 @@ void enqueue_chunk(gpe_batch *B, uint32_t chunk, hipStream_t st) {
     hipLaunchKernelGGL((k_batch_real<T, TY, NT, GS>), dim3(grid), dim3(64, TY), 0, st, R, B->d_items,     \
                        grp.first, grp.second, (int)s, (int)chunk, half)
                 const bool gs = B->real_order == 2;
+#define GPE_BATCH_QUAD(TY, NT, GS)                                                                      \
+    hipLaunchKernelGGL((k_batch_quad<T, TY, NT, GS>), dim3(grid), dim3(64, TY), 0, st, R, B->d_items,     \
+                       grp.first, grp.second, (int)s, (int)chunk, half)
+                if (B->real_quad) {
+                    switch (B->real_ty * 4 + (B->real_nt ? 2 : 0) + (gs ? 1 : 0)) {
+                        case 12: GPE_BATCH_QUAD(3, false, false); break;
+                        case 13: GPE_BATCH_QUAD(3, false, true); break;
+                        case 14: GPE_BATCH_QUAD(3, true, false); break;
+                        case 15: GPE_BATCH_QUAD(3, true, true); break;
+                        case 24: GPE_BATCH_QUAD(6, false, false); break;
+                        case 25: GPE_BATCH_QUAD(6, false, true); break;
+                        case 26: GPE_BATCH_QUAD(6, true, false); break;
+                        case 27: GPE_BATCH_QUAD(6, true, true); break;
+                        case 48: GPE_BATCH_QUAD(12, false, false); break;
+                        case 49: GPE_BATCH_QUAD(12, false, true); break;
+                        case 50: GPE_BATCH_QUAD(12, true, false); break;
+                        default: GPE_BATCH_QUAD(12, true, true);
+                    }
+                    continue;
+                }
+#undef GPE_BATCH_QUAD
                 switch (B->real_ty * 4 + (B->real_nt ? 2 : 0) + (gs ? 1 : 0)) {
                     case 4: GPE_BATCH_REAL(1, false, false); break;
                     case 5: GPE_BATCH_REAL(1, false, true); break;
@@ -695,7 +778,8 @@ void gpe_batch_destroy(gpe_batch *b) { delete b; }
 
 uint64_t gpe_batch_instance_bytes(const gpe_batch *B, const gpe_instance_config *c) {
     if (!B || !valid_instance(c)) return 0;
-    return instance_bytes(c, B->elem(), B->real_ty, B->real_order, nullptr, nullptr, nullptr, nullptr, nullptr);
+    return instance_bytes(c, B->elem(), B->real_ty, B->real_order, B->real_quad, nullptr, nullptr, nullptr, nullptr,
+                          nullptr);
 }
 
 gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id) {
@@ -705,7 +789,8 @@ gpe_status gpe_batch_add(gpe_batch *B, const gpe_instance_config *c, int32_t *id
     }
     BInst b;
     b.cfg = *c;
-    b.bytes = instance_bytes(c, B->elem(), B->real_ty, B->real_order, &b.L, &b.gg, &b.zg, &b.zcz, &b.ntiles);
+    b.bytes = instance_bytes(c, B->elem(), B->real_ty, B->real_order, B->real_quad, &b.L, &b.gg, &b.zg, &b.zcz,
+                             &b.ntiles);
     if (B->budget && B->resident + b.bytes > B->budget) {
         set_error("gpe_batch_add: %zu bytes would exceed the budget (%llu of %llu resident)", b.bytes,
                   (unsigned long long)B->resident, (unsigned long long)B->budget);
@@ -1013,11 +1098,15 @@ uint64_t gpe_batch_group_bytes(const gpe_batch *B) { return B ? B->group_bytes :
 
 gpe_status gpe_batch_set_tile(gpe_batch *B, uint32_t tile) {
     const uint32_t waves = tile & 0xff;
-    if (!B || (waves != 1 && waves != 2 && waves != 4 && waves != 8) ||
-        (tile & ~(0xffu | GPE_BATCH_TILE_CACHED | GPE_BATCH_TILE_SLABS | GPE_BATCH_TILE_GROUP_SLABS)) ||
+    const bool quad = (tile & GPE_BATCH_TILE_QUAD) != 0;
+    const bool waves_ok = quad ? (waves == 3 || waves == 6 || waves == 12)
+                               : (waves == 1 || waves == 2 || waves == 4 || waves == 8);
+    if (!B || !waves_ok ||
+        (tile & ~(0xffu | GPE_BATCH_TILE_CACHED | GPE_BATCH_TILE_SLABS | GPE_BATCH_TILE_GROUP_SLABS |
+                  GPE_BATCH_TILE_QUAD)) ||
         ((tile & GPE_BATCH_TILE_SLABS) && (tile & GPE_BATCH_TILE_GROUP_SLABS))) {
-        set_error("gpe_batch_set_tile: waves must be 1, 2, 4 or 8 (optionally | GPE_BATCH_TILE_CACHED, "
-                  "| GPE_BATCH_TILE_SLABS or | GPE_BATCH_TILE_GROUP_SLABS)");
+        set_error("gpe_batch_set_tile: waves must be 1, 2, 4 or 8, or 3, 6 or 12 with GPE_BATCH_TILE_QUAD "
+                  "(optionally | GPE_BATCH_TILE_CACHED, | GPE_BATCH_TILE_SLABS or | GPE_BATCH_TILE_GROUP_SLABS)");
         return GPE_EINVAL;
     }
     for (auto &b : B->inst)
@@ -1026,6 +1115,7 @@ gpe_status gpe_batch_set_tile(gpe_batch *B, uint32_t tile) {
             return GPE_EBUSY;
         }
     B->real_ty = (int)waves;
+    B->real_quad = quad;
     B->real_nt = !(tile & GPE_BATCH_TILE_CACHED);
     B->real_order = (tile & GPE_BATCH_TILE_GROUP_SLABS) ? 2 : (tile & GPE_BATCH_TILE_SLABS) ? 1 : 0;
     B->graph_dirty = true;
