@@ -129,7 +129,7 @@ def tune_batch_tile(make_batch, chunk_steps: int = 10, chunks: int = 2, candidat
                    for (w, c, sl), v in ((_tile_triple(k), v) for k, v in timings.items())}}
 
 
-BATCH_GROUP_BYTES = (0, 256 << 20, 384 << 20, 512 << 20, 640 << 20, 1 << 30, 2 << 30)
+BATCH_GROUP_BYTES = (0, 192 << 20, 256 << 20, 384 << 20, 512 << 20, 640 << 20, 1 << 30, 2 << 30)
 
 
 def tune_batch_groups(batch, chunk_steps: int = 10, chunks: int = 2, candidates=BATCH_GROUP_BYTES) -> dict:
