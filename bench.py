@@ -111,9 +111,10 @@ def measured_traffic(nodes, precision, info):
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic_128cubed.md), when the
     workload and kernel variant are the ones profiled; else None."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_128cubed.json")
-    if nodes != 25165824 or info["kernel"] != "gather" or info["tile"] != 11 or not os.path.exists(path):
+    # the default variants at 128^3 cells and their kernels
+    tile, key = {"fp64": (24, "k_quad<double,0,1,0,6,true>"), "fp32": (11, "k_gather<float,1,1,0,4,true>")}[precision]
+    if nodes != 25165824 or info["kernel"] != "gather" or info["tile"] != tile or not os.path.exists(path):
         return None
-    key = "k_gather<double,0,1,0,4,true>" if precision == "fp64" else "k_gather<float,1,1,0,4,true>"
     try:
         k = json.load(open(path))["kernels"][key]
     except (OSError, ValueError, KeyError):

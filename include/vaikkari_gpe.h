@@ -94,10 +94,12 @@ typedef struct gpe_config {
                               256 MiB: tile 18 FP64, 19 FP32, or 14 on grids of at most 1024
                               one-wave blocks; imaginary time with both fields in 256 MiB:
                               tile 20 FP64 from 256 four-wave blocks, 3 FP32 from 1024;
+                              FP64 real time beyond: tile 24 when the wave rows split into 8
+                              y-bands of whole 6-wave blocks, else 22;
                               else the y-band tile with the most waves whose block rows divide
                               by 8: 13, 11, 12, 14 for 8 (FP64 imaginary time only), 4, 2, 1
-                              waves; none: tile 7).  128^3 cells: tile 11, FP64 imaginary
-                              time 13 */
+                              waves; none: tile 7).  128^3 cells: FP64 tile 24 (imaginary
+                              time 13), FP32 tile 11 */
 } gpe_config;
 
 typedef struct gpe_solver gpe_solver;

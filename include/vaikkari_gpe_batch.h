@@ -109,7 +109,7 @@ uint64_t gpe_batch_chunk_node_steps(const gpe_batch *b, uint32_t chunk_steps);
 #define GPE_BATCH_TILE_GROUP_SLABS 0x400u
 /* | GPE_BATCH_TILE_QUAD: one thread per cell and group of 4 classes (three
  * times the waves of one thread per cell, for about the same loads), waves
- * per block 3, 6 or 12; the single solver's gather tiles 19-22.  Same bits. */
+ * per block 3, 6 or 12; the single solver's gather tiles 19-24.  Same bits. */
 #define GPE_BATCH_TILE_QUAD 0x800u
 gpe_status gpe_batch_set_tile(gpe_batch *b, uint32_t tile);
 /* How gpe_batch_run issues a chunk: GPE_BATCH_GRAPH replays a captured

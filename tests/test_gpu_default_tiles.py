@@ -20,11 +20,13 @@ CASES = [
     ((48, 48, 48), "fp32", 19),   # cached
     ((80, 80, 80), "fp64", 18),   # 197 MB of fields: cached
     ((96, 96, 96), "fp32", 19),   # 170 MB: cached
-    ((96, 96, 96), "fp64", 12),   # 340 MB: 4-wave rows 12, 2-wave rows 24
+    ((96, 96, 96), "fp64", 24),   # 340 MB, FP64: class groups in y-bands (48 wave rows)
     ((112, 112, 112), "fp32", 7),  # 270 MB; rows 7, 14, 28: no y-bands
-    ((128, 128, 128), "fp64", 11),
+    ((128, 128, 128), "fp64", 24),
     ((128, 128, 128), "fp32", 11),
-    ((160, 160, 160), "fp64", 12),
+    ((160, 160, 160), "fp64", 24),
+    ((160, 160, 160), "fp32", 12),  # 4-wave rows 20, 2-wave rows 40
+    ((87, 87, 302), "fp64", 22),    # 44 wave rows: no y-bands, class groups in plain order
 ]
 
 

@@ -140,7 +140,11 @@ def tune_batch_groups(batch, chunk_steps: int = 10, chunks: int = 2, candidates=
     for gb in candidates:
         batch.set_group_bytes(gb)
         batch.run(chunk_steps, 1)  # re-grouping, capture + warm-up
-        timings[gb] = batch.run(chunk_steps, chunks)
+        ms = batch.run(chunk_steps, chunks)
+        # a run in which no instance had steps left took no time and measures nothing
+        timings[gb] = ms if ms > 0 else float("inf")
+    if all(v == float("inf") for v in timings.values()):
+        raise ValueError("tune_batch_groups: the instances ran out of steps; give them more")
     best = min(timings, key=timings.get)
     batch.set_group_bytes(best)
     return {"group_bytes": best, "ms": {str(k >> 20) + "MiB": v for k, v in timings.items()}}

@@ -589,8 +589,8 @@ __device__ __forceinline__ double quad_update(cplx_t<T> *__restrict__ next, cons
 }
 
 // Imaginary time: block partials of sum|out|^2 in logical block order.
-// G.remap 1: XCD x takes the x-th contiguous eighth of the blocks; G.rev
-// walks the blocks backwards.
+// G.remap 1: XCD x takes the x-th contiguous eighth of the blocks; 2: y-bands
+// per XCD; G.rev walks the blocks (the layers) backwards.
 template <typename T, int FORM, int POT, int MODE, int TY, bool NT = false>
 __global__ void __launch_bounds__(64 * TY) k_quad(cplx_t<T> *__restrict__ next, const cplx_t<T> *__restrict__ prev,
                                                   const T *__restrict__ pot, Layout L, Hamiltonian<T, FORM> H,
@@ -599,11 +599,21 @@ __global__ void __launch_bounds__(64 * TY) k_quad(cplx_t<T> *__restrict__ next, 
     __shared__ double lds[TY];
     const int nb = (int)gridDim.x, lin = (int)blockIdx.x;
     int b = lin;
-    if (G.remap == 1) {
-        const int q = nb >> 3, r = nb & 7, xcd = lin & 7, slot = lin >> 3;
-        b = (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+    if (G.remap == 2) {
+        // XCD x owns the y-band [x gy/8, (x+1) gy/8) of wave rows of every
+        // layer and walks it layer by layer (G.rev: layers backwards); the
+        // band of one layer is bb whole blocks (the host checks gy % 8 == 0
+        // and 3 gx gy/8 % TY == 0)
+        const int bb = 3 * G.gx * (G.gy >> 3) / TY, xcd = lin & 7, slot = lin >> 3;
+        const int layer = slot / bb, within = slot - layer * bb;
+        b = ((G.rev ? G.gz - 1 - layer : layer) * 8 + xcd) * bb + within;
+    } else {
+        if (G.remap == 1) {
+            const int q = nb >> 3, r = nb & 7, xcd = lin & 7, slot = lin >> 3;
+            b = (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+        }
+        if (G.rev) b = nb - 1 - b;
     }
-    if (G.rev) b = nb - 1 - b;
     const int wv = b * TY + (int)threadIdx.y;  // logical wave
     const int grp = __builtin_amdgcn_readfirstlane(wv % 3);
     int t = wv / 3;

@@ -72,14 +72,15 @@ struct TileShape {
     int node;               // gather family: 1 one thread per node (k_node, ty waves = ty classes),
                             // 2 one thread per cell and group of 4 classes (k_quad, ty (group, rows) waves)
 };
-// gather tiles 15-18: one thread per node, 12 or 4 classes per block; 19-22:
-// one thread per group of 4 classes
+// gather tiles 15-18: one thread per node, 12 or 4 classes per block; 19-24:
+// one thread per group of 4 classes (23, 24: y-bands per XCD)
 constexpr TileShape kGatherTiles[] = {{4, 1, 0, 1}, {4, 1, 0, 0}, {4, 1, 1, 0}, {1, 1, 1, 0}, {2, 1, 1, 0},
                                       {8, 1, 1, 0}, {4, 1, 1, 1}, {2, 1, 0, 1}, {8, 1, 0, 1}, {1, 1, 0, 1},
                                       {4, 1, 2, 1}, {2, 1, 2, 1}, {8, 1, 2, 1}, {1, 1, 2, 1},
                                       {12, 1, 0, 1, 0, 1}, {12, 1, 0, 0, 0, 1}, {4, 1, 0, 1, 0, 1},
                                       {4, 1, 1, 0, 0, 1}, {3, 1, 1, 0, 0, 2}, {6, 1, 1, 0, 0, 2},
-                                      {12, 1, 1, 0, 0, 2}, {6, 1, 0, 1, 0, 2}};
+                                      {12, 1, 1, 0, 0, 2}, {6, 1, 0, 1, 0, 2}, {3, 1, 2, 1, 0, 2},
+                                      {6, 1, 2, 1, 0, 2}};
 constexpr TileShape kZmarchTiles[] = {{4, 8, 1, 0}, {4, 8, 0, 1}, {1, 8, 1, 0}, {2, 8, 1, 1}, {8, 8, 1, 1},
                                       {4, 4, 1, 1}, {4, 16, 1, 1}, {2, 16, 1, 0}, {1, 16, 1, 1}, {2, 4, 1, 1},
                                       {4, 4, 2, 1}, {8, 8, 2, 1}};
@@ -125,31 +126,55 @@ int gather_rows(int ty, const Layout &L) {
 //    input and a streaming store sends it to HBM; FP64 from 256 blocks of 4
 //    waves, one thread per group of 4 classes, 6 waves per block (tile 20);
 //    FP32 from 1024, 4 waves per block (tile 3) (48^3-96^3: +5 to +27 %);
+//  - FP64 real time beyond the cache: one thread per cell and group of 4
+//    classes with streaming read-modify-write, in y-bands per XCD when the
+//    wave rows split into them (tile 24, 6 waves per block), else in plain
+//    order (tile 22): +2 to +6 % over one thread per cell at 96^3-256^3;
 //  - otherwise y-bands per XCD with streaming read-modify-write, the most
 //    waves per block (up to 8 in FP64 imaginary time, 4 else: tiles 13, 11,
 //    12, 14 for 8, 4, 2, 1) whose block rows divide by the 8 XCDs; when none
 //    does, {4 waves, contiguous eighths, streaming} (tile 7).
-// At 128^3 cells this is tile 11 (tile 13 in FP64 imaginary time).
-int default_tile(int kernel, uint32_t precision, int mode, const Layout &L) {
-    if (kernel == GPE_KERNEL_FUSED) return precision == GPE_FP64 ? 10 : 13;
-    if (kernel != GPE_KERNEL_GATHER) return 0;
-    const unsigned long long both = 2ull * (unsigned long long)L.elems * (precision == GPE_FP64 ? 16u : 8u);
-    const long long gx = (L.nx + (1 << row_lanes_log2(L.nx)) - 1) >> row_lanes_log2(L.nx);
-    if (mode == MODE_REAL && both <= (256ull << 20)) {
-        const int gy1 = gather_rows(1, L);
-        if (gy1 % 8 == 0 && gx * gy1 * L.nz <= 1024) return 13;
-        return precision == GPE_FP64 ? 17 : 18;
-    }
-    if (mode == MODE_IMAG && both <= (256ull << 20)) {
-        const long long b4 = gx * gather_rows(4, L) * L.nz;
-        if (precision == GPE_FP64 && b4 >= 256) return 19;
-        if (precision != GPE_FP64 && b4 >= 1024) return 2;
-    }
+// At 128^3 cells this is tile 24 in FP64 (13 in imaginary time), tile 11 in
+// FP32.
+struct GridFacts {
+    bool cached;     // both time levels fit 256 MiB
+    long long gx;    // wave columns per row
+    bool tiny_band;  // one-wave blocks split into y-bands, at most 1024 of them
+    long long b4;    // blocks of 4 waves per launch
+};
+GridFacts grid_facts(uint32_t precision, const Layout &L) {
+    GridFacts f;
+    f.cached = 2ull * (unsigned long long)L.elems * (precision == GPE_FP64 ? 16u : 8u) <= (256ull << 20);
+    f.gx = (L.nx + (1 << row_lanes_log2(L.nx)) - 1) >> row_lanes_log2(L.nx);
+    const int gy1 = gather_rows(1, L);
+    f.tiny_band = gy1 % 8 == 0 && f.gx * gy1 * L.nz <= 1024;
+    f.b4 = f.gx * gather_rows(4, L) * L.nz;
+    return f;
+}
+
+// The default among the one-thread-per-cell gather tiles (1-14).
+int cell_default_tile(uint32_t precision, int mode, const Layout &L) {
+    const GridFacts f = grid_facts(precision, L);
+    if (mode == MODE_REAL && f.cached) return f.tiny_band ? 13 : 3;
+    if (mode == MODE_IMAG && f.cached && f.b4 >= 1024) return precision == GPE_FP64 ? 5 : 2;
     const int wmax = precision == GPE_FP64 && mode == MODE_IMAG ? 8 : 4;
     static constexpr int kWaves[4] = {8, 4, 2, 1}, kBandTile[4] = {12, 10, 11, 13};
     for (int i = 0; i < 4; i++)
         if (kWaves[i] <= wmax && gather_rows(kWaves[i], L) % 8 == 0) return kBandTile[i];
     return 6;
+}
+
+int default_tile(int kernel, uint32_t precision, int mode, const Layout &L) {
+    if (kernel == GPE_KERNEL_FUSED) return precision == GPE_FP64 ? 10 : 13;
+    if (kernel != GPE_KERNEL_GATHER) return 0;
+    const GridFacts f = grid_facts(precision, L);
+    if (mode == MODE_REAL && f.cached) return f.tiny_band ? 13 : precision == GPE_FP64 ? 17 : 18;
+    if (mode == MODE_IMAG && f.cached && precision == GPE_FP64 && f.b4 >= 256) return 19;
+    if (mode == MODE_REAL && precision == GPE_FP64 && !f.cached) {
+        const int rows = gather_rows(1, L);  // wave rows per layer
+        return rows % 8 == 0 && (3 * f.gx * (rows / 8)) % 6 == 0 ? 23 : 21;
+    }
+    return cell_default_tile(precision, mode, L);
 }
 
 TileShape tile_shape(int kernel, int tile) {
@@ -271,7 +296,9 @@ Grid3 grid_of(int kernel, int tile, const Layout &L) {
         G.gx = (L.nx + (1 << G.lsh) - 1) >> G.lsh;
         G.gy = (L.ny + (64 >> G.lsh) - 1) / (64 >> G.lsh);
         G.gz = L.nz;
-        G.remap = t.remap == 1 ? 1 : 0;
+        G.remap = t.remap;
+        // y-bands (k_quad only): whole blocks per band and layer
+        if (G.remap == 2 && (t.node != 2 || G.gy % 8 != 0 || (3 * G.gx * (G.gy / 8)) % t.ty != 0)) G.remap = 0;
         return G;
     }
     const int lw = 1 << G.lsh, rows = t.ty * (64 >> G.lsh);
@@ -357,6 +384,7 @@ hipError_t launch_update(int kernel, int tile, cplx_t<T> *next, const cplx_t<T> 
             case 30: launch_quad<T, FORM, POT, MODE, 3>(GPE_ARGS); break;
             case 60: launch_quad<T, FORM, POT, MODE, 6>(GPE_ARGS); break;
             case 61: launch_quad<T, FORM, POT, MODE, 6, true>(GPE_ARGS); break;
+            case 31: launch_quad<T, FORM, POT, MODE, 3, true>(GPE_ARGS); break;
             case 120: launch_quad<T, FORM, POT, MODE, 12>(GPE_ARGS); break;
             default: return hipErrorInvalidValue;
         }
@@ -707,10 +735,11 @@ gpe_status gpe_solver::enqueue_steps(uint64_t nsteps, uint64_t *launches) {
     return GPE_OK;
 }
 
-// One real-time step whose odd half-update (gather, `tile`) also reduces the
-// observables of the even field into scal[4..7].
+// One real-time step whose odd half-update (gather, `tile`, one thread per
+// cell) also reduces the observables of the even field into scal[4..7]; the
+// even half-update runs gather tile `even_tile`.
 template <typename T, int FORM>
-gpe_status gpe_solver::observe_step(int tile) {
+gpe_status gpe_solver::observe_step(int tile, int even_tile) {
     using C = cplx_t<T>;
     Hamiltonian<T, FORM> H;
     H.lapfac = (T)co.lapfac_dt;
@@ -751,11 +780,11 @@ gpe_status gpe_solver::observe_step(int tile) {
     GPE_HIP_TRY(hipGetLastError());
     hipError_t e;
     if (arr)
-        e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(GPE_KERNEL_GATHER, tile, ev, od, p, L, H, trap, co.dt,
+        e = launch_update<T, FORM, POT_ARRAY, MODE_REAL>(GPE_KERNEL_GATHER, even_tile, ev, od, p, L, H, trap, co.dt,
                                                           nullptr, nullptr, stream, true);
     else
-        e = launch_update<T, FORM, POT_TRAP, MODE_REAL>(GPE_KERNEL_GATHER, tile, ev, od, p, L, H, trap, co.dt, nullptr,
-                                                         nullptr, stream, true);
+        e = launch_update<T, FORM, POT_TRAP, MODE_REAL>(GPE_KERNEL_GATHER, even_tile, ev, od, p, L, H, trap, co.dt,
+                                                         nullptr, nullptr, stream, true);
     GPE_HIP_TRY(e);
     steps_done += 1;
     return GPE_OK;
@@ -1088,12 +1117,14 @@ gpe_status gpe_step_observe(gpe_solver *s, uint64_t nsteps, double out[4]) {
     }
     // the last step as two gather passes (any family gives the same bits),
     // the odd one accumulating the observables of the even field it reads
-    int tile =
+    const int tile =
         s->kernel == GPE_KERNEL_GATHER ? s->tile : default_tile(GPE_KERNEL_GATHER, s->cfg.precision, MODE_REAL, s->L);
-    // the observables are summed per cell: a node-parallel tile hands the pass to tile 4
-    if (tile_shape(GPE_KERNEL_GATHER, tile).node) tile = 3;
-    const gpe_status st = s->cfg.precision == GPE_FP64 ? s->observe_step<double, FORM_REF>(tile)
-                                                       : s->observe_step<float, FORM_DIFF>(tile);
+    // the observables are summed per cell: a node or class-group tile hands the
+    // odd pass to the default one-thread-per-cell tile
+    const int obs_tile =
+        tile_shape(GPE_KERNEL_GATHER, tile).node ? cell_default_tile(s->cfg.precision, MODE_REAL, s->L) : tile;
+    const gpe_status st = s->cfg.precision == GPE_FP64 ? s->observe_step<double, FORM_REF>(obs_tile, tile)
+                                                       : s->observe_step<float, FORM_DIFF>(obs_tile, tile);
     if (st != GPE_OK) return st;
     GPE_HIP_TRY(hipMemcpyAsync(out, s->scal + 4, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     GPE_HIP_TRY(hipStreamSynchronize(s->stream));

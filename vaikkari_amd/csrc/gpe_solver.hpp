@@ -56,5 +56,5 @@ struct gpe_solver {
     gpe_status step(uint64_t nsteps, uint64_t *launches);
     void imag_variant(int &k, int &t) const;
     template <typename T> gpe_status observables_t(int which, double out[4]);
-    template <typename T, int FORM> gpe_status observe_step(int tile);
+    template <typename T, int FORM> gpe_status observe_step(int tile, int even_tile);
 };
