@@ -110,9 +110,9 @@ int gather_rows(int ty, const Layout &L) {
 // 32 layers} in FP64 and {2 waves, 16 layers} in FP32.  Z-march: tile 1.
 // Gather family, a rule fitted to the gather tiles timed on MI355X (tiles
 // 1-14 over 24^3 to 256^3 cells, 87x87x302, 100x60x140 and 300x300x200,
-// tiles 15-18 over 16^3 to 128^3 and 87x87x302; both precisions, both modes;
-// against tiles 1-14 the rule's choice ran at 99 % of the fastest on
-// geometric mean, 92 % at worst):
+// tiles 15-24 over 16^3 to 256^3 and 87x87x302; both precisions, both modes;
+// against tiles 1-14 its one-thread-per-cell part, cell_default_tile, ran at
+// 99 % of the fastest on geometric mean, 92 % at worst):
 //  - real time with both time levels in 256 MiB (the Infinity Cache keeps
 //    them between the two passes): a cached read-modify-write in contiguous
 //    eighths, in FP64 one thread per node with 4 classes per block (tile 18:
