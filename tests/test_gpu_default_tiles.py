@@ -69,10 +69,10 @@ def test_default_imaginary_time_tile_by_grid(dims, precision, want, kernel):
     assert info["kernel"] == "gather" and info["tile"] == want
 
 
-@pytest.mark.parametrize("n,precision", [(32, "fp64"), (40, "fp64"), (40, "fp32")])
+@pytest.mark.parametrize("n,precision", [(32, "fp64"), (40, "fp64"), (40, "fp32"), (96, "fp64")])
 def test_default_tile_steps_equal_other_tiles(n, precision):
-    """The small-grid defaults (tile 14 at 32^3, the node tile 18 at 40^3 FP64, the class-group
-    tile 19 at 40^3 FP32) step the same bits as tile 11, the 128^3 default."""
+    """The defaults (tile 14 at 32^3, the node tile 18 at 40^3 FP64, the class-group tile 19 at
+    40^3 FP32, the class-group y-band tile 24 at 96^3 FP64) step the same bits as tile 11."""
     announce(f"default tile {n}^3 {precision} bits")
     dims, h = (n, n, n), 0.25
     b = h / np.sqrt(8.0)
