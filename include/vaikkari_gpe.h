@@ -91,8 +91,8 @@ typedef struct gpe_config {
     uint32_t kernel;       /* gpe_kernel */
     uint32_t tile;         /* tile shape 1..gpe_variant_tiles(kernel), 0 = the family's defaults;
                               gather: by grid, precision and mode (real time with both fields in
-                              256 MiB: tile 18 FP64, 19 FP32, or 14 on grids of at most 1024
-                              one-wave blocks; imaginary time with both fields in 256 MiB:
+                              256 MiB: tile 18 FP64, 19 FP32, or 23 on grids of at most 1024
+                              one-wave blocks whose rows split into 8 y-bands; imaginary time with both fields in 256 MiB:
                               tile 20 FP64 from 256 four-wave blocks, 3 FP32 from 1024;
                               FP64 real time beyond: tile 24 when the wave rows split into 8
                               y-bands of whole 6-wave blocks, else 22;

@@ -14,7 +14,8 @@ def announce(msg):
 
 # (dims, precision) -> real-time default tile (1-based)
 CASES = [
-    ((32, 32, 32), "fp64", 14),   # both fields cached, 512 one-wave blocks in y-bands
+    ((32, 32, 32), "fp64", 23),   # both fields cached, 512 one-wave blocks: class groups in y-bands
+    ((32, 32, 32), "fp32", 23),
     ((24, 24, 24), "fp64", 18),   # cached: one thread per node in FP64 (one-wave rows 12, no y-bands)
     ((24, 24, 24), "fp32", 19),   # cached: one thread per group of 4 classes in FP32
     ((48, 48, 48), "fp32", 19),   # cached
@@ -71,8 +72,9 @@ def test_default_imaginary_time_tile_by_grid(dims, precision, want, kernel):
 
 @pytest.mark.parametrize("n,precision", [(32, "fp64"), (40, "fp64"), (40, "fp32"), (96, "fp64")])
 def test_default_tile_steps_equal_other_tiles(n, precision):
-    """The defaults (tile 14 at 32^3, the node tile 18 at 40^3 FP64, the class-group tile 19 at
-    40^3 FP32, the class-group y-band tile 24 at 96^3 FP64) step the same bits as tile 11."""
+    """The defaults (the class-group y-band tile 23 at 32^3, the node tile 18 at 40^3 FP64, the
+    class-group tile 19 at 40^3 FP32, the class-group y-band tile 24 at 96^3 FP64) step the same
+    bits as tile 11."""
     announce(f"default tile {n}^3 {precision} bits")
     dims, h = (n, n, n), 0.25
     b = h / np.sqrt(8.0)

@@ -118,9 +118,10 @@ int gather_rows(int ty, const Layout &L) {
 //    eighths, in FP64 one thread per node with 4 classes per block (tile 18:
 //    +4 to +15 % over one thread per cell at 56^3-80^3), in FP32 one thread
 //    per cell and group of 4 classes, 3 waves per block (tile 19: +2 to +14 %
-//    at 24^3-96^3); {1 wave, y-bands, streaming} (tile 14) instead when its
-//    block rows divide into y-bands and it launches at most 1024 blocks
-//    (32^3: 1.5x tile 4);
+//    at 24^3-96^3); on grids of at most 1024 one-wave blocks whose rows
+//    split into y-bands (32^3: latency-bound), y-bands instead: one thread per
+//    cell and group of 4 classes, 3 waves per block (tile 23; 32^3: +6 % FP64,
+//    +10 % FP32 over {1 wave, y-bands} (tile 14), itself 1.5x tile 4);
 //  - imaginary time with both time levels in 256 MiB: cached stores in
 //    contiguous eighths, since the Euler step's output is the next step's
 //    input and a streaming store sends it to HBM; FP64 from 256 blocks of 4
@@ -168,7 +169,9 @@ int default_tile(int kernel, uint32_t precision, int mode, const Layout &L) {
     if (kernel == GPE_KERNEL_FUSED) return precision == GPE_FP64 ? 10 : 13;
     if (kernel != GPE_KERNEL_GATHER) return 0;
     const GridFacts f = grid_facts(precision, L);
-    if (mode == MODE_REAL && f.cached) return f.tiny_band ? 13 : precision == GPE_FP64 ? 17 : 18;
+    // tiny grids: class groups in y-bands (3-wave blocks always split the bands
+    // of wave rows evenly, as their count divides by 8)
+    if (mode == MODE_REAL && f.cached) return f.tiny_band ? 22 : precision == GPE_FP64 ? 17 : 18;
     if (mode == MODE_IMAG && f.cached && precision == GPE_FP64 && f.b4 >= 256) return 19;
     if (mode == MODE_REAL && precision == GPE_FP64 && !f.cached) {
         const int rows = gather_rows(1, L);  // wave rows per layer
