@@ -93,7 +93,7 @@ typedef struct gpe_config {
                               gather: by grid, precision and mode (real time with both fields in
                               256 MiB: tile 18 FP64, 19 FP32, or 23 on grids of at most 1024
                               one-wave blocks whose rows split into 8 y-bands; imaginary time with both fields in 256 MiB:
-                              tile 20 FP64 from 256 four-wave blocks, 3 FP32 from 1024;
+                              tile 19 FP64 from 256 four-wave blocks, 6 FP32 from 700;
                               FP64 real time beyond: tile 24 when the wave rows split into 8
                               y-bands of whole 6-wave blocks, else 22;
                               else the y-band tile with the most waves whose block rows divide

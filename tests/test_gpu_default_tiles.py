@@ -45,11 +45,12 @@ def test_default_gather_tile_by_grid(dims, precision, want):
 # (dims, precision) -> imaginary-time default tile (1-based)
 IMAG_CASES = [
     ((32, 32, 32), "fp64", 12),    # 128 four-wave blocks: y-bands of 2 waves
-    ((48, 48, 48), "fp64", 20),    # cached, 432 four-wave blocks: cached stores, groups of 4 classes
-    ((48, 48, 48), "fp32", 7),     # FP32 needs 1024 four-wave blocks for cached stores; no y-bands
-    ((64, 64, 64), "fp64", 20),    # cached, 1024 four-wave blocks
-    ((64, 64, 64), "fp32", 3),     # cached stores, 4 waves
-    ((96, 96, 96), "fp32", 3),     # 170 MB: cached
+    ((48, 48, 48), "fp64", 19),    # cached, 432 four-wave blocks: cached stores, groups of 4 classes
+    ((48, 48, 48), "fp32", 7),     # FP32 needs 700 four-wave blocks for cached stores; no y-bands
+    ((56, 56, 56), "fp32", 6),     # cached, 784 four-wave blocks: cached stores, 8 waves
+    ((64, 64, 64), "fp64", 19),    # cached, 1024 four-wave blocks
+    ((64, 64, 64), "fp32", 6),
+    ((96, 96, 96), "fp32", 6),     # 170 MB: cached
     ((96, 96, 96), "fp64", 12),    # 340 MB: y-bands, 8- and 4-wave rows (6, 12) do not divide by 8
     ((128, 128, 128), "fp64", 13),
     ((128, 128, 128), "fp32", 11),

@@ -125,8 +125,9 @@ int gather_rows(int ty, const Layout &L) {
 //  - imaginary time with both time levels in 256 MiB: cached stores in
 //    contiguous eighths, since the Euler step's output is the next step's
 //    input and a streaming store sends it to HBM; FP64 from 256 blocks of 4
-//    waves, one thread per group of 4 classes, 6 waves per block (tile 20);
-//    FP32 from 1024, 4 waves per block (tile 3) (48^3-96^3: +5 to +27 %);
+//    waves, one thread per group of 4 classes, 3 waves per block (tile 19);
+//    FP32 from 700, one thread per cell, 8 waves per block (tile 6)
+//    (48^3-96^3: +5 to +30 % over streaming stores);
 //  - FP64 real time beyond the cache: one thread per cell and group of 4
 //    classes with streaming read-modify-write, in y-bands per XCD when the
 //    wave rows split into them (tile 24, 6 waves per block), else in plain
@@ -157,7 +158,7 @@ GridFacts grid_facts(uint32_t precision, const Layout &L) {
 int cell_default_tile(uint32_t precision, int mode, const Layout &L) {
     const GridFacts f = grid_facts(precision, L);
     if (mode == MODE_REAL && f.cached) return f.tiny_band ? 13 : 3;
-    if (mode == MODE_IMAG && f.cached && f.b4 >= 1024) return precision == GPE_FP64 ? 5 : 2;
+    if (mode == MODE_IMAG && f.cached && f.b4 >= 700) return 5;
     const int wmax = precision == GPE_FP64 && mode == MODE_IMAG ? 8 : 4;
     static constexpr int kWaves[4] = {8, 4, 2, 1}, kBandTile[4] = {12, 10, 11, 13};
     for (int i = 0; i < 4; i++)
@@ -172,7 +173,7 @@ int default_tile(int kernel, uint32_t precision, int mode, const Layout &L) {
     // tiny grids: class groups in y-bands (3-wave blocks always split the bands
     // of wave rows evenly, as their count divides by 8)
     if (mode == MODE_REAL && f.cached) return f.tiny_band ? 22 : precision == GPE_FP64 ? 17 : 18;
-    if (mode == MODE_IMAG && f.cached && precision == GPE_FP64 && f.b4 >= 256) return 19;
+    if (mode == MODE_IMAG && f.cached && precision == GPE_FP64 && f.b4 >= 256) return 18;
     if (mode == MODE_REAL && precision == GPE_FP64 && !f.cached) {
         const int rows = gather_rows(1, L);  // wave rows per layer
         return rows % 8 == 0 && (3 * f.gx * (rows / 8)) % 6 == 0 ? 23 : 21;
