@@ -95,7 +95,7 @@ typedef struct gpe_config {
                               one-wave blocks whose rows split into 8 y-bands; imaginary time with both fields in 256 MiB:
                               tile 19 FP64 from 256 four-wave blocks, 6 FP32 from 700;
                               FP64 real time beyond: tile 24 when the wave rows split into 8
-                              y-bands of whole 6-wave blocks, else 22;
+                              y-bands of whole 6-wave blocks;
                               else the y-band tile with the most waves whose block rows divide
                               by 8: 13, 11, 12, 14 for 8 (FP64 imaginary time only), 4, 2, 1
                               waves; none: tile 7).  128^3 cells: FP64 tile 24 (imaginary

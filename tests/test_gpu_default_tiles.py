@@ -27,7 +27,8 @@ CASES = [
     ((128, 128, 128), "fp32", 11),
     ((160, 160, 160), "fp64", 24),
     ((160, 160, 160), "fp32", 12),  # 4-wave rows 20, 2-wave rows 40
-    ((87, 87, 302), "fp64", 22),    # 44 wave rows: no y-bands, class groups in plain order
+    ((87, 87, 302), "fp64", 7),     # 44 wave rows: no y-bands; one thread per cell in eighths
+    ((112, 112, 112), "fp64", 7),   # 28 wave rows: the same
 ]
 
 

@@ -128,10 +128,12 @@ int gather_rows(int ty, const Layout &L) {
 //    waves, one thread per group of 4 classes, 3 waves per block (tile 19);
 //    FP32 from 700, one thread per cell, 8 waves per block (tile 6)
 //    (48^3-96^3: +5 to +30 % over streaming stores);
-//  - FP64 real time beyond the cache: one thread per cell and group of 4
-//    classes with streaming read-modify-write, in y-bands per XCD when the
-//    wave rows split into them (tile 24, 6 waves per block), else in plain
-//    order (tile 22): +2 to +6 % over one thread per cell at 96^3-256^3;
+//  - FP64 real time beyond the cache, when the wave rows split into y-bands
+//    of whole 6-wave blocks: one thread per cell and group of 4 classes with
+//    streaming read-modify-write in y-bands per XCD (tile 24): +2 to +6 %
+//    over one thread per cell at 96^3-256^3 (in plain order, tile 22, the
+//    class groups are level with tile 7 where no y-bands fit: -4 % at 112^3,
+//    +1.5 % at 87x87x302);
 //  - otherwise y-bands per XCD with streaming read-modify-write, the most
 //    waves per block (up to 8 in FP64 imaginary time, 4 else: tiles 13, 11,
 //    12, 14 for 8, 4, 2, 1) whose block rows divide by the 8 XCDs; when none
@@ -176,7 +178,7 @@ int default_tile(int kernel, uint32_t precision, int mode, const Layout &L) {
     if (mode == MODE_IMAG && f.cached && precision == GPE_FP64 && f.b4 >= 256) return 18;
     if (mode == MODE_REAL && precision == GPE_FP64 && !f.cached) {
         const int rows = gather_rows(1, L);  // wave rows per layer
-        return rows % 8 == 0 && (3 * f.gx * (rows / 8)) % 6 == 0 ? 23 : 21;
+        if (rows % 8 == 0 && (3 * f.gx * (rows / 8)) % 6 == 0) return 23;
     }
     return cell_default_tile(precision, mode, L);
 }
